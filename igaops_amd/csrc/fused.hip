@@ -1,0 +1,769 @@
+// Fused sum-factorised 3-D operator  y = sum_{a,b} T_a^T diag(D_ab) I_b x
+// (trial interpolation I_b with the banded basis, point-wise 4x4 coefficient
+// contraction D at the quadrature points, test contraction T_a with the banded
+// weighted-quadrature weights), in one pass over the coefficient stream.
+//
+// Decomposition (all indices direction-1 fastest):
+//   * a work-group owns a TQ1 x TQ2 tile of (q1, q2) quadrature columns and a
+//     slab of q3 planes; thread (l1, l2) owns one column.
+//   * sweeping q3 upwards, each thread keeps two register rings of P = p+1
+//     DOF planes c3: the trial partials T2(q1,q2,c3) (3 fields) and the test
+//     partials S2(q1,q2,c3) (3 fields).  A trial plane is built from the DOF
+//     sub-block x(c1 in band(tile), c2 in band(tile), c3) staged in LDS; a test
+//     plane leaves the ring when no later q3 touches it and is contracted in
+//     directions 2 and 1 through LDS into a partial DOF block of the tile.
+//   * partial blocks of neighbouring tiles / slabs overlap by the band halo; a
+//     second kernel adds them in a fixed order (deterministic result).
+// Coefficients are stored per (tile, q3, field) as contiguous runs over the
+// tile's columns, so every load of the stream is a coalesced full-wave read.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "common.hpp"
+
+namespace iga {
+
+enum FusedForm { FORM_M1 = 0, FORM_K6 = 1, FORM_KM7 = 2, FORM_SYM10 = 3, FORM_GEN16 = 4 };
+
+__host__ __device__ constexpr int form_ncoef(int f) {
+  return f == FORM_M1 ? 1 : f == FORM_K6 ? 6 : f == FORM_KM7 ? 7 : f == FORM_SYM10 ? 10 : 16;
+}
+
+constexpr int kTQ1 = 16, kTQ2 = 16, kThreads = kTQ1 * kTQ2;
+constexpr int kDMax = 32;  // max DOF footprint of a tile in directions 1, 2
+constexpr int kPMax = 9;   // max band width p+1
+constexpr int kKMax = 4;   // max tiles/slabs whose footprint contains one DOF index
+
+template <typename T>
+struct DirTab {
+  const int32_t* first;  // [nq] band start
+  const T* tab;          // [4][nq][P]: B0, B1, W0, W1 (W are the transposed test weights)
+};
+
+template <typename T>
+struct FusedArgs {
+  int nc[3], nq[3];
+  int nt1, nt2, nslab, nrhs, nlogical;
+  const int32_t *t1_q0, *t1_c0, *t1_nc;  // [nt1+1], [nt1], [nt1]
+  const int32_t *t2_q0, *t2_c0, *t2_nc;
+  const int32_t *s_q0, *s_c0, *s_nc;  // [nslab+1], [nslab], [nslab]
+  const int64_t* coef_base;           // [nt1*nt2]
+  const int64_t* part_base;           // [nt1*nt2*nslab]
+  int64_t part_stride;                // elements per right-hand side
+  const T* coef;
+  const T* x;
+  const T* w;  // NURBS weights or nullptr
+  T* partial;
+  DirTab<T> d[3];
+};
+
+// logical block id: the blocks dispatched to one XCD (round-robin over 8) get a
+// contiguous logical range, so the right-hand sides of one tile-slab, adjacent
+// in logical order, share that XCD's L2 for the coefficient stream.
+__device__ inline int xcd_logical(int b, int nblk) {
+  const int per = nblk / 8;
+  return (b % 8) * per + b / 8;
+}
+
+template <typename T, int P, int FORM>
+__device__ inline void point_contract(const T* __restrict__ c, int64_t stride, const T u[4], T f[4]) {
+  if constexpr (FORM == FORM_M1) {
+    f[0] = c[0] * u[0];
+    f[1] = f[2] = f[3] = T(0);
+  } else if constexpr (FORM == FORM_K6) {
+    const T c11 = c[0], c22 = c[stride], c33 = c[2 * stride];
+    const T c12 = c[3 * stride], c13 = c[4 * stride], c23 = c[5 * stride];
+    f[0] = T(0);
+    f[1] = c11 * u[1] + c12 * u[2] + c13 * u[3];
+    f[2] = c12 * u[1] + c22 * u[2] + c23 * u[3];
+    f[3] = c13 * u[1] + c23 * u[2] + c33 * u[3];
+  } else if constexpr (FORM == FORM_KM7) {
+    const T m = c[0], c11 = c[stride], c22 = c[2 * stride], c33 = c[3 * stride];
+    const T c12 = c[4 * stride], c13 = c[5 * stride], c23 = c[6 * stride];
+    f[0] = m * u[0];
+    f[1] = c11 * u[1] + c12 * u[2] + c13 * u[3];
+    f[2] = c12 * u[1] + c22 * u[2] + c23 * u[3];
+    f[3] = c13 * u[1] + c23 * u[2] + c33 * u[3];
+  } else if constexpr (FORM == FORM_SYM10) {
+    const T d00 = c[0], d11 = c[stride], d22 = c[2 * stride], d33 = c[3 * stride];
+    const T d01 = c[4 * stride], d02 = c[5 * stride], d03 = c[6 * stride];
+    const T d12 = c[7 * stride], d13 = c[8 * stride], d23 = c[9 * stride];
+    f[0] = d00 * u[0] + d01 * u[1] + d02 * u[2] + d03 * u[3];
+    f[1] = d01 * u[0] + d11 * u[1] + d12 * u[2] + d13 * u[3];
+    f[2] = d02 * u[0] + d12 * u[1] + d22 * u[2] + d23 * u[3];
+    f[3] = d03 * u[0] + d13 * u[1] + d23 * u[2] + d33 * u[3];
+  } else {
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      T acc = T(0);
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc += c[(a * 4 + b) * stride] * u[b];
+      f[a] = acc;
+    }
+  }
+}
+
+template <typename T, int P, int FORM>
+__global__ void __launch_bounds__(kThreads) fused_apply_kernel(FusedArgs<T> a) {
+  constexpr int NC = form_ncoef(FORM);
+  __shared__ T xs[kDMax][kDMax + 1];
+  __shared__ T t1s[2][kDMax][kTQ1];
+  __shared__ T s2s[3][kTQ2][kTQ1];
+  __shared__ T r1s[2][kDMax][kTQ1];
+  __shared__ T b1s[4][kTQ1][P];
+  __shared__ T b2s[4][kTQ2][P];
+  __shared__ int f1s[kTQ1];
+  __shared__ int f2s[kTQ2];
+
+  const int L = xcd_logical(blockIdx.x, gridDim.x);
+  if (L >= a.nlogical) return;  // padding blocks of the XCD remap (whole block exits)
+  int rem = L;
+  const int k = rem % a.nrhs;
+  rem /= a.nrhs;
+  const int s = rem % a.nslab;
+  rem /= a.nslab;
+  const int t1 = rem % a.nt1;
+  const int t2 = rem / a.nt1;
+  const int tile = t2 * a.nt1 + t1;
+
+  const int tid = threadIdx.x;
+  const int l1 = tid % kTQ1, l2 = tid / kTQ1;
+  const int q1s = a.t1_q0[t1], q1n = a.t1_q0[t1 + 1] - q1s;
+  const int q2s = a.t2_q0[t2], q2n = a.t2_q0[t2 + 1] - q2s;
+  const int c1lo = a.t1_c0[t1], D1 = a.t1_nc[t1];
+  const int c2lo = a.t2_c0[t2], D2 = a.t2_nc[t2];
+  const int q3s = a.s_q0[s], q3e = a.s_q0[s + 1];
+  const int c3lo = a.s_c0[s];
+  const bool active = (l1 < q1n) && (l2 < q2n);
+  const int n1 = a.nc[0], n2 = a.nc[1];
+  const int nr = a.nrhs;
+
+  // stage the direction-1 and -2 tables of this tile
+  for (int i = tid; i < 4 * kTQ1 * P; i += kThreads) {
+    const int m = i / (kTQ1 * P), l = (i / P) % kTQ1, j = i % P;
+    b1s[m][l][j] = (l < q1n) ? a.d[0].tab[((int64_t)m * a.nq[0] + q1s + l) * P + j] : T(0);
+  }
+  for (int i = tid; i < 4 * kTQ2 * P; i += kThreads) {
+    const int m = i / (kTQ2 * P), l = (i / P) % kTQ2, j = i % P;
+    b2s[m][l][j] = (l < q2n) ? a.d[1].tab[((int64_t)m * a.nq[1] + q2s + l) * P + j] : T(0);
+  }
+  if (tid < kTQ1) f1s[tid] = (tid < q1n) ? a.d[0].first[q1s + tid] - c1lo : 0;
+  if (tid < kTQ2) f2s[tid] = (tid < q2n) ? a.d[1].first[q2s + tid] - c2lo : 0;
+  __syncthreads();
+
+  T tr[3][P];  // trial partials of the window planes: (val,val), (d1,val), (val,d2)
+  T te[3][P];  // test partials: (val,val), (d1,val), (val,d2) in directions (1,2)
+
+  // Builds the trial partials of DOF plane c3 for this thread's column.
+  auto trial_plane = [&](int c3, T& v00, T& v10, T& v01) {
+    __syncthreads();  // xs / t1s are reused
+    const T* xk = a.x + k;
+    for (int i = tid; i < D1 * D2; i += kThreads) {
+      const int i2 = i / D1, i1 = i % D1;
+      const int64_t dof = (int64_t)(c1lo + i1) + (int64_t)n1 * ((int64_t)(c2lo + i2) + (int64_t)n2 * c3);
+      T v = xk[dof * nr];
+      if (a.w) v *= a.w[dof];
+      xs[i2][i1] = v;
+    }
+    __syncthreads();
+    for (int i = tid; i < D2 * kTQ1; i += kThreads) {
+      const int i2 = i / kTQ1, l = i % kTQ1;
+      const int f = f1s[l];
+      T acc0 = T(0), acc1 = T(0);
+      if (l < q1n) {
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+          const T xv = xs[i2][f + j];
+          acc0 += b1s[0][l][j] * xv;
+          acc1 += b1s[1][l][j] * xv;
+        }
+      }
+      t1s[0][i2][l] = acc0;
+      t1s[1][i2][l] = acc1;
+    }
+    __syncthreads();
+    v00 = v10 = v01 = T(0);
+    if (active) {
+      const int f = f2s[l2];
+#pragma unroll
+      for (int j = 0; j < P; ++j) {
+        const T b0 = b2s[0][l2][j], b1 = b2s[1][l2][j];
+        const T a0 = t1s[0][f + j][l1], a1 = t1s[1][f + j][l1];
+        v00 += b0 * a0;
+        v10 += b0 * a1;
+        v01 += b1 * a0;
+      }
+    }
+  };
+
+  // Contracts the oldest test plane (ring slot 0, DOF plane c3) in directions
+  // 2 and 1 and writes the tile's partial block of that plane.
+  T* part = a.partial + (int64_t)k * a.part_stride + a.part_base[(int64_t)tile * a.nslab + s];
+  auto flush_plane = [&](int c3) {
+    __syncthreads();  // s2s / r1s are reused
+    s2s[0][l2][l1] = active ? te[0][0] : T(0);
+    s2s[1][l2][l1] = active ? te[1][0] : T(0);
+    s2s[2][l2][l1] = active ? te[2][0] : T(0);
+    __syncthreads();
+    for (int i = tid; i < D2 * kTQ1; i += kThreads) {
+      const int c2l = i / kTQ1, l = i % kTQ1;
+      T r0 = T(0), r1 = T(0);
+      for (int m = 0; m < q2n; ++m) {
+        const int j = c2l - f2s[m];
+        if (j >= 0 && j < P) {
+          r0 += b2s[2][m][j] * s2s[0][m][l] + b2s[3][m][j] * s2s[2][m][l];
+          r1 += b2s[2][m][j] * s2s[1][m][l];
+        }
+      }
+      r1s[0][c2l][l] = r0;
+      r1s[1][c2l][l] = r1;
+    }
+    __syncthreads();
+    T* blk = part + (int64_t)(c3 - c3lo) * D2 * D1;
+    for (int i = tid; i < D2 * D1; i += kThreads) {
+      const int c2l = i / D1, c1l = i % D1;
+      T acc = T(0);
+      for (int l = 0; l < q1n; ++l) {
+        const int j = c1l - f1s[l];
+        if (j >= 0 && j < P) acc += b1s[2][l][j] * r1s[0][c2l][l] + b1s[3][l][j] * r1s[1][c2l][l];
+      }
+      blk[i] = acc;
+    }
+  };
+
+  const int32_t* first3 = a.d[2].first;
+  const T* tab3 = a.d[2].tab;
+  const int64_t nq3 = a.nq[2];
+  int cur = first3[q3s];
+#pragma unroll
+  for (int j = 0; j < P; ++j) {
+    trial_plane(cur + j, tr[0][j], tr[1][j], tr[2][j]);
+    te[0][j] = te[1][j] = te[2][j] = T(0);
+  }
+
+  const int npts = q1n * q2n;
+  const T* cptr = a.coef + a.coef_base[tile] + (l1 + q1n * l2);
+  for (int q3 = q3s; q3 < q3e; ++q3) {
+    const int f = first3[q3];
+    while (cur < f) {
+      flush_plane(cur);
+#pragma unroll
+      for (int j = 0; j < P - 1; ++j) {
+#pragma unroll
+        for (int g = 0; g < 3; ++g) {
+          te[g][j] = te[g][j + 1];
+          tr[g][j] = tr[g][j + 1];
+        }
+      }
+      te[0][P - 1] = te[1][P - 1] = te[2][P - 1] = T(0);
+      trial_plane(cur + P, tr[0][P - 1], tr[1][P - 1], tr[2][P - 1]);
+      ++cur;
+    }
+    if (active) {
+      T bb0[P], bb1[P], ww0[P], ww1[P];
+#pragma unroll
+      for (int j = 0; j < P; ++j) {
+        bb0[j] = tab3[(0 * nq3 + q3) * P + j];
+        bb1[j] = tab3[(1 * nq3 + q3) * P + j];
+        ww0[j] = tab3[(2 * nq3 + q3) * P + j];
+        ww1[j] = tab3[(3 * nq3 + q3) * P + j];
+      }
+      T u[4] = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+      for (int j = 0; j < P; ++j) {
+        u[0] += bb0[j] * tr[0][j];
+        u[1] += bb0[j] * tr[1][j];
+        u[2] += bb0[j] * tr[2][j];
+        u[3] += bb1[j] * tr[0][j];
+      }
+      T fv[4];
+      point_contract<T, P, FORM>(cptr + (int64_t)q3 * NC * npts, npts, u, fv);
+#pragma unroll
+      for (int j = 0; j < P; ++j) {
+        te[0][j] += ww0[j] * fv[0] + ww1[j] * fv[3];
+        te[1][j] += ww0[j] * fv[1];
+        te[2][j] += ww0[j] * fv[2];
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < P; ++j) {
+    flush_plane(cur);
+#pragma unroll
+    for (int jj = 0; jj < P - 1; ++jj) {
+#pragma unroll
+      for (int g = 0; g < 3; ++g) te[g][jj] = te[g][jj + 1];
+    }
+    ++cur;
+  }
+}
+
+// y[a*r + k] = alpha * wscale[a] * sum of the partial blocks covering DOF a (+ y)
+template <typename T>
+struct GatherArgs {
+  int nc[3], nt1, nt2, nslab, nrhs;
+  const int32_t *l1_cnt, *l1_t, *l1_off;  // [nc1], [nc1][kKMax] x2
+  const int32_t *l2_cnt, *l2_t, *l2_off;
+  const int32_t *l3_cnt, *l3_t, *l3_off;
+  const int32_t *t1_nc, *t2_nc;
+  const int64_t* part_base;
+  int64_t part_stride;
+  const T* partial;
+  const T* w;
+  T* y;
+  T alpha;
+  int accumulate;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256) fused_gather_kernel(GatherArgs<T> g) {
+  const int64_t ndof = (int64_t)g.nc[0] * g.nc[1] * g.nc[2];
+  const int64_t total = ndof * g.nrhs;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int k = (int)(idx % g.nrhs);
+    const int64_t dof = idx / g.nrhs;
+    const int c1 = (int)(dof % g.nc[0]);
+    const int c2 = (int)((dof / g.nc[0]) % g.nc[1]);
+    const int c3 = (int)(dof / ((int64_t)g.nc[0] * g.nc[1]));
+    const T* pk = g.partial + (int64_t)k * g.part_stride;
+    T acc = T(0);
+    for (int i3 = 0; i3 < g.l3_cnt[c3]; ++i3) {
+      const int s = g.l3_t[c3 * kKMax + i3], o3 = g.l3_off[c3 * kKMax + i3];
+      for (int i2 = 0; i2 < g.l2_cnt[c2]; ++i2) {
+        const int t2 = g.l2_t[c2 * kKMax + i2], o2 = g.l2_off[c2 * kKMax + i2];
+        const int D2 = g.t2_nc[t2];
+        for (int i1 = 0; i1 < g.l1_cnt[c1]; ++i1) {
+          const int t1 = g.l1_t[c1 * kKMax + i1], o1 = g.l1_off[c1 * kKMax + i1];
+          const int D1 = g.t1_nc[t1];
+          const int64_t base = g.part_base[((int64_t)t2 * g.nt1 + t1) * g.nslab + s];
+          acc += pk[base + ((int64_t)o3 * D2 + o2) * D1 + o1];
+        }
+      }
+    }
+    T v = g.alpha * acc;
+    if (g.w) v *= g.w[dof];
+    if (g.accumulate) v += g.y[idx];
+    g.y[idx] = v;
+  }
+}
+
+// Packs reference-layout coefficients into the fused layout of one FORM.
+//   C: (3,3,nq) with C[i][j] multiplying trial d_j and test d_i; m: (nq);
+//   W, dW: NURBS weight function and its gradient at the points (nullptr for
+//   B-splines).  NURBS folding (operators/nurbs_ops.py:343-390):
+//   D_ij = C_ij / W^2, D_00 = z^T C z + m / W^2, D_0j = -(C^T z)_j / W,
+//   D_i0 = -(C z)_i / W with z = dW / W^2.
+template <typename T>
+struct PackArgs {
+  int nq[3], nt1, form, ncoef;
+  int tq1, tq2;
+  const int32_t *t1_q0, *t2_q0;
+  const int64_t* coef_base;
+  const double* C;
+  const double* m;
+  const double* W;
+  const double* dW;
+  T* out;
+};
+
+template <typename T>
+__global__ void fused_pack_kernel(PackArgs<T> p) {
+  const int64_t nqt = (int64_t)p.nq[0] * p.nq[1] * p.nq[2];
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < nqt; q += (int64_t)gridDim.x * blockDim.x) {
+    const int q1 = (int)(q % p.nq[0]);
+    const int q2 = (int)((q / p.nq[0]) % p.nq[1]);
+    const int64_t q3 = q / ((int64_t)p.nq[0] * p.nq[1]);
+    const int t1 = q1 / p.tq1, t2 = q2 / p.tq2;
+    const int l1 = q1 - p.t1_q0[t1], l2 = q2 - p.t2_q0[t2];
+    const int q1n = p.t1_q0[t1 + 1] - p.t1_q0[t1];
+    const int q2n = p.t2_q0[t2 + 1] - p.t2_q0[t2];
+    const int64_t npts = (int64_t)q1n * q2n;
+    T* dst = p.out + p.coef_base[t2 * p.nt1 + t1] + q3 * p.ncoef * npts + (l1 + (int64_t)q1n * l2);
+    double D[4][4] = {{0}};
+    double c[3][3] = {{0}};
+    if (p.C)
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) c[i][j] = p.C[(int64_t)(i * 3 + j) * nqt + q];
+    const double mm = p.m ? p.m[q] : 0.0;
+    if (p.W) {
+      const double wi = 1.0 / p.W[q];
+      double z[3];
+      for (int i = 0; i < 3; ++i) z[i] = p.dW[(int64_t)i * nqt + q] * wi * wi;
+      double cz[3] = {0, 0, 0}, ctz[3] = {0, 0, 0}, zcz = 0.0;
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+          cz[i] += c[i][j] * z[j];
+          ctz[j] += c[i][j] * z[i];
+          zcz += z[i] * c[i][j] * z[j];
+        }
+      D[0][0] = zcz + mm * wi * wi;
+      for (int j = 0; j < 3; ++j) {
+        D[0][j + 1] = -ctz[j] * wi;
+        D[j + 1][0] = -cz[j] * wi;
+      }
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) D[i + 1][j + 1] = c[i][j] * wi * wi;
+    } else {
+      D[0][0] = mm;
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) D[i + 1][j + 1] = c[i][j];
+    }
+    const int64_t st = npts;
+    switch (p.form) {
+      case FORM_M1:
+        dst[0] = (T)D[0][0];
+        break;
+      case FORM_K6:
+        dst[0] = (T)D[1][1]; dst[st] = (T)D[2][2]; dst[2 * st] = (T)D[3][3];
+        dst[3 * st] = (T)D[1][2]; dst[4 * st] = (T)D[1][3]; dst[5 * st] = (T)D[2][3];
+        break;
+      case FORM_KM7:
+        dst[0] = (T)D[0][0]; dst[st] = (T)D[1][1]; dst[2 * st] = (T)D[2][2]; dst[3 * st] = (T)D[3][3];
+        dst[4 * st] = (T)D[1][2]; dst[5 * st] = (T)D[1][3]; dst[6 * st] = (T)D[2][3];
+        break;
+      case FORM_SYM10:
+        dst[0] = (T)D[0][0]; dst[st] = (T)D[1][1]; dst[2 * st] = (T)D[2][2]; dst[3 * st] = (T)D[3][3];
+        dst[4 * st] = (T)D[0][1]; dst[5 * st] = (T)D[0][2]; dst[6 * st] = (T)D[0][3];
+        dst[7 * st] = (T)D[1][2]; dst[8 * st] = (T)D[1][3]; dst[9 * st] = (T)D[2][3];
+        break;
+      default:
+        for (int a = 0; a < 4; ++a)
+          for (int b = 0; b < 4; ++b) dst[(a * 4 + b) * st] = (T)D[a][b];
+    }
+  }
+}
+
+}  // namespace iga
+
+using namespace iga;
+
+struct iga_fused_s {
+  int dtype = IGA_F64, form = FORM_KM7, P = 0;
+  int nc[3] = {0, 0, 0}, nq[3] = {0, 0, 0};
+  int nt1 = 0, nt2 = 0, nslab = 0;
+  int64_t coef_elems = 0, part_stride = 0;
+  // device tables
+  int32_t* d_first[3] = {nullptr, nullptr, nullptr};
+  void* d_tab[3] = {nullptr, nullptr, nullptr};  // T[4][nq][P]
+  int32_t *d_t1_q0 = nullptr, *d_t1_c0 = nullptr, *d_t1_nc = nullptr;
+  int32_t *d_t2_q0 = nullptr, *d_t2_c0 = nullptr, *d_t2_nc = nullptr;
+  int32_t *d_s_q0 = nullptr, *d_s_c0 = nullptr, *d_s_nc = nullptr;
+  int64_t *d_coef_base = nullptr, *d_part_base = nullptr;
+  int32_t* d_list[3][3] = {{nullptr}};  // [dir][cnt, t, off]
+  std::vector<void*> owned;
+};
+
+namespace iga {
+
+template <typename V>
+static V* upload(iga_fused_s* f, const std::vector<V>& h) {
+  V* d = nullptr;
+  IGA_HIP(hipMalloc(&d, sizeof(V) * std::max<size_t>(h.size(), 1)));
+  f->owned.push_back(d);
+  if (!h.empty()) IGA_HIP(hipMemcpy(d, h.data(), sizeof(V) * h.size(), hipMemcpyHostToDevice));
+  return d;
+}
+
+static void free_fused(iga_fused_s* f) {
+  for (void* p : f->owned) (void)hipFree(p);
+  f->owned.clear();
+}
+
+// Tiling of one direction into runs of `tq` points, with the DOF footprint of each run.
+struct DirTiling {
+  std::vector<int32_t> q0, c0, nc;                  // q0 has ntile+1 entries
+  std::vector<int32_t> cnt, t, off;                 // per DOF: covering tiles (kKMax slots)
+};
+
+static DirTiling tile_direction(const int32_t* first, int nq, int ncdof, int P, int tq, int dmax) {
+  DirTiling d;
+  for (int q = 0; q < nq; q += tq) d.q0.push_back(q);
+  d.q0.push_back(nq);
+  const int nt = (int)d.q0.size() - 1;
+  d.cnt.assign(ncdof, 0);
+  d.t.assign((size_t)ncdof * kKMax, 0);
+  d.off.assign((size_t)ncdof * kKMax, 0);
+  for (int t = 0; t < nt; ++t) {
+    int lo = first[d.q0[t]], hi = first[d.q0[t]];
+    for (int q = d.q0[t]; q < d.q0[t + 1]; ++q) {
+      lo = std::min(lo, (int)first[q]);
+      hi = std::max(hi, (int)first[q]);
+    }
+    const int n = hi + P - lo;
+    IGA_CHECK_ARG(dmax <= 0 || n <= dmax, "tile footprint exceeds the kernel's LDS block");
+    d.c0.push_back(lo);
+    d.nc.push_back(n);
+    for (int c = lo; c < lo + n; ++c) {
+      IGA_CHECK_ARG(d.cnt[c] < kKMax, "a DOF is covered by too many tiles");
+      d.t[(size_t)c * kKMax + d.cnt[c]] = t;
+      d.off[(size_t)c * kKMax + d.cnt[c]] = c - lo;
+      d.cnt[c]++;
+    }
+  }
+  for (int c = 0; c < ncdof; ++c) IGA_CHECK_ARG(d.cnt[c] > 0, "a DOF is not covered by any tile");
+  return d;
+}
+
+template <typename T, int P>
+static void launch_apply_p(const iga_fused_s* f, const FusedArgs<T>& a, int nblk, hipStream_t s) {
+  switch (f->form) {
+    case FORM_M1: fused_apply_kernel<T, P, FORM_M1><<<nblk, kThreads, 0, s>>>(a); break;
+    case FORM_K6: fused_apply_kernel<T, P, FORM_K6><<<nblk, kThreads, 0, s>>>(a); break;
+    case FORM_KM7: fused_apply_kernel<T, P, FORM_KM7><<<nblk, kThreads, 0, s>>>(a); break;
+    case FORM_SYM10: fused_apply_kernel<T, P, FORM_SYM10><<<nblk, kThreads, 0, s>>>(a); break;
+    default: fused_apply_kernel<T, P, FORM_GEN16><<<nblk, kThreads, 0, s>>>(a); break;
+  }
+}
+
+template <typename T>
+static void launch_apply(const iga_fused_s* f, const FusedArgs<T>& a, int nblk, hipStream_t s) {
+  switch (f->P) {
+    case 3: launch_apply_p<T, 3>(f, a, nblk, s); break;
+    case 4: launch_apply_p<T, 4>(f, a, nblk, s); break;
+    case 5: launch_apply_p<T, 5>(f, a, nblk, s); break;
+    case 6: launch_apply_p<T, 6>(f, a, nblk, s); break;
+    case 7: launch_apply_p<T, 7>(f, a, nblk, s); break;
+    case 8: launch_apply_p<T, 8>(f, a, nblk, s); break;
+    default: throw Error(IGA_ERR_UNSUPPORTED, "band width p+1 must be in 3..8");
+  }
+}
+
+// stages: bit 1 = main sweep (partial blocks), bit 2 = gather into y
+template <typename T>
+static void run_apply(const iga_fused_s* f, const void* coef, const void* x, void* y, int nrhs, const void* w,
+                      double alpha, int accumulate, void* work, hipStream_t s, int stages = 3) {
+  if (stages & 1) {
+  FusedArgs<T> a{};
+  for (int d = 0; d < 3; ++d) {
+    a.nc[d] = f->nc[d];
+    a.nq[d] = f->nq[d];
+    a.d[d].first = f->d_first[d];
+    a.d[d].tab = (const T*)f->d_tab[d];
+  }
+  a.nt1 = f->nt1;
+  a.nt2 = f->nt2;
+  a.nslab = f->nslab;
+  a.nrhs = nrhs;
+  a.t1_q0 = f->d_t1_q0; a.t1_c0 = f->d_t1_c0; a.t1_nc = f->d_t1_nc;
+  a.t2_q0 = f->d_t2_q0; a.t2_c0 = f->d_t2_c0; a.t2_nc = f->d_t2_nc;
+  a.s_q0 = f->d_s_q0; a.s_c0 = f->d_s_c0; a.s_nc = f->d_s_nc;
+  a.coef_base = f->d_coef_base;
+  a.part_base = f->d_part_base;
+  a.part_stride = f->part_stride;
+  a.coef = (const T*)coef;
+  a.x = (const T*)x;
+  a.w = (const T*)w;
+  a.partial = (T*)work;
+  const int64_t nlog = (int64_t)f->nt1 * f->nt2 * f->nslab * nrhs;
+  IGA_CHECK_ARG(nlog < (1LL << 30), "too many work-groups");
+  a.nlogical = (int)nlog;
+  const int nblk = (int)((nlog + 7) / 8 * 8);
+  launch_apply<T>(f, a, nblk, s);
+  IGA_LAUNCH_CHECK();
+  }
+  if (!(stages & 2)) return;
+
+  GatherArgs<T> g{};
+  for (int d = 0; d < 3; ++d) g.nc[d] = f->nc[d];
+  g.nt1 = f->nt1;
+  g.nt2 = f->nt2;
+  g.nslab = f->nslab;
+  g.nrhs = nrhs;
+  g.l1_cnt = f->d_list[0][0]; g.l1_t = f->d_list[0][1]; g.l1_off = f->d_list[0][2];
+  g.l2_cnt = f->d_list[1][0]; g.l2_t = f->d_list[1][1]; g.l2_off = f->d_list[1][2];
+  g.l3_cnt = f->d_list[2][0]; g.l3_t = f->d_list[2][1]; g.l3_off = f->d_list[2][2];
+  g.t1_nc = f->d_t1_nc;
+  g.t2_nc = f->d_t2_nc;
+  g.part_base = f->d_part_base;
+  g.part_stride = f->part_stride;
+  g.partial = (const T*)work;
+  g.w = (const T*)w;
+  g.y = (T*)y;
+  g.alpha = (T)alpha;
+  g.accumulate = accumulate;
+  const int64_t total = (int64_t)f->nc[0] * f->nc[1] * f->nc[2] * nrhs;
+  fused_gather_kernel<T><<<grid_for(total, 256), 256, 0, s>>>(g);
+  IGA_LAUNCH_CHECK();
+}
+
+template <typename T>
+static void run_pack(const iga_fused_s* f, const double* C, const double* m, const double* W, const double* dW,
+                     void* out, hipStream_t s) {
+  PackArgs<T> p{};
+  for (int d = 0; d < 3; ++d) p.nq[d] = f->nq[d];
+  p.nt1 = f->nt1;
+  p.form = f->form;
+  p.ncoef = form_ncoef(f->form);
+  p.tq1 = kTQ1;
+  p.tq2 = kTQ2;
+  p.t1_q0 = f->d_t1_q0;
+  p.t2_q0 = f->d_t2_q0;
+  p.coef_base = f->d_coef_base;
+  p.C = C;
+  p.m = m;
+  p.W = W;
+  p.dW = dW;
+  p.out = (T*)out;
+  const int64_t nqt = (int64_t)f->nq[0] * f->nq[1] * f->nq[2];
+  fused_pack_kernel<T><<<grid_for(nqt, 256), 256, 0, s>>>(p);
+  IGA_LAUNCH_CHECK();
+}
+
+}  // namespace iga
+
+extern "C" {
+
+int iga_fused_create(iga_fused_t* out, const int32_t* nc, const int32_t* nq, int P, const int32_t* const* first,
+                     const double* const* tabs, int form, int dtype, int nslab) {
+  return guard([&] {
+    IGA_CHECK_ARG(out && nc && nq && first && tabs, "NULL argument");
+    IGA_CHECK_ARG(P >= 3 && P <= 8, "band width p+1 must be in 3..8");
+    IGA_CHECK_ARG(form >= FORM_M1 && form <= FORM_GEN16, "unknown coefficient form");
+    IGA_CHECK_ARG(dtype == IGA_F64 || dtype == IGA_F32, "unknown dtype");
+    for (int d = 0; d < 3; ++d) {
+      IGA_CHECK_ARG(nc[d] >= P && nq[d] >= 1, "bad sizes");
+      for (int q = 0; q < nq[d]; ++q) {
+        IGA_CHECK_ARG(first[d][q] >= 0 && first[d][q] + P <= nc[d], "band leaves the DOF range");
+        if (q > 0) IGA_CHECK_ARG(first[d][q] >= first[d][q - 1], "band starts must be non-decreasing");
+      }
+    }
+    auto* f = new iga_fused_s();
+    try {
+      f->dtype = dtype;
+      f->form = form;
+      f->P = P;
+      for (int d = 0; d < 3; ++d) {
+        f->nc[d] = nc[d];
+        f->nq[d] = nq[d];
+      }
+      DirTiling d1 = tile_direction(first[0], nq[0], nc[0], P, kTQ1, kDMax);
+      DirTiling d2 = tile_direction(first[1], nq[1], nc[1], P, kTQ2, kDMax);
+      f->nt1 = (int)d1.c0.size();
+      f->nt2 = (int)d2.c0.size();
+      // slabs of q3: enough work-groups to fill the device (>= ~2048 with one rhs)
+      int ns = nslab;
+      if (ns <= 0) {
+        const int tiles = f->nt1 * f->nt2;
+        ns = std::max(1, std::min(nq[2] / (3 * P), (2048 + tiles - 1) / tiles));
+      }
+      ns = std::max(1, std::min(ns, nq[2]));
+      const int slab = (nq[2] + ns - 1) / ns;
+      DirTiling d3 = tile_direction(first[2], nq[2], nc[2], P, slab, 0);
+      f->nslab = (int)d3.c0.size();
+      // packed coefficient offsets per tile
+      const int ncoef = form_ncoef(form);
+      std::vector<int64_t> coef_base(f->nt1 * f->nt2), part_base((size_t)f->nt1 * f->nt2 * f->nslab);
+      int64_t cb = 0, pb = 0;
+      for (int t2 = 0; t2 < f->nt2; ++t2)
+        for (int t1 = 0; t1 < f->nt1; ++t1) {
+          const int64_t npts = (int64_t)(d1.q0[t1 + 1] - d1.q0[t1]) * (d2.q0[t2 + 1] - d2.q0[t2]);
+          coef_base[t2 * f->nt1 + t1] = cb;
+          cb += npts * ncoef * nq[2];
+          for (int sl = 0; sl < f->nslab; ++sl) {
+            part_base[((size_t)t2 * f->nt1 + t1) * f->nslab + sl] = pb;
+            pb += (int64_t)d1.nc[t1] * d2.nc[t2] * d3.nc[sl];
+          }
+        }
+      f->coef_elems = cb;
+      f->part_stride = pb;
+      // band tables in the apply precision
+      for (int d = 0; d < 3; ++d) {
+        f->d_first[d] = upload(f, std::vector<int32_t>(first[d], first[d] + nq[d]));
+        const size_t nt = (size_t)4 * nq[d] * P;
+        if (dtype == IGA_F64) {
+          f->d_tab[d] = upload(f, std::vector<double>(tabs[d], tabs[d] + nt));
+        } else {
+          std::vector<float> t32(nt);
+          for (size_t i = 0; i < nt; ++i) t32[i] = (float)tabs[d][i];
+          f->d_tab[d] = upload(f, t32);
+        }
+      }
+      f->d_t1_q0 = upload(f, d1.q0); f->d_t1_c0 = upload(f, d1.c0); f->d_t1_nc = upload(f, d1.nc);
+      f->d_t2_q0 = upload(f, d2.q0); f->d_t2_c0 = upload(f, d2.c0); f->d_t2_nc = upload(f, d2.nc);
+      f->d_s_q0 = upload(f, d3.q0); f->d_s_c0 = upload(f, d3.c0); f->d_s_nc = upload(f, d3.nc);
+      f->d_coef_base = upload(f, coef_base);
+      f->d_part_base = upload(f, part_base);
+      const DirTiling* dt[3] = {&d1, &d2, &d3};
+      for (int d = 0; d < 3; ++d) {
+        f->d_list[d][0] = upload(f, dt[d]->cnt);
+        f->d_list[d][1] = upload(f, dt[d]->t);
+        f->d_list[d][2] = upload(f, dt[d]->off);
+      }
+    } catch (...) {
+      free_fused(f);
+      delete f;
+      throw;
+    }
+    *out = f;
+  });
+}
+
+int iga_fused_destroy(iga_fused_t f) {
+  return guard([&] {
+    if (!f) return;
+    free_fused(f);
+    delete f;
+  });
+}
+
+int iga_fused_info(iga_fused_t f, int64_t* coef_elems, int64_t* part_elems_per_rhs, int* ntiles1, int* ntiles2,
+                   int* nslabs) {
+  return guard([&] {
+    IGA_CHECK_ARG(f, "NULL plan");
+    if (coef_elems) *coef_elems = f->coef_elems;
+    if (part_elems_per_rhs) *part_elems_per_rhs = f->part_stride;
+    if (ntiles1) *ntiles1 = f->nt1;
+    if (ntiles2) *ntiles2 = f->nt2;
+    if (nslabs) *nslabs = f->nslab;
+  });
+}
+
+int64_t iga_fused_workspace_bytes(iga_fused_t f, int nrhs) {
+  if (!f || nrhs < 1) return 0;
+  return f->part_stride * (int64_t)nrhs * (f->dtype == IGA_F64 ? 8 : 4);
+}
+
+int iga_fused_pack(iga_fused_t f, const double* C, const double* m, const double* W, const double* dW, void* packed,
+                   void* stream) {
+  return guard([&] {
+    IGA_CHECK_ARG(f && packed, "NULL argument");
+    IGA_CHECK_ARG((W == nullptr) == (dW == nullptr), "W and dW go together");
+    if (f->form != FORM_M1) IGA_CHECK_ARG(C, "stiffness forms need C");
+    if (f->form == FORM_M1 || f->form == FORM_KM7) IGA_CHECK_ARG(m, "mass forms need m");
+    if (f->dtype == IGA_F64)
+      run_pack<double>(f, C, m, W, dW, packed, as_stream(stream));
+    else
+      run_pack<float>(f, C, m, W, dW, packed, as_stream(stream));
+  });
+}
+
+int iga_fused_apply(iga_fused_t f, const void* packed, const void* x, void* y, int nrhs, const void* nurbs_w,
+                    double alpha, int accumulate, void* work, void* stream) {
+  return guard([&] {
+    IGA_CHECK_ARG(f && packed && x && y && work, "NULL argument");
+    IGA_CHECK_ARG(nrhs >= 1, "nrhs must be >= 1");
+    if (f->dtype == IGA_F64)
+      run_apply<double>(f, packed, x, y, nrhs, nurbs_w, alpha, accumulate, work, as_stream(stream));
+    else
+      run_apply<float>(f, packed, x, y, nrhs, nurbs_w, alpha, accumulate, work, as_stream(stream));
+  });
+}
+
+int iga_fused_apply_stage(iga_fused_t f, int stage, const void* packed, const void* x, void* y, int nrhs,
+                          const void* nurbs_w, double alpha, int accumulate, void* work, void* stream) {
+  return guard([&] {
+    IGA_CHECK_ARG(f && work && (stage == 1 || stage == 2), "bad argument");
+    IGA_CHECK_ARG(stage == 2 || (packed && x), "main stage needs coefficients and x");
+    IGA_CHECK_ARG(stage == 1 || y, "gather stage needs y");
+    IGA_CHECK_ARG(nrhs >= 1, "nrhs must be >= 1");
+    if (f->dtype == IGA_F64)
+      run_apply<double>(f, packed, x, y, nrhs, nurbs_w, alpha, accumulate, work, as_stream(stream), stage);
+    else
+      run_apply<float>(f, packed, x, y, nrhs, nurbs_w, alpha, accumulate, work, as_stream(stream), stage);
+  });
+}
+
+}  // extern "C"

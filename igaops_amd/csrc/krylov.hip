@@ -1,0 +1,189 @@
+// Krylov vector kernels: many-dot products (Gram-Schmidt projections, norms),
+// many-axpy updates, axpby and precision casts.  Reductions are deterministic:
+// fixed per-block chunks, a 64-lane wavefront shuffle tree, then a fixed-order
+// second pass over the per-block partials.
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+
+namespace iga {
+
+constexpr int kRedThreads = 256;
+constexpr int kRedPerThread = 8;
+constexpr int kRedChunk = kRedThreads * kRedPerThread;  // elements per block
+constexpr int kRedMaxBlocks = 2048;
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// Block-wide sum; the result is valid in thread 0.
+__device__ inline double block_sum(double v, double* lds) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) lds[wave] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += lds[w];
+  return s;
+}
+
+static int reduce_blocks(int64_t n) {
+  int64_t g = (n + kRedChunk - 1) / kRedChunk;
+  if (g < 1) g = 1;
+  if (g > kRedMaxBlocks) g = kRedMaxBlocks;
+  return (int)g;
+}
+
+// partial[b * k + j] = sum over block b's elements of V[j][i] * w[i]
+template <typename T>
+__global__ void __launch_bounds__(kRedThreads) dot_many_stage1(int64_t n, int k, const T* __restrict__ V,
+                                                               int64_t ldv, const T* __restrict__ w,
+                                                               double* __restrict__ partial) {
+  __shared__ double lds[kRedThreads / 64];
+  const int nb = gridDim.x;
+  // contiguous, equal-size chunks per block: deterministic for given (n, grid)
+  const int64_t per = (n + nb - 1) / nb;
+  const int64_t lo = (int64_t)blockIdx.x * per;
+  const int64_t hi = lo + per < n ? lo + per : n;
+  for (int j = 0; j < k; ++j) {
+    const T* v = V + (int64_t)j * ldv;
+    double acc = 0.0;
+    for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) acc += (double)v[i] * (double)w[i];
+    const double s = block_sum(acc, lds);
+    if (threadIdx.x == 0) partial[(int64_t)blockIdx.x * k + j] = s;
+  }
+}
+
+__global__ void __launch_bounds__(kRedThreads) dot_many_stage2(int nb, int k, const double* __restrict__ partial,
+                                                               double* __restrict__ out) {
+  __shared__ double lds[kRedThreads / 64];
+  const int j = blockIdx.x;
+  double acc = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) acc += partial[(int64_t)b * k + j];
+  const double s = block_sum(acc, lds);
+  if (threadIdx.x == 0) out[j] = s;
+}
+
+template <typename T>
+__global__ void axpy_many_kernel(int64_t n, int k, const T* __restrict__ V, int64_t ldv,
+                                 const double* __restrict__ h, double alpha, T* __restrict__ w) {
+  extern __shared__ double hs[];
+  for (int j = threadIdx.x; j < k; j += blockDim.x) hs[j] = h[j];
+  __syncthreads();
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double acc = 0.0;
+    for (int j = 0; j < k; ++j) acc += hs[j] * (double)V[(int64_t)j * ldv + i];
+    w[i] = (T)((double)w[i] + alpha * acc);
+  }
+}
+
+template <typename T>
+__global__ void axpby_kernel(int64_t n, double a, const double* a_dev, int invert_a, const T* __restrict__ x,
+                             double b, T* __restrict__ y) {
+  double av = a_dev ? a_dev[0] : a;
+  if (invert_a) av = 1.0 / av;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double yv = (b == 0.0) ? 0.0 : b * (double)y[i];
+    y[i] = (T)(av * (double)x[i] + yv);
+  }
+}
+
+template <typename S, typename D>
+__global__ void cast_kernel(int64_t n, const S* __restrict__ x, D* __restrict__ y) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = (D)x[i];
+}
+
+}  // namespace iga
+
+using namespace iga;
+
+extern "C" {
+
+int64_t iga_reduce_workspace_bytes(int64_t n, int k) {
+  if (n < 0 || k < 0) return 0;
+  return (int64_t)reduce_blocks(n) * (k > 0 ? k : 1) * (int64_t)sizeof(double);
+}
+
+int iga_dot_many(int dtype, int64_t n, int k, const void* V, int64_t ldv, const void* w, double* out,
+                 void* work, void* stream) {
+  return guard([&] {
+    IGA_CHECK_ARG(V && w && out && work && n >= 0 && k >= 0 && ldv >= n, "bad argument");
+    if (k == 0) return;
+    hipStream_t s = as_stream(stream);
+    const int nb = reduce_blocks(n);
+    double* partial = (double*)work;
+    if (dtype == IGA_F64)
+      dot_many_stage1<double><<<nb, kRedThreads, 0, s>>>(n, k, (const double*)V, ldv, (const double*)w, partial);
+    else if (dtype == IGA_F32)
+      dot_many_stage1<float><<<nb, kRedThreads, 0, s>>>(n, k, (const float*)V, ldv, (const float*)w, partial);
+    else
+      throw Error(IGA_ERR_INVALID, "unknown dtype");
+    IGA_LAUNCH_CHECK();
+    dot_many_stage2<<<k, kRedThreads, 0, s>>>(nb, k, partial, out);
+    IGA_LAUNCH_CHECK();
+  });
+}
+
+int iga_axpy_many(int dtype, int64_t n, int k, const void* V, int64_t ldv, const double* h, double alpha, void* w,
+                  void* stream) {
+  return guard([&] {
+    IGA_CHECK_ARG(V && w && h && n >= 0 && k >= 0 && ldv >= n, "bad argument");
+    IGA_CHECK_ARG(k <= 8192, "too many vectors");
+    if (k == 0 || n == 0) return;
+    hipStream_t s = as_stream(stream);
+    dim3 grid(grid_for(n, 256, 8192)), block(256);
+    const size_t shm = sizeof(double) * (size_t)k;
+    if (dtype == IGA_F64)
+      axpy_many_kernel<double><<<grid, block, shm, s>>>(n, k, (const double*)V, ldv, h, alpha, (double*)w);
+    else if (dtype == IGA_F32)
+      axpy_many_kernel<float><<<grid, block, shm, s>>>(n, k, (const float*)V, ldv, h, alpha, (float*)w);
+    else
+      throw Error(IGA_ERR_INVALID, "unknown dtype");
+    IGA_LAUNCH_CHECK();
+  });
+}
+
+int iga_axpby(int dtype, int64_t n, double a, const double* a_dev, int invert_a, const void* x, double b, void* y,
+              void* stream) {
+  return guard([&] {
+    IGA_CHECK_ARG(x && y && n >= 0, "bad argument");
+    if (n == 0) return;
+    dim3 grid(grid_for(n, 256, 8192)), block(256);
+    hipStream_t s = as_stream(stream);
+    if (dtype == IGA_F64)
+      axpby_kernel<double><<<grid, block, 0, s>>>(n, a, a_dev, invert_a, (const double*)x, b, (double*)y);
+    else if (dtype == IGA_F32)
+      axpby_kernel<float><<<grid, block, 0, s>>>(n, a, a_dev, invert_a, (const float*)x, b, (float*)y);
+    else
+      throw Error(IGA_ERR_INVALID, "unknown dtype");
+    IGA_LAUNCH_CHECK();
+  });
+}
+
+int iga_cast(int dtype_x, int dtype_y, int64_t n, const void* x, void* y, void* stream) {
+  return guard([&] {
+    IGA_CHECK_ARG(x && y && n >= 0, "bad argument");
+    if (n == 0) return;
+    dim3 grid(grid_for(n, 256, 8192)), block(256);
+    hipStream_t s = as_stream(stream);
+    if (dtype_x == IGA_F64 && dtype_y == IGA_F32)
+      cast_kernel<double, float><<<grid, block, 0, s>>>(n, (const double*)x, (float*)y);
+    else if (dtype_x == IGA_F32 && dtype_y == IGA_F64)
+      cast_kernel<float, double><<<grid, block, 0, s>>>(n, (const float*)x, (double*)y);
+    else if (dtype_x == IGA_F64 && dtype_y == IGA_F64)
+      cast_kernel<double, double><<<grid, block, 0, s>>>(n, (const double*)x, (double*)y);
+    else if (dtype_x == IGA_F32 && dtype_y == IGA_F32)
+      cast_kernel<float, float><<<grid, block, 0, s>>>(n, (const float*)x, (float*)y);
+    else
+      throw Error(IGA_ERR_INVALID, "unknown dtype");
+    IGA_LAUNCH_CHECK();
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,251 @@
+"""Fast-diagonalization (FD) preconditioner: host LAPACK set-up, device apply.
+
+Set-up follows ``fastdiag/utils/space_precond.py:124-236`` (Gauss rewrite of the
+rules, Dirichlet trimming, ``+1e-8 M`` regularisation of untrimmed directions,
+``scipy.linalg.eigh`` of each 1-D pencil, Kronecker-sum eigenvalues with correctors,
+clip at ``1e-14 max``).  The apply ``y[free] = U (Lambda^{-1} o U^T x[free])``,
+``y = 0`` elsewhere (``space_precond.py:273-320``) runs on the device: the free nodes
+form a box (one interior range per direction), so gather/scatter are box copies, the
+Kronecker transforms are dense mode products and ``Lambda^{-1}`` is a point-wise scale.
+"""
+from __future__ import annotations
+
+from typing import List, Optional, Sequence
+
+import numpy as np
+import scipy.linalg as sla
+
+from . import _lib
+from . import device as dv
+from .common import Constants, ParametricDirection, combine_arrays, kron_nonzero_indices, validate_entry
+from .quadrature import StandardGauss
+
+
+def rewrite_quadrature(quadrule_list) -> List[StandardGauss]:
+    """Gauss-Legendre rules on the same spaces (``fastdiag/utils/template.py:28-45``)."""
+    if not isinstance(quadrule_list, (list, tuple)):
+        raise ValueError("quadrule_list should be iterable")
+    out = []
+    for q in quadrule_list:
+        if getattr(q, "quadrature_class", None) == "GAUSS" and getattr(q, "quadrature_type", None) == "legendre":
+            out.append(q)
+        else:
+            out.append(StandardGauss(q.degree, q.knotvector, quadtype="legendre"))
+    return out
+
+
+class SpaceFD:
+    """Spatial FD for one DOF per node (``nbDoFsPerNode = 1``, the scalar problems of
+    the path; the reference's vector-valued loop over DOFs per node is the same apply
+    repeated on each component)."""
+
+    def __init__(self):
+        self.nbDoFsPerNode = 0
+        self.space_scalar_coefs = (0.0, 1.0)
+        self._eigvals_cache = None
+        self._dev = {}
+
+    def compute_space_eigendecomposition(self, space_quadrule_list, space_table_dirichlet):
+        table = np.asarray(space_table_dirichlet, dtype=bool)
+        if table.ndim != 3:
+            raise ValueError("space_table_dirichlet must have shape (ndof_per_node, ndim, 2)")
+        if table.shape[0] != 1:
+            raise NotImplementedError("one DOF per node is supported")
+        qs = rewrite_quadrature(space_quadrule_list)
+        nbdirs = min(table.shape[1], len(qs))
+        self.nbDoFsPerNode = table.shape[0]
+        self.nnz = [q.nbctrlpts for q in qs[:nbdirs]]
+        self.space_dirs = [ParametricDirection(i) for i in range(nbdirs)]
+        self.indices: List[np.ndarray] = []
+        self.eigenvec_by_dir_space: List[np.ndarray] = []
+        self.eigenval_by_dir_space: List[np.ndarray] = []
+        for d in range(nbdirs):
+            q = qs[d]
+            M = (q.weights[0] @ q.basis[0]).toarray()
+            K = (q.weights[-1] @ q.basis[-1]).toarray()
+            lo = 1 if table[0, d, 0] else 0
+            hi = self.nnz[d] - (1 if table[0, d, 1] else 0)
+            idx = np.arange(lo, hi)
+            Mi = M[np.ix_(idx, idx)]
+            Ki = K[np.ix_(idx, idx)]
+            if len(idx) == self.nnz[d]:
+                Ki = Ki + Constants.TINY * Mi
+            lam, U = sla.eigh(Ki, Mi)
+            self.indices.append(idx)
+            self.eigenvec_by_dir_space.append(np.real(U))
+            self.eigenval_by_dir_space.append(np.real(lam))
+        self.space_table_dirichlet = table
+        self.mass_space_corrector = [1.0]
+        self.stiff_space_corrector = [np.ones(nbdirs)]
+        self._eigvals_cache = None
+        self._dev = {}
+
+    # -- reference-compatible accessors (fastdiag/utils/space_precond.py:40-102) --
+    @property
+    def sp_nnz(self) -> int:
+        return int(np.prod(self.nnz))
+
+    @property
+    def space_free_nodes(self) -> List[List[int]]:
+        nnz = list(self.nnz)[::-1]
+        return [kron_nonzero_indices([idx.tolist() for idx in self.indices[::-1]], nnz)]
+
+    def update_space_eigenvalues(self, scalar_coefs: Sequence[float]) -> List[np.ndarray]:
+        """``space_precond.py:195-236``."""
+        if len(scalar_coefs) < 2:
+            return []
+        mixed = combine_arrays(self.eigenval_by_dir_space, self.stiff_space_corrector[0])
+        cur = scalar_coefs[0] * self.mass_space_corrector[0] * np.ones_like(mixed) + scalar_coefs[1] * mixed
+        cur = np.maximum(cur, Constants.SAFEGUARD * np.max(cur))
+        self.space_scalar_coefs = (float(scalar_coefs[0]), float(scalar_coefs[1]))
+        self._eigvals_cache = [cur]
+        self._dev.pop("inv_eig", None)
+        return [cur]
+
+    @property
+    def space_eigenvalues(self) -> List[np.ndarray]:
+        if self._eigvals_cache is None:
+            self.update_space_eigenvalues(self.space_scalar_coefs)
+        return self._eigvals_cache
+
+    def add_scalar_space_correctors(self, mass_corrector=None, stiffness_corrector=None):
+        ok = lambda e: isinstance(e, (list, tuple)) and len(e) == self.nbDoFsPerNode
+        if mass_corrector is not None and ok(mass_corrector):
+            self.mass_space_corrector = list(mass_corrector)
+            self._eigvals_cache = None
+            self._dev.pop("inv_eig", None)
+        if stiffness_corrector is not None and ok(stiffness_corrector):
+            self.stiff_space_corrector = [np.asarray(s, dtype=float) for s in stiffness_corrector]
+            self._eigvals_cache = None
+            self._dev.pop("inv_eig", None)
+
+    # -- device apply ---------------------------------------------------------
+    def box(self):
+        lo = np.array([int(idx[0]) if len(idx) else 0 for idx in self.indices], dtype=np.int64)
+        ln = np.array([len(idx) for idx in self.indices], dtype=np.int64)
+        return np.array(self.nnz, dtype=np.int64), lo, ln
+
+    def device_factors(self, dtype: str):
+        from .operators import Band
+
+        key = ("U", dtype)
+        if key not in self._dev:
+            Ut = [Band([U.T.copy()]) for U in self.eigenvec_by_dir_space]
+            Un = [Band([U]) for U in self.eigenvec_by_dir_space]
+            self._dev[key] = (Ut, Un)
+        if "inv_eig" not in self._dev or self._dev["inv_eig"][0] != dtype:
+            inv = 1.0 / self.space_eigenvalues[0]
+            self._dev["inv_eig"] = (dtype, dv.to_device(inv, dtype))
+        return self._dev[key][0], self._dev[key][1], self._dev["inv_eig"][1]
+
+    def apply_device(self, x, out=None, nrhs: int = 1, stream=None):
+        """``out = P x`` for device arrays ((N, nrhs) C order)."""
+        from .operators import mf_apply_device
+
+        t = dv.torch()
+        dtype = "f32" if x.dtype == t.float32 else "f64"
+        Ut, Un, inv = self.device_factors(dtype)
+        n, lo, ln = self.box()
+        nsub = int(np.prod(ln))
+        na, npt = _lib.i64_array(n)
+        la, lpt = _lib.i64_array(lo)
+        lna, lnpt = _lib.i64_array(ln)
+        dt = dv.iga_dtype(x)
+        s = dv.stream_ptr(stream)
+        sub = t.empty(nsub * nrhs, dtype=x.dtype, device=x.device)
+        _lib.call("iga_box_gather", dt, len(n), npt, lpt, lnpt, nrhs, dv.ptr(x), dv.ptr(sub), s)
+        tmp = mf_apply_device(Ut, sub, nrhs, stream=stream)
+        _lib.call("iga_qp_scale", dt, nsub, nrhs, dv.ptr(inv), dv.ptr(tmp), dv.ptr(tmp), 0, 1.0, s)
+        res = mf_apply_device(Un, tmp, nrhs, stream=stream)
+        if out is None:
+            out = t.empty_like(x)
+        _lib.call("iga_box_scatter", dt, len(n), npt, lpt, lnpt, nrhs, dv.ptr(res), dv.ptr(out), 1, s)
+        return out
+
+
+class SingleFastDiagonalization:
+    """``fastdiag/preconditioner.py:15-197`` (spatial type; the space-time type is in
+    ``igaops_amd.spacetime``)."""
+
+    def __init__(self, preconditioner_type: str):
+        self._preconditioner_type = validate_entry(preconditioner_type, ["spatial", "spacetime"])
+        self._space_fd: Optional[SpaceFD] = None
+        self._sptime_fd = None
+
+    @property
+    def preconditioner_type(self):
+        return self._preconditioner_type
+
+    @property
+    def space_preconditioner(self) -> SpaceFD:
+        if self._space_fd is None:
+            raise RuntimeError("Space preconditioner not defined.")
+        return self._space_fd
+
+    @property
+    def sptm_preconditioner(self):
+        if self._preconditioner_type == "spatial" or self._sptime_fd is None:
+            raise RuntimeError("Space-time preconditioner not defined.")
+        return self._sptime_fd
+
+    def compute_space_eigendecomposition(self, space_quadrule_list, space_table_dirichlet):
+        fd = SpaceFD()
+        fd.compute_space_eigendecomposition(space_quadrule_list, space_table_dirichlet)
+        self._space_fd = fd
+
+    def compute_time_schurdecomposition(self, time_quadrule, use_arrowhead: bool = True):
+        if self._preconditioner_type == "spatial":
+            raise RuntimeError("Space-time preconditioner not defined.")
+        if not use_arrowhead:
+            raise NotImplementedError("only the arrowhead space-time FD is provided")
+        from .spacetime import SpaceTimeFD
+
+        st = SpaceTimeFD(self.space_preconditioner)
+        st.compute_time_schurdecomposition(time_quadrule)
+        self._sptime_fd = st
+
+    def update_space_eigenvalues(self, scalar_coefs):
+        self.space_preconditioner.update_space_eigenvalues(scalar_coefs)
+        if self._sptime_fd is not None:
+            self._sptime_fd.invalidate()
+
+    def add_scalar_space_time_correctors(self, mass_corrector=None, stiffness_corrector=None, advection_corrector=None):
+        if self._space_fd is not None:
+            self._space_fd.add_scalar_space_correctors(mass_corrector, stiffness_corrector)
+        if self._sptime_fd is not None:
+            self._sptime_fd.add_scalar_time_correctors(advection_corrector)
+
+    def apply_device(self, x, out=None, nrhs=1, stream=None):
+        if self._preconditioner_type == "spatial":
+            return self.space_preconditioner.apply_device(x, out, nrhs, stream)
+        return self.sptm_preconditioner.apply_device(x, out, nrhs, stream)
+
+    def apply_transpose_device(self, x, out=None, nrhs=1, stream=None):
+        if self._preconditioner_type == "spatial":
+            return self.space_preconditioner.apply_device(x, out, nrhs, stream)
+        return self.sptm_preconditioner.apply_device(x, out, nrhs, stream, transpose=True)
+
+    def _host_call(self, fn, array_in):
+        dv.require_gpu()
+        t = dv.torch()
+        was_np = not isinstance(array_in, t.Tensor)
+        shape = tuple(array_in.shape)
+        r = int(np.prod(shape[1:])) if len(shape) > 1 else 1
+        dtype = "f32" if (not was_np and array_in.dtype == t.float32) else "f64"
+        x = dv.to_device(array_in, dtype).reshape(-1)
+        y = fn(x, None, r)
+        y = y.reshape(shape)
+        return y.double().cpu().numpy() if was_np else y
+
+    def apply(self, array_in):
+        """``preconditioner.py:147-163``."""
+        return self._host_call(self.apply_device, array_in)
+
+    def apply_transpose(self, array_in):
+        """``preconditioner.py:165-182``."""
+        return self._host_call(self.apply_transpose_device, array_in)
+
+    __call__ = apply
+
+
+SingleFD = SingleFastDiagonalization

@@ -1,0 +1,664 @@
+"""Matrix-free operators on the GPU, behind the reference's call signatures.
+
+* ``MatrixFree.apply(matrix_list, array_in, is_transpose=False)``
+  (``operators/utils/matrix_free.py:77-119``): d banded mode products on the device.
+* ``BsplineOperations`` / ``NurbsOperations`` ``.compute_mf_scalar_{u_v, gradu_gradv,
+  gradu_v, u_gradv}(quadrule_list, coefficients, array_in, time_ders=None,
+  nurbs_weights=np.array([]))`` (``operators/utils/template.py:312-454``,
+  ``operators/bsplines_ops.py:290-429``, ``operators/nurbs_ops.py:320-463``): same
+  arguments and results; numpy inputs give numpy outputs (uploaded per call), torch
+  device tensors stay on the device.
+* ``StiffnessMassOperator``: a device-resident plan for ``y = a K x + b M x`` (the
+  benchmark matvec), coefficients packed once; used by the solvers and the server.
+
+Two execution paths, both HIP:
+  - fused (``csrc/fused.hip``): 3-D, rules whose test weights depend only on the test
+    derivative (Gauss, WQ type "2"), one pass over the coefficient stream;
+  - composed (``csrc/lmode.hip``): any dimension / time derivatives / WQ type "1",
+    mirroring the reference's sequence of L-mode products and point-wise products
+    (terms with the same trial interpolation share it, terms with the same test
+    weights share the test contraction).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+from scipy import sparse as sp
+
+from . import _lib
+from . import device as dv
+from .bands import common_band, fused_direction_tables, test_weights_depend_on_alpha_only
+
+# ---------------------------------------------------------------------------
+# Device bands
+# ---------------------------------------------------------------------------
+
+
+class Band:
+    """Device copy of ``nmat`` matrices sharing one band (``iga_band_t``)."""
+
+    def __init__(self, mats: Sequence):
+        first, width, vals = common_band([_as_csr(m) for m in mats])
+        self.nrows, self.ncols = _as_csr(mats[0]).shape
+        self.width = width
+        self.nmat = len(mats)
+        h = C.c_void_p()
+        fa, fp = _lib.i32_array(first)
+        va, vp = _lib.f64_array(vals)
+        _lib.call("iga_band_create", C.byref(h), self.nrows, self.ncols, width, self.nmat, fp, vp)
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib._lib is not None:
+            _lib.load().iga_band_destroy(h)
+            self._h = None
+
+    def apply(self, x, ys: Sequence, outer: int, inner: int, accumulate: bool = False, alpha: float = 1.0, stream=None):
+        arr = (C.c_void_p * 4)(*[dv.ptr(y) for y in ys] + [0] * (4 - len(ys)))
+        _lib.call(
+            "iga_band_mode_product",
+            self._h,
+            dv.iga_dtype(x),
+            int(outer),
+            int(inner),
+            dv.ptr(x),
+            arr,
+            int(bool(accumulate)),
+            float(alpha),
+            dv.stream_ptr(stream),
+        )
+
+
+def _as_csr(m):
+    return sp.csr_array(m) if not isinstance(m, np.ndarray) else sp.csr_array(np.asarray(m))
+
+
+_BAND_CACHE: Dict[Tuple, Tuple[list, Band]] = {}
+
+
+def band_of(*mats, transpose: bool = False) -> Band:
+    """Cached device band of the given matrices (kept alive with the band)."""
+    key = (tuple(id(m) for m in mats), transpose)
+    hit = _BAND_CACHE.get(key)
+    if hit is not None:
+        return hit[1]
+    src = [(_as_csr(m).T if transpose else m) for m in mats]
+    b = Band(src)
+    _BAND_CACHE[key] = (list(mats), b)
+    return b
+
+
+def clear_caches():
+    _BAND_CACHE.clear()
+    _FUSED_CACHE.clear()
+
+
+# ---------------------------------------------------------------------------
+# MatrixFree.apply on the device
+# ---------------------------------------------------------------------------
+
+
+def mf_apply_device(bands: Sequence[Band], x, nrhs: int, out=None, accumulate=False, alpha=1.0, stream=None):
+    """``(M_d (x) ... (x) M_1) x`` for ``bands = [M_1, .., M_d]`` on an ``(N, nrhs)``
+    device array.  Directions are processed slowest first, like the reference."""
+    t = dv.torch()
+    cur = [b.ncols for b in bands]
+    tensor = x
+    nd = len(bands)
+    for d in reversed(range(nd)):
+        b = bands[d]
+        outer = int(np.prod(cur[d + 1:], dtype=np.int64))
+        inner = int(np.prod(cur[:d], dtype=np.int64)) * nrhs
+        new = list(cur)
+        new[d] = b.nrows
+        last = d == 0
+        n_out = int(np.prod(new, dtype=np.int64)) * nrhs
+        if last and out is not None:
+            y = out
+        else:
+            y = t.empty(n_out, dtype=x.dtype, device=x.device)
+        b.apply(tensor, [y], outer, inner, accumulate=(accumulate and last), alpha=(alpha if last else 1.0), stream=stream)
+        tensor = y
+        cur = new
+    return tensor
+
+
+def _to_dev_2d(array_in, dtype="f64"):
+    """Device (N*r,) view of an (N,) or (N, r) input; returns (tensor, r, was_numpy, shape)."""
+    t = dv.torch()
+    was_np = not isinstance(array_in, t.Tensor)
+    shape = tuple(array_in.shape)
+    r = int(np.prod(shape[1:], dtype=np.int64)) if len(shape) > 1 else 1
+    x = dv.to_device(array_in, dtype).reshape(-1)
+    return x, r, was_np, shape
+
+
+def _from_dev(y, was_np, shape):
+    y = y.reshape(shape)
+    if was_np:
+        return y.double().cpu().numpy()
+    return y
+
+
+class MatrixFree:
+    """``operators/utils/matrix_free.py:7-119`` on the device."""
+
+    @staticmethod
+    def apply(matrix_list, array_in, is_transpose: bool = False):
+        dv.require_gpu()
+        t = dv.torch()
+        if isinstance(array_in, t.Tensor):
+            dtype = "f32" if array_in.dtype == t.float32 else "f64"
+        else:
+            if np.iscomplexobj(array_in) or any(np.iscomplexobj(m) for m in matrix_list if isinstance(m, np.ndarray)):
+                raise NotImplementedError("complex MatrixFree.apply: use the real and imaginary parts")
+            dtype = "f64"
+        x, r, was_np, shape = _to_dev_2d(array_in, dtype)
+        bands = [band_of(m, transpose=is_transpose) for m in matrix_list]
+        if x.numel() != int(np.prod([b.ncols for b in bands])) * r:
+            raise ValueError("array_in does not match the Kronecker product")
+        y = mf_apply_device(bands, x, r)
+        out_shape = (int(np.prod([b.nrows for b in bands])),) + tuple(shape[1:])
+        return _from_dev(y, was_np, out_shape)
+
+
+# ---------------------------------------------------------------------------
+# Composed operator path
+# ---------------------------------------------------------------------------
+
+
+class Term:
+    """``y += W^{test} diag(coef) B^{trial} x``: per-direction basis index ``trial``,
+    weights index ``test``, coefficient device array ``coef`` (N_q,)."""
+
+    __slots__ = ("trial", "test", "coef")
+
+    def __init__(self, trial, test, coef):
+        self.trial = tuple(int(v) for v in trial)
+        self.test = tuple(int(v) for v in test)
+        self.coef = coef
+
+
+def composed_apply(quadrule_list, terms: List[Term], x, nrhs: int, out=None, stream=None):
+    """Evaluate ``sum_terms`` with shared interpolations / test contractions."""
+    t = dv.torch()
+    nq = int(np.prod([q.nbquadpts for q in quadrule_list], dtype=np.int64))
+    ncdof = int(np.prod([q.nbctrlpts for q in quadrule_list], dtype=np.int64))
+    interp: Dict[Tuple, object] = {}
+    acc: Dict[Tuple, object] = {}
+    dt = dv.iga_dtype(x)
+    for term in terms:
+        if term.trial not in interp:
+            bands = [band_of(q.basis[b]) for q, b in zip(quadrule_list, term.trial)]
+            interp[term.trial] = mf_apply_device(bands, x, nrhs, stream=stream)
+        u = interp[term.trial]
+        first = term.test not in acc
+        if first:
+            acc[term.test] = t.empty(nq * nrhs, dtype=x.dtype, device=x.device)
+        _lib.call(
+            "iga_qp_scale",
+            dt,
+            nq,
+            nrhs,
+            dv.ptr(term.coef),
+            dv.ptr(u),
+            dv.ptr(acc[term.test]),
+            0 if first else 1,
+            1.0,
+            dv.stream_ptr(stream),
+        )
+    y = out if out is not None else t.zeros(ncdof * nrhs, dtype=x.dtype, device=x.device)
+    started = out is None
+    for i, (test, a) in enumerate(acc.items()):
+        bands = [band_of(q.weights[z]) for q, z in zip(quadrule_list, test)]
+        mf_apply_device(bands, a, nrhs, out=y, accumulate=(i > 0 or started), stream=stream)
+    if not acc and out is not None:
+        y.zero_()
+    return y
+
+
+def _spacetime_dims(quadrule_list, time_ders):
+    nd = len(quadrule_list)
+    return nd, (nd - 1 if time_ders is not None else nd)
+
+
+def terms_u_v(quadrule_list, c, time_ders=None):
+    """``bsplines_ops.py:290-317``."""
+    nd, _ = _spacetime_dims(quadrule_list, time_ders)
+    tr = [0] * nd
+    te = [0] * nd
+    if time_ders is not None:
+        tr[-1] = time_ders[1]
+        te[-1] = time_ders[0]
+    return [Term(tr, te, c)]
+
+
+def terms_gradu_gradv(quadrule_list, C, time_ders=None):
+    """``bsplines_ops.py:319-355`` (``C[i][j]``: test d_i, trial d_j)."""
+    nd, sd = _spacetime_dims(quadrule_list, time_ders)
+    out = []
+    for j in range(sd):
+        beta = [0] * nd
+        beta[j] = 1
+        if time_ders is not None:
+            beta[-1] = time_ders[1]
+        for i in range(sd):
+            alpha = [0] * nd
+            alpha[i] = 1
+            if time_ders is not None:
+                alpha[-1] = time_ders[0]
+            out.append(Term(beta, [b + 2 * a for a, b in zip(alpha, beta)], C[i][j]))
+    return out
+
+
+def terms_gradu_v(quadrule_list, c, time_ders=None):
+    """``bsplines_ops.py:357-390``: test d_i, trial value."""
+    nd, sd = _spacetime_dims(quadrule_list, time_ders)
+    beta = [0] * nd
+    if time_ders is not None:
+        beta[-1] = time_ders[1]
+    out = []
+    for i in range(sd):
+        alpha = [0] * nd
+        alpha[i] = 1
+        if time_ders is not None:
+            alpha[-1] = time_ders[0]
+        out.append(Term(beta, [b + 2 * a for a, b in zip(alpha, beta)], c[i]))
+    return out
+
+
+def terms_u_gradv(quadrule_list, c, time_ders=None):
+    """``bsplines_ops.py:392-429``: trial d_j, test weights index ``alpha`` (value)."""
+    nd, sd = _spacetime_dims(quadrule_list, time_ders)
+    alpha = [0] * nd
+    if time_ders is not None:
+        alpha[-1] = time_ders[0]
+    out = []
+    for i in range(sd):
+        beta = [0] * nd
+        beta[i] = 1
+        if time_ders is not None:
+            beta[-1] = time_ders[1]
+        out.append(Term(beta, alpha, c[i]))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# NURBS weight projection (device)
+# ---------------------------------------------------------------------------
+
+
+def nurbs_weight_fields(quadrule_list, w_dev):
+    """``W`` and ``grad W`` at the quadrature points (``nurbs_ops.py:14-62``), float64."""
+    nd = len(quadrule_list)
+    W = mf_apply_device([band_of(q.basis[0]) for q in quadrule_list], w_dev, 1)
+    dW = []
+    for j in range(nd):
+        bands = [band_of(q.basis[1 if d == j else 0]) for d, q in enumerate(quadrule_list)]
+        dW.append(mf_apply_device(bands, w_dev, 1))
+    return W, dW
+
+
+# ---------------------------------------------------------------------------
+# Fused plans
+# ---------------------------------------------------------------------------
+
+FORM_M1, FORM_K6, FORM_KM7, FORM_SYM10, FORM_GEN16 = 0, 1, 2, 3, 4
+FORM_NCOEF = {0: 1, 1: 6, 2: 7, 3: 10, 4: 16}
+
+
+class FusedPlan:
+    """Device plan of ``csrc/fused.hip`` for three 1-D rules, one form and dtype."""
+
+    def __init__(self, quadrule_list, form: int, dtype: str = "f64", nslab: int = 0):
+        if len(quadrule_list) != 3:
+            raise ValueError("the fused operator is three-dimensional")
+        tabs = [fused_direction_tables(q) for q in quadrule_list]
+        P = max(t[1] for t in tabs)
+        tabs = [fused_direction_tables(q, P) if t[1] < P else t for q, t in zip(quadrule_list, tabs)]
+        self.P = P
+        self.form = form
+        self.dtype = dtype
+        self.nc = np.array([q.nbctrlpts for q in quadrule_list], dtype=np.int32)
+        self.nq = np.array([q.nbquadpts for q in quadrule_list], dtype=np.int32)
+        self._keep = []
+        firsts = (_lib._i32p * 3)()
+        tabps = (_lib._f64p * 3)()
+        for d, (first, _, vals) in enumerate(tabs):
+            fa, fp = _lib.i32_array(first)
+            va, vp = _lib.f64_array(vals)
+            self._keep += [fa, va]
+            firsts[d] = fp
+            tabps[d] = vp
+        nca, ncp = _lib.i32_array(self.nc)
+        nqa, nqp = _lib.i32_array(self.nq)
+        h = C.c_void_p()
+        _lib.call("iga_fused_create", C.byref(h), ncp, nqp, P, firsts, tabps, form, dv.DTYPES[dtype], int(nslab))
+        self._h = h
+        ce, pe = C.c_int64(), C.c_int64()
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        _lib.call("iga_fused_info", h, C.byref(ce), C.byref(pe), C.byref(a), C.byref(b), C.byref(c))
+        self.coef_elems = ce.value
+        self.part_elems = pe.value
+        self.tiles = (a.value, b.value)
+        self.nslab = c.value
+        self._work = {}
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib._lib is not None:
+            _lib.load().iga_fused_destroy(h)
+            self._h = None
+
+    def workspace_bytes(self, nrhs: int) -> int:
+        return int(_lib.load().iga_fused_workspace_bytes(self._h, int(nrhs)))
+
+    def workspace(self, nrhs: int):
+        w = self._work.get(nrhs)
+        if w is None:
+            w = dv.workspace(self.workspace_bytes(nrhs))
+            self._work[nrhs] = w
+        return w
+
+    def pack(self, C=None, m=None, W=None, dW=None, stream=None):
+        """Packed device coefficients from reference-layout float64 device arrays."""
+        out = dv.empty(self.coef_elems, self.dtype)
+        _lib.call(
+            "iga_fused_pack",
+            self._h,
+            dv.ptr(C),
+            dv.ptr(m),
+            dv.ptr(W),
+            dv.ptr(dW),
+            dv.ptr(out),
+            dv.stream_ptr(stream),
+        )
+        return out
+
+    def apply_stage(self, stage, packed, x, y, nrhs=1, nurbs_w=None, alpha=1.0, accumulate=False, work=None, stream=None):
+        """One of the two launches of ``apply`` (1: coefficient sweep, 2: gather)."""
+        work = work if work is not None else self.workspace(nrhs)
+        _lib.call("iga_fused_apply_stage", self._h, int(stage), dv.ptr(packed), dv.ptr(x), dv.ptr(y), int(nrhs),
+                  dv.ptr(nurbs_w), float(alpha), int(bool(accumulate)), dv.ptr(work), dv.stream_ptr(stream))
+        return y
+
+    def apply(self, packed, x, y, nrhs: int = 1, nurbs_w=None, alpha=1.0, accumulate=False, work=None, stream=None):
+        work = work if work is not None else self.workspace(nrhs)
+        _lib.call(
+            "iga_fused_apply",
+            self._h,
+            dv.ptr(packed),
+            dv.ptr(x),
+            dv.ptr(y),
+            int(nrhs),
+            dv.ptr(nurbs_w),
+            float(alpha),
+            int(bool(accumulate)),
+            dv.ptr(work),
+            dv.stream_ptr(stream),
+        )
+        return y
+
+
+_FUSED_CACHE: Dict[Tuple, Tuple[list, FusedPlan]] = {}
+
+
+def fused_plan(quadrule_list, form, dtype="f64", nslab=0) -> FusedPlan:
+    key = (tuple(id(q) for q in quadrule_list), form, dtype, nslab)
+    hit = _FUSED_CACHE.get(key)
+    if hit is None:
+        hit = (list(quadrule_list), FusedPlan(quadrule_list, form, dtype, nslab))
+        _FUSED_CACHE[key] = hit
+    return hit[1]
+
+
+def fused_applicable(quadrule_list, time_ders=None) -> bool:
+    return (
+        len(quadrule_list) == 3
+        and time_ders is None
+        and all(test_weights_depend_on_alpha_only(q) for q in quadrule_list)
+        and all(2 <= q.degree <= 7 for q in quadrule_list)
+    )
+
+
+# ---------------------------------------------------------------------------
+# Device operator: y = a K x + b M x
+# ---------------------------------------------------------------------------
+
+
+def _is_symmetric(Cd) -> bool:
+    t = dv.torch()
+    return bool(t.equal(Cd, Cd.transpose(0, 1)))
+
+
+class StiffnessMassOperator:
+    """Device-resident ``y = K x + M x`` (either part may be omitted).
+
+    ``C`` (3,3,N_q) stiffness coefficients (``C[i][j]``: test d_i, trial d_j), ``m``
+    (N_q,) mass coefficients, ``nurbs_weights`` (N_dof,) for a NURBS patch.  Inputs may
+    be numpy or device tensors; they are uploaded / packed once.  ``dtype`` is the
+    arithmetic of the apply ("f64" or "f32")."""
+
+    def __init__(self, quadrule_list, C=None, m=None, nurbs_weights=None, dtype="f64", force_composed=False, nslab=0):
+        dv.require_gpu()
+        t = dv.torch()
+        self.quadrule_list = list(quadrule_list)
+        self.dtype = dtype
+        self.ndof = int(np.prod([q.nbctrlpts for q in quadrule_list]))
+        self.nq = int(np.prod([q.nbquadpts for q in quadrule_list]))
+        self.is_nurbs = nurbs_weights is not None and np.size(nurbs_weights) > 0
+        Cd = None if C is None else dv.to_device(C, "f64").reshape(3, 3, -1)
+        md = None if m is None else dv.to_device(m, "f64").reshape(-1)
+        self.w64 = dv.to_device(nurbs_weights, "f64") if self.is_nurbs else None
+        self.w = None if not self.is_nurbs else self.w64.to(dv.torch_dtype(dtype))
+        self.fused = (not force_composed) and fused_applicable(self.quadrule_list)
+        if self.fused:
+            sym = Cd is None or _is_symmetric(Cd)
+            if self.is_nurbs:
+                form = FORM_SYM10 if sym else FORM_GEN16
+            elif Cd is None:
+                form = FORM_M1
+            elif md is None:
+                form = FORM_K6 if sym else FORM_GEN16
+            else:
+                form = FORM_KM7 if sym else FORM_GEN16
+            if form == FORM_GEN16 and md is None:
+                md = t.zeros(self.nq, dtype=t.float64, device=Cd.device)
+            if form == FORM_M1 and md is None:
+                raise ValueError("no coefficients")
+            self.plan = fused_plan(self.quadrule_list, form, dtype, nslab)
+            W = dW = None
+            if self.is_nurbs:
+                Wf, dWf = nurbs_weight_fields(self.quadrule_list, self.w64)
+                W, dW = Wf, t.cat(dWf)
+                if Cd is None:
+                    Cd = t.zeros((3, 3, self.nq), dtype=t.float64, device=W.device)
+                if md is None:
+                    md = t.zeros(self.nq, dtype=t.float64, device=W.device)
+            self.packed = self.plan.pack(Cd, md, W, dW)
+            self.form = form
+        else:
+            self.terms = self._composed_terms(Cd, md)
+        self.coef_count = FORM_NCOEF[self.form] if self.fused else len(self.terms)
+
+    def _composed_terms(self, Cd, md):
+        t = dv.torch()
+        qs = self.quadrule_list
+        tdt = dv.torch_dtype(self.dtype)
+        terms = []
+        if not self.is_nurbs:
+            if Cd is not None:
+                Cc = [[Cd[i, j].to(tdt).contiguous() for j in range(3)] for i in range(3)]
+                terms += terms_gradu_gradv(qs, Cc)
+            if md is not None:
+                terms += terms_u_v(qs, md.to(tdt).contiguous())
+            return terms
+        Wf, dWf = nurbs_weight_fields(qs, self.w64)
+        wi = 1.0 / Wf
+        z = [g * wi * wi for g in dWf]
+        if Cd is not None:
+            Cw = [[(Cd[i, j] * wi * wi).to(tdt).contiguous() for j in range(3)] for i in range(3)]
+            terms += terms_gradu_gradv(qs, Cw)
+            zcz = sum(Cd[i, j] * z[i] * z[j] for i in range(3) for j in range(3))
+            terms += terms_u_v(qs, zcz.to(tdt).contiguous())
+            ctz = [(-sum(Cd[i, j] * z[i] for i in range(3)) * wi).to(tdt).contiguous() for j in range(3)]
+            terms += terms_u_gradv(qs, ctz)
+            cz = [(-sum(Cd[i, j] * z[j] for j in range(3)) * wi).to(tdt).contiguous() for i in range(3)]
+            terms += terms_gradu_v(qs, cz)
+        if md is not None:
+            terms += terms_u_v(qs, (md * wi * wi).to(tdt).contiguous())
+        return terms
+
+    def apply(self, x, out=None, nrhs: int = 1, alpha: float = 1.0, accumulate: bool = False, stream=None):
+        """``out = alpha A x (+ out)`` on device arrays of the operator dtype."""
+        t = dv.torch()
+        if out is None:
+            out = t.empty_like(x)
+        if self.fused:
+            self.plan.apply(self.packed, x, out, nrhs, self.w, alpha, accumulate, stream=stream)
+            return out
+        xin = x
+        if self.is_nurbs:
+            xin = (x.reshape(self.ndof, nrhs) * self.w[:, None]).reshape(-1)
+        tmp = composed_apply(self.quadrule_list, self.terms, xin, nrhs, stream=stream)
+        if self.is_nurbs:
+            tmp = (tmp.reshape(self.ndof, nrhs) * self.w[:, None]).reshape(-1)
+        if accumulate:
+            out.add_(tmp, alpha=alpha)
+        else:
+            out.copy_(tmp).mul_(alpha) if alpha != 1.0 else out.copy_(tmp)
+        return out
+
+    __call__ = apply
+
+    def apply_device(self, x, out=None, nrhs: int = 1, stream=None):
+        return self.apply(x, out=out, nrhs=nrhs, stream=stream)
+
+    def algorithmic_bytes(self, nrhs: int = 1) -> int:
+        """``b (2 N_dof r + c N_q)`` with c = 7 (B-spline K+M) / 11 (NURBS K+M)."""
+        b = 8 if self.dtype == "f64" else 4
+        has_k = self.form != FORM_M1 if self.fused else True
+        if self.is_nurbs:
+            c = 11 if has_k else 1
+        else:
+            c = 7 if (has_k and self.form in (FORM_KM7, FORM_GEN16)) else (6 if has_k else 1)
+        return b * (2 * self.ndof * nrhs + c * self.nq)
+
+
+# ---------------------------------------------------------------------------
+# Reference-signature operator classes
+# ---------------------------------------------------------------------------
+
+
+def _operator_call(kind, quadrule_list, coefficients, array_in, time_ders, nurbs_weights):
+    dv.require_gpu()
+    t = dv.torch()
+    qs = list(quadrule_list)
+    nd = len(qs)
+    is_nurbs = nurbs_weights is not None and np.size(nurbs_weights) > 0
+    dtype = "f32" if isinstance(array_in, t.Tensor) and array_in.dtype == t.float32 else "f64"
+    x, r, was_np, shape = _to_dev_2d(array_in, dtype)
+    ndof = int(np.prod([q.nbctrlpts for q in qs]))
+    if x.numel() != ndof * r:
+        raise ValueError("array_in does not match the number of control points")
+    coef = dv.to_device(coefficients, "f64")
+    if (
+        kind in ("u_v", "gradu_gradv")
+        and fused_applicable(qs, time_ders)
+        and (kind == "u_v" or coefficients.shape[:2] == (3, 3))
+    ):
+        if kind == "u_v":
+            op = StiffnessMassOperator(qs, C=None, m=coef.reshape(-1), nurbs_weights=nurbs_weights, dtype=dtype)
+        else:
+            op = StiffnessMassOperator(qs, C=coef.reshape(3, 3, -1), m=None, nurbs_weights=nurbs_weights, dtype=dtype)
+        y = op.apply(x, nrhs=r)
+        return _from_dev(y, was_np, shape)
+    tdt = dv.torch_dtype(dtype)
+    xin = x
+    w = None
+    if is_nurbs:
+        w64 = dv.to_device(nurbs_weights, "f64")
+        w = w64.to(tdt)
+        xin = (x.reshape(ndof, r) * w[:, None]).reshape(-1)
+        Wf, dWf = nurbs_weight_fields(qs, w64)
+        wi = 1.0 / Wf
+        sd = nd - 1 if time_ders is not None else nd
+        z = [g * wi * wi for g in dWf][:sd]
+    c = coef
+    terms: List[Term] = []
+    cast = lambda a: a.to(tdt).contiguous()
+    if kind == "u_v":
+        cc = c.reshape(-1)
+        terms = terms_u_v(qs, cast(cc * wi * wi if is_nurbs else cc), time_ders)
+    elif kind == "gradu_gradv":
+        sd = c.shape[0]
+        if is_nurbs:
+            terms += terms_gradu_gradv(qs, [[cast(c[i, j] * wi * wi) for j in range(sd)] for i in range(sd)], time_ders)
+            zcz = sum(c[i, j] * z[i] * z[j] for i in range(sd) for j in range(sd))
+            terms += terms_u_v(qs, cast(zcz), time_ders)
+            terms += terms_u_gradv(qs, [cast(-sum(c[i, j] * z[i] for i in range(sd)) * wi) for j in range(sd)], time_ders)
+            terms += terms_gradu_v(qs, [cast(-sum(c[i, j] * z[j] for j in range(sd)) * wi) for i in range(sd)], time_ders)
+        else:
+            terms = terms_gradu_gradv(qs, [[cast(c[i, j]) for j in range(sd)] for i in range(sd)], time_ders)
+    elif kind in ("gradu_v", "u_gradv"):
+        sd = c.shape[0]
+        build = terms_gradu_v if kind == "gradu_v" else terms_u_gradv
+        if is_nurbs:
+            terms += build(qs, [cast(c[i] * wi * wi) for i in range(sd)], time_ders)
+            terms += terms_u_v(qs, cast(-sum(c[i] * z[i] for i in range(sd)) * wi), time_ders)
+        else:
+            terms = build(qs, [cast(c[i]) for i in range(sd)], time_ders)
+    else:
+        raise ValueError(kind)
+    y = composed_apply(qs, terms, xin, r)
+    if is_nurbs:
+        y = (y.reshape(ndof, r) * w[:, None]).reshape(-1)
+    return _from_dev(y, was_np, shape)
+
+
+class BsplineOperations:
+    """``operators/bsplines_ops.py:56-429`` (matrix-free part) on the device."""
+
+    @staticmethod
+    def compute_mf_scalar_u_v(quadrule_list, coefficients, array_in, time_ders=None, nurbs_weights=np.array([])):
+        return _operator_call("u_v", quadrule_list, coefficients, array_in, time_ders, None)
+
+    @staticmethod
+    def compute_mf_scalar_gradu_gradv(quadrule_list, coefficients, array_in, time_ders=None, nurbs_weights=np.array([])):
+        return _operator_call("gradu_gradv", quadrule_list, coefficients, array_in, time_ders, None)
+
+    @staticmethod
+    def compute_mf_scalar_gradu_v(quadrule_list, coefficients, array_in, time_ders=None, nurbs_weights=np.array([])):
+        return _operator_call("gradu_v", quadrule_list, coefficients, array_in, time_ders, None)
+
+    @staticmethod
+    def compute_mf_scalar_u_gradv(quadrule_list, coefficients, array_in, time_ders=None, nurbs_weights=np.array([])):
+        return _operator_call("u_gradv", quadrule_list, coefficients, array_in, time_ders, None)
+
+
+class NurbsOperations:
+    """``operators/nurbs_ops.py:80-463`` (matrix-free part) on the device."""
+
+    @staticmethod
+    def compute_mf_scalar_u_v(quadrule_list, coefficients, array_in, time_ders=None, nurbs_weights=np.array([])):
+        return _operator_call("u_v", quadrule_list, coefficients, array_in, time_ders, nurbs_weights)
+
+    @staticmethod
+    def compute_mf_scalar_gradu_gradv(quadrule_list, coefficients, array_in, time_ders=None, nurbs_weights=np.array([])):
+        return _operator_call("gradu_gradv", quadrule_list, coefficients, array_in, time_ders, nurbs_weights)
+
+    @staticmethod
+    def compute_mf_scalar_gradu_v(quadrule_list, coefficients, array_in, time_ders=None, nurbs_weights=np.array([])):
+        return _operator_call("gradu_v", quadrule_list, coefficients, array_in, time_ders, nurbs_weights)
+
+    @staticmethod
+    def compute_mf_scalar_u_gradv(quadrule_list, coefficients, array_in, time_ders=None, nurbs_weights=np.array([])):
+        return _operator_call("u_gradv", quadrule_list, coefficients, array_in, time_ders, nurbs_weights)
+
+
+def operator_engine(nurbs_weights):
+    """``geometry/mixin/template.py:74-77``: NURBS engine iff weights are given."""
+    return NurbsOperations if nurbs_weights is not None and np.size(nurbs_weights) > 0 else BsplineOperations

@@ -1,0 +1,173 @@
+/*
+ * libigaops_hip — C ABI of the MI355X (gfx950) implementation of igaops'
+ * weighted-quadrature matrix-free operator path.
+ *
+ * Conventions
+ *   - Every function returns IGA_OK (0) or a negative IGA_ERR_* code; the text of
+ *     the last failure of the calling thread is available from iga_last_error().
+ *     No C++ exception crosses this boundary.
+ *   - Pointers named x, y, coef, V, w, ... are device pointers (HBM) unless the
+ *     parameter is documented as host.  `stream` is a hipStream_t (NULL = default).
+ *   - Tensor-product arrays use direction-1-fastest order: the DOF a = (a1,a2,a3)
+ *     lives at a1 + n1*(a2 + n2*a3); with r right-hand sides, element (a, k) lives
+ *     at a*r + k (C order of a (N, r) array), as in the reference.
+ *   - dtype selects the arithmetic: IGA_F64 (double) or IGA_F32 (float); all
+ *     vectors and coefficients of one call share it.
+ *
+ * Reference citations are into JoaquinCORNEJO/igaops, src/igaops/.
+ */
+#ifndef IGAOPS_HIP_H
+#define IGAOPS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IGA_OK 0
+#define IGA_ERR_INVALID -1     /* bad argument / shape (Python: ValueError)        */
+#define IGA_ERR_HIP -2         /* HIP runtime failure (Python: RuntimeError)        */
+#define IGA_ERR_NOMEM -3       /* allocation failure (Python: MemoryError)          */
+#define IGA_ERR_UNSUPPORTED -4 /* configuration not provided (NotImplementedError)  */
+#define IGA_ERR_INTERNAL -5    /* unexpected failure (RuntimeError)                 */
+
+#define IGA_F64 0
+#define IGA_F32 1
+
+/* ---- library ------------------------------------------------------------- */
+
+/* Library version (major*10000 + minor*100 + patch). */
+int iga_version(void);
+
+/* Copies the last error message of this thread into buf (NUL terminated). */
+int iga_last_error(char* buf, int buflen);
+
+/* Number of visible HIP devices. */
+int iga_device_count(int* count);
+
+/* ---- banded 1-D factors and L-mode products -------------------------------
+ * A band holds nmat matrices of shape (nrows x ncols) that share one sparsity
+ * band: row i has its possibly-non-zero entries in columns
+ * [first[i], first[i] + width).  vals (host, float64) is [nmat][nrows][width].
+ * This is the device form of the reference's CSR basis / weight matrices
+ * (quadrature/utils/operations.py:52-105, weighted_quad.py:353-363) and of the
+ * dense FD eigenvector matrices (width = ncols).
+ */
+typedef struct iga_band_s* iga_band_t;
+
+int iga_band_create(iga_band_t* out, int nrows, int ncols, int width, int nmat,
+                    const int32_t* first, const double* vals);
+int iga_band_destroy(iga_band_t band);
+
+/* One n-mode product of a tensor viewed as (outer, ncols, inner):
+ *   y_m(o, i, t) = alpha * sum_j V_m[i, j] * x(o, first[i]+j, t)   [+ y_m(o,i,t)]
+ * for m < nmat, output viewed as (outer, nrows, inner).  Replaces
+ * MatrixFree.tensor_matrix_product (operators/utils/matrix_free.py:39-75); the
+ * reference's MatrixFree.apply (matrix_free.py:77-119) is d such calls. */
+int iga_band_mode_product(iga_band_t band, int dtype, int64_t outer, int64_t inner,
+                          const void* x, void* const* y, int accumulate, double alpha,
+                          void* stream);
+
+/* Point-wise coefficient product at the quadrature points with r right-hand
+ * sides:  y[q*r + k] = alpha * coef[q] * x[q*r + k]  [+ y[q*r + k]].
+ * Replaces the broadcast einsums of operators/bsplines_ops.py:311,341-343,378,416-418. */
+int iga_qp_scale(int dtype, int64_t nq, int64_t nrhs, const void* coef, const void* x,
+                 void* y, int accumulate, double alpha, void* stream);
+
+/* Dirichlet masks (solvers/linear_solver.py:119-124): zero every entry outside a
+ * box (iga_box_mask), or the entries listed in idx (device int64, iga_index_zero). */
+int iga_box_mask(int dtype, int ndim, const int64_t* n, const int64_t* lo, const int64_t* len,
+                 int64_t nrhs, void* full, void* stream);
+int iga_index_zero(int dtype, int64_t nidx, const int64_t* idx, int64_t nrhs, void* x, void* stream);
+
+/* ---- Krylov reductions (FP64 or FP32 vectors, FP64 results) ----------------
+ * Deterministic two-stage wavefront-shuffle reductions.  `work` is caller-owned
+ * device scratch of iga_reduce_workspace_bytes(n, k) bytes.  They replace the
+ * numpy dot / norm / axpy loops of solvers/linear_solver.py:481-495 and
+ * solvers/deflated/helpers.py:44-58,116-133. */
+int64_t iga_reduce_workspace_bytes(int64_t n, int k);
+
+/* out[j] = sum_i V[j*ldv + i] * w[i]  for j < k  (out: device, float64[k]) */
+int iga_dot_many(int dtype, int64_t n, int k, const void* V, int64_t ldv, const void* w,
+                 double* out, void* work, void* stream);
+
+/* w[i] += alpha * sum_j h[j] * V[j*ldv + i]   (h: device, float64[k]) */
+int iga_axpy_many(int dtype, int64_t n, int k, const void* V, int64_t ldv, const double* h,
+                  double alpha, void* w, void* stream);
+
+/* y[i] = a * x[i] + b * y[i]; a and b are read from device memory when
+ * a_dev / b_dev are non-NULL (then a[0] / b[0] is used, optionally inverted
+ * with invert_a), else the host values are used. */
+int iga_axpby(int dtype, int64_t n, double a, const double* a_dev, int invert_a,
+              const void* x, double b, void* y, void* stream);
+
+/* Casts between float64 and float32 vectors: y = (dtype_y) x. */
+int iga_cast(int dtype_x, int dtype_y, int64_t n, const void* x, void* y, void* stream);
+
+/* Gather / scatter of a box of a 3-D (or lower) tensor-product array (the free
+ * nodes of the FD preconditioner, fastdiag/utils/space_precond.py:104-122,296,315):
+ *   gather:  sub[s] = full[offset(s)]        (box lo[d]..lo[d]+len[d]-1, dims n[d])
+ *   scatter: full[offset(s)] = sub[s]; entries of full outside the box are set to 0
+ *            when zero_outside != 0.  ndim <= 4, r right-hand sides. */
+int iga_box_gather(int dtype, int ndim, const int64_t* n, const int64_t* lo, const int64_t* len,
+                   int64_t nrhs, const void* full, void* sub, void* stream);
+int iga_box_scatter(int dtype, int ndim, const int64_t* n, const int64_t* lo, const int64_t* len,
+                    int64_t nrhs, const void* sub, void* full, int zero_outside, void* stream);
+
+/* ---- fused 3-D sum-factorised operator -------------------------------------
+ * One plan per (three 1-D quadrature rules, coefficient form, dtype).  It replaces
+ * the chain of MatrixFree.apply + einsum calls of
+ * BsplineOperations.compute_mf_scalar_{u_v,gradu_gradv} (operators/bsplines_ops.py:290-355)
+ * and NurbsOperations.compute_mf_scalar_{u_v,gradu_gradv} (operators/nurbs_ops.py:320-390)
+ * for 3-D rules whose test weights depend only on the test derivative
+ * (Gauss rules and WQ type "2": weights = [W0, W0, W1, W1]).
+ *
+ * Direction d is described by its band start first[d][q] (non-decreasing,
+ * first + P <= nc[d]) and tabs[d] = float64 [4][nq[d]][P] holding, for each
+ * quadrature point q and band slot j (DOF first[d][q] + j):
+ *   B0[q][j] = basis[0][q, c], B1[q][j] = basis[1][q, c],
+ *   W0[q][j] = weights[0][c, q], W1[q][j] = weights[2][c, q].
+ * form: 0 mass (1 coefficient/point), 1 stiffness with symmetric C (6),
+ *       2 stiffness + mass (7), 3 symmetric 4x4 (10: NURBS stiffness [+ mass]),
+ *       4 general 4x4 (16).
+ * nslab: number of q3 slabs (0 = automatic). */
+typedef struct iga_fused_s* iga_fused_t;
+
+int iga_fused_create(iga_fused_t* out, const int32_t* nc, const int32_t* nq, int P,
+                     const int32_t* const* first, const double* const* tabs, int form, int dtype,
+                     int nslab);
+int iga_fused_destroy(iga_fused_t plan);
+
+/* Sizes: packed coefficient length (elements), partial-block workspace per
+ * right-hand side (elements), tile and slab counts. */
+int iga_fused_info(iga_fused_t plan, int64_t* coef_elems, int64_t* part_elems_per_rhs,
+                   int* ntiles1, int* ntiles2, int* nslabs);
+int64_t iga_fused_workspace_bytes(iga_fused_t plan, int nrhs);
+
+/* Packs coefficients given in the reference layout (float64 device arrays):
+ * C (3,3,nq) [C[i][j] multiplies trial d_j and test d_i], m (nq), and for NURBS
+ * the weight function W (nq) and its parametric gradient dW (3,nq) at the
+ * quadrature points (both NULL for B-splines).  NURBS terms are folded once here
+ * instead of on every call (operators/nurbs_ops.py:355-387). */
+int iga_fused_pack(iga_fused_t plan, const double* C, const double* m, const double* W,
+                   const double* dW, void* packed, void* stream);
+
+/* y = alpha * A x [+ y] for nrhs right-hand sides ((N, nrhs) C order).  For NURBS
+ * pass nurbs_w (per-DOF weights, plan dtype): input and output are scaled by it
+ * as in operators/nurbs_ops.py:359-361,388-390.  work: device scratch of
+ * iga_fused_workspace_bytes(plan, nrhs) bytes. */
+int iga_fused_apply(iga_fused_t plan, const void* packed, const void* x, void* y, int nrhs,
+                    const void* nurbs_w, double alpha, int accumulate, void* work, void* stream);
+
+/* The two launches of iga_fused_apply, separately (for timing): stage 1 runs the
+ * coefficient sweep into the partial blocks of `work`, stage 2 adds them into y. */
+int iga_fused_apply_stage(iga_fused_t plan, int stage, const void* packed, const void* x, void* y,
+                          int nrhs, const void* nurbs_w, double alpha, int accumulate, void* work,
+                          void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* IGAOPS_HIP_H */

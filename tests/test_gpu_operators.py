@@ -1,0 +1,176 @@
+"""GPU parity of the operator path against the oracle (float64: <= 1e-12 relative;
+float32: <= 5e-6 relative, measured against the float64 oracle)."""
+import numpy as np
+import pytest
+
+from igaops_amd.geometry import lognormal_conductivity, quarter_annulus, unit_cube
+from igaops_amd.quadrature import StandardGauss, WeightedQuadrature, make_open_knotvector
+from oracle import matrix_free as omf
+from oracle import operators as oo
+
+pytestmark = pytest.mark.gpu
+
+F64_TOL = 1e-12
+F32_TOL = 5e-6
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def spd_coefficients(nq, seed):
+    rng = np.random.default_rng(seed)
+    A = rng.standard_normal((3, 3, nq))
+    C = np.einsum("ikq,jkq->ijq", A, A) + 0.5 * np.eye(3)[:, :, None]
+    m = rng.random(nq) + 0.5
+    return C, m
+
+
+def device_op(patch, C, m, dtype="f64", force_composed=False, nslab=0):
+    from igaops_amd.operators import StiffnessMassOperator
+
+    w = patch.nurbs_weights if patch.is_nurbs else None
+    return StiffnessMassOperator(patch.quadrule_list, C=C, m=m, nurbs_weights=w, dtype=dtype,
+                                 force_composed=force_composed, nslab=nslab)
+
+
+def run_op(op, x, nrhs=1, dtype="f64"):
+    import torch
+
+    xd = torch.as_tensor(x, device="cuda").to(torch.float64 if dtype == "f64" else torch.float32).reshape(-1)
+    y = op.apply(xd, nrhs=nrhs)
+    torch.cuda.synchronize()
+    return y.double().cpu().numpy().reshape(x.shape)
+
+
+CASES = [
+    ("cube_p3_n6_wq2", lambda: unit_cube(3, 6)),
+    ("cube_p2_n12_wq2", lambda: unit_cube(2, 12)),
+    ("annulus_p4_n5_wq2", lambda: quarter_annulus(4, 5)),
+    ("annulus_p3_n7_gauss", lambda: quarter_annulus(3, 7, quadclass="gauss", quadtype="legendre")),
+    ("cube_p6_n9_wq2", lambda: unit_cube(6, 9)),
+    ("annulus_p7_n3_wq2", lambda: quarter_annulus(7, 3)),
+]
+
+
+@pytest.mark.parametrize("name,make", CASES, ids=[c[0] for c in CASES])
+@pytest.mark.parametrize("path", ["fused", "composed"])
+def test_stiffness_plus_mass_f64(name, make, path):
+    print(f"case {name} {path}", flush=True)
+    patch = make()
+    C, m = spd_coefficients(patch.nbquadpts_total, 1)
+    x = np.random.default_rng(2).standard_normal(patch.nbctrlpts_total)
+    w = patch.nurbs_weights if patch.is_nurbs else None
+    ref = oo.stiffness_plus_mass(patch.quadrule_list, C, m, x, nurbs_weights=w)
+    op = device_op(patch, C, m, force_composed=(path == "composed"))
+    assert op.fused == (path == "fused")
+    assert rel(run_op(op, x), ref) <= F64_TOL
+
+
+@pytest.mark.parametrize("nslab", [1, 2, 5])
+def test_fused_slabs_and_multi_rhs(nslab):
+    patch = quarter_annulus(3, 6)
+    C, m = spd_coefficients(patch.nbquadpts_total, 3)
+    X = np.random.default_rng(4).standard_normal((patch.nbctrlpts_total, 3))
+    ref = oo.stiffness_plus_mass(patch.quadrule_list, C, m, X, nurbs_weights=patch.nurbs_weights)
+    op = device_op(patch, C, m, nslab=nslab)
+    assert rel(run_op(op, X, nrhs=3), ref) <= F64_TOL
+
+
+@pytest.mark.parametrize("name,make", CASES[:4], ids=[c[0] for c in CASES[:4]])
+def test_stiffness_plus_mass_f32(name, make):
+    patch = make()
+    C, m = spd_coefficients(patch.nbquadpts_total, 5)
+    x = np.random.default_rng(6).standard_normal(patch.nbctrlpts_total)
+    w = patch.nurbs_weights if patch.is_nurbs else None
+    ref = oo.stiffness_plus_mass(patch.quadrule_list, C, m, x, nurbs_weights=w)
+    op = device_op(patch, C, m, dtype="f32")
+    assert rel(run_op(op, x, dtype="f32"), ref) <= F32_TOL
+
+
+@pytest.mark.parametrize("kind", ["u_v", "gradu_gradv", "gradu_v", "u_gradv"])
+@pytest.mark.parametrize("geom", ["cube", "annulus"])
+def test_reference_signature_operators(kind, geom):
+    from igaops_amd.operators import BsplineOperations, NurbsOperations
+
+    patch = unit_cube(3, 4) if geom == "cube" else quarter_annulus(3, 4)
+    qs = patch.quadrule_list
+    nq = patch.nbquadpts_total
+    rng = np.random.default_rng(8)
+    x = rng.standard_normal(patch.nbctrlpts_total)
+    coef = {"u_v": rng.random(nq), "gradu_gradv": rng.standard_normal((3, 3, nq)),
+            "gradu_v": rng.standard_normal((3, nq)), "u_gradv": rng.standard_normal((3, nq))}[kind]
+    eng = BsplineOperations if geom == "cube" else NurbsOperations
+    kw = {} if geom == "cube" else {"nurbs_weights": patch.nurbs_weights}
+    got = getattr(eng, f"compute_mf_scalar_{kind}")(qs, coef, x, **kw)
+    fn = getattr(oo, ("bspline_" if geom == "cube" else "nurbs_") + kind)
+    ref = fn(qs, coef, x, **kw)
+    assert isinstance(got, np.ndarray) and got.shape == x.shape
+    assert rel(got, ref) <= F64_TOL
+
+
+def test_wq_type1_and_spacetime_composed():
+    """WQ type "1" (test weights depend on the trial derivative) and a 4-D space-time
+    operator go through the composed path."""
+    from igaops_amd.operators import BsplineOperations
+
+    q1 = WeightedQuadrature(3, make_open_knotvector(3, 4), "1")
+    qs = [q1] * 3
+    nq = q1.nbquadpts ** 3
+    rng = np.random.default_rng(9)
+    C = rng.standard_normal((3, 3, nq))
+    x = rng.standard_normal(q1.nbctrlpts ** 3)
+    assert rel(BsplineOperations.compute_mf_scalar_gradu_gradv(qs, C, x), oo.bspline_gradu_gradv(qs, C, x)) <= F64_TOL
+    qsp = WeightedQuadrature(2, make_open_knotvector(2, 4), "2")
+    qt = WeightedQuadrature(2, make_open_knotvector(2, 5), "2")
+    qst = [qsp, qsp, qsp, qt]
+    nq = qsp.nbquadpts ** 3 * qt.nbquadpts
+    x = rng.standard_normal(qsp.nbctrlpts ** 3 * qt.nbctrlpts)
+    c = rng.random(nq)
+    Cs = rng.standard_normal((3, 3, nq))
+    got = BsplineOperations.compute_mf_scalar_u_v(qst, c, x, time_ders=(0, 1)) + \
+        BsplineOperations.compute_mf_scalar_gradu_gradv(qst, Cs, x, time_ders=(0, 0))
+    ref = oo.bspline_u_v(qst, c, x, time_ders=(0, 1)) + oo.bspline_gradu_gradv(qst, Cs, x, time_ders=(0, 0))
+    assert rel(got, ref) <= F64_TOL
+
+
+@pytest.mark.parametrize("transpose", [False, True])
+def test_matrixfree_apply(transpose):
+    from igaops_amd.operators import MatrixFree
+
+    rng = np.random.default_rng(10)
+    q = WeightedQuadrature(3, make_open_knotvector(3, 5), "2")
+    mats = [q.basis[0], q.basis[1], q.weights[2].T.tocsr()]
+    n_in = int(np.prod([m.shape[0 if transpose else 1] for m in mats]))
+    X = rng.standard_normal((n_in, 2))
+    got = MatrixFree.apply(mats, X, is_transpose=transpose)
+    ref = omf.apply(mats, X, is_transpose=transpose)
+    assert rel(got, ref) <= F64_TOL
+    dense = [rng.standard_normal((4, 3)), rng.standard_normal((2, 5))]
+    v = rng.standard_normal(15)
+    assert rel(MatrixFree.apply(dense, v), omf.apply(dense, v)) <= F64_TOL
+
+
+def test_lognormal_annulus_config_small():
+    """The benchmark's coefficient field on a small curved patch."""
+    patch = quarter_annulus(4, 8)
+    kappa = lognormal_conductivity(patch, 0.5, seed=0)
+    C, m = patch.conductivity_coefficients(kappa)
+    x = np.random.default_rng(11).standard_normal(patch.nbctrlpts_total)
+    ref = oo.stiffness_plus_mass(patch.quadrule_list, C, m, x, nurbs_weights=patch.nurbs_weights)
+    assert rel(run_op(device_op(patch, C, m), x), ref) <= F64_TOL
+    assert rel(run_op(device_op(patch, C, m, dtype="f32"), x, dtype="f32"), ref) <= F32_TOL
+
+
+def test_zero_and_repeat_calls_are_deterministic():
+    import torch
+
+    patch = unit_cube(3, 6)
+    C, m = spd_coefficients(patch.nbquadpts_total, 12)
+    op = device_op(patch, C, m)
+    z = torch.zeros(patch.nbctrlpts_total, dtype=torch.float64, device="cuda")
+    assert float(op.apply(z).abs().max()) == 0.0
+    x = torch.randn(patch.nbctrlpts_total, dtype=torch.float64, device="cuda")
+    y1 = op.apply(x).clone()
+    y2 = op.apply(x).clone()
+    assert torch.equal(y1, y2)

@@ -1,0 +1,117 @@
+"""GPU parity of the FD apply and the Krylov drivers against the oracle."""
+import numpy as np
+import pytest
+
+from igaops_amd.geometry import lognormal_conductivity, quarter_annulus, unit_cube
+from igaops_amd.quadrature import StandardGauss, make_open_knotvector
+from oracle import fastdiag as ofd
+from oracle import operators as oo
+from oracle import solvers as osv
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+@pytest.mark.parametrize("table", ["all", "mixed", "none"])
+@pytest.mark.parametrize("p,nel", [(3, 6), (2, 9)])
+def test_fd_apply_matches_oracle(table, p, nel):
+    from igaops_amd.fastdiag import SingleFD
+
+    patch = unit_cube(p, nel)
+    qs = patch.quadrule_list
+    tab = {"all": np.ones((1, 3, 2), bool), "none": np.zeros((1, 3, 2), bool)}.get(table)
+    if tab is None:
+        tab = np.zeros((1, 3, 2), bool)
+        tab[0, 0, 0] = True
+        tab[0, 2, 1] = True
+    fd = SingleFD("spatial")
+    fd.compute_space_eigendecomposition(qs, tab)
+    fd.update_space_eigenvalues([0.3, 1.0])
+    gq = [StandardGauss(q.degree, q.knotvector, "legendre") for q in qs]
+    ofd_ = ofd.SpaceFD(gq, tab)
+    ofd_.scalar_coefs = (0.3, 1.0)
+    X = np.random.default_rng(1).standard_normal((patch.nbctrlpts_total, 2))
+    np.testing.assert_array_equal(np.asarray(fd.space_preconditioner.space_free_nodes[0]), ofd_.free_nodes())
+    assert rel(fd.apply(X), ofd_.apply(X)) <= 1e-12
+    assert rel(fd.apply(X[:, 0]), ofd_.apply(X[:, 0])) <= 1e-12
+
+
+def build_problem(geom, p, nel, sigma=0.5):
+    patch = unit_cube(p, nel) if geom == "cube" else quarter_annulus(p, nel)
+    kappa = lognormal_conductivity(patch, sigma, seed=1) if sigma > 0 else None
+    C, m = patch.conductivity_coefficients(kappa)
+    return patch, C, m
+
+
+def test_gmres_device_matches_oracle_gmres():
+    """Same preconditioned problem, same stopping rule: iteration counts within one,
+    solutions within the attainable accuracy."""
+    import torch
+
+    from igaops_amd.fastdiag import SingleFD
+    from igaops_amd.operators import StiffnessMassOperator
+    from igaops_amd.solvers import LinearSolver
+
+    patch, C, m = build_problem("annulus", 3, 6)
+    qs = patch.quadrule_list
+    w = patch.nurbs_weights
+    n = patch.nbctrlpts_total
+    fd = SingleFD("spatial")
+    fd.compute_space_eigendecomposition(qs, np.ones((1, 3, 2), bool))
+    fd.update_space_eigenvalues([0.0, 1.0])
+    free = np.asarray(fd.space_preconditioner.space_free_nodes[0])
+    dod = np.setdiff1d(np.arange(n), free).tolist()
+    op = StiffnessMassOperator(qs, C=C, m=None, nurbs_weights=w)
+    b = np.random.default_rng(3).standard_normal(n)
+    solver = LinearSolver(1e-10, 200, cleandod=dod)
+    out = solver.solve(op, b, Pfun=fd)
+    gq = [StandardGauss(q.degree, q.knotvector, "legendre") for q in qs]
+    ofd_ = ofd.SpaceFD(gq, np.ones((1, 3, 2), bool))
+    ref = osv.gmres(lambda v: oo.nurbs_gradu_gradv(qs, C, v, nurbs_weights=w), b, ofd_.apply, tol=1e-10,
+                    maxiters=200, cleandod=dod)
+    assert out.success and ref.success
+    assert abs(out.iterations - ref.iterations) <= 1
+    assert rel(out.solution, ref.solution) <= 1e-8
+    # true residual of the device solution, masked as the solver sees it
+    Ax = oo.nurbs_gradu_gradv(qs, C, out.solution, nurbs_weights=w)
+    r = b.copy()
+    r[dod] = 0
+    Ax[dod] = 0
+    assert np.linalg.norm(r - Ax) / np.linalg.norm(r) <= 1e-9
+
+
+def test_mixed_precision_refinement_reaches_1e10():
+    import torch
+
+    from igaops_amd.fastdiag import SingleFD
+    from igaops_amd.operators import StiffnessMassOperator
+    from igaops_amd.solvers import Mask, PrecisionAdapter, refinement_solve
+
+    patch, C, m = build_problem("annulus", 4, 8)
+    qs = patch.quadrule_list
+    w = patch.nurbs_weights
+    n = patch.nbctrlpts_total
+    fd = SingleFD("spatial")
+    fd.compute_space_eigendecomposition(qs, np.ones((1, 3, 2), bool))
+    sp = fd.space_preconditioner
+    A64 = StiffnessMassOperator(qs, C=C, m=m, nurbs_weights=w, dtype="f64")
+    A32 = StiffnessMassOperator(qs, C=C, m=m, nurbs_weights=w, dtype="f32")
+    fd.update_space_eigenvalues([1.0, 1.0])
+    mask = Mask(box=sp.box())
+    b = torch.as_tensor(np.random.default_rng(4).standard_normal(n), device="cuda")
+    mask(b)
+    P32 = PrecisionAdapter(fd, n, "f32")
+    A32a = PrecisionAdapter(A32, n, "f32")
+    out = refinement_solve(lambda x, o: A64.apply(x, out=o), lambda x, o: A32a.apply_device(x, out=o),
+                           lambda x, o: P32.apply_device(x, out=o), b, tol=1e-10, inner_tol=1e-5, mask=mask)
+    assert out.success
+    x = out.solution.cpu().numpy()
+    bh = b.cpu().numpy()
+    Ax = oo.stiffness_plus_mass(qs, C, m, x, nurbs_weights=w)
+    free = np.zeros(n, bool)
+    free[np.asarray(sp.space_free_nodes[0])] = True
+    Ax[~free] = 0
+    assert np.linalg.norm(bh - Ax) / np.linalg.norm(bh) <= 1e-10

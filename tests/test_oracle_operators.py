@@ -1,0 +1,189 @@
+"""Pins of the oracle's matrix-free operators (oracle/operators.py)."""
+import numpy as np
+import pytest
+
+from igaops_amd.geometry import quarter_annulus, unit_cube
+from igaops_amd.quadrature import StandardGauss, WeightedQuadrature, make_open_knotvector
+from oracle import assembly as asm
+from oracle import matrix_free as mf
+from oracle import operators as oo
+
+
+def gauss_list(ndim, p=3, nel=4):
+    q = StandardGauss(p, make_open_knotvector(p, nel), "legendre")
+    return [q] * ndim
+
+
+@pytest.mark.parametrize("ndim", [1, 2, 3])
+def test_mass_matches_assembly_and_kron_sum(ndim):
+    """``tests/test_tensor_operations.py:44-65`` restated: MF mass == assembled mass;
+    with unit coefficients on [0,1]^d the mass is the Kronecker product of 1-D masses."""
+    qs = gauss_list(ndim)
+    nq = int(np.prod([q.nbquadpts for q in qs]))
+    c = np.ones(nq)
+    M = asm.bspline_mass(qs, c)
+    M1 = (qs[0].weights[0] @ qs[0].basis[0]).toarray()
+    Mk = M1
+    for _ in range(ndim - 1):
+        Mk = np.kron(M1, Mk)
+    np.testing.assert_allclose(M.toarray(), Mk, rtol=1e-12, atol=1e-14)
+    v = np.random.default_rng(0).random(M.shape[0])
+    np.testing.assert_allclose(oo.bspline_u_v(qs, c, v), M @ v, rtol=1e-12)
+
+
+def test_mass_times_one_is_basis_integral():
+    """``M 1 = int N_A``: on [0,1]^3 the integral of a B-spline is (k_{i+p+1}-k_i)/(p+1)."""
+    p, nel = 3, 5
+    kv = make_open_knotvector(p, nel)
+    q = WeightedQuadrature(p, kv, "2")
+    qs = [q] * 3
+    nq = q.nbquadpts ** 3
+    y = oo.bspline_u_v(qs, np.ones(nq), np.ones(q.nbctrlpts ** 3))
+    i1 = np.array([(kv[i + p + 1] - kv[i]) / (p + 1) for i in range(q.nbctrlpts)])
+    np.testing.assert_allclose(y, np.kron(np.kron(i1, i1), i1), rtol=1e-12)
+
+
+@pytest.mark.parametrize("quad", ["gauss", "wq2", "wq1"])
+def test_stiffness_kron_sum_and_null_space(quad):
+    """Constant diagonal conductivity on the identity map: K is the Kronecker sum
+    ``sum_d c_d M..K_d..M`` (exact for Gauss, 1e-8 for WQ) and ``K 1 = 0``."""
+    p, nel = 3, 4
+    kv = make_open_knotvector(p, nel)
+    if quad == "gauss":
+        q = StandardGauss(p, kv, "legendre")
+        tol = 1e-12
+    else:
+        q = WeightedQuadrature(p, kv, quad[-1])
+        tol = 1e-8
+    qs = [q] * 3
+    nq = q.nbquadpts ** 3
+    vals = [3.0, 2.0, 5.0]
+    C = np.zeros((3, 3, nq))
+    for i in range(3):
+        C[i, i] = vals[i]
+    g = StandardGauss(p, kv, "legendre")
+    M1 = (g.weights[0] @ g.basis[0]).toarray()
+    K1 = (g.weights[3] @ g.basis[1]).toarray()
+    def kron3(f):  # f = [F_1, F_2, F_3] -> F_3 (x) F_2 (x) F_1
+        return np.kron(f[2], np.kron(f[1], f[0]))
+
+    ref = sum(vals[d] * kron3([K1 if e == d else M1 for e in range(3)]) for d in range(3))
+    rng = np.random.default_rng(1)
+    v = rng.standard_normal(q.nbctrlpts ** 3)
+    y = oo.bspline_gradu_gradv(qs, C, v)
+    assert np.linalg.norm(y - ref @ v) <= tol * np.linalg.norm(ref @ v)
+    y1 = oo.bspline_gradu_gradv(qs, C, np.ones(q.nbctrlpts ** 3))
+    assert np.max(np.abs(y1)) < 1e-10
+
+
+@pytest.mark.parametrize("op", ["gradu_gradv", "gradu_v", "u_gradv", "u_v"])
+@pytest.mark.parametrize("quad", ["gauss", "wq1"])
+def test_operators_match_assembly_random_coefficients(op, quad):
+    """L-mode operator == explicitly assembled Kronecker matrix, random variable
+    coefficients (non-symmetric C), including WQ type 1 where the test weights
+    depend on the trial derivative."""
+    p, nel = 2, 3
+    kv = make_open_knotvector(p, nel)
+    q = StandardGauss(p, kv, "legendre") if quad == "gauss" else WeightedQuadrature(p, kv, "1")
+    qs = [q] * 3
+    nq = q.nbquadpts ** 3
+    rng = np.random.default_rng(7)
+    v = rng.standard_normal(q.nbctrlpts ** 3)
+    if op == "gradu_gradv":
+        C = rng.standard_normal((3, 3, nq))
+        ref = asm.bspline_stiffness(qs, C) @ v
+        got = oo.bspline_gradu_gradv(qs, C, v)
+    elif op == "u_v":
+        c = rng.standard_normal(nq)
+        ref = asm.bspline_mass(qs, c) @ v
+        got = oo.bspline_u_v(qs, c, v)
+    else:
+        c = rng.standard_normal((3, nq))
+        A = None
+        for i in range(3):
+            if op == "gradu_v":
+                tr, te = np.zeros(3, int), 2 * np.eye(3, dtype=int)[i]
+            else:
+                tr, te = np.eye(3, dtype=int)[i], np.zeros(3, int)
+            T = asm.weights_kron(qs, te) @ asm.sp.diags_array(c[i]) @ asm.basis_kron(qs, tr)
+            A = T if A is None else A + T
+        ref = A @ v
+        got = (oo.bspline_gradu_v if op == "gradu_v" else oo.bspline_u_gradv)(qs, c, v)
+    np.testing.assert_allclose(got, ref, rtol=1e-11, atol=1e-12 * np.abs(ref).max())
+
+
+def test_stiffness_symmetric_for_symmetric_coefficients():
+    q = StandardGauss(3, make_open_knotvector(3, 3), "legendre")
+    qs = [q] * 3
+    rng = np.random.default_rng(2)
+    nq = q.nbquadpts ** 3
+    A = rng.standard_normal((3, 3, nq))
+    C = np.einsum("ikq,jkq->ijq", A, A)
+    K = asm.bspline_stiffness(qs, C).toarray()
+    np.testing.assert_allclose(K, K.T, rtol=1e-12, atol=1e-12 * np.abs(K).max())
+
+
+def test_nurbs_unit_weights_equal_bspline():
+    q = WeightedQuadrature(3, make_open_knotvector(3, 4), "2")
+    qs = [q] * 3
+    rng = np.random.default_rng(4)
+    nq, nc = q.nbquadpts ** 3, q.nbctrlpts ** 3
+    A = rng.standard_normal((3, 3, nq))
+    C = np.einsum("ikq,jkq->ijq", A, A)
+    v = rng.standard_normal(nc)
+    w = np.ones(nc)
+    np.testing.assert_allclose(
+        oo.nurbs_gradu_gradv(qs, C, v, nurbs_weights=w), oo.bspline_gradu_gradv(qs, C, v), rtol=1e-12, atol=1e-12
+    )
+    m = rng.random(nq)
+    np.testing.assert_allclose(oo.nurbs_u_v(qs, m, v, nurbs_weights=w), oo.bspline_u_v(qs, m, v), rtol=1e-12)
+
+
+@pytest.mark.parametrize("kind", ["gradu_gradv", "u_v"])
+def test_nurbs_chain_rule_against_rational_basis(kind):
+    """NURBS operators (built from B-spline ones on w o x with the folded terms) ==
+    direct assembly from the rational basis R_A = w_A N_A / W, Gauss rules, annulus
+    geometry (``tests/test_assemble.py:93-134`` restated in 3-D)."""
+    patch = quarter_annulus(3, 2, quadclass="gauss", quadtype="legendre")
+    qs = patch.quadrule_list
+    w = patch.nurbs_weights
+    C, m = patch.conductivity_coefficients()
+    v = np.random.default_rng(5).standard_normal(patch.nbctrlpts_total)
+    if kind == "gradu_gradv":
+        ref = asm.nurbs_stiffness_gauss(qs, C, w) @ v
+        got = oo.nurbs_gradu_gradv(qs, C, v, nurbs_weights=w)
+    else:
+        ref = asm.nurbs_mass_gauss(qs, m, w) @ v
+        got = oo.nurbs_u_v(qs, m, v, nurbs_weights=w)
+    np.testing.assert_allclose(got, ref, rtol=1e-10, atol=1e-12 * np.abs(ref).max())
+
+
+def test_annulus_volume_from_nurbs_mass():
+    """``1^T M 1 = |Omega| = 3 pi / 4`` for r in [1, 2], quarter turn, height 1
+    (the integrand is rational, so quadrature is not exact: the error must shrink
+    with refinement)."""
+    errs = []
+    for quadclass, nel in [("weighted", 4), ("weighted", 8), ("gauss", 8)]:
+        patch = quarter_annulus(4, nel, quadclass=quadclass, quadtype="2" if quadclass == "weighted" else "legendre")
+        qs = patch.quadrule_list
+        _, m = patch.conductivity_coefficients()
+        one = np.ones(patch.nbctrlpts_total)
+        vol = one @ oo.nurbs_u_v(qs, m, one, nurbs_weights=patch.nurbs_weights)
+        errs.append(abs(vol - 3 * np.pi / 4))
+    assert errs[1] < errs[0] / 10 and errs[1] < 1e-7
+    assert errs[2] < 1e-10
+
+
+def test_planted_index_error_is_detected():
+    """A dropped coefficient term (C01 zeroed) must fail the assembly comparison."""
+    q = StandardGauss(2, make_open_knotvector(2, 3), "legendre")
+    qs = [q] * 3
+    rng = np.random.default_rng(9)
+    C = rng.standard_normal((3, 3, q.nbquadpts ** 3))
+    v = rng.standard_normal(q.nbctrlpts ** 3)
+    ref = asm.bspline_stiffness(qs, C) @ v
+    C2 = C.copy()
+    C2[0, 1] = 0.0
+    assert not np.allclose(oo.bspline_gradu_gradv(qs, C2, v), ref, rtol=1e-8)
+    Ct = np.transpose(C, (1, 0, 2)).copy()
+    assert not np.allclose(oo.bspline_gradu_gradv(qs, Ct, v), ref, rtol=1e-8)
