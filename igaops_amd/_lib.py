@@ -31,6 +31,11 @@ SIGNATURES = {
     "iga_reduce_workspace_bytes": (_i64, [_i64, _i32]),
     "iga_dot_many": (_i32, [_i32, _i64, _i32, _vp, _i64, _vp, _vp, _vp, _vp]),
     "iga_axpy_many": (_i32, [_i32, _i64, _i32, _vp, _i64, _vp, _f64, _vp, _vp]),
+    "iga_zgram_workspace_bytes": (_i64, [_i64, _i32, _i32]),
+    "iga_zgram": (_i32, [_i64, _i32, _vp, _i32, _vp, _vp, _vp, _vp]),
+    "iga_zcombine": (_i32, [_i64, _i32, _vp, _i32, _vp, _f64, _vp, _vp]),
+    "iga_zdot_many": (_i32, [_i64, _i32, _vp, _i64, _vp, _vp, _vp, _vp]),
+    "iga_zaxpy_many": (_i32, [_i64, _i32, _vp, _i64, _vp, _f64, _f64, _vp, _vp]),
     "iga_axpby": (_i32, [_i32, _i64, _f64, _vp, _i32, _vp, _f64, _vp, _vp]),
     "iga_cast": (_i32, [_i32, _i32, _i64, _vp, _vp, _vp]),
     "iga_box_gather": (_i32, [_i32, _i32, _i64p, _i64p, _i64p, _i64, _vp, _vp, _vp]),
@@ -47,6 +52,11 @@ SIGNATURES = {
     "iga_fused_pack": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "iga_fused_apply": (_i32, [_vp, _vp, _vp, _vp, _i32, _vp, _f64, _i32, _vp, _vp]),
     "iga_fused_apply_stage": (_i32, [_vp, _i32, _vp, _vp, _vp, _i32, _vp, _f64, _i32, _vp, _vp]),
+    "iga_fd_create": (_i32, [C.POINTER(_vp), _i32, _i64p, _i64p, _i64p, C.POINTER(_f64p), _i32]),
+    "iga_fd_set_eigenvalues": (_i32, [_vp, _f64p]),
+    "iga_fd_workspace_bytes": (_i64, [_vp, _i32]),
+    "iga_fd_apply": (_i32, [_vp, _vp, _vp, _i32, _vp, _vp]),
+    "iga_fd_destroy": (_i32, [_vp]),
 }
 
 _ERRORS = {-1: ValueError, -2: RuntimeError, -3: MemoryError, -4: NotImplementedError, -5: RuntimeError}

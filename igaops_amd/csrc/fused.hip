@@ -32,9 +32,12 @@ __host__ __device__ constexpr int form_ncoef(int f) {
 }
 
 constexpr int kTQ1 = 16, kTQ2 = 16, kThreads = kTQ1 * kTQ2;
-constexpr int kDMax = 32;  // max DOF footprint of a tile in directions 1, 2
+// Max DOF footprint of a 16-point tile in directions 1, 2: 16 points span at most 9
+// elements for rules with >= 2 points per element, so p + 10 <= 17 DOFs for p <= 7.
+constexpr int kDMax = 20;
 constexpr int kPMax = 9;   // max band width p+1
 constexpr int kKMax = 4;   // max tiles/slabs whose footprint contains one DOF index
+constexpr int kSlabMax = 64;  // max q3 planes of one slab (direction-3 table staged in LDS)
 
 template <typename T>
 struct DirTab {
@@ -106,7 +109,10 @@ __device__ inline void point_contract(const T* __restrict__ c, int64_t stride, c
 }
 
 template <typename T, int P, int FORM>
-__global__ void __launch_bounds__(kThreads) fused_apply_kernel(FusedArgs<T> a) {
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8)))
+fused_apply_kernel(FusedArgs<T> a) {
+  // prefetch depth (q3 planes) of the coefficient stream
+  constexpr int PD = sizeof(T) == 8 ? 2 : 3;
   constexpr int NC = form_ncoef(FORM);
   __shared__ T xs[kDMax][kDMax + 1];
   __shared__ T t1s[2][kDMax][kTQ1];
@@ -116,6 +122,9 @@ __global__ void __launch_bounds__(kThreads) fused_apply_kernel(FusedArgs<T> a) {
   __shared__ T b2s[4][kTQ2][P];
   __shared__ int f1s[kTQ1];
   __shared__ int f2s[kTQ2];
+  __shared__ int l1r[kDMax][2];  // tile points l whose direction-1 band holds DOF c1l: [lo, hi)
+  __shared__ int m2r[kDMax][2];  // same for direction 2
+  __shared__ T b3s[kSlabMax][4][P];  // direction-3 band values of the slab's planes
 
   const int L = xcd_logical(blockIdx.x, gridDim.x);
   if (L >= a.nlogical) return;  // padding blocks of the XCD remap (whole block exits)
@@ -151,21 +160,77 @@ __global__ void __launch_bounds__(kThreads) fused_apply_kernel(FusedArgs<T> a) {
   }
   if (tid < kTQ1) f1s[tid] = (tid < q1n) ? a.d[0].first[q1s + tid] - c1lo : 0;
   if (tid < kTQ2) f2s[tid] = (tid < q2n) ? a.d[1].first[q2s + tid] - c2lo : 0;
+  // per DOF index of the footprint, the range of tile points whose band holds it
+  // (band starts are non-decreasing, so the range is contiguous)
+  if (tid < kDMax) {
+    int lo = 0, hi = 0;
+    if (tid < D1) {
+      lo = q1n;
+      for (int l = q1n - 1; l >= 0; --l) {
+        const int f = a.d[0].first[q1s + l] - c1lo;
+        if (f <= tid && tid < f + P) {
+          lo = l;
+          if (hi == 0) hi = l + 1;
+        }
+      }
+      if (lo == q1n) lo = hi = 0;
+    }
+    l1r[tid][0] = lo;
+    l1r[tid][1] = hi;
+  } else if (tid >= 64 && tid < 64 + kDMax) {
+    const int c = tid - 64;
+    int lo = 0, hi = 0;
+    if (c < D2) {
+      lo = q2n;
+      for (int m = q2n - 1; m >= 0; --m) {
+        const int f = a.d[1].first[q2s + m] - c2lo;
+        if (f <= c && c < f + P) {
+          lo = m;
+          if (hi == 0) hi = m + 1;
+        }
+      }
+      if (lo == q2n) lo = hi = 0;
+    }
+    m2r[c][0] = lo;
+    m2r[c][1] = hi;
+  }
+  for (int i = tid; i < (q3e - q3s) * 4 * P; i += kThreads) {
+    const int lq = i / (4 * P), m = (i / P) % 4, j = i % P;
+    b3s[lq][m][j] = a.d[2].tab[((int64_t)m * a.nq[2] + q3s + lq) * P + j];
+  }
   __syncthreads();
 
   T tr[3][P];  // trial partials of the window planes: (val,val), (d1,val), (val,d2)
   T te[3][P];  // test partials: (val,val), (d1,val), (val,d2) in directions (1,2)
 
-  // Builds the trial partials of DOF plane c3 for this thread's column.
-  auto trial_plane = [&](int c3, T& v00, T& v10, T& v01) {
-    __syncthreads();  // xs / t1s are reused
-    const T* xk = a.x + k;
-    for (int i = tid; i < D1 * D2; i += kThreads) {
+  // x block of one DOF plane, held in registers between its (early) load and its
+  // use, so that the load latency overlaps the work of the current step.
+  // Loads are unconditional from clamped (always valid) addresses so that the
+  // compiler does not wait on them at the point of issue; NURBS weights are kept
+  // apart and applied when the block is used.
+  constexpr int XR = (kDMax * kDMax + kThreads - 1) / kThreads;
+  const T* xk = a.x + k;
+  const bool nurbs = a.w != nullptr;
+  auto load_x = [&](int c3, T (&xr)[XR], T (&wr)[XR]) {
+#pragma unroll
+    for (int e = 0; e < XR; ++e) {
+      int i = tid + e * kThreads;
+      i = i < D1 * D2 ? i : 0;
       const int i2 = i / D1, i1 = i % D1;
       const int64_t dof = (int64_t)(c1lo + i1) + (int64_t)n1 * ((int64_t)(c2lo + i2) + (int64_t)n2 * c3);
-      T v = xk[dof * nr];
-      if (a.w) v *= a.w[dof];
-      xs[i2][i1] = v;
+      xr[e] = xk[dof * nr];
+      if (nurbs) wr[e] = a.w[dof];
+    }
+  };
+
+  // Builds the trial partials of a DOF plane (its x block given in registers) for
+  // this thread's column.
+  auto trial_plane = [&](const T (&xr)[XR], const T (&wr)[XR], T& v00, T& v10, T& v01) {
+    __syncthreads();  // xs / t1s are reused
+#pragma unroll
+    for (int e = 0; e < XR; ++e) {
+      const int i = tid + e * kThreads;
+      if (i < D1 * D2) xs[i / D1][i % D1] = nurbs ? xr[e] * wr[e] : xr[e];
     }
     __syncthreads();
     for (int i = tid; i < D2 * kTQ1; i += kThreads) {
@@ -207,15 +272,15 @@ __global__ void __launch_bounds__(kThreads) fused_apply_kernel(FusedArgs<T> a) {
     s2s[1][l2][l1] = active ? te[1][0] : T(0);
     s2s[2][l2][l1] = active ? te[2][0] : T(0);
     __syncthreads();
+    // only the rows m of the tile whose band holds DOF c2l contribute: [m2lo, m2hi)
     for (int i = tid; i < D2 * kTQ1; i += kThreads) {
       const int c2l = i / kTQ1, l = i % kTQ1;
       T r0 = T(0), r1 = T(0);
-      for (int m = 0; m < q2n; ++m) {
+      const int mhi = m2r[c2l][1];
+      for (int m = m2r[c2l][0]; m < mhi; ++m) {
         const int j = c2l - f2s[m];
-        if (j >= 0 && j < P) {
-          r0 += b2s[2][m][j] * s2s[0][m][l] + b2s[3][m][j] * s2s[2][m][l];
-          r1 += b2s[2][m][j] * s2s[1][m][l];
-        }
+        r0 += b2s[2][m][j] * s2s[0][m][l] + b2s[3][m][j] * s2s[2][m][l];
+        r1 += b2s[2][m][j] * s2s[1][m][l];
       }
       r1s[0][c2l][l] = r0;
       r1s[1][c2l][l] = r1;
@@ -225,28 +290,54 @@ __global__ void __launch_bounds__(kThreads) fused_apply_kernel(FusedArgs<T> a) {
     for (int i = tid; i < D2 * D1; i += kThreads) {
       const int c2l = i / D1, c1l = i % D1;
       T acc = T(0);
-      for (int l = 0; l < q1n; ++l) {
+      const int lhi = l1r[c1l][1];
+      for (int l = l1r[c1l][0]; l < lhi; ++l) {
         const int j = c1l - f1s[l];
-        if (j >= 0 && j < P) acc += b1s[2][l][j] * r1s[0][c2l][l] + b1s[3][l][j] * r1s[1][c2l][l];
+        acc += b1s[2][l][j] * r1s[0][c2l][l] + b1s[3][l][j] * r1s[1][c2l][l];
       }
       blk[i] = acc;
     }
   };
 
   const int32_t* first3 = a.d[2].first;
-  const T* tab3 = a.d[2].tab;
   const int64_t nq3 = a.nq[2];
+  const int c3hi = c3lo + a.s_nc[s] - 1;  // last DOF plane this slab touches
   int cur = first3[q3s];
+  {
+    // all P window planes: issue every load first, then build them one by one
+    T xr[P][XR], wr[P][XR];
 #pragma unroll
-  for (int j = 0; j < P; ++j) {
-    trial_plane(cur + j, tr[0][j], tr[1][j], tr[2][j]);
-    te[0][j] = te[1][j] = te[2][j] = T(0);
+    for (int j = 0; j < P; ++j) load_x(cur + j, xr[j], wr[j]);
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+      trial_plane(xr[j], wr[j], tr[0][j], tr[1][j], tr[2][j]);
+      te[0][j] = te[1][j] = te[2][j] = T(0);
+    }
   }
+  T xnext[XR], wnext[XR];  // x block of the next plane to enter the trial window
+  if (cur + P <= c3hi) load_x(cur + P, xnext, wnext);
 
   const int npts = q1n * q2n;
-  const T* cptr = a.coef + a.coef_base[tile] + (l1 + q1n * l2);
-  for (int q3 = q3s; q3 < q3e; ++q3) {
-    const int f = first3[q3];
+  // inactive lanes read the tile's first point (a valid address) and discard it
+  const T* cptr = a.coef + a.coef_base[tile] + (active ? (l1 + q1n * l2) : 0);
+  constexpr int NCF = form_ncoef(FORM);
+  // Coefficients and band values of plane q3 are loaded PD planes ahead into a
+  // ring of buffers; the loop is unrolled PD times so that buffer b is always
+  // addressed statically and no register with a load in flight is ever copied.
+  T cbuf[PD][NCF];
+  int fbuf[PD];
+  auto load_step = [&](int q3, T (&c)[NCF], int& fb) {
+    const T* p = cptr + (int64_t)q3 * NCF * npts;
+#pragma unroll
+    for (int f = 0; f < NCF; ++f) c[f] = p[(int64_t)f * npts];
+    fb = first3[q3];
+  };
+#pragma unroll
+  for (int b = 0; b < PD; ++b)
+    if (q3s + b < q3e) load_step(q3s + b, cbuf[b], fbuf[b]);
+
+  auto step = [&](int q3, T (&c)[NCF], int& fb) {
+    const int f = fb;
     while (cur < f) {
       flush_plane(cur);
 #pragma unroll
@@ -258,35 +349,36 @@ __global__ void __launch_bounds__(kThreads) fused_apply_kernel(FusedArgs<T> a) {
         }
       }
       te[0][P - 1] = te[1][P - 1] = te[2][P - 1] = T(0);
-      trial_plane(cur + P, tr[0][P - 1], tr[1][P - 1], tr[2][P - 1]);
+      trial_plane(xnext, wnext, tr[0][P - 1], tr[1][P - 1], tr[2][P - 1]);
       ++cur;
+      if (cur + P <= c3hi) load_x(cur + P, xnext, wnext);
     }
     if (active) {
-      T bb0[P], bb1[P], ww0[P], ww1[P];
-#pragma unroll
-      for (int j = 0; j < P; ++j) {
-        bb0[j] = tab3[(0 * nq3 + q3) * P + j];
-        bb1[j] = tab3[(1 * nq3 + q3) * P + j];
-        ww0[j] = tab3[(2 * nq3 + q3) * P + j];
-        ww1[j] = tab3[(3 * nq3 + q3) * P + j];
-      }
+      const T (&tb)[4][P] = b3s[q3 - q3s];
       T u[4] = {T(0), T(0), T(0), T(0)};
 #pragma unroll
       for (int j = 0; j < P; ++j) {
-        u[0] += bb0[j] * tr[0][j];
-        u[1] += bb0[j] * tr[1][j];
-        u[2] += bb0[j] * tr[2][j];
-        u[3] += bb1[j] * tr[0][j];
+        u[0] += tb[0][j] * tr[0][j];
+        u[1] += tb[0][j] * tr[1][j];
+        u[2] += tb[0][j] * tr[2][j];
+        u[3] += tb[1][j] * tr[0][j];
       }
       T fv[4];
-      point_contract<T, P, FORM>(cptr + (int64_t)q3 * NC * npts, npts, u, fv);
+      point_contract<T, P, FORM>(c, 1, u, fv);
 #pragma unroll
       for (int j = 0; j < P; ++j) {
-        te[0][j] += ww0[j] * fv[0] + ww1[j] * fv[3];
-        te[1][j] += ww0[j] * fv[1];
-        te[2][j] += ww0[j] * fv[2];
+        te[0][j] += tb[2][j] * fv[0] + tb[3][j] * fv[3];
+        te[1][j] += tb[2][j] * fv[1];
+        te[2][j] += tb[2][j] * fv[2];
       }
     }
+    // refill this buffer with the plane PD steps ahead
+    if (q3 + PD < q3e) load_step(q3 + PD, c, fb);
+  };
+  for (int q3 = q3s; q3 < q3e; q3 += PD) {
+#pragma unroll
+    for (int b = 0; b < PD; ++b)
+      if (q3 + b < q3e) step(q3 + b, cbuf[b], fbuf[b]);
   }
 #pragma unroll
   for (int j = 0; j < P; ++j) {
@@ -650,6 +742,7 @@ int iga_fused_create(iga_fused_t* out, const int32_t* nc, const int32_t* nq, int
         ns = std::max(1, std::min(nq[2] / (3 * P), (2048 + tiles - 1) / tiles));
       }
       ns = std::max(1, std::min(ns, nq[2]));
+      ns = std::max(ns, (nq[2] + kSlabMax - 1) / kSlabMax);  // the slab table lives in LDS
       const int slab = (nq[2] + ns - 1) / ns;
       DirTiling d3 = tile_direction(first[2], nq[2], nc[2], P, slab, 0);
       f->nslab = (int)d3.c0.size();

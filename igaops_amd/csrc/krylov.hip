@@ -93,6 +93,61 @@ __global__ void axpby_kernel(int64_t n, double a, const double* a_dev, int inver
   }
 }
 
+// Complex (interleaved re, im) variants: partial[b][2j..2j+1] = sum conj(V_j[i]) w[i]
+__global__ void __launch_bounds__(kRedThreads) zdot_many_stage1(int64_t n, int k, const double* __restrict__ V,
+                                                                int64_t ldv, const double* __restrict__ w,
+                                                                double* __restrict__ partial) {
+  __shared__ double lds[kRedThreads / 64];
+  const int nb = gridDim.x;
+  const int64_t per = (n + nb - 1) / nb;
+  const int64_t lo = (int64_t)blockIdx.x * per;
+  const int64_t hi = lo + per < n ? lo + per : n;
+  for (int j = 0; j < k; ++j) {
+    const double* v = V + 2 * (int64_t)j * ldv;
+    double re = 0.0, im = 0.0;
+    for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+      const double vr = v[2 * i], vi = v[2 * i + 1], wr = w[2 * i], wi = w[2 * i + 1];
+      re += vr * wr + vi * wi;
+      im += vr * wi - vi * wr;
+    }
+    const double sr = block_sum(re, lds);
+    const double si = block_sum(im, lds);
+    if (threadIdx.x == 0) {
+      partial[((int64_t)blockIdx.x * k + j) * 2] = sr;
+      partial[((int64_t)blockIdx.x * k + j) * 2 + 1] = si;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kRedThreads) zdot_many_stage2(int nb, int k, const double* __restrict__ partial,
+                                                                double* __restrict__ out) {
+  __shared__ double lds[kRedThreads / 64];
+  const int j = blockIdx.x >> 1, part = blockIdx.x & 1;
+  double acc = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) acc += partial[((int64_t)b * k + j) * 2 + part];
+  const double s = block_sum(acc, lds);
+  if (threadIdx.x == 0) out[2 * j + part] = s;
+}
+
+// w[i] += alpha * sum_j h[j] V_j[i]   (complex h, complex alpha)
+__global__ void zaxpy_many_kernel(int64_t n, int k, const double* __restrict__ V, int64_t ldv,
+                                  const double* __restrict__ h, double ar, double ai, double* __restrict__ w) {
+  extern __shared__ double hs[];
+  for (int j = threadIdx.x; j < 2 * k; j += blockDim.x) hs[j] = h[j];
+  __syncthreads();
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double re = 0.0, im = 0.0;
+    for (int j = 0; j < k; ++j) {
+      const double vr = V[2 * ((int64_t)j * ldv + i)], vi = V[2 * ((int64_t)j * ldv + i) + 1];
+      const double hr = hs[2 * j], hi = hs[2 * j + 1];
+      re += hr * vr - hi * vi;
+      im += hr * vi + hi * vr;
+    }
+    w[2 * i] += ar * re - ai * im;
+    w[2 * i + 1] += ar * im + ai * re;
+  }
+}
+
 template <typename S, typename D>
 __global__ void cast_kernel(int64_t n, const S* __restrict__ x, D* __restrict__ y) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -145,6 +200,33 @@ int iga_axpy_many(int dtype, int64_t n, int k, const void* V, int64_t ldv, const
       axpy_many_kernel<float><<<grid, block, shm, s>>>(n, k, (const float*)V, ldv, h, alpha, (float*)w);
     else
       throw Error(IGA_ERR_INVALID, "unknown dtype");
+    IGA_LAUNCH_CHECK();
+  });
+}
+
+int iga_zdot_many(int64_t n, int k, const double* V, int64_t ldv, const double* w, double* out, void* work,
+                  void* stream) {
+  return guard([&] {
+    IGA_CHECK_ARG(V && w && out && work && n >= 0 && k >= 0 && ldv >= n, "bad argument");
+    if (k == 0) return;
+    hipStream_t s = as_stream(stream);
+    const int nb = reduce_blocks(n);
+    zdot_many_stage1<<<nb, kRedThreads, 0, s>>>(n, k, V, ldv, w, (double*)work);
+    IGA_LAUNCH_CHECK();
+    zdot_many_stage2<<<2 * k, kRedThreads, 0, s>>>(nb, k, (const double*)work, out);
+    IGA_LAUNCH_CHECK();
+  });
+}
+
+int iga_zaxpy_many(int64_t n, int k, const double* V, int64_t ldv, const double* h, double alpha_re,
+                   double alpha_im, double* w, void* stream) {
+  return guard([&] {
+    IGA_CHECK_ARG(V && w && h && n >= 0 && k >= 0 && ldv >= n, "bad argument");
+    IGA_CHECK_ARG(k <= 4096, "too many vectors");
+    if (k == 0 || n == 0) return;
+    dim3 grid(grid_for(n, 256, 8192)), block(256);
+    zaxpy_many_kernel<<<grid, block, sizeof(double) * 2 * (size_t)k, as_stream(stream)>>>(n, k, V, ldv, h, alpha_re,
+                                                                                       alpha_im, w);
     IGA_LAUNCH_CHECK();
   });
 }

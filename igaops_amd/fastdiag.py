@@ -120,6 +120,17 @@ class SpaceFD:
             self._dev.pop("inv_eig", None)
 
     # -- device apply ---------------------------------------------------------
+    def plan(self, dtype: str) -> "FDPlan":
+        key = ("plan", dtype)
+        p = self._dev.get(key)
+        if p is None:
+            n, lo, ln = self.box()
+            p = FDPlan(n, lo, ln, self.eigenvec_by_dir_space, dtype)
+            self._dev[key] = p
+        if p.eig_version != id(self.space_eigenvalues[0]):
+            p.set_eigenvalues(self.space_eigenvalues[0])
+        return p
+
     def box(self):
         lo = np.array([int(idx[0]) if len(idx) else 0 for idx in self.indices], dtype=np.int64)
         ln = np.array([len(idx) for idx in self.indices], dtype=np.int64)
@@ -139,7 +150,15 @@ class SpaceFD:
         return self._dev[key][0], self._dev[key][1], self._dev["inv_eig"][1]
 
     def apply_device(self, x, out=None, nrhs: int = 1, stream=None):
-        """``out = P x`` for device arrays ((N, nrhs) C order)."""
+        """``out = P x`` for device arrays ((N, nrhs) C order), MFMA mode products."""
+        t = dv.torch()
+        dtype = "f32" if x.dtype == t.float32 else "f64"
+        if out is None:
+            out = t.empty_like(x)
+        return self.plan(dtype).apply(x, out, nrhs, stream)
+
+    def apply_device_bands(self, x, out=None, nrhs: int = 1, stream=None):
+        """``out = P x`` through the generic banded mode-product kernel (dense bands)."""
         from .operators import mf_apply_device
 
         t = dv.torch()
@@ -160,6 +179,47 @@ class SpaceFD:
         if out is None:
             out = t.empty_like(x)
         _lib.call("iga_box_scatter", dt, len(n), npt, lpt, lnpt, nrhs, dv.ptr(res), dv.ptr(out), 1, s)
+        return out
+
+
+class FDPlan:
+    """Device FD plan (``iga_fd_t``): eigenvectors uploaded once, ``Lambda^{-1}`` on set."""
+
+    def __init__(self, n, lo, ln, eigvecs, dtype: str):
+        import ctypes as C
+
+        self.dtype = dtype
+        self.nsub = int(np.prod(ln))
+        na, npt = _lib.i64_array(n)
+        la, lpt = _lib.i64_array(lo)
+        lna, lnpt = _lib.i64_array(ln)
+        keep = [np.ascontiguousarray(U, dtype=np.float64) for U in eigvecs]
+        arr = (_lib._f64p * len(keep))(*[k.ctypes.data_as(_lib._f64p) for k in keep])
+        h = C.c_void_p()
+        _lib.call("iga_fd_create", C.byref(h), len(n), npt, lpt, lnpt, arr, dv.DTYPES[dtype])
+        self._h = h
+        self.eig_version = None
+        self._work = {}
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib._lib is not None:
+            _lib.load().iga_fd_destroy(h)
+            self._h = None
+
+    def set_eigenvalues(self, lam: np.ndarray):
+        la, lp = _lib.f64_array(lam)
+        if la.size != self.nsub:
+            raise ValueError("eigenvalues do not match the free box")
+        _lib.call("iga_fd_set_eigenvalues", self._h, lp)
+        self.eig_version = id(lam)
+
+    def apply(self, x, out, nrhs: int = 1, stream=None):
+        w = self._work.get(nrhs)
+        if w is None:
+            w = dv.workspace(_lib.load().iga_fd_workspace_bytes(self._h, int(nrhs)))
+            self._work[nrhs] = w
+        _lib.call("iga_fd_apply", self._h, dv.ptr(x), dv.ptr(out), int(nrhs), dv.ptr(w), dv.stream_ptr(stream))
         return out
 
 

@@ -96,6 +96,27 @@ int iga_dot_many(int dtype, int64_t n, int k, const void* V, int64_t ldv, const 
 int iga_axpy_many(int dtype, int64_t n, int k, const void* V, int64_t ldv, const double* h,
                   double alpha, void* w, void* stream);
 
+/* Complex (interleaved re, im; float64) versions for the recycled solver
+ * (solvers/deflated/helpers.py:44-58,116-133; gcrodr.py:266-276,340-345):
+ *   out[2j], out[2j+1] = re, im of sum_i conj(V_j[i]) w[i]   (work: bytes for 2k vectors)
+ *   w[i] += alpha * sum_j h[j] V_j[i]                          (h: device complex[k]) */
+int iga_zdot_many(int64_t n, int k, const double* V, int64_t ldv, const double* w, double* out,
+                  void* work, void* stream);
+int iga_zaxpy_many(int64_t n, int k, const double* V, int64_t ldv, const double* h, double alpha_re,
+                   double alpha_im, double* w, void* stream);
+
+/* One-pass block operations on sets of complex vectors given as device arrays of
+ * row addresses (the tall products of gcrodr.py:266-276,288-292,327-363 and
+ * helpers.py:240-253):
+ *   iga_zgram:    out[a*nb + b] = sum_i conj(X_a[i]) Y_b[i]   (complex, 1 <= na, nb <= 64)
+ *   iga_zcombine: Out_c = beta Out_c + sum_j coef[j*nout + c] V_j   (1 <= nout <= 24)
+ * work for iga_zgram: iga_zgram_workspace_bytes(n, na, nb) bytes. */
+int64_t iga_zgram_workspace_bytes(int64_t n, int na, int nb);
+int iga_zgram(int64_t n, int na, const double* const* X, int nb, const double* const* Y, double* out,
+              void* work, void* stream);
+int iga_zcombine(int64_t n, int nv, const double* const* V, int nout, const double* coef, double beta,
+                 double* const* out, void* stream);
+
 /* y[i] = a * x[i] + b * y[i]; a and b are read from device memory when
  * a_dev / b_dev are non-NULL (then a[0] / b[0] is used, optionally inverted
  * with invert_a), else the host values are used. */
@@ -165,6 +186,24 @@ int iga_fused_apply(iga_fused_t plan, const void* packed, const void* x, void* y
 int iga_fused_apply_stage(iga_fused_t plan, int stage, const void* packed, const void* x, void* y,
                           int nrhs, const void* nurbs_w, double alpha, int accumulate, void* work,
                           void* stream);
+
+/* ---- fast-diagonalization apply --------------------------------------------
+ * Replaces SpaceFD.apply (fastdiag/utils/space_precond.py:273-320) for one DOF per
+ * node: y[free] = U (Lambda^{-1} o U^T x[free]), y = 0 outside the free box, where
+ * U = U_ndim (x) ... (x) U_1.  The free box of direction d is [lo[d], lo[d]+len[d])
+ * of n[d] DOFs (the Dirichlet trimming of space_precond.py:160-171); U[d] is the
+ * host float64 row-major len[d] x len[d] eigenvector matrix of eigh(K_d, M_d)
+ * (space_precond.py:175).  Mode products run as MFMA GEMM tiles in the plan dtype. */
+typedef struct iga_fd_s* iga_fd_t;
+
+int iga_fd_create(iga_fd_t* out, int ndim, const int64_t* n, const int64_t* lo, const int64_t* len,
+                  const double* const* U, int dtype);
+/* Lambda over the free box (host float64, direction 1 fastest), e.g. the output of
+ * SpaceFD.update_space_eigenvalues (space_precond.py:195-236); stored inverted. */
+int iga_fd_set_eigenvalues(iga_fd_t plan, const double* eigenvalues);
+int64_t iga_fd_workspace_bytes(iga_fd_t plan, int nrhs);
+int iga_fd_apply(iga_fd_t plan, const void* x, void* y, int nrhs, void* work, void* stream);
+int iga_fd_destroy(iga_fd_t plan);
 
 #ifdef __cplusplus
 }

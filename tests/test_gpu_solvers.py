@@ -37,6 +37,11 @@ def test_fd_apply_matches_oracle(table, p, nel):
     np.testing.assert_array_equal(np.asarray(fd.space_preconditioner.space_free_nodes[0]), ofd_.free_nodes())
     assert rel(fd.apply(X), ofd_.apply(X)) <= 1e-12
     assert rel(fd.apply(X[:, 0]), ofd_.apply(X[:, 0])) <= 1e-12
+    import torch
+
+    x32 = torch.as_tensor(X[:, 0], device="cuda").float()
+    y32 = fd.apply(x32).double().cpu().numpy()
+    assert rel(y32, ofd_.apply(X[:, 0])) <= 1e-4  # float32 GEMMs on a conditioned inverse
 
 
 def build_problem(geom, p, nel, sigma=0.5):
