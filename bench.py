@@ -1,26 +1,31 @@
 """Benchmark of the weighted-quadrature matrix-free operator path on one MI355X.
 
-A step is one stiffness + mass matvec ``Y = (K + M) X`` over a batch of right-hand
-sides, inputs resident in HBM.  Default workload (the FP64 GPU drop-in case):
-quarter thick-walled annulus (r in [1,2]) extruded to height 1 as a NURBS map,
-degree 4, 64 elements per direction (68^3 = 314,432 DOF), WQ type "2" rules
-(199^3 = 7,880,599 points), conductivity ``kappa detJ J^+ J^+T`` with a lognormal
-``kappa`` (sigma 0.5, seeded Fourier modes, 8 per axis), 16 seeded N(0,1)
-right-hand sides, float64.
+A step is one stiffness + mass matvec ``y = (K + M) x`` on one right-hand side,
+inputs resident in HBM.  Default workload (the FP64 GPU drop-in case): quarter
+thick-walled annulus (r in [1,2]) extruded to height 1 as a NURBS map, degree 4,
+64 elements per direction (68^3 = 314,432 DOF), WQ type "2" rules (199^3 =
+7,880,599 points), conductivity ``kappa detJ J^-1 J^-T`` with a lognormal ``kappa``
+(sigma 0.5, seeded Fourier modes, 8 per axis), seeded N(0,1) vectors, float64.
 
-Printed: one JSON line with GDOF/s (= N_dof * nrhs / t / 1e9), the roofline of the
-fused sweep kernel (algorithmic bytes ``8 (2 N_dof r + 11 N_q)`` over its average
-event-timed launch), the relative error against the CPU oracle on one right-hand
-side, and the oracle's own single-thread throughput on the host (cpu_baseline).
-``--solve`` adds FD-preconditioned GMRES solves of the 16 right-hand sides to
-1e-10 (plain FP64 and FP32/IR) with solves/s and p50/p99 latency.
+Printed: one JSON line with GDOF/s (= N_dof / t_step / 1e9); the roofline of the
+fused sweep kernel (algorithmic bytes ``8 (2 N_dof + 11 N_q)`` over its average
+event-timed launch, and the HBM bytes it moves, read from two rocprofv3 counter
+passes over a child run and scaled by the same counters on a 512 MiB device copy);
+the 16 right-hand sides of the workload batched in one launch; the relative error
+against the CPU oracle; the oracle's own single-thread throughput on the host
+(cpu_baseline); and FD-preconditioned GMRES solves of the 16 right-hand sides to
+1e-10 (plain FP64, and FP32 operators inside FP64 iterative refinement) with
+solves/s and p50/p99 latency.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -112,7 +117,7 @@ def solve_section(patch, C, m, nrhs, tol=1e-10):
             t0 = time.perf_counter()
             if mode == "fp64":
                 out = gmres_device(lambda x, o: A64.apply(x, out=o), b, lambda x, o: fd.apply_device(x, out=o),
-                                   tol, 400, mask=mask, restart=None)
+                                   tol, 400, mask=mask, restart=None, use_increment=False)
             else:
                 out = refinement_solve(lambda x, o: A64.apply(x, out=o), lambda x, o: A32a.apply_device(x, out=o),
                                        lambda x, o: P32.apply_device(x, out=o), b, tol=tol, inner_tol=1e-5,
@@ -137,6 +142,76 @@ def solve_section(patch, C, m, nrhs, tol=1e-10):
     return res
 
 
+PMC_COPY_BYTES = 512 * 1024 * 1024
+
+
+def pmc_child(args):
+    """Body of the counter runs: three launches of the sweep kernel on one
+    right-hand side, then three 512 MiB device copies (the counters' scale)."""
+    import torch
+
+    from igaops_amd.operators import StiffnessMassOperator
+
+    cfg = dict(WORKLOADS[args.workload])
+    patch, C, m = build_patch(cfg)
+    w = patch.nurbs_weights if patch.is_nurbs else None
+    op = StiffnessMassOperator(patch.quadrule_list, C=C, m=m, nurbs_weights=w, dtype=args.dtype)
+    tdt = torch.float64 if args.dtype == "f64" else torch.float32
+    x = torch.randn(patch.nbctrlpts_total, dtype=tdt, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        op.plan.apply_stage(1, op.packed, x, y, 1, op.w)
+    a = torch.ones(PMC_COPY_BYTES // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print("pmc child done", flush=True)
+
+
+def _counter_values(csv_path, name_part):
+    import csv
+
+    with open(csv_path) as fh:
+        return [float(r["Counter_Value"]) for r in csv.DictReader(fh) if name_part in r["Kernel_Name"]]
+
+
+def pmc_traffic(args):
+    """HBM bytes per launch of the sweep kernel from two rocprofv3 passes
+    (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled by the ratio of
+    the bytes a 512 MiB copy moves to what the same counter reads for it."""
+    prof = shutil.which("rocprofv3")
+    if prof is None:
+        return None, "rocprofv3 not found"
+    out = {}
+    tmp = tempfile.mkdtemp(prefix="igaops_pmc_", dir="/tmp")
+    env = dict(os.environ, TMPDIR="/tmp")
+    try:
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            d = os.path.join(tmp, counter)
+            cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "run", "--", sys.executable,
+                   os.path.abspath(__file__), "--pmc-child", "--workload", args.workload, "--dtype", args.dtype]
+            subprocess.run(cmd, cwd="/tmp", env=env, timeout=150, stdout=subprocess.DEVNULL,
+                           stderr=subprocess.DEVNULL, check=False)
+            csvs = [os.path.join(r, f) for r, _, fs in os.walk(d) for f in fs if f.endswith("counter_collection.csv")]
+            if not csvs:
+                return None, f"no {counter} counters collected"
+            kv = _counter_values(csvs[0], "fused_apply_kernel")
+            # the three large copies (set-up uploads also run the copy kernel, at small sizes)
+            cv = _counter_values(csvs[0], "copyBuffer")
+            kern = sum(kv) / len(kv) if kv else None
+            copy = max(cv) if cv else None
+            if not kern or not copy:
+                return None, f"{counter}: kernel or copy dispatch missing"
+            scale = PMC_COPY_BYTES / (copy * 1024.0)
+            out[counter] = {"kib": kern, "scale": scale, "bytes": kern * 1024.0 * scale}
+    except (subprocess.TimeoutExpired, OSError) as e:
+        return None, f"counter pass failed: {type(e).__name__}"
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    return out, None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,8 +221,13 @@ def main():
     ap.add_argument("--dtype", default="f64", choices=["f64", "f32"])
     ap.add_argument("--nrhs", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--solve", action="store_true")
+    ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--no-traffic", action="store_true")
+    ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.pmc_child:
+        pmc_child(args)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)
         sys.exit(2)
@@ -159,7 +239,7 @@ def main():
     cfg = dict(WORKLOADS[args.workload])
     if args.nrhs:
         cfg["nrhs"] = args.nrhs
-    r = cfg["nrhs"]
+    r_batch = cfg["nrhs"]
     t_setup = time.perf_counter()
     patch, C, m = build_patch(cfg)
     w = patch.nurbs_weights if patch.is_nurbs else None
@@ -168,14 +248,16 @@ def main():
     setup_s = time.perf_counter() - t_setup
     n, nq = patch.nbctrlpts_total, patch.nbquadpts_total
     tdt = torch.float64 if args.dtype == "f64" else torch.float32
-    X = torch.as_tensor(np.random.default_rng(1).standard_normal((n, r)), device="cuda").to(tdt).contiguous()
+    rng = np.random.default_rng(1)
+    X = torch.as_tensor(rng.standard_normal((n, r_batch)), device="cuda").to(tdt).contiguous()
+    x1 = X[:, 0].contiguous()
+    y1 = torch.empty_like(x1)
     Y = torch.empty_like(X)
-    Xf, Yf = X.reshape(-1), Y.reshape(-1)
 
+    # timed region: K single-right-hand-side matvecs bracketed by synchronisations
     def step():
-        op.apply(Xf, out=Yf, nrhs=r)
+        op.apply(x1, out=y1, nrhs=1)
 
-    # timed region: K steps bracketed by synchronisations
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -185,19 +267,32 @@ def main():
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     ms_step = elapsed * 1e3 / args.steps
-    value = n * r * args.steps / elapsed / 1e9
+    value = n * args.steps / elapsed / 1e9
 
     # dominant kernel (the coefficient sweep) timed alone with events on its stream
     plan, packed = op.plan, op.packed
-    main_ms = time_events(lambda: plan.apply_stage(1, packed, Xf, Yf, r, op.w), args.steps, 1)
-    gather_ms = time_events(lambda: plan.apply_stage(2, packed, Xf, Yf, r, op.w), args.steps, 1)
+    main_ms = time_events(lambda: plan.apply_stage(1, packed, x1, y1, 1, op.w), args.steps, 1)
+    gather_ms = time_events(lambda: plan.apply_stage(2, packed, x1, y1, 1, op.w), args.steps, 1)
     t_main = float(np.mean(main_ms)) * 1e-3
-    alg_bytes = op.algorithmic_bytes(r)
+    alg_bytes = op.algorithmic_bytes(1)
     achieved = alg_bytes / t_main / 1e9
 
+    # the batch of right-hand sides of the workload in one launch
+    batched = None
+    if r_batch > 1:
+        Xf, Yf = X.reshape(-1), Y.reshape(-1)
+        bms = time_events(lambda: op.apply(Xf, out=Yf, nrhs=r_batch), max(3, args.steps // 2), 1)
+        tb = float(np.mean(bms)) * 1e-3
+        batched = {
+            "nrhs": r_batch,
+            "ms": tb * 1e3,
+            "gdofs": n * r_batch / tb / 1e9,
+            "pct_hbm_roofline": op.algorithmic_bytes(r_batch) / tb / 1e9 / HBM_PEAK_GBS,
+        }
+
     # parity on one right-hand side against the oracle, and the CPU baseline
-    x0 = X[:, 0].double().cpu().numpy()
-    y0 = op.apply(X[:, 0].contiguous()).double().cpu().numpy()
+    x0 = x1.double().cpu().numpy()
+    y0 = op.apply(x1).double().cpu().numpy()
     cpu = None
     relerr = None
     if not args.no_cpu:
@@ -225,12 +320,15 @@ def main():
         "dtype": args.dtype,
         "data": "synthetic (seeded geometry, lognormal conductivity and N(0,1) right-hand sides)",
         "config": {
-            "workload": f"{args.workload}: {cfg['geometry']} NURBS, degree {cfg['degree']}, {cfg['nel']} elements/dir, "
-            f"WQ type 2, stiffness+mass, {r} rhs",
+            "workload": f"{args.workload}: {cfg['geometry']} NURBS map, degree {cfg['degree']}, {cfg['nel']} "
+            f"elements/dir, WQ type 2, stiffness+mass matvec, one right-hand side per step "
+            f"({r_batch} right-hand sides of the workload batched in 'batched')",
             "ndof": n,
             "nq": nq,
-            "nrhs": r,
+            "nrhs_per_step": 1,
             "coef_form": op.form,
+            "slabs": op.plan.nslab,
+            "tiles": list(op.plan.tiles),
         },
         "roofline": {
             "bound": "hbm",
@@ -245,12 +343,27 @@ def main():
             "algorithmic_bytes": alg_bytes,
         },
         "pct_hbm_roofline_matvec": alg_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "batched": batched,
         "rel_err_vs_oracle": relerr,
         "cpu_baseline": cpu,
         "setup_s": setup_s,
     }
-    if args.solve:
-        out["solve"] = solve_section(patch, C, m, r)
+    if not args.no_traffic:
+        pm, why = pmc_traffic(args)
+        if pm is not None:
+            t = pm["FETCH_SIZE"]["bytes"] + pm["WRITE_SIZE"]["bytes"]
+            out["roofline"]["traffic"] = t
+            out["roofline"]["traffic_detail"] = {
+                "fetch_bytes": pm["FETCH_SIZE"]["bytes"],
+                "write_bytes": pm["WRITE_SIZE"]["bytes"],
+                "fetch_scale": pm["FETCH_SIZE"]["scale"],
+                "write_scale": pm["WRITE_SIZE"]["scale"],
+                "traffic_over_algorithmic": t / alg_bytes,
+            }
+        else:
+            out["roofline"]["traffic_note"] = why
+    if not args.no_solve:
+        out["solve"] = solve_section(patch, C, m, r_batch)
     print(json.dumps(out))
 
 

@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <vector>
 
 #include "common.hpp"
@@ -109,8 +110,7 @@ __device__ inline void point_contract(const T* __restrict__ c, int64_t stride, c
 }
 
 template <typename T, int P, int FORM>
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8)))
-fused_apply_kernel(FusedArgs<T> a) {
+__device__ __attribute__((always_inline)) inline void fused_apply_body(const FusedArgs<T>& a) {
   // prefetch depth (q3 planes) of the coefficient stream
   constexpr int PD = sizeof(T) == 8 ? 2 : 3;
   constexpr int NC = form_ncoef(FORM);
@@ -211,135 +211,168 @@ fused_apply_kernel(FusedArgs<T> a) {
   constexpr int XR = (kDMax * kDMax + kThreads - 1) / kThreads;
   const T* xk = a.x + k;
   const bool nurbs = a.w != nullptr;
-  auto load_x = [&](int c3, T (&xr)[XR], T (&wr)[XR]) {
+  // Per-thread offsets of its x-block elements within a DOF plane and their LDS
+  // slots, computed once (no division in the sweep).  Lanes without an element
+  // load the block's first entry (a valid address) and do not store it.
+  // The same (i2, i1) split of the footprint serves the flush of test planes.
+  int64_t xoff[XR];
+  int xij[XR];  // (i2 << 8) | i1, or -1 past the footprint
+#pragma unroll
+  for (int e = 0; e < XR; ++e) {
+    const int i = tid + e * kThreads;
+    const bool ok = i < D1 * D2;
+    const int ii = ok ? i : 0;
+    const int i2 = ii / D1, i1 = ii % D1;
+    xoff[e] = (int64_t)(c1lo + i1) + (int64_t)n1 * (c2lo + i2);
+    xij[e] = ok ? (i2 << 8) | i1 : -1;
+  }
+  const int64_t plane = (int64_t)n1 * n2;
+  struct XBlock {
+    T x[XR], w[XR];
+  };
+  struct Trial {
+    T v00, v10, v01;
+  };
+  auto load_x = [&](int c3) __attribute__((always_inline)) {
+    XBlock b;
 #pragma unroll
     for (int e = 0; e < XR; ++e) {
-      int i = tid + e * kThreads;
-      i = i < D1 * D2 ? i : 0;
-      const int i2 = i / D1, i1 = i % D1;
-      const int64_t dof = (int64_t)(c1lo + i1) + (int64_t)n1 * ((int64_t)(c2lo + i2) + (int64_t)n2 * c3);
-      xr[e] = xk[dof * nr];
-      if (nurbs) wr[e] = a.w[dof];
+      const int64_t dof = xoff[e] + plane * c3;
+      b.x[e] = xk[dof * nr];
+      b.w[e] = nurbs ? a.w[dof] : T(1);
     }
+    return b;
   };
 
-  // Builds the trial partials of a DOF plane (its x block given in registers) for
-  // this thread's column.
-  auto trial_plane = [&](const T (&xr)[XR], const T (&wr)[XR], T& v00, T& v10, T& v01) {
-    __syncthreads();  // xs / t1s are reused
+  // One LDS phase of the sweep.  It may (a) contract the oldest test plane (ring
+  // slot 0, DOF plane c3f) in directions 2 and 1 into the tile's partial block and
+  // (b) build the trial partials of a new DOF plane from an x block held in
+  // registers.  Both contractions share the phase's three barriers.
+  T* part = a.partial + (int64_t)k * a.part_stride + a.part_base[(int64_t)tile * a.nslab + s];
+  T* xsf = &xs[0][0];
+  auto plane_phase = [&](bool do_flush, int c3f, bool do_trial, XBlock xb) __attribute__((always_inline)) {
+    Trial out{T(0), T(0), T(0)};
+    __syncthreads();  // LDS buffers are reused
+    if (do_flush) {
+      s2s[0][l2][l1] = active ? te[0][0] : T(0);
+      s2s[1][l2][l1] = active ? te[1][0] : T(0);
+      s2s[2][l2][l1] = active ? te[2][0] : T(0);
+    }
+    if (do_trial) {
 #pragma unroll
-    for (int e = 0; e < XR; ++e) {
-      const int i = tid + e * kThreads;
-      if (i < D1 * D2) xs[i / D1][i % D1] = nurbs ? xr[e] * wr[e] : xr[e];
+      for (int e = 0; e < XR; ++e)
+        if (xij[e] >= 0) xsf[(xij[e] >> 8) * (kDMax + 1) + (xij[e] & 255)] = nurbs ? xb.x[e] * xb.w[e] : xb.x[e];
     }
     __syncthreads();
-    for (int i = tid; i < D2 * kTQ1; i += kThreads) {
-      const int i2 = i / kTQ1, l = i % kTQ1;
-      const int f = f1s[l];
-      T acc0 = T(0), acc1 = T(0);
-      if (l < q1n) {
+    // stage A: direction-2 test contraction (R1) and direction-1 trial contraction (T1)
+    const int na = do_flush ? D2 * kTQ1 : 0;
+    const int nb = do_trial ? D2 * kTQ1 : 0;
+    for (int it = tid; it < na + nb; it += kThreads) {
+      if (it < na) {
+        const int c2l = it / kTQ1, l = it % kTQ1;
+        T r0 = T(0), r1 = T(0);
+        const int mhi = m2r[c2l][1];
+        for (int m = m2r[c2l][0]; m < mhi; ++m) {
+          const int j = c2l - f2s[m];
+          r0 += b2s[2][m][j] * s2s[0][m][l] + b2s[3][m][j] * s2s[2][m][l];
+          r1 += b2s[2][m][j] * s2s[1][m][l];
+        }
+        r1s[0][c2l][l] = r0;
+        r1s[1][c2l][l] = r1;
+      } else {
+        const int i2 = (it - na) / kTQ1, l = (it - na) % kTQ1;
+        const int f = f1s[l];
+        T acc0 = T(0), acc1 = T(0);
+        if (l < q1n) {
 #pragma unroll
-        for (int j = 0; j < P; ++j) {
-          const T xv = xs[i2][f + j];
-          acc0 += b1s[0][l][j] * xv;
-          acc1 += b1s[1][l][j] * xv;
+          for (int j = 0; j < P; ++j) {
+            const T xv = xs[i2][f + j];
+            acc0 += b1s[0][l][j] * xv;
+            acc1 += b1s[1][l][j] * xv;
+          }
+        }
+        t1s[0][i2][l] = acc0;
+        t1s[1][i2][l] = acc1;
+      }
+    }
+    __syncthreads();
+    // stage B: direction-1 test contraction into the partial block, and the
+    // direction-2 trial contraction of this thread's column
+    if (do_flush) {
+      T* blk = part + (int64_t)(c3f - c3lo) * D2 * D1;
+#pragma unroll
+      for (int e = 0; e < XR; ++e) {
+        if (xij[e] >= 0) {
+          const int c2l = xij[e] >> 8, c1l = xij[e] & 255;
+          T acc = T(0);
+          const int lhi = l1r[c1l][1];
+          for (int l = l1r[c1l][0]; l < lhi; ++l) {
+            const int j = c1l - f1s[l];
+            acc += b1s[2][l][j] * r1s[0][c2l][l] + b1s[3][l][j] * r1s[1][c2l][l];
+          }
+          blk[tid + e * kThreads] = acc;
         }
       }
-      t1s[0][i2][l] = acc0;
-      t1s[1][i2][l] = acc1;
     }
-    __syncthreads();
-    v00 = v10 = v01 = T(0);
-    if (active) {
+    if (do_trial && active) {
       const int f = f2s[l2];
 #pragma unroll
       for (int j = 0; j < P; ++j) {
         const T b0 = b2s[0][l2][j], b1 = b2s[1][l2][j];
         const T a0 = t1s[0][f + j][l1], a1 = t1s[1][f + j][l1];
-        v00 += b0 * a0;
-        v10 += b0 * a1;
-        v01 += b1 * a0;
+        out.v00 += b0 * a0;
+        out.v10 += b0 * a1;
+        out.v01 += b1 * a0;
       }
     }
-  };
-
-  // Contracts the oldest test plane (ring slot 0, DOF plane c3) in directions
-  // 2 and 1 and writes the tile's partial block of that plane.
-  T* part = a.partial + (int64_t)k * a.part_stride + a.part_base[(int64_t)tile * a.nslab + s];
-  auto flush_plane = [&](int c3) {
-    __syncthreads();  // s2s / r1s are reused
-    s2s[0][l2][l1] = active ? te[0][0] : T(0);
-    s2s[1][l2][l1] = active ? te[1][0] : T(0);
-    s2s[2][l2][l1] = active ? te[2][0] : T(0);
-    __syncthreads();
-    // only the rows m of the tile whose band holds DOF c2l contribute: [m2lo, m2hi)
-    for (int i = tid; i < D2 * kTQ1; i += kThreads) {
-      const int c2l = i / kTQ1, l = i % kTQ1;
-      T r0 = T(0), r1 = T(0);
-      const int mhi = m2r[c2l][1];
-      for (int m = m2r[c2l][0]; m < mhi; ++m) {
-        const int j = c2l - f2s[m];
-        r0 += b2s[2][m][j] * s2s[0][m][l] + b2s[3][m][j] * s2s[2][m][l];
-        r1 += b2s[2][m][j] * s2s[1][m][l];
-      }
-      r1s[0][c2l][l] = r0;
-      r1s[1][c2l][l] = r1;
-    }
-    __syncthreads();
-    T* blk = part + (int64_t)(c3 - c3lo) * D2 * D1;
-    for (int i = tid; i < D2 * D1; i += kThreads) {
-      const int c2l = i / D1, c1l = i % D1;
-      T acc = T(0);
-      const int lhi = l1r[c1l][1];
-      for (int l = l1r[c1l][0]; l < lhi; ++l) {
-        const int j = c1l - f1s[l];
-        acc += b1s[2][l][j] * r1s[0][c2l][l] + b1s[3][l][j] * r1s[1][c2l][l];
-      }
-      blk[i] = acc;
-    }
+    return out;
   };
 
   const int32_t* first3 = a.d[2].first;
-  const int64_t nq3 = a.nq[2];
   const int c3hi = c3lo + a.s_nc[s] - 1;  // last DOF plane this slab touches
   int cur = first3[q3s];
   {
     // all P window planes: issue every load first, then build them one by one
-    T xr[P][XR], wr[P][XR];
+    XBlock xw[P];
 #pragma unroll
-    for (int j = 0; j < P; ++j) load_x(cur + j, xr[j], wr[j]);
+    for (int j = 0; j < P; ++j) xw[j] = load_x(cur + j);
 #pragma unroll
     for (int j = 0; j < P; ++j) {
-      trial_plane(xr[j], wr[j], tr[0][j], tr[1][j], tr[2][j]);
+      const Trial t = plane_phase(false, 0, true, xw[j]);
+      tr[0][j] = t.v00;
+      tr[1][j] = t.v10;
+      tr[2][j] = t.v01;
       te[0][j] = te[1][j] = te[2][j] = T(0);
     }
   }
-  T xnext[XR], wnext[XR];  // x block of the next plane to enter the trial window
-  if (cur + P <= c3hi) load_x(cur + P, xnext, wnext);
+  // x block of the next plane to enter the trial window
+  XBlock xnext = load_x(cur + P <= c3hi ? cur + P : cur);
 
   const int npts = q1n * q2n;
   // inactive lanes read the tile's first point (a valid address) and discard it
   const T* cptr = a.coef + a.coef_base[tile] + (active ? (l1 + q1n * l2) : 0);
   constexpr int NCF = form_ncoef(FORM);
-  // Coefficients and band values of plane q3 are loaded PD planes ahead into a
-  // ring of buffers; the loop is unrolled PD times so that buffer b is always
-  // addressed statically and no register with a load in flight is ever copied.
+  const int64_t step_stride = (int64_t)NCF * npts;
+  // Coefficients of plane q3 are loaded PD planes ahead into a ring of buffers;
+  // the loop is unrolled PD times so that buffer b is always addressed statically
+  // and no register with a load in flight is ever copied.
   T cbuf[PD][NCF];
   int fbuf[PD];
-  auto load_step = [&](int q3, T (&c)[NCF], int& fb) {
-    const T* p = cptr + (int64_t)q3 * NCF * npts;
+  auto load_step = [&](int q3, T (&c)[NCF], int& fb) __attribute__((always_inline)) {
+    const T* p = cptr + (int64_t)q3 * step_stride;
 #pragma unroll
-    for (int f = 0; f < NCF; ++f) c[f] = p[(int64_t)f * npts];
+    for (int f = 0; f < NCF; ++f) c[f] = p[f * npts];
     fb = first3[q3];
   };
 #pragma unroll
   for (int b = 0; b < PD; ++b)
     if (q3s + b < q3e) load_step(q3s + b, cbuf[b], fbuf[b]);
 
-  auto step = [&](int q3, T (&c)[NCF], int& fb) {
+  auto step = [&](int q3, T (&c)[NCF], int& fb) __attribute__((always_inline)) {
     const int f = fb;
     while (cur < f) {
-      flush_plane(cur);
+      // the oldest test plane leaves the window, the next trial plane enters it
+      const Trial t = plane_phase(true, cur, true, xnext);
 #pragma unroll
       for (int j = 0; j < P - 1; ++j) {
 #pragma unroll
@@ -349,9 +382,11 @@ fused_apply_kernel(FusedArgs<T> a) {
         }
       }
       te[0][P - 1] = te[1][P - 1] = te[2][P - 1] = T(0);
-      trial_plane(xnext, wnext, tr[0][P - 1], tr[1][P - 1], tr[2][P - 1]);
+      tr[0][P - 1] = t.v00;
+      tr[1][P - 1] = t.v10;
+      tr[2][P - 1] = t.v01;
       ++cur;
-      if (cur + P <= c3hi) load_x(cur + P, xnext, wnext);
+      if (cur + P <= c3hi) xnext = load_x(cur + P);
     }
     if (active) {
       const T (&tb)[4][P] = b3s[q3 - q3s];
@@ -382,7 +417,7 @@ fused_apply_kernel(FusedArgs<T> a) {
   }
 #pragma unroll
   for (int j = 0; j < P; ++j) {
-    flush_plane(cur);
+    plane_phase(true, cur, false, xnext);
 #pragma unroll
     for (int jj = 0; jj < P - 1; ++jj) {
 #pragma unroll
@@ -390,6 +425,20 @@ fused_apply_kernel(FusedArgs<T> a) {
     }
     ++cur;
   }
+}
+
+// Two launch variants of the same sweep: with an occupancy target of three waves
+// per SIMD (used when the band is narrow enough to fit without spilling) and
+// without (wider FP64 bands, two waves per SIMD).
+template <typename T, int P, int FORM>
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8)))
+fused_apply_kernel(FusedArgs<T> a) {
+  fused_apply_body<T, P, FORM>(a);
+}
+
+template <typename T, int P, int FORM>
+__global__ void __launch_bounds__(kThreads) fused_apply_kernel_wide(FusedArgs<T> a) {
+  fused_apply_body<T, P, FORM>(a);
 }
 
 // y[a*r + k] = alpha * wscale[a] * sum of the partial blocks covering DOF a (+ y)
@@ -599,14 +648,73 @@ static DirTiling tile_direction(const int32_t* first, int nq, int ncdof, int P, 
   return d;
 }
 
+// The three-wave variant is used unless the register budget it imposes makes the
+// compiler spill (private scratch), which costs more than the occupancy gains;
+// decided once per instance from the code object's attributes.
+template <typename T, int P, int FORM>
+static bool use_wide() {
+  static std::atomic<int> wide{-1};
+  int w = wide.load(std::memory_order_relaxed);
+  if (w < 0) {
+    hipFuncAttributes fa{};
+    w = (hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&fused_apply_kernel<T, P, FORM>)) == hipSuccess &&
+         fa.localSizeBytes > 0)
+            ? 1
+            : 0;
+    wide.store(w, std::memory_order_relaxed);
+  }
+  return w != 0;
+}
+
+template <typename T, int P, int FORM>
+static void launch_apply_pf(const FusedArgs<T>& a, int nblk, hipStream_t s) {
+  if (use_wide<T, P, FORM>())
+    fused_apply_kernel_wide<T, P, FORM><<<nblk, kThreads, 0, s>>>(a);
+  else
+    fused_apply_kernel<T, P, FORM><<<nblk, kThreads, 0, s>>>(a);
+}
+
+// resident work-groups per CU of the variant launch_apply_pf picks (0 if unknown)
+template <typename T, int P, int FORM>
+static int resident_blocks_pf() {
+  int nb = 0;
+  const void* fn = use_wide<T, P, FORM>() ? reinterpret_cast<const void*>(&fused_apply_kernel_wide<T, P, FORM>)
+                                          : reinterpret_cast<const void*>(&fused_apply_kernel<T, P, FORM>);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, kThreads, 0) != hipSuccess) return 0;
+  return nb;
+}
+
+template <typename T, int P>
+static int resident_blocks_p(int form) {
+  switch (form) {
+    case FORM_M1: return resident_blocks_pf<T, P, FORM_M1>();
+    case FORM_K6: return resident_blocks_pf<T, P, FORM_K6>();
+    case FORM_KM7: return resident_blocks_pf<T, P, FORM_KM7>();
+    case FORM_SYM10: return resident_blocks_pf<T, P, FORM_SYM10>();
+    default: return resident_blocks_pf<T, P, FORM_GEN16>();
+  }
+}
+
+template <typename T>
+static int resident_blocks(int P, int form) {
+  switch (P) {
+    case 3: return resident_blocks_p<T, 3>(form);
+    case 4: return resident_blocks_p<T, 4>(form);
+    case 5: return resident_blocks_p<T, 5>(form);
+    case 6: return resident_blocks_p<T, 6>(form);
+    case 7: return resident_blocks_p<T, 7>(form);
+    default: return resident_blocks_p<T, 8>(form);
+  }
+}
+
 template <typename T, int P>
 static void launch_apply_p(const iga_fused_s* f, const FusedArgs<T>& a, int nblk, hipStream_t s) {
   switch (f->form) {
-    case FORM_M1: fused_apply_kernel<T, P, FORM_M1><<<nblk, kThreads, 0, s>>>(a); break;
-    case FORM_K6: fused_apply_kernel<T, P, FORM_K6><<<nblk, kThreads, 0, s>>>(a); break;
-    case FORM_KM7: fused_apply_kernel<T, P, FORM_KM7><<<nblk, kThreads, 0, s>>>(a); break;
-    case FORM_SYM10: fused_apply_kernel<T, P, FORM_SYM10><<<nblk, kThreads, 0, s>>>(a); break;
-    default: fused_apply_kernel<T, P, FORM_GEN16><<<nblk, kThreads, 0, s>>>(a); break;
+    case FORM_M1: launch_apply_pf<T, P, FORM_M1>(a, nblk, s); break;
+    case FORM_K6: launch_apply_pf<T, P, FORM_K6>(a, nblk, s); break;
+    case FORM_KM7: launch_apply_pf<T, P, FORM_KM7>(a, nblk, s); break;
+    case FORM_SYM10: launch_apply_pf<T, P, FORM_SYM10>(a, nblk, s); break;
+    default: launch_apply_pf<T, P, FORM_GEN16>(a, nblk, s); break;
   }
 }
 
@@ -735,14 +843,35 @@ int iga_fused_create(iga_fused_t* out, const int32_t* nc, const int32_t* nq, int
       DirTiling d2 = tile_direction(first[1], nq[1], nc[1], P, kTQ2, kDMax);
       f->nt1 = (int)d1.c0.size();
       f->nt2 = (int)d2.c0.size();
-      // slabs of q3: enough work-groups to fill the device (>= ~2048 with one rhs)
+      // Slabs of q3 (one work-group per tile and slab with one right-hand side).  A
+      // work-group costs about (its q3 planes + 6P) plane steps, the 6P being the
+      // window's fill and drain; work-groups run in rounds of the device's resident
+      // slots.  The count minimising rounds x cost is taken (the slab table lives in
+      // LDS, which bounds a slab to kSlabMax planes).
+      const int ns_min = (nq[2] + kSlabMax - 1) / kSlabMax;
       int ns = nslab;
       if (ns <= 0) {
-        const int tiles = f->nt1 * f->nt2;
-        ns = std::max(1, std::min(nq[2] / (3 * P), (2048 + tiles - 1) / tiles));
+        const int64_t tiles = (int64_t)f->nt1 * f->nt2;
+        int dev = 0, ncu = 0;
+        const int per_cu = dtype == IGA_F64 ? resident_blocks<double>(P, form) : resident_blocks<float>(P, form);
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+          ncu = 0;
+        const int64_t slots = per_cu > 0 && ncu > 0 ? (int64_t)per_cu * ncu : 1024;
+        double best = 0.0;
+        // slabs of at least ~3P planes keep each DOF plane within a few slabs
+        const int ns_max = std::max(ns_min, std::min(nq[2] / (3 * P), 4 * ns_min + 64));
+        for (int c = ns_min; c <= ns_max; ++c) {
+          const double rounds = (double)((tiles * c + slots - 1) / slots);
+          const double cost = rounds * ((double)nq[2] / c + 6.0 * P);
+          if (ns <= 0 || cost < best) {
+            best = cost;
+            ns = c;
+          }
+        }
       }
       ns = std::max(1, std::min(ns, nq[2]));
-      ns = std::max(ns, (nq[2] + kSlabMax - 1) / kSlabMax);  // the slab table lives in LDS
+      ns = std::max(ns, ns_min);
       const int slab = (nq[2] + ns - 1) / ns;
       DirTiling d3 = tile_direction(first[2], nq[2], nc[2], P, slab, 0);
       f->nslab = (int)d3.c0.size();
