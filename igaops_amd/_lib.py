@@ -57,6 +57,11 @@ SIGNATURES = {
     "iga_fd_workspace_bytes": (_i64, [_vp, _i32]),
     "iga_fd_apply": (_i32, [_vp, _vp, _vp, _i32, _vp, _vp]),
     "iga_fd_destroy": (_i32, [_vp]),
+    "iga_stfd_create": (_i32, [C.POINTER(_vp), _i64p, _i64p, _i64p, C.POINTER(_f64p), _f64p, _f64p, _i32]),
+    "iga_stfd_set_blocks": (_i32, [_vp, _f64p, _f64p, _f64p, _f64p, _f64p, C.c_double]),
+    "iga_stfd_workspace_bytes": (_i64, [_vp, _i32]),
+    "iga_stfd_apply": (_i32, [_vp, _vp, _vp, _i32, _i32, _vp, _vp]),
+    "iga_stfd_destroy": (_i32, [_vp]),
 }
 
 _ERRORS = {-1: ValueError, -2: RuntimeError, -3: MemoryError, -4: NotImplementedError, -5: RuntimeError}

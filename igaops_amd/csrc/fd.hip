@@ -12,6 +12,7 @@
 // into the epilogue of the last forward product.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "common.hpp"
@@ -139,37 +140,51 @@ struct iga_fd_s {
 
 namespace iga {
 
+// Y(o, i, t) = sum_k A[i][k] X(o, k, t) with A of shape n_out x n_in (row-major).
 template <typename T>
-static void mode_gemm(const T* A, const T* X, T* Y, int64_t outer, int64_t n, int64_t inner, const T* scale,
-                      hipStream_t s) {
+static void mode_gemm_rect(const T* A, const T* X, T* Y, int64_t outer, int64_t n_out, int64_t n_in, int64_t inner,
+                           const T* scale, hipStream_t s) {
   GemmDesc g{};
   dim3 grid;
   if (inner > 1) {
-    g.M = (int)n;
+    g.M = (int)n_out;
     g.N = (int)inner;
-    g.K = (int)n;
-    g.lm = n; g.lk = 1;            // L = A
+    g.K = (int)n_in;
+    g.lm = n_in; g.lk = 1;         // L = A
     g.rk = inner; g.rj = 1;        // R = X(o)
     g.cm = inner; g.cj = 1;
-    g.l_batch = 0; g.r_batch = n * inner; g.c_batch = n * inner;
-    g.scale_m = inner; g.scale_j = 1; g.scale_batch = n * inner;
-    IGA_CHECK_ARG(outer < 65536 && inner < (int64_t)1 << 30, "mode product too large for the grid");
-    grid = dim3((unsigned)((n + kTile - 1) / kTile), (unsigned)((inner + kTile - 1) / kTile), (unsigned)outer);
+    g.l_batch = 0; g.r_batch = n_in * inner; g.c_batch = n_out * inner;
+    g.scale_m = inner; g.scale_j = 1; g.scale_batch = n_out * inner;
+    IGA_CHECK_ARG(inner < (int64_t)1 << 30, "mode product too large for the grid");
+    // batches beyond the grid's z limit go in further launches
+    constexpr int64_t kZMax = 65535;
+    for (int64_t o0 = 0; o0 < outer; o0 += kZMax) {
+      const int64_t nb = std::min(kZMax, outer - o0);
+      grid = dim3((unsigned)((n_out + kTile - 1) / kTile), (unsigned)((inner + kTile - 1) / kTile), (unsigned)nb);
+      mfma_gemm_kernel<T><<<grid, 256, 0, s>>>(g, A, X + o0 * g.r_batch, Y + o0 * g.c_batch,
+                                               scale ? scale + o0 * g.scale_batch : nullptr);
+      IGA_LAUNCH_CHECK();
+    }
+    return;
   } else {
     g.M = (int)outer;
-    g.N = (int)n;
-    g.K = (int)n;
-    g.lm = n; g.lk = 1;            // L = X rows
-    g.rk = 1; g.rj = n;            // R(k, i) = A[i][k]
-    g.cm = n; g.cj = 1;
-    g.scale_m = n; g.scale_j = 1;
+    g.N = (int)n_out;
+    g.K = (int)n_in;
+    g.lm = n_in; g.lk = 1;         // L = X rows
+    g.rk = 1; g.rj = n_in;         // R(k, i) = A[i][k]
+    g.cm = n_out; g.cj = 1;
+    g.scale_m = n_out; g.scale_j = 1;
     IGA_CHECK_ARG(outer < (int64_t)1 << 30, "mode product too large for the grid");
-    grid = dim3((unsigned)((outer + kTile - 1) / kTile), (unsigned)((n + kTile - 1) / kTile), 1);
+    grid = dim3((unsigned)((outer + kTile - 1) / kTile), (unsigned)((n_out + kTile - 1) / kTile), 1);
   }
-  const T* Lp = inner > 1 ? A : X;
-  const T* Rp = inner > 1 ? X : A;
-  mfma_gemm_kernel<T><<<grid, 256, 0, s>>>(g, Lp, Rp, Y, scale);
+  mfma_gemm_kernel<T><<<grid, 256, 0, s>>>(g, X, A, Y, scale);
   IGA_LAUNCH_CHECK();
+}
+
+template <typename T>
+static void mode_gemm(const T* A, const T* X, T* Y, int64_t outer, int64_t n, int64_t inner, const T* scale,
+                      hipStream_t s) {
+  mode_gemm_rect<T>(A, X, Y, outer, n, n, inner, scale, s);
 }
 
 template <typename T>
@@ -305,6 +320,284 @@ int iga_fd_destroy(iga_fd_t f) {
       IGA_HIP(hipFree(f->d_Ut[d]));
     }
     IGA_HIP(hipFree(f->d_inv));
+    delete f;
+  });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Space-time arrowhead FD (time slowest):
+//   y[free] = Re[(U_s (x) U_t) A^{-1} (U_s (x) U_t)^H x[free]],  y = 0 elsewhere,
+// where, per spatial eigenvalue lambda_j, A_j = [[diag(D_i + lambda_j), b], [-b^H, D_L + lambda_j]]
+// is the arrowhead of the time pencil (D complex, b complex).  The transposed apply
+// swaps U_t and conj(U_t) and uses b -> -conj(b).  The space transforms are real;
+// the time transform maps the real tensor to a complex one stored as a real block
+// followed by an imaginary block, so every product is a real MFMA GEMM:
+//   forward  [Y_re; Y_im] = F X,  F = [Re U_t^H; Im U_t^H]       (2L x L)
+//   backward Re(U_t Z)    = G [Z_re; Z_im],  G = [Re U_t, -Im U_t]  (L x 2L)
+// ---------------------------------------------------------------------------
+
+namespace iga {
+
+constexpr int kArrowMax = 512;  // time eigen-indices held in LDS
+
+// One thread per column j (spatial eigen-index j / nrhs).  z holds the complex
+// tensor as [2][nt][ncol] and is overwritten with the solution.
+template <typename T>
+__global__ void __launch_bounds__(256) arrowhead_kernel(int nt, int64_t ncol, int nrhs, const T* __restrict__ lam,
+                                                         const T* __restrict__ dre, const T* __restrict__ dim,
+                                                         const T* __restrict__ bre, const T* __restrict__ bim,
+                                                         T* __restrict__ z) {
+  __shared__ T sd[2][kArrowMax];
+  __shared__ T sb[2][kArrowMax];
+  for (int i = threadIdx.x; i < nt; i += blockDim.x) {
+    sd[0][i] = dre[i];
+    sd[1][i] = dim[i];
+    sb[0][i] = i < nt - 1 ? bre[i] : T(0);
+    sb[1][i] = i < nt - 1 ? bim[i] : T(0);
+  }
+  __syncthreads();
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ncol) return;
+  const T lj = lam[j / nrhs];
+  T* zr = z + j;
+  T* zi = z + (int64_t)nt * ncol + j;
+  const int L = nt - 1;
+  // Schur complement S = H_L + sum |b_i|^2 / H_i and  a = sum conj(b_i) r_i / H_i
+  T s_re = sd[0][L] + lj, s_im = sd[1][L];
+  T a_re = T(0), a_im = T(0);
+  for (int i = 0; i < L; ++i) {
+    const T hr = sd[0][i] + lj, hi = sd[1][i];
+    const T den = T(1) / (hr * hr + hi * hi);
+    const T ir = hr * den, ii = -hi * den;  // 1 / H_i
+    const T br = sb[0][i], bi = sb[1][i];
+    const T bb = br * br + bi * bi;
+    s_re += bb * ir;
+    s_im += bb * ii;
+    const T rr = zr[(int64_t)i * ncol], ri = zi[(int64_t)i * ncol];
+    const T yr = rr * ir - ri * ii, yi = rr * ii + ri * ir;
+    a_re += br * yr + bi * yi;
+    a_im += br * yi - bi * yr;
+  }
+  const T lr = zr[(int64_t)L * ncol] + a_re, li = zi[(int64_t)L * ncol] + a_im;
+  const T sden = T(1) / (s_re * s_re + s_im * s_im);
+  const T xr = (lr * s_re + li * s_im) * sden, xi = (li * s_re - lr * s_im) * sden;
+  zr[(int64_t)L * ncol] = xr;
+  zi[(int64_t)L * ncol] = xi;
+  // x_i = (r_i - b_i x_L) / H_i
+  for (int i = 0; i < L; ++i) {
+    const T hr = sd[0][i] + lj, hi = sd[1][i];
+    const T den = T(1) / (hr * hr + hi * hi);
+    const T br = sb[0][i], bi = sb[1][i];
+    const T tr = zr[(int64_t)i * ncol] - (br * xr - bi * xi);
+    const T ti = zi[(int64_t)i * ncol] - (br * xi + bi * xr);
+    zr[(int64_t)i * ncol] = (tr * hr + ti * hi) * den;
+    zi[(int64_t)i * ncol] = (ti * hr - tr * hi) * den;
+  }
+}
+
+}  // namespace iga
+
+struct iga_stfd_s {
+  int dtype = IGA_F64;
+  int64_t n[4] = {0, 0, 0, 0}, lo[4] = {0, 0, 0, 0}, len[4] = {0, 0, 0, 0};  // 3 space + time
+  int64_t nspace = 0, nsub = 0;
+  void* d_Ut[3] = {nullptr, nullptr, nullptr};
+  void* d_U[3] = {nullptr, nullptr, nullptr};
+  void* d_F[2] = {nullptr, nullptr};  // [apply, transpose]
+  void* d_G[2] = {nullptr, nullptr};
+  void* d_lam = nullptr;              // spatial eigenvalues over the space box
+  void* d_D[2] = {nullptr, nullptr};  // adv * D  (re, im), length L
+  void* d_b[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [apply/transpose][re/im], length L-1
+  bool blocks_set = false;
+  std::vector<void*> owned;
+};
+
+namespace iga {
+
+static void* st_upload(iga_stfd_s* f, const std::vector<double>& h) {
+  void* d = nullptr;
+  const size_t es = f->dtype == IGA_F64 ? 8 : 4;
+  IGA_HIP(hipMalloc(&d, es * std::max<size_t>(h.size(), 1)));
+  f->owned.push_back(d);
+  if (h.empty()) return d;
+  if (f->dtype == IGA_F64) {
+    IGA_HIP(hipMemcpy(d, h.data(), 8 * h.size(), hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> h32(h.begin(), h.end());
+    IGA_HIP(hipMemcpy(d, h32.data(), 4 * h.size(), hipMemcpyHostToDevice));
+  }
+  return d;
+}
+
+static void st_write(iga_stfd_s* f, void* d, const std::vector<double>& h) {
+  if (f->dtype == IGA_F64) {
+    IGA_HIP(hipMemcpy(d, h.data(), 8 * h.size(), hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> h32(h.begin(), h.end());
+    IGA_HIP(hipMemcpy(d, h32.data(), 4 * h.size(), hipMemcpyHostToDevice));
+  }
+}
+
+template <typename T>
+static void stfd_apply_t(const iga_stfd_s* f, const T* x, T* y, int nrhs, int tr, T* w0, T* w1, hipStream_t s) {
+  const int dt = std::is_same<T, double>::value ? IGA_F64 : IGA_F32;
+  int rc = iga_box_gather(dt, 4, f->n, f->lo, f->len, nrhs, x, w0, s);
+  if (rc != IGA_OK) throw Error(rc, "box gather failed");
+  const int64_t L = f->len[3];
+  const int64_t cols = f->nspace * nrhs;
+  T* cur = w0;
+  T* nxt = w1;
+  auto space_pass = [&](int d, bool transpose) {
+    int64_t outer = L, inner = nrhs;
+    for (int e = d + 1; e < 3; ++e) outer *= f->len[e];
+    for (int e = 0; e < d; ++e) inner *= f->len[e];
+    const T* A = (const T*)(transpose ? f->d_Ut[d] : f->d_U[d]);
+    mode_gemm<T>(A, cur, nxt, outer, f->len[d], inner, nullptr, s);
+    std::swap(cur, nxt);
+  };
+  for (int d = 0; d < 3; ++d) space_pass(d, true);
+  mode_gemm_rect<T>((const T*)f->d_F[tr], cur, nxt, 1, 2 * L, L, cols, nullptr, s);
+  std::swap(cur, nxt);
+  arrowhead_kernel<T><<<grid_for(cols, 256), 256, 0, s>>>((int)L, cols, nrhs, (const T*)f->d_lam,
+                                                          (const T*)f->d_D[0], (const T*)f->d_D[1],
+                                                          (const T*)f->d_b[tr][0], (const T*)f->d_b[tr][1], cur);
+  IGA_LAUNCH_CHECK();
+  mode_gemm_rect<T>((const T*)f->d_G[tr], cur, nxt, 1, L, 2 * L, cols, nullptr, s);
+  std::swap(cur, nxt);
+  for (int d = 2; d >= 0; --d) space_pass(d, false);
+  rc = iga_box_scatter(dt, 4, f->n, f->lo, f->len, nrhs, cur, y, 1, s);
+  if (rc != IGA_OK) throw Error(rc, "box scatter failed");
+}
+
+static void free_stfd(iga_stfd_s* f) {
+  for (int d = 0; d < 3; ++d) {
+    (void)hipFree(f->d_U[d]);
+    (void)hipFree(f->d_Ut[d]);
+  }
+  for (void* p : f->owned) (void)hipFree(p);
+  f->owned.clear();
+}
+
+}  // namespace iga
+
+extern "C" {
+
+int iga_stfd_create(iga_stfd_t* out, const int64_t* n, const int64_t* lo, const int64_t* len, const double* const* U,
+                    const double* ut_re, const double* ut_im, int dtype) {
+  return guard([&] {
+    IGA_CHECK_ARG(out && n && lo && len && U && ut_re && ut_im, "NULL argument");
+    IGA_CHECK_ARG(dtype == IGA_F64 || dtype == IGA_F32, "unknown dtype");
+    for (int d = 0; d < 4; ++d)
+      IGA_CHECK_ARG(len[d] >= 1 && lo[d] >= 0 && lo[d] + len[d] <= n[d], "free box does not fit");
+    IGA_CHECK_ARG(len[3] >= 2 && len[3] <= kArrowMax, "time block size must be in 2..512");
+    auto* f = new iga_stfd_s();
+    f->dtype = dtype;
+    try {
+      f->nspace = 1;
+      for (int d = 0; d < 4; ++d) {
+        f->n[d] = n[d];
+        f->lo[d] = lo[d];
+        f->len[d] = len[d];
+        if (d < 3) f->nspace *= len[d];
+      }
+      f->nsub = f->nspace * len[3];
+      const size_t es = dtype == IGA_F64 ? 8 : 4;
+      for (int d = 0; d < 3; ++d) {
+        const int64_t m = len[d];
+        std::vector<double> u((size_t)m * m), ut((size_t)m * m);
+        for (int64_t i = 0; i < m; ++i)
+          for (int64_t k = 0; k < m; ++k) {
+            u[i * m + k] = U[d][i * m + k];
+            ut[i * m + k] = U[d][k * m + i];
+          }
+        IGA_HIP(hipMalloc(&f->d_U[d], es * m * m));
+        IGA_HIP(hipMalloc(&f->d_Ut[d], es * m * m));
+        st_write(f, f->d_U[d], u);
+        st_write(f, f->d_Ut[d], ut);
+      }
+      const int64_t L = len[3];
+      for (int tr = 0; tr < 2; ++tr) {
+        // apply: U_t^H forward, U_t backward; transpose: U_t^T forward, conj(U_t) backward
+        const double sg = tr == 0 ? -1.0 : 1.0;
+        std::vector<double> F((size_t)2 * L * L), G((size_t)L * 2 * L);
+        for (int64_t i = 0; i < L; ++i)
+          for (int64_t k = 0; k < L; ++k) {
+            F[i * L + k] = ut_re[k * L + i];
+            F[(L + i) * L + k] = sg * ut_im[k * L + i];
+            G[i * 2 * L + k] = ut_re[i * L + k];
+            G[i * 2 * L + L + k] = sg * ut_im[i * L + k];
+          }
+        f->d_F[tr] = st_upload(f, F);
+        f->d_G[tr] = st_upload(f, G);
+      }
+      f->d_lam = st_upload(f, std::vector<double>((size_t)f->nspace, 0.0));
+      f->d_D[0] = st_upload(f, std::vector<double>((size_t)L, 0.0));
+      f->d_D[1] = st_upload(f, std::vector<double>((size_t)L, 0.0));
+      for (int tr = 0; tr < 2; ++tr)
+        for (int c = 0; c < 2; ++c) f->d_b[tr][c] = st_upload(f, std::vector<double>((size_t)L, 0.0));
+    } catch (...) {
+      free_stfd(f);
+      delete f;
+      throw;
+    }
+    *out = f;
+  });
+}
+
+int iga_stfd_set_blocks(iga_stfd_t f, const double* lam_space, const double* d_re, const double* d_im,
+                        const double* g_re, const double* g_im, double adv) {
+  return guard([&] {
+    IGA_CHECK_ARG(f && lam_space && d_re && d_im && g_re && g_im, "NULL argument");
+    const int64_t L = f->len[3];
+    st_write(f, f->d_lam, std::vector<double>(lam_space, lam_space + f->nspace));
+    std::vector<double> dr(L), di(L), b[2][2];
+    for (int64_t i = 0; i < L; ++i) {
+      dr[i] = adv * d_re[i];
+      di[i] = adv * d_im[i];
+    }
+    for (int tr = 0; tr < 2; ++tr)
+      for (int c = 0; c < 2; ++c) b[tr][c].assign((size_t)L, 0.0);
+    for (int64_t i = 0; i + 1 < L; ++i) {
+      b[0][0][i] = adv * g_re[i];   // apply: b = adv g
+      b[0][1][i] = adv * g_im[i];
+      b[1][0][i] = -adv * g_re[i];  // transpose: b = -adv conj(g)
+      b[1][1][i] = adv * g_im[i];
+    }
+    st_write(f, f->d_D[0], dr);
+    st_write(f, f->d_D[1], di);
+    for (int tr = 0; tr < 2; ++tr)
+      for (int c = 0; c < 2; ++c) st_write(f, f->d_b[tr][c], b[tr][c]);
+    f->blocks_set = true;
+  });
+}
+
+int64_t iga_stfd_workspace_bytes(iga_stfd_t f, int nrhs) {
+  if (!f || nrhs < 1) return 0;
+  return 2 * (2 * f->nsub * (int64_t)nrhs) * (f->dtype == IGA_F64 ? 8 : 4) + 512;
+}
+
+int iga_stfd_apply(iga_stfd_t f, const void* x, void* y, int nrhs, int transpose, void* work, void* stream) {
+  return guard([&] {
+    IGA_CHECK_ARG(f && x && y && work && nrhs >= 1, "NULL argument");
+    IGA_CHECK_ARG(f->blocks_set, "time blocks and eigenvalues are not set");
+    const int64_t half = 2 * f->nsub * nrhs;
+    const int tr = transpose ? 1 : 0;
+    if (f->dtype == IGA_F64) {
+      double* w0 = (double*)work;
+      stfd_apply_t<double>(f, (const double*)x, (double*)y, nrhs, tr, w0, w0 + half, as_stream(stream));
+    } else {
+      float* w0 = (float*)work;
+      stfd_apply_t<float>(f, (const float*)x, (float*)y, nrhs, tr, w0, w0 + half, as_stream(stream));
+    }
+  });
+}
+
+int iga_stfd_destroy(iga_stfd_t f) {
+  return guard([&] {
+    if (!f) return;
+    free_stfd(f);
     delete f;
   });
 }

@@ -36,7 +36,6 @@ constexpr int kTQ1 = 16, kTQ2 = 16, kThreads = kTQ1 * kTQ2;
 // Max DOF footprint of a 16-point tile in directions 1, 2: 16 points span at most 9
 // elements for rules with >= 2 points per element, so p + 10 <= 17 DOFs for p <= 7.
 constexpr int kDMax = 20;
-constexpr int kPMax = 9;   // max band width p+1
 constexpr int kKMax = 4;   // max tiles/slabs whose footprint contains one DOF index
 constexpr int kSlabMax = 64;  // max q3 planes of one slab (direction-3 table staged in LDS)
 
@@ -113,7 +112,6 @@ template <typename T, int P, int FORM>
 __device__ __attribute__((always_inline)) inline void fused_apply_body(const FusedArgs<T>& a) {
   // prefetch depth (q3 planes) of the coefficient stream
   constexpr int PD = sizeof(T) == 8 ? 2 : 3;
-  constexpr int NC = form_ncoef(FORM);
   __shared__ T xs[kDMax][kDMax + 1];
   __shared__ T t1s[2][kDMax][kTQ1];
   __shared__ T s2s[3][kTQ2][kTQ1];

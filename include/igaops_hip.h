@@ -205,6 +205,31 @@ int64_t iga_fd_workspace_bytes(iga_fd_t plan, int nrhs);
 int iga_fd_apply(iga_fd_t plan, const void* x, void* y, int nrhs, void* work, void* stream);
 int iga_fd_destroy(iga_fd_t plan);
 
+/* ---- space-time arrowhead fast-diagonalization apply ----------------------------
+ * Replaces SpTimeFD.apply / apply_transpose (fastdiag/utils/time_arrow.py:111-192)
+ * for one DOF per node, time the slowest direction:
+ *   y[free] = Re[(U_s (x) U_t) A^{-1} (U_s (x) conj(U_t))^T x[free]],  y = 0 elsewhere,
+ * A^{-1} being the arrowhead solve of solve_special_arrowhead
+ * (fastdiag/utils/operations.py:35-93) with H = adv D (+) Lambda_s and b = adv g
+ * (time_arrow.py:82-97); the transposed apply swaps U_t and conj(U_t) and uses
+ * b = -adv conj(g).  n/lo/len describe the 4-D free box (three space directions, then
+ * time, whose first DOF is constrained: lo[3] = 1, time_template.py:52-56).  U[d]
+ * are the real space eigenvectors (row-major len[d] x len[d]); ut_re/ut_im the time
+ * eigenvectors U_t of compute_time_schurdecomposition (time_arrow.py:44-80),
+ * row-major len[3] x len[3]. */
+typedef struct iga_stfd_s* iga_stfd_t;
+
+int iga_stfd_create(iga_stfd_t* out, const int64_t* n, const int64_t* lo, const int64_t* len,
+                    const double* const* U, const double* ut_re, const double* ut_im, int dtype);
+/* lam_space: the spatial Lambda over the space box (direction 1 fastest); d_re/d_im:
+ * the diagonal of D_t (len[3]); g_re/g_im: its border g (len[3] - 1); adv: the
+ * advection corrector (time_arrow.py:99-109). */
+int iga_stfd_set_blocks(iga_stfd_t plan, const double* lam_space, const double* d_re, const double* d_im,
+                        const double* g_re, const double* g_im, double adv);
+int64_t iga_stfd_workspace_bytes(iga_stfd_t plan, int nrhs);
+int iga_stfd_apply(iga_stfd_t plan, const void* x, void* y, int nrhs, int transpose, void* work, void* stream);
+int iga_stfd_destroy(iga_stfd_t plan);
+
 #ifdef __cplusplus
 }
 #endif

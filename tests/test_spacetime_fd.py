@@ -1,0 +1,125 @@
+"""Space-time arrowhead FD: host factors (CPU) and device apply (GPU) against the oracle."""
+import numpy as np
+import pytest
+
+from igaops_amd.quadrature import StandardGauss, WeightedQuadrature, make_open_knotvector
+from oracle import fastdiag as ofd
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def rules(p_s=2, nel_s=4, p_t=2, nel_t=5, wq=True):
+    mk = (lambda p, n: WeightedQuadrature(p, make_open_knotvector(p, n), quadtype="2")) if wq else (
+        lambda p, n: StandardGauss(p, make_open_knotvector(p, n), "legendre"))
+    return [mk(p_s, nel_s) for _ in range(3)], mk(p_t, nel_t)
+
+
+def gauss_of(q):
+    return StandardGauss(q.degree, q.knotvector, "legendre")
+
+
+def test_time_factors_match_oracle_and_invariants():
+    from igaops_amd.spacetime import arrowhead_factors, time_pencil
+
+    _, qt = rules()
+    Wt, Mt, nt = time_pencil(qt)
+    Wo, Mo = ofd.time_pencil(gauss_of(qt))
+    np.testing.assert_allclose(Wt, Wo, rtol=0, atol=1e-14)
+    np.testing.assert_allclose(Mt, Mo, rtol=0, atol=1e-14)
+    Ut, D, g = arrowhead_factors(Wt, Mt)
+    Uo, Do, go = ofd.arrowhead_decomposition(Wo, Mo)
+    np.testing.assert_allclose(D, Do, rtol=1e-12)
+    np.testing.assert_allclose(Ut, Uo, rtol=1e-10, atol=1e-12)
+    np.testing.assert_allclose(g, go, rtol=1e-10, atol=1e-12)
+    # U^H M U = I and U^H W U = [[diag, g], [-g^H, sigma]]
+    np.testing.assert_allclose(Ut.conj().T @ Mt @ Ut, np.eye(nt - 1), atol=1e-10)
+    Dt = np.diag(D).astype(complex)
+    Dt[:-1, -1] = g
+    Dt[-1, :-1] = -g.conj()
+    np.testing.assert_allclose(Ut.conj().T @ Wt @ Ut, Dt, atol=1e-9)
+
+
+def test_free_nodes_match_oracle():
+    from igaops_amd.fastdiag import SingleFD
+
+    qs, qt = rules()
+    tab = np.ones((1, 3, 2), bool)
+    fd = SingleFD("spacetime")
+    fd.compute_space_eigendecomposition(qs, tab)
+    fd.compute_time_schurdecomposition(qt)
+    ost = ofd.SpaceTimeFD(ofd.SpaceFD([gauss_of(q) for q in qs], tab), gauss_of(qt))
+    np.testing.assert_array_equal(fd.sptm_preconditioner.sptm_free_nodes[0], ost.free_nodes())
+    n, lo, ln = fd.sptm_preconditioner.box()
+    assert list(lo) == [1, 1, 1, 1] and int(np.prod(ln)) == ost.free_nodes().size
+
+
+def _pair(tab, adv=1.0, coefs=(0.0, 1.0)):
+    from igaops_amd.fastdiag import SingleFD
+
+    qs, qt = rules()
+    fd = SingleFD("spacetime")
+    fd.compute_space_eigendecomposition(qs, tab)
+    fd.compute_time_schurdecomposition(qt)
+    fd.update_space_eigenvalues(list(coefs))
+    fd.add_scalar_space_time_correctors(advection_corrector=[adv])
+    sfd = ofd.SpaceFD([gauss_of(q) for q in qs], tab)
+    sfd.scalar_coefs = coefs
+    ost = ofd.SpaceTimeFD(sfd, gauss_of(qt))
+    ost.adv_corr = adv
+    n = int(np.prod([q.nbctrlpts for q in qs])) * qt.nbctrlpts
+    return fd, ost, n
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("table", ["all", "mixed"])
+@pytest.mark.parametrize("adv", [1.0, 0.7])
+def test_spacetime_apply_matches_oracle(table, adv):
+    tab = np.ones((1, 3, 2), bool)
+    if table == "mixed":
+        tab[0, 1, 1] = False
+        tab[0, 2, 0] = False
+    fd, ost, n = _pair(tab, adv, (0.2, 1.0))
+    print("case", table, adv, n, flush=True)
+    x = np.random.default_rng(3).standard_normal(n)
+    assert rel(fd.apply(x), ost.apply(x)) <= 1e-12
+    assert rel(fd.apply_transpose(x), ost.apply_transpose(x)) <= 1e-12
+
+
+@pytest.mark.gpu
+def test_spacetime_apply_multi_rhs_and_f32():
+    import torch
+
+    fd, ost, n = _pair(np.ones((1, 3, 2), bool))
+    X = np.random.default_rng(4).standard_normal((n, 3))
+    ref = np.stack([ost.apply(X[:, k]) for k in range(3)], axis=1)
+    assert rel(fd.apply(X), ref) <= 1e-12
+    x32 = torch.as_tensor(X[:, 0], device="cuda").float()
+    y32 = fd.apply(x32).double().cpu().numpy()
+    assert rel(y32, ref[:, 0]) <= 1e-4
+
+
+@pytest.mark.gpu
+def test_spacetime_fd_inverts_separable_operator_on_device():
+    """Gauss rules: the device apply is the exact inverse of W_t (x) M_s + M_t (x) K_s
+    on the free nodes (the property pinning the oracle)."""
+    from igaops_amd.fastdiag import SingleFD
+
+    qs = StandardGauss(2, make_open_knotvector(2, 3), "legendre")
+    qt = StandardGauss(2, make_open_knotvector(2, 4), "legendre")
+    fd = SingleFD("spacetime")
+    fd.compute_space_eigendecomposition([qs] * 3, np.ones((1, 3, 2), bool))
+    fd.compute_time_schurdecomposition(qt)
+    fd.update_space_eigenvalues([0.0, 1.0])
+    M = (qs.weights[0] @ qs.basis[0]).toarray()
+    K = (qs.weights[-1] @ qs.basis[-1]).toarray()
+    Mt = (qt.weights[0] @ qt.basis[0]).toarray()
+    Wt = (qt.weights[1] @ qt.basis[1]).toarray()
+    kron3 = lambda a, b, c: np.kron(c, np.kron(b, a))
+    A = np.kron(Wt, kron3(M, M, M)) + np.kron(Mt, kron3(K, M, M) + kron3(M, K, M) + kron3(M, M, K))
+    free = fd.sptm_preconditioner.sptm_free_nodes[0]
+    v = np.random.default_rng(5).standard_normal(A.shape[0])
+    ref = np.zeros_like(v)
+    ref[free] = np.linalg.solve(A[np.ix_(free, free)], v[free])
+    assert rel(fd.apply(v), ref) <= 1e-10
