@@ -145,6 +145,72 @@ def solve_section(patch, C, m, nrhs, tol=1e-10):
 PMC_COPY_BYTES = 512 * 1024 * 1024
 
 
+def spacetime_section(steps=5, nrhs_seq=10, tol=1e-10, space=(3, 32), time_=(2, 64)):
+    """Space-time heat equation: 3-D unit cube (degree 3, 32 elements/dir) x time
+    (degree 2, 64 elements), WQ type "2" rules, unit diffusivity,
+    ``A = u_v(time_ders=(0,1)) + gradu_gradv(time_ders=(0,0))``, Dirichlet on the
+    space boundary and the initial time; arrowhead space-time FD; GCRODR (m=40, k=20)
+    over 10 right-hand sides ``b_{i+1} = b_i + 0.05 |b_i| / sqrt(N) xi_i``."""
+    import torch
+
+    from igaops_amd.fastdiag import SingleFD
+    from igaops_amd.gcrodr import GCRODR
+    from igaops_amd.quadrature import WeightedQuadrature, make_open_knotvector
+    from igaops_amd.spacetime import BoxMasked, SpaceTimeOperator
+
+    t0 = time.perf_counter()
+    qs = [WeightedQuadrature(space[0], make_open_knotvector(*space), quadtype="2") for _ in range(3)]
+    qt = WeightedQuadrature(time_[0], make_open_knotvector(*time_), quadtype="2")
+    ql = qs + [qt]
+    fd = SingleFD("spacetime")
+    fd.compute_space_eigendecomposition(qs, np.ones((1, 3, 2), bool))
+    fd.compute_time_schurdecomposition(qt)
+    fd.update_space_eigenvalues([0.0, 1.0])
+    op = SpaceTimeOperator(ql, 1.0, np.eye(3))
+    A = BoxMasked(op, fd.sptm_preconditioner.box())
+    n = op.ndof
+    rng = np.random.default_rng(5)
+    x = torch.as_tensor(rng.standard_normal(n), device="cuda")
+    y = torch.empty_like(x)
+    setup_s = time.perf_counter() - t0
+    mv = float(np.mean(time_events(lambda: op.apply_device(x, out=y), steps, 1)))
+    pc = float(np.mean(time_events(lambda: fd.sptm_preconditioner.apply_device(x, out=y), steps, 1)))
+    b = torch.as_tensor(rng.standard_normal(n), device="cuda")
+    A._mask(b, 1)
+    g = GCRODR(tol, 2000, krylov_size=40, recycled_size=20)
+    lat, its, true_res = [], [], []
+    for i in range(nrhs_seq):
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        out = g.solve_device(A, b, Pfun=fd.sptm_preconditioner)
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - t1)
+        its.append(out.iterations)
+        r = b - A.apply_device(out.solution)
+        true_res.append(float(torch.linalg.vector_norm(r) / torch.linalg.vector_norm(b)))
+        xi = torch.as_tensor(rng.standard_normal(n), device="cuda")
+        A._mask(xi, 1)
+        b = b + 0.05 * torch.linalg.vector_norm(b) / n ** 0.5 * xi
+    nq = op.nq
+    return {
+        "workload": f"space-time heat, space p={space[0]} nel={space[1]}, time p={time_[0]} nel={time_[1]}, "
+                    f"WQ type 2, unit cube, unit diffusivity",
+        "ndof": n,
+        "nq": nq,
+        "matvec_ms": mv,
+        "matvec_gdofs": n / mv / 1e6,
+        "matvec_resident_fields": len(op.terms),
+        "matvec_pct_hbm_roofline": op.algorithmic_bytes(1) / (mv * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "fd_apply_ms": pc,
+        "gcrodr_iterations": its,
+        "solve_s": lat,
+        "solves_per_s": len(lat) / sum(lat),
+        "preconditioned_tol": tol,
+        "max_true_rel_residual": max(true_res),
+        "setup_s": setup_s,
+    }
+
+
 def pmc_child(args):
     """Body of the counter runs: three launches of the sweep kernel on one
     right-hand side, then three 512 MiB device copies (the counters' scale)."""
@@ -223,14 +289,18 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-traffic", action="store_true")
+    ap.add_argument("--spacetime", action="store_true", help="only the space-time GCRODR section")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
-    if args.pmc_child:
-        pmc_child(args)
-        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)
         sys.exit(2)
+    if args.pmc_child:
+        pmc_child(args)
+        return
+    if args.spacetime:
+        print(json.dumps({"spacetime": spacetime_section()}))
+        return
 
     import torch
 

@@ -182,3 +182,84 @@ class SpaceTimeFD:
 
     def apply_transpose_device(self, x, out=None, nrhs: int = 1, stream=None):
         return self.apply_device(x, out, nrhs, stream, transpose=True)
+
+
+class SpaceTimeOperator:
+    """The space-time heat operator ``A x = u_v(c; time_ders=(0, 1)) x +
+    gradu_gradv(C; time_ders=(0, 0)) x`` (``examples/performance_matvec_sptm.py:44-57``;
+    the terms are ``bsplines_ops.py:290-355`` with the time direction last) on the
+    device, coefficients resident in HBM.
+
+    ``c`` is an ``(N_q,)`` array or a scalar; ``C`` a ``(d, d, N_q)`` array or a
+    ``(d, d)`` matrix of constants (``d`` space directions).  Coefficient fields that
+    are identically zero are left out of the sum (they contribute nothing)."""
+
+    def __init__(self, quadrule_list, c=1.0, C=None, dtype: str = "f64"):
+        from .operators import Term, terms_gradu_gradv, terms_u_v
+
+        t = dv.torch()
+        self.quadrule_list = list(quadrule_list)
+        nd = len(self.quadrule_list)
+        sd = nd - 1
+        self.nq = int(np.prod([q.nbquadpts for q in self.quadrule_list], dtype=np.int64))
+        self.ndof = int(np.prod([q.nbctrlpts for q in self.quadrule_list], dtype=np.int64))
+        self.dtype = dtype
+        tdt = t.float64 if dtype == "f64" else t.float32
+        if C is None:
+            C = np.eye(sd)
+
+        def field(v):
+            v = np.asarray(v, dtype=np.float64)
+            if v.ndim == 0:
+                if float(v) == 0.0:
+                    return None
+                return t.full((self.nq,), float(v), dtype=tdt, device=dv.device())
+            if v.shape != (self.nq,):
+                raise ValueError("coefficient field does not match the quadrature points")
+            if not np.any(v != 0.0):
+                return None
+            return dv.to_device(v, dtype)
+
+        C = np.asarray(C, dtype=np.float64)
+        if C.shape[:2] != (sd, sd):
+            raise ValueError("C must be (d, d) or (d, d, N_q) over the space directions")
+        cfields = [[field(C[i, j]) for j in range(sd)] for i in range(sd)]
+        terms = [tm for tm in terms_u_v(self.quadrule_list, field(c), (0, 1)) if tm.coef is not None]
+        terms += [tm for tm in terms_gradu_gradv(self.quadrule_list, cfields, (0, 0)) if tm.coef is not None]
+        self.terms: List[Term] = terms
+
+    def apply_device(self, x, out=None, nrhs: int = 1, stream=None):
+        from .operators import composed_apply
+
+        return composed_apply(self.quadrule_list, self.terms, x, nrhs, out=out, stream=stream)
+
+    apply = apply_device
+
+    def algorithmic_bytes(self, nrhs: int = 1) -> int:
+        """``b (2 N_dof r + c N_q)`` with ``c`` = the resident coefficient fields."""
+        b = 8 if self.dtype == "f64" else 4
+        return b * (2 * self.ndof * nrhs + len(self.terms) * self.nq)
+
+
+class BoxMasked:
+    """``out = mask(A mask(x))`` for a device operator, the free set a box (the
+    cleandod wrappers of ``solvers/linear_solver.py:119-124``); the input is copied,
+    not masked in place."""
+
+    def __init__(self, op, box):
+        self.op = op
+        self.box = box
+
+    def _mask(self, v, nrhs, stream=None):
+        n, lo, ln = self.box
+        na, npt = _lib.i64_array(n)
+        la, lpt = _lib.i64_array(lo)
+        lna, lnpt = _lib.i64_array(ln)
+        _lib.call("iga_box_mask", dv.iga_dtype(v), len(n), npt, lpt, lnpt, int(nrhs), dv.ptr(v), dv.stream_ptr(stream))
+
+    def apply_device(self, x, out=None, nrhs: int = 1, stream=None):
+        xm = x.clone()
+        self._mask(xm, nrhs, stream)
+        out = self.op.apply_device(xm, out=out, nrhs=nrhs, stream=stream)
+        self._mask(out, nrhs, stream)
+        return out
