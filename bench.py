@@ -92,7 +92,7 @@ def solve_section(patch, C, m, nrhs, tol=1e-10):
 
     from igaops_amd.fastdiag import SingleFD
     from igaops_amd.operators import StiffnessMassOperator
-    from igaops_amd.solvers import Mask, PrecisionAdapter, gmres_device, refinement_solve
+    from igaops_amd.solvers import Mask, gmres_device, mixed_solve
 
     qs = patch.quadrule_list
     w = patch.nurbs_weights if patch.is_nurbs else None
@@ -103,8 +103,6 @@ def solve_section(patch, C, m, nrhs, tol=1e-10):
     mask = Mask(box=fd.space_preconditioner.box())
     A64 = StiffnessMassOperator(qs, C=C, m=m, nurbs_weights=w, dtype="f64")
     A32 = StiffnessMassOperator(qs, C=C, m=m, nurbs_weights=w, dtype="f32")
-    P32 = PrecisionAdapter(fd, n, "f32")
-    A32a = PrecisionAdapter(A32, n, "f32")
     rng = np.random.default_rng(2)
     B = [torch.as_tensor(rng.standard_normal(n), device="cuda") for _ in range(nrhs)]
     for b in B:
@@ -119,9 +117,8 @@ def solve_section(patch, C, m, nrhs, tol=1e-10):
                 out = gmres_device(lambda x, o: A64.apply(x, out=o), b, lambda x, o: fd.apply_device(x, out=o),
                                    tol, 400, mask=mask, restart=None, use_increment=False)
             else:
-                out = refinement_solve(lambda x, o: A64.apply(x, out=o), lambda x, o: A32a.apply_device(x, out=o),
-                                       lambda x, o: P32.apply_device(x, out=o), b, tol=tol, inner_tol=1e-5,
-                                       mask=mask)
+                out = mixed_solve(lambda x, o: A64.apply(x, out=o), lambda x, o: A32.apply(x, out=o),
+                                  lambda x, o: fd.apply_device(x, out=o), b, tol=tol, inner_tol=1e-5, mask=mask)
             torch.cuda.synchronize()
             lat.append(time.perf_counter() - t0)
             its.append(out.iterations)

@@ -57,6 +57,9 @@ SIGNATURES = {
     "iga_fd_workspace_bytes": (_i64, [_vp, _i32]),
     "iga_fd_apply": (_i32, [_vp, _vp, _vp, _i32, _vp, _vp]),
     "iga_fd_destroy": (_i32, [_vp]),
+    "iga_gmres_state_doubles": (_i64, [_i32]),
+    "iga_gmres_start": (_i32, [_i32, _i64, _i32, _vp, _vp, _vp, _vp, _vp]),
+    "iga_gmres_step": (_i32, [_i32, _i64, _i32, _i32, _vp, _i64, _vp, _vp, _vp, _vp]),
     "iga_stfd_create": (_i32, [C.POINTER(_vp), _i64p, _i64p, _i64p, C.POINTER(_f64p), _f64p, _f64p, _i32]),
     "iga_stfd_set_blocks": (_i32, [_vp, _f64p, _f64p, _f64p, _f64p, _f64p, C.c_double]),
     "iga_stfd_workspace_bytes": (_i64, [_vp, _i32]),

@@ -35,6 +35,17 @@ struct Error : std::runtime_error {
 
 #define IGA_LAUNCH_CHECK() IGA_HIP(hipGetLastError())
 
+// A nested entry point's failure propagates with its own message.
+#define IGA_CHECK(call)                                                     \
+  do {                                                                      \
+    const int rc_ = (call);                                                 \
+    if (rc_ != IGA_OK) {                                                    \
+      char buf_[512];                                                       \
+      iga_last_error(buf_, (int)sizeof(buf_));                              \
+      throw ::iga::Error(rc_, std::string(#call) + ": " + buf_);            \
+    }                                                                       \
+  } while (0)
+
 // Every extern "C" entry point runs its body through this guard, so no C++
 // exception crosses the ABI; the return value is IGA_OK or a negative code.
 template <typename F>

@@ -39,23 +39,50 @@ static int reduce_blocks(int64_t n) {
   return (int)g;
 }
 
-// partial[b * k + j] = sum over block b's elements of V[j][i] * w[i]
+// Many-dot products V w: blocks tile (element range) x (group of kDotVec vectors);
+// every thread keeps one partial sum per vector of its group over its elements, the
+// 64 lanes of a wave are combined by shuffles, and each wave writes its partials
+// (deterministic: fixed ranges, fixed combination order).
+constexpr int kDotVec = 16;
+constexpr int kDotChunk = 1024;  // elements per block when n is small enough
+constexpr int kDotMaxChunks = 4096;
+
+static int dot_chunks(int64_t n) {
+  int64_t g = (n + kDotChunk - 1) / kDotChunk;
+  if (g < 1) g = 1;
+  if (g > kDotMaxChunks) g = kDotMaxChunks;
+  return (int)g;
+}
+
+// partial[(b * 4 + wave) * k + j] for chunk b
 template <typename T>
 __global__ void __launch_bounds__(kRedThreads) dot_many_stage1(int64_t n, int k, const T* __restrict__ V,
                                                                int64_t ldv, const T* __restrict__ w,
                                                                double* __restrict__ partial) {
-  __shared__ double lds[kRedThreads / 64];
   const int nb = gridDim.x;
-  // contiguous, equal-size chunks per block: deterministic for given (n, grid)
-  const int64_t per = (n + nb - 1) / nb;
+  const int64_t per = ((n + nb - 1) / nb + kRedThreads - 1) / kRedThreads * kRedThreads;
   const int64_t lo = (int64_t)blockIdx.x * per;
   const int64_t hi = lo + per < n ? lo + per : n;
-  for (int j = 0; j < k; ++j) {
-    const T* v = V + (int64_t)j * ldv;
-    double acc = 0.0;
-    for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) acc += (double)v[i] * (double)w[i];
-    const double s = block_sum(acc, lds);
-    if (threadIdx.x == 0) partial[(int64_t)blockIdx.x * k + j] = s;
+  const int j0 = blockIdx.y * kDotVec;
+  const int kb = k - j0 < kDotVec ? k - j0 : kDotVec;
+  double acc[kDotVec];
+#pragma unroll
+  for (int jj = 0; jj < kDotVec; ++jj) acc[jj] = 0.0;
+  const T* Vb = V + (int64_t)j0 * ldv;
+  for (int64_t i = lo + threadIdx.x; i < hi; i += kRedThreads) {
+    const double wv = (double)w[i];
+#pragma unroll
+    for (int jj = 0; jj < kDotVec; ++jj)
+      if (jj < kb) acc[jj] += (double)Vb[(int64_t)jj * ldv + i] * wv;
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double* out = partial + ((int64_t)blockIdx.x * (kRedThreads / 64) + wave) * k + j0;
+#pragma unroll
+  for (int jj = 0; jj < kDotVec; ++jj) {
+    if (jj < kb) {
+      const double v = wave_sum(acc[jj]);
+      if (lane == 0) out[jj] = v;
+    }
   }
 }
 
@@ -154,6 +181,88 @@ __global__ void cast_kernel(int64_t n, const S* __restrict__ x, D* __restrict__ 
     y[i] = (D)x[i];
 }
 
+// ---- device-side GMRES step -------------------------------------------------
+// Per-solve state (float64) for Krylov dimension m, at fixed offsets:
+//   [0, m)       first Gram-Schmidt pass, [m, 2m) second pass,
+//   2m           ||w||^2, 2m+1 the scale of the new basis vector (1/||w||, or 1/beta),
+//   R            m x m, column-major, the rotated Hessenberg columns,
+//   cs, sn       the Givens rotations, g (m+1) the rotated right-hand side beta e_1,
+//   hist         2 (m+1): (g[k], ||w_k||) per step, the values a host polls.
+constexpr int kGmMaxDim = 2048;  // the rotation kernel stages 3 k doubles in LDS
+
+struct GmState {
+  int m;
+  __host__ __device__ int64_t R() const { return 2 * (int64_t)m + 2; }
+  __host__ __device__ int64_t cs() const { return R() + (int64_t)m * m; }
+  __host__ __device__ int64_t sn() const { return cs() + m; }
+  __host__ __device__ int64_t g() const { return sn() + m; }
+  __host__ __device__ int64_t hist() const { return g() + m + 1; }
+  __host__ __device__ int64_t size() const { return hist() + 2 * ((int64_t)m + 1); }
+};
+
+__global__ void gm_zero_kernel(int64_t n, double* __restrict__ p) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0.0;
+}
+
+// beta = ||r|| from ls[2m]; g[0] = beta, scale = 1/beta (0 for r = 0)
+__global__ void gm_start_kernel(GmState st, double* ls) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double beta = sqrt(fmax(ls[2 * st.m], 0.0));
+  ls[st.g()] = beta;
+  ls[2 * st.m + 1] = beta > 0.0 ? 1.0 / beta : 0.0;
+  ls[st.hist()] = beta;
+  ls[st.hist() + 1] = beta;
+}
+
+// Step k (1-based): Hessenberg column h = pass1 + pass2, h[k] = ||w||, rotated by the
+// k-1 previous rotations; a new rotation annihilates h[k] and updates g.  The column
+// and the rotations are staged through LDS by one wave, the sequential rotation
+// sweep runs on one lane.
+__global__ void __launch_bounds__(64) gm_rotate_kernel(GmState st, int k, double* ls) {
+  extern __shared__ double sh[];  // h[k], c[k], s[k]
+  double* h = sh;
+  double* c = sh + k;
+  double* sv = sh + 2 * k;
+  const int m = st.m;
+  double* Rk = ls + st.R() + (int64_t)(k - 1) * m;
+  double* cs = ls + st.cs();
+  double* sn = ls + st.sn();
+  double* g = ls + st.g();
+  for (int i = threadIdx.x; i < k; i += blockDim.x) {
+    h[i] = ls[i] + ls[m + i];
+    c[i] = cs[i];
+    sv[i] = sn[i];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double nw = sqrt(fmax(ls[2 * m], 0.0));
+    for (int i = 0; i + 1 < k; ++i) {
+      const double a = h[i], b = h[i + 1];
+      h[i] = c[i] * a + sv[i] * b;
+      h[i + 1] = -sv[i] * a + c[i] * b;
+    }
+    const double a = h[k - 1];
+    double cc = 1.0, ss = 0.0;
+    if (nw != 0.0) {
+      const double r = hypot(a, nw);
+      cc = a / r;
+      ss = nw / r;
+    }
+    cs[k - 1] = cc;
+    sn[k - 1] = ss;
+    h[k - 1] = cc * a + ss * nw;
+    const double gk1 = g[k - 1];
+    g[k] = -ss * gk1;
+    g[k - 1] = cc * gk1;
+    ls[2 * m + 1] = nw > 0.0 ? 1.0 / nw : 0.0;
+    ls[st.hist() + 2 * k] = -ss * gk1;
+    ls[st.hist() + 2 * k + 1] = nw;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < k; i += blockDim.x) Rk[i] = h[i];
+}
+
 }  // namespace iga
 
 using namespace iga;
@@ -162,7 +271,10 @@ extern "C" {
 
 int64_t iga_reduce_workspace_bytes(int64_t n, int k) {
   if (n < 0 || k < 0) return 0;
-  return (int64_t)reduce_blocks(n) * (k > 0 ? k : 1) * (int64_t)sizeof(double);
+  const int64_t kk = k > 0 ? k : 1;
+  const int64_t dots = (int64_t)dot_chunks(n) * (kRedThreads / 64) * kk;  // iga_dot_many
+  const int64_t zdots = (int64_t)reduce_blocks(n) * 2 * kk;              // iga_zdot_many
+  return (dots > zdots ? dots : zdots) * (int64_t)sizeof(double);
 }
 
 int iga_dot_many(int dtype, int64_t n, int k, const void* V, int64_t ldv, const void* w, double* out,
@@ -171,16 +283,17 @@ int iga_dot_many(int dtype, int64_t n, int k, const void* V, int64_t ldv, const 
     IGA_CHECK_ARG(V && w && out && work && n >= 0 && k >= 0 && ldv >= n, "bad argument");
     if (k == 0) return;
     hipStream_t s = as_stream(stream);
-    const int nb = reduce_blocks(n);
+    const int nb = dot_chunks(n);
     double* partial = (double*)work;
+    const dim3 grid(nb, (k + kDotVec - 1) / kDotVec);
     if (dtype == IGA_F64)
-      dot_many_stage1<double><<<nb, kRedThreads, 0, s>>>(n, k, (const double*)V, ldv, (const double*)w, partial);
+      dot_many_stage1<double><<<grid, kRedThreads, 0, s>>>(n, k, (const double*)V, ldv, (const double*)w, partial);
     else if (dtype == IGA_F32)
-      dot_many_stage1<float><<<nb, kRedThreads, 0, s>>>(n, k, (const float*)V, ldv, (const float*)w, partial);
+      dot_many_stage1<float><<<grid, kRedThreads, 0, s>>>(n, k, (const float*)V, ldv, (const float*)w, partial);
     else
       throw Error(IGA_ERR_INVALID, "unknown dtype");
     IGA_LAUNCH_CHECK();
-    dot_many_stage2<<<k, kRedThreads, 0, s>>>(nb, k, partial, out);
+    dot_many_stage2<<<k, kRedThreads, 0, s>>>(nb * (kRedThreads / 64), k, partial, out);
     IGA_LAUNCH_CHECK();
   });
 }
@@ -265,6 +378,48 @@ int iga_cast(int dtype_x, int dtype_y, int64_t n, const void* x, void* y, void* 
     else
       throw Error(IGA_ERR_INVALID, "unknown dtype");
     IGA_LAUNCH_CHECK();
+  });
+}
+
+int64_t iga_gmres_state_doubles(int m) {
+  if (m < 1) return 0;
+  return GmState{m}.size();
+}
+
+int iga_gmres_start(int dtype, int64_t n, int m, const void* r, void* V0, double* ls, void* work, void* stream) {
+  return guard([&] {
+    IGA_CHECK_ARG(r && V0 && ls && work && n >= 1 && m >= 1, "bad argument");
+    IGA_CHECK_ARG(m <= kGmMaxDim, "Krylov dimension above 2048");
+    hipStream_t s = as_stream(stream);
+    const GmState st{m};
+    gm_zero_kernel<<<grid_for(st.size(), 256, 1024), 256, 0, s>>>(st.size(), ls);
+    IGA_LAUNCH_CHECK();
+    IGA_CHECK(iga_dot_many(dtype, n, 1, r, n, r, ls + 2 * m, work, stream));
+    gm_start_kernel<<<1, 64, 0, s>>>(st, ls);
+    IGA_LAUNCH_CHECK();
+    IGA_CHECK(iga_axpby(dtype, n, 0.0, ls + 2 * m + 1, 0, r, 0.0, V0, stream));
+  });
+}
+
+int iga_gmres_step(int dtype, int64_t n, int k, int m, void* V, int64_t ldv, void* w, double* ls, void* work,
+                   void* stream) {
+  return guard([&] {
+    IGA_CHECK_ARG(V && w && ls && work && n >= 1 && ldv >= n, "bad argument");
+    IGA_CHECK_ARG(k >= 1 && k <= m, "step outside the Krylov dimension");
+    IGA_CHECK_ARG(m <= kGmMaxDim, "Krylov dimension above 2048");
+    hipStream_t s = as_stream(stream);
+    const GmState st{m};
+    // classical Gram-Schmidt, two passes
+    IGA_CHECK(iga_dot_many(dtype, n, k, V, ldv, w, ls, work, stream));
+    IGA_CHECK(iga_axpy_many(dtype, n, k, V, ldv, ls, -1.0, w, stream));
+    IGA_CHECK(iga_dot_many(dtype, n, k, V, ldv, w, ls + m, work, stream));
+    IGA_CHECK(iga_axpy_many(dtype, n, k, V, ldv, ls + m, -1.0, w, stream));
+    IGA_CHECK(iga_dot_many(dtype, n, 1, w, n, w, ls + 2 * m, work, stream));
+    gm_rotate_kernel<<<1, 64, 3 * sizeof(double) * (size_t)k, s>>>(st, k, ls);
+    IGA_LAUNCH_CHECK();
+    const size_t es = dtype == IGA_F64 ? 8 : 4;
+    void* vk = (char*)V + (size_t)k * ldv * es;
+    IGA_CHECK(iga_axpby(dtype, n, 0.0, ls + 2 * m + 1, 0, w, 0.0, vk, stream));
   });
 }
 

@@ -1,0 +1,398 @@
+"""Device-resident Krylov solves advanced step by step, without a host round trip
+per iteration.
+
+``GmresRun`` is the reference's un-restarted, right-preconditioned GMRES
+(``solvers/linear_solver.py:438-525``) with its relative and absolute residual
+criteria (``solvers/utils/convergence_manager.py:156-327``).  Each Arnoldi step is
+queued on the run's stream: preconditioner, operator, both Gram-Schmidt passes,
+the Givens update of the least-squares problem and the new basis vector all stay
+on the device (``iga_gmres_step``).  The host learns each step's residual estimate
+``|g_k|`` from a two-value asynchronous copy and an event, so it can queue the next
+steps before the previous ones finish; steps queued past convergence do not change
+the solution, which is formed from the first ``k`` columns only.
+
+``RefinementRun`` is mixed-precision iterative refinement: inner ``GmresRun`` solves
+of ``A_lo d = r`` in float32 vectors and operators, the residual ``r = b - A x`` and
+the solution in float64, until ``||r|| / ||b|| <= tol``.
+
+Both expose ``advance()``, which queues whatever work is possible and returns
+whether anything happened, so that one host thread can interleave many runs on
+separate streams (the serving loop does).
+"""
+from __future__ import annotations
+
+import contextlib
+from typing import Callable, Optional
+
+import numpy as np
+
+from . import _lib
+from . import device as dv
+from .common import Constants
+
+
+def _back_substitute(R: np.ndarray, g: np.ndarray) -> np.ndarray:
+    k = len(g)
+    y = np.zeros(k)
+    for i in range(k - 1, -1, -1):
+        y[i] = (g[i] - R[i, i + 1:k] @ y[i + 1:k]) / R[i, i]
+    return y
+
+
+def _relative_or_absolute(tol: float, rel: float, absv: float) -> str:
+    if rel < tol:
+        return "relative"
+    if absv < Constants.SAFEGUARD:
+        return "absolute"
+    return ""
+
+
+class GmresRun:
+    """One GMRES solve of ``A x = b`` right-preconditioned by ``P``.
+
+    ``A(x, out)`` / ``P(x, out)`` write into ``out`` (vectors of ``dtype``); ``mask``
+    zeroes the constrained entries (``linear_solver.py:119-124``).  ``maxiters`` is
+    also the Krylov dimension (the reference does not restart)."""
+
+    def __init__(self, A: Callable, P: Optional[Callable], b, tol: float, maxiters: int, mask=None, x0=None,
+                 stream=None, lookahead: int = 2, dtype: Optional[str] = None):
+        t = dv.torch()
+        self.A, self.P, self.mask = A, P, mask
+        self.tol, self.m = float(tol), int(maxiters)
+        if self.m < 1:
+            raise ValueError("maxiters must be positive")
+        self.stream = stream
+        self.lookahead = max(1, int(lookahead))
+        self.dtype = dtype or ("f32" if b.dtype == t.float32 else "f64")
+        tdt = dv.torch_dtype(self.dtype)
+        self.n = n = b.numel()
+        lib = _lib.load()
+        with self._on_stream():
+            self.b = b
+            self.V = t.empty((self.m + 1, n), dtype=tdt, device=b.device)
+            self.Z = t.empty((self.m, n), dtype=tdt, device=b.device)
+            self.w = t.empty(n, dtype=tdt, device=b.device)
+            self.x = t.zeros(n, dtype=tdt, device=b.device)
+            self.x0 = x0
+            self.nls = int(lib.iga_gmres_state_doubles(self.m))
+            self.ls = t.zeros(self.nls, dtype=t.float64, device=b.device)
+            self.work = dv.workspace(lib.iga_reduce_workspace_bytes(n, self.m + 1))
+            self.nb = t.zeros(1, dtype=t.float64, device=b.device)
+        self.hist_host = t.zeros(2 * (self.m + 2), dtype=t.float64, pin_memory=True)
+        self._hist_off = self.nls - 2 * (self.m + 1)
+        self.launched = 0      # steps queued
+        self.known = -1        # last step whose residual estimate has been read
+        self.events = {}
+        self.kfinal: Optional[int] = None
+        self.reason = ""
+        self.res_hist = []
+        self.ref = None
+        self.beta = None
+        self._sol_event = None
+        self._sol_host = None
+        self.result = None
+
+    # -- plumbing --
+    def _on_stream(self):
+        return dv.torch().cuda.stream(self.stream) if self.stream is not None else contextlib.nullcontext()
+
+    def _sp(self):
+        return dv.stream_ptr(self.stream)
+
+    def _record(self, step: int, src_off: int, dst_off: int, count: int):
+        t = dv.torch()
+        self.hist_host[dst_off:dst_off + count].copy_(self.ls[src_off:src_off + count], non_blocking=True)
+        ev = t.cuda.Event()
+        ev.record(self.stream if self.stream is not None else t.cuda.current_stream())
+        self.events[step] = ev
+
+    # -- phases --
+    def start(self):
+        t = dv.torch()
+        dt = dv.DTYPES[self.dtype]
+        with self._on_stream():
+            r = self.w
+            r.copy_(self.b)
+            if self.x0 is not None:
+                self.x.copy_(self.x0)
+                tmp = t.empty_like(self.w)
+                self.A(self.x, tmp)
+                if self.mask:
+                    self.mask(tmp)
+                r.sub_(tmp)
+                _lib.call("iga_dot_many", dt, self.n, 1, dv.ptr(self.b), self.n, dv.ptr(self.b), dv.ptr(self.nb),
+                          dv.ptr(self.work), self._sp())
+            _lib.call("iga_gmres_start", dt, self.n, self.m, dv.ptr(r), dv.ptr(self.V[0]), dv.ptr(self.ls),
+                      dv.ptr(self.work), self._sp())
+            if self.x0 is not None:
+                self.hist_host[2 * (self.m + 1):2 * (self.m + 1) + 1].copy_(self.nb, non_blocking=True)
+            self._record(0, self._hist_off, 0, 2)
+        self.launched = 0
+
+    def can_launch(self) -> bool:
+        return self.kfinal is None and self.launched < self.m and self.launched - self.known < self.lookahead
+
+    def launch(self):
+        """Queue Arnoldi step k = launched + 1."""
+        k = self.launched + 1
+        dt = dv.DTYPES[self.dtype]
+        with self._on_stream():
+            z = self.Z[k - 1]
+            if self.P is not None:
+                self.P(self.V[k - 1], z)
+            else:
+                z.copy_(self.V[k - 1])
+            if self.mask:
+                self.mask(z)
+            self.A(z, self.w)
+            if self.mask:
+                self.mask(self.w)
+            _lib.call("iga_gmres_step", dt, self.n, k, self.m, dv.ptr(self.V), self.n, dv.ptr(self.w),
+                      dv.ptr(self.ls), dv.ptr(self.work), self._sp())
+            self._record(k, self._hist_off + 2 * k, 2 * k, 2)
+        self.launched = k
+
+    def poll(self, block: bool = False) -> bool:
+        """Read the residual estimates of finished steps; decide convergence."""
+        progressed = False
+        while self.kfinal is None and (self.known + 1) in self.events:
+            j = self.known + 1
+            ev = self.events[j]
+            if not ev.query():
+                if not block:
+                    break
+                ev.synchronize()
+            block = False
+            del self.events[j]
+            self.known = j
+            progressed = True
+            g, nw = float(self.hist_host[2 * j]), float(self.hist_host[2 * j + 1])
+            if j == 0:
+                self.beta = abs(g)
+                nb = float(self.hist_host[2 * (self.m + 1)]) ** 0.5 if self.x0 is not None else self.beta
+                self.ref = nb if nb > 0 else self.beta
+                self.res_hist.append(1.0)
+                if self.beta < Constants.SAFEGUARD:
+                    self.kfinal, self.reason = 0, "absolute"
+                continue
+            res = abs(g)
+            self.res_hist.append(res / self.ref)
+            reason = _relative_or_absolute(self.tol, res / self.ref, res)
+            if reason or j >= self.m or nw == 0.0:
+                self.kfinal = j
+                self.reason = reason or ("breakdown" if nw == 0.0 else "maxiters")
+        return progressed
+
+    @property
+    def converged_known(self) -> bool:
+        return self.kfinal is not None
+
+    def request_solution(self):
+        """Queue the copy of the least-squares factors of the first kfinal steps."""
+        t = dv.torch()
+        k = self.kfinal
+        m = self.m
+        if k == 0:
+            self._sol_host = None
+            return
+        with self._on_stream():
+            off_r, off_g = 2 * m + 2, 2 * m + 2 + m * m + 2 * m
+            self._sol_host = t.empty(k * m + k, dtype=t.float64, pin_memory=True)
+            self._sol_host[:k * m].copy_(self.ls[off_r:off_r + k * m], non_blocking=True)
+            self._sol_host[k * m:].copy_(self.ls[off_g:off_g + k], non_blocking=True)
+            self._sol_event = t.cuda.Event()
+            self._sol_event.record(self.stream if self.stream is not None else t.cuda.current_stream())
+
+    def solution_ready(self, block: bool = False) -> bool:
+        if self._sol_event is None:
+            return True
+        if block:
+            self._sol_event.synchronize()
+            return True
+        return self._sol_event.query()
+
+    def complete(self):
+        """x += Z[:k]^T y (queued); returns the solve's OutputArgs."""
+        from .solvers import OutputArgs
+
+        t = dv.torch()
+        k = self.kfinal
+        if k and k > 0:
+            self.solution_ready(block=True)
+            h = self._sol_host.numpy()
+            R = h[:k * self.m].reshape(k, self.m).T[:k, :k]
+            y = _back_substitute(R, h[k * self.m:k * self.m + k])
+            with self._on_stream():
+                # the pinned source and the device copy are kept with the run until it is dropped
+                self._y_host = t.from_numpy(y).pin_memory()
+                self._y_dev = self._y_host.to(self.x.device, non_blocking=True)
+                _lib.call("iga_axpy_many", dv.DTYPES[self.dtype], self.n, k, dv.ptr(self.Z), self.n,
+                          dv.ptr(self._y_dev), 1.0, dv.ptr(self.x), self._sp())
+        success = self.reason in ("relative", "absolute", "breakdown")
+        self.result = OutputArgs(self.x, np.asarray(self.res_hist), success, int(k or 0), self.reason)
+        return self.result
+
+    def release(self):
+        """Drop the Krylov basis (the solution tensor is kept)."""
+        self.V = self.Z = None
+
+    # -- driver --
+    def advance(self) -> bool:
+        progressed = False
+        if self.launched == 0 and not self.events and self.known < 0:
+            self.start()
+            progressed = True
+        if self.result is not None:
+            return False
+        progressed |= self.poll()
+        while self.can_launch():
+            self.launch()
+            progressed = True
+        if self.kfinal is not None:
+            if self._sol_event is None and self._sol_host is None:
+                self.request_solution()
+                progressed = True
+            if self.solution_ready():
+                self.complete()
+                progressed = True
+        return progressed
+
+    @property
+    def done(self) -> bool:
+        return self.result is not None
+
+    def run(self):
+        """Synchronous driver: the same steps with the host waiting on each event."""
+        self.start()
+        while self.kfinal is None:
+            while self.can_launch():
+                self.launch()
+            self.poll(block=True)
+        self.request_solution()
+        return self.complete()
+
+
+class RefinementRun:
+    """``x += GMRES_lo(A_lo, r; P_lo)`` with ``r = b - A64 x`` in float64 until
+    ``||r|| / ||b|| <= tol``; the inner solves use float32 vectors and operators."""
+
+    def __init__(self, A64: Callable, A_lo: Callable, P_lo: Callable, b, tol: float = 1e-10,
+                 inner_tol: float = 1e-5, inner_maxiters: int = 200, max_refinements: int = 30, mask=None,
+                 stream=None, lookahead: int = 2, lo_dtype: str = "f32"):
+        t = dv.torch()
+        self.A64, self.A_lo, self.P_lo, self.mask = A64, A_lo, P_lo, mask
+        self.tol, self.inner_tol = float(tol), float(inner_tol)
+        self.inner_maxiters, self.max_ref = int(inner_maxiters), int(max_refinements)
+        self.stream, self.lookahead, self.lo_dtype = stream, lookahead, lo_dtype
+        self.n = b.numel()
+        with self._on_stream():
+            self.b = b
+            self.x = t.zeros(self.n, dtype=t.float64, device=b.device)
+            self.r = b.clone()
+            if mask:
+                mask(self.r)
+            self.r_lo = t.empty(self.n, dtype=dv.torch_dtype(lo_dtype), device=b.device)
+            self.w = t.empty_like(self.x)
+            self.nrm = t.zeros(2, dtype=t.float64, device=b.device)
+            self.work = dv.workspace(_lib.load().iga_reduce_workspace_bytes(self.n, 1))
+        self.nrm_host = t.zeros(2, dtype=t.float64, pin_memory=True)
+        self.phase = "norm"
+        self.inner: Optional[GmresRun] = None
+        self.inner_its = []
+        self.hist = []
+        self.nb = None
+        self._ev = None
+        self.result = None
+
+    def _on_stream(self):
+        return dv.torch().cuda.stream(self.stream) if self.stream is not None else contextlib.nullcontext()
+
+    def _queue_norm(self, v, slot: int):
+        t = dv.torch()
+        with self._on_stream():
+            _lib.call("iga_dot_many", _lib.IGA_F64, self.n, 1, dv.ptr(v), self.n, dv.ptr(v), dv.ptr(self.nrm[slot:]),
+                      dv.ptr(self.work), dv.stream_ptr(self.stream))
+            self.nrm_host[slot:slot + 1].copy_(self.nrm[slot:slot + 1], non_blocking=True)
+            self._ev = t.cuda.Event()
+            self._ev.record(self.stream if self.stream is not None else t.cuda.current_stream())
+
+    def _start_inner(self):
+        with self._on_stream():
+            _lib.call("iga_cast", _lib.IGA_F64, dv.DTYPES[self.lo_dtype], self.n, dv.ptr(self.r), dv.ptr(self.r_lo),
+                      dv.stream_ptr(self.stream))
+        self.inner = GmresRun(self.A_lo, self.P_lo, self.r_lo, self.inner_tol, self.inner_maxiters, mask=self.mask,
+                              stream=self.stream, lookahead=self.lookahead, dtype=self.lo_dtype)
+        self.phase = "inner"
+
+    def _finish(self, success: bool):
+        from .solvers import OutputArgs
+
+        self.result = OutputArgs(self.x, np.asarray(self.hist), success, int(sum(self.inner_its)),
+                                 "true-residual" if success else "max_refinements",
+                                 history={"inner_iterations": list(self.inner_its)})
+
+    def advance(self) -> bool:
+        if self.result is not None:
+            return False
+        if self.phase == "norm":  # ||b||
+            self._queue_norm(self.r, 0)
+            self.phase = "wait_nb"
+            return True
+        if self.phase == "wait_nb":
+            if not self._ev.query():
+                return False
+            self.nb = float(self.nrm_host[0]) ** 0.5
+            self.hist.append(1.0)
+            if self.nb < Constants.SAFEGUARD:
+                self._finish(True)
+                return True
+            self._start_inner()
+            return True
+        if self.phase == "inner":
+            progressed = self.inner.advance()
+            if self.inner.done:
+                out = self.inner.result
+                self.inner_its.append(out.iterations)
+                with self._on_stream():
+                    _lib.call("iga_cast", dv.DTYPES[self.lo_dtype], _lib.IGA_F64, self.n, dv.ptr(out.solution),
+                              dv.ptr(self.w), dv.stream_ptr(self.stream))
+                    self.x.add_(self.w)
+                    self.A64(self.x, self.w)
+                    if self.mask:
+                        self.mask(self.w)
+                    self.r.copy_(self.b)
+                    self.r.sub_(self.w)
+                    if self.mask:
+                        self.mask(self.r)
+                self.inner.release()
+                self._queue_norm(self.r, 1)
+                self.phase = "wait_r"
+                return True
+            return progressed
+        if self.phase == "wait_r":
+            if not self._ev.query():
+                return False
+            rel = float(self.nrm_host[1]) ** 0.5 / self.nb
+            self.hist.append(rel)
+            if rel <= self.tol:
+                self._finish(True)
+            elif len(self.inner_its) >= self.max_ref:
+                self._finish(False)
+            else:
+                self._start_inner()
+            return True
+        return False
+
+    @property
+    def done(self) -> bool:
+        return self.result is not None
+
+    def run(self):
+        while not self.done:
+            if not self.advance():
+                # nothing to do until the stream catches up
+                if self.phase == "inner" and self.inner is not None:
+                    self.inner.poll(block=True)
+                elif self._ev is not None:
+                    self._ev.synchronize()
+        return self.result

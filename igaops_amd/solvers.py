@@ -209,7 +209,12 @@ def gmres_device(
 
     ``A(x, out)`` and ``P(x, out)`` write into ``out``.  Without ``restart`` this is the
     reference's un-restarted GMRES; with it, cycles of ``restart`` iterations restart
-    from the true residual ``b - A x``."""
+    from the true residual ``b - A x``.  Without ``restart`` and without the
+    increment criterion the Arnoldi steps run device-side (``runs.GmresRun``)."""
+    if restart is None and not use_increment:
+        from .runs import GmresRun
+
+        return GmresRun(A, P, b, tol, maxiters, mask=mask, x0=x0, stream=stream).run()
     t = dv.torch()
     n = b.numel()
     m = min(maxiters, restart) if restart else maxiters
@@ -401,6 +406,17 @@ def refinement_solve(
             break
     return OutputArgs(x, np.asarray(hist), success, int(sum(inner_its)), "true-residual" if success else "max_refinements",
                       history={"inner_iterations": inner_its})
+
+
+def mixed_solve(A64: Callable, A_lo: Callable, P_lo: Callable, b, tol: float = 1e-10, inner_tol: float = 1e-5,
+                inner_maxiters: int = 200, max_refinements: int = 30, mask: Optional[Mask] = None,
+                lo_dtype: str = "f32") -> OutputArgs:
+    """Iterative refinement with float64 residuals around inner GMRES solves whose
+    vectors, operator ``A_lo`` and preconditioner ``P_lo`` are ``lo_dtype``."""
+    from .runs import RefinementRun
+
+    return RefinementRun(A64, A_lo, P_lo, b, tol, inner_tol, inner_maxiters, max_refinements, mask=mask,
+                         lo_dtype=lo_dtype).run()
 
 
 # ---------------------------------------------------------------------------

@@ -126,6 +126,24 @@ int iga_axpby(int dtype, int64_t n, double a, const double* a_dev, int invert_a,
 /* Casts between float64 and float32 vectors: y = (dtype_y) x. */
 int iga_cast(int dtype_x, int dtype_y, int64_t n, const void* x, void* y, void* stream);
 
+/* ---- device-side GMRES Arnoldi step ------------------------------------------
+ * Replaces the inner loop of _GMRES (solvers/linear_solver.py:473-508): the two
+ * Gram-Schmidt passes (481-490), the Hessenberg column and its least-squares
+ * residual, with the least-squares problem kept in Givens-rotated form instead of
+ * lstsq (501; the residual estimate |g_k| equals its residual norm).  `ls` is a
+ * float64 device state of iga_gmres_state_doubles(m) values:
+ *   [0,m) and [m,2m) the two projection passes, 2m ||w||^2, 2m+1 the next basis scale,
+ *   R = ls + 2m+2 (m x m column-major, the rotated Hessenberg), cs, sn (m each),
+ *   g (m+1), hist (2(m+1): g[k] and ||w_k|| after step k, for the host to poll).
+ * iga_gmres_start: beta = ||r||, g[0] = beta, V0 = r / beta (state cleared first).
+ * iga_gmres_step (k = 1..m, m <= 2048): orthogonalises w against V[0..k) in place, updates
+ * R, cs, sn, g and writes V[k] = w / ||w||.  `work` holds
+ * iga_reduce_workspace_bytes(n, m) bytes; vectors are `dtype`, the state float64. */
+int64_t iga_gmres_state_doubles(int m);
+int iga_gmres_start(int dtype, int64_t n, int m, const void* r, void* V0, double* ls, void* work, void* stream);
+int iga_gmres_step(int dtype, int64_t n, int k, int m, void* V, int64_t ldv, void* w, double* ls, void* work,
+                   void* stream);
+
 /* Gather / scatter of a box of a 3-D (or lower) tensor-product array (the free
  * nodes of the FD preconditioner, fastdiag/utils/space_precond.py:104-122,296,315):
  *   gather:  sub[s] = full[offset(s)]        (box lo[d]..lo[d]+len[d]-1, dims n[d])

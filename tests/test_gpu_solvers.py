@@ -120,3 +120,70 @@ def test_mixed_precision_refinement_reaches_1e10():
     free[np.asarray(sp.space_free_nodes[0])] = True
     Ax[~free] = 0
     assert np.linalg.norm(bh - Ax) / np.linalg.norm(bh) <= 1e-10
+
+
+def _annulus_fd_problem(p=3, nel=6, mass=True):
+    from igaops_amd.fastdiag import SingleFD
+    from igaops_amd.operators import StiffnessMassOperator
+    from igaops_amd.solvers import Mask
+
+    patch, C, m = build_problem("annulus", p, nel)
+    qs = patch.quadrule_list
+    w = patch.nurbs_weights
+    fd = SingleFD("spatial")
+    fd.compute_space_eigendecomposition(qs, np.ones((1, 3, 2), bool))
+    fd.update_space_eigenvalues([1.0 if mass else 0.0, 1.0])
+    sp = fd.space_preconditioner
+    ops = {dt: StiffnessMassOperator(qs, C=C, m=m if mass else None, nurbs_weights=w, dtype=dt)
+           for dt in ("f64", "f32")}
+    return patch, C, m if mass else None, fd, Mask(box=sp.box()), ops
+
+
+def test_device_step_gmres_matches_host_step_gmres():
+    """The device-side Arnoldi/Givens steps (runs.GmresRun) against the host-side
+    Givens loop on the same problem: same iteration count, same residual history,
+    solutions within the attainable accuracy; both with and without lookahead."""
+    import torch
+
+    from igaops_amd.runs import GmresRun
+    from igaops_amd.solvers import gmres_device
+
+    patch, C, m, fd, mask, ops = _annulus_fd_problem()
+    n = patch.nbctrlpts_total
+    A = lambda x, o: ops["f64"].apply(x, out=o)
+    P = lambda x, o: fd.apply_device(x, out=o)
+    b = torch.as_tensor(np.random.default_rng(11).standard_normal(n), device="cuda")
+    mask(b)
+    ref = gmres_device(A, b, P, 1e-10, 200, mask=mask, restart=400)  # host Givens path (one cycle)
+    for la in (1, 3):
+        out = GmresRun(A, P, b, 1e-10, 200, mask=mask, lookahead=la).run()
+        assert out.success and ref.success
+        assert out.iterations == ref.iterations
+        np.testing.assert_allclose(out.residual[:len(ref.residual)], ref.residual[:len(out.residual)], rtol=1e-6,
+                                   atol=1e-14)
+        assert rel(out.solution.cpu().numpy(), ref.solution.cpu().numpy()) <= 1e-9
+    # true residual
+    r = b - ops["f64"].apply(out.solution)
+    mask(r)
+    assert float(torch.linalg.vector_norm(r) / torch.linalg.vector_norm(b)) <= 1e-9
+
+
+def test_refinement_run_float32_vectors_reaches_1e10():
+    import torch
+
+    from igaops_amd.solvers import mixed_solve
+
+    patch, C, m, fd, mask, ops = _annulus_fd_problem(4, 8)
+    n = patch.nbctrlpts_total
+    b = torch.as_tensor(np.random.default_rng(12).standard_normal(n), device="cuda")
+    mask(b)
+    out = mixed_solve(lambda x, o: ops["f64"].apply(x, out=o), lambda x, o: ops["f32"].apply(x, out=o),
+                      lambda x, o: fd.apply_device(x, out=o), b, tol=1e-10, inner_tol=1e-5, mask=mask)
+    assert out.success and len(out.history["inner_iterations"]) >= 2
+    x = out.solution.cpu().numpy()
+    Ax = oo.stiffness_plus_mass(patch.quadrule_list, C, m, x, nurbs_weights=patch.nurbs_weights)
+    bh = b.cpu().numpy()
+    free = np.zeros(n, bool)
+    free[np.asarray(fd.space_preconditioner.space_free_nodes[0])] = True
+    Ax[~free] = 0
+    assert np.linalg.norm(bh - Ax) / np.linalg.norm(bh) <= 1e-10
