@@ -142,6 +142,32 @@ def solve_section(patch, C, m, nrhs, tol=1e-10):
 PMC_COPY_BYTES = 512 * 1024 * 1024
 
 
+def serve_section(nreq=100, qps=20.0, seed=0, max_active=16, budget_gb=200.0):
+    """Poisson-arrival stream of Poisson/diffusion solves (nel U{16..96}, p U{2..5},
+    cube or annulus, sigma U[0.2,1.0]) served continuously, plain float64 and mixed
+    precision on the same stream; latency from arrival to a checked 1e-10 residual."""
+    from igaops_amd.serving import SolveServer, poisson_stream, summarize, wq_rule
+
+    reqs = poisson_stream(nreq, qps, seed=seed)
+    t0 = time.perf_counter()
+    for r in reqs:  # the 1-D rule cache (host) is warm in a running server
+        wq_rule(r.degree, r.nel)
+    warm = time.perf_counter() - t0
+    out = {"requests": nreq, "qps": qps, "seed": seed, "max_active": max_active, "rule_cache_warmup_s": warm}
+    for mode in ("fp64", "mixed"):
+        srv = SolveServer(mode=mode, max_active=max_active, hbm_budget_gb=budget_gb)
+        t1 = time.perf_counter()
+        res = srv.serve(poisson_stream(nreq, qps, seed=seed))
+        wall = time.perf_counter() - t1
+        s = summarize(res, 1e-10, wall)
+        s.update({"max_concurrent": srv.stats["max_active"], "peak_admitted_gb": srv.stats["max_bytes"] / 1e9,
+                  "deferred_for_memory": srv.stats["deferred_for_memory"], "shapes": len(srv.shapes)})
+        out[mode] = s
+        print(f"serve {mode}: {json.dumps(s)}", file=sys.stderr, flush=True)
+    out["speedup_mixed_vs_fp64"] = out["mixed"]["solves_per_s"] / max(out["fp64"]["solves_per_s"], 1e-30)
+    return out
+
+
 def spacetime_section(steps=5, nrhs_seq=10, tol=1e-10, space=(3, 32), time_=(2, 64)):
     """Space-time heat equation: 3-D unit cube (degree 3, 32 elements/dir) x time
     (degree 2, 64 elements), WQ type "2" rules, unit diffusivity,
@@ -287,6 +313,8 @@ def main():
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-traffic", action="store_true")
     ap.add_argument("--spacetime", action="store_true", help="only the space-time GCRODR section")
+    ap.add_argument("--serve", type=int, default=0, help="only the serving section, with this many requests")
+    ap.add_argument("--qps", type=float, default=20.0)
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -297,6 +325,9 @@ def main():
         return
     if args.spacetime:
         print(json.dumps({"spacetime": spacetime_section()}))
+        return
+    if args.serve:
+        print(json.dumps({"serve": serve_section(args.serve, args.qps)}))
         return
 
     import torch

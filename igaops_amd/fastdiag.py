@@ -214,10 +214,17 @@ class FDPlan:
         _lib.call("iga_fd_set_eigenvalues", self._h, lp)
         self.eig_version = id(lam)
 
-    def apply(self, x, out, nrhs: int = 1, stream=None):
-        w = self._work.get(nrhs)
+    def workspace(self, nrhs: int = 1):
+        return dv.workspace(_lib.load().iga_fd_workspace_bytes(self._h, int(nrhs)))
+
+    def apply(self, x, out, nrhs: int = 1, stream=None, work=None):
+        """``out = P x``; ``work`` (from ``workspace``) lets concurrent callers of one
+        plan keep their scratch apart (the plan's own buffer is used otherwise)."""
+        w = work
         if w is None:
-            w = dv.workspace(_lib.load().iga_fd_workspace_bytes(self._h, int(nrhs)))
+            w = self._work.get(nrhs)
+        if w is None:
+            w = self.workspace(nrhs)
             self._work[nrhs] = w
         _lib.call("iga_fd_apply", self._h, dv.ptr(x), dv.ptr(out), int(nrhs), dv.ptr(w), dv.stream_ptr(stream))
         return out

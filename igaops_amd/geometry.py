@@ -207,6 +207,78 @@ class ExtrudedPatch:
         return C, m
 
 
+    def device_coefficients(self, sigma: float = 0.0, seed: int = 0, modes: int = 8, mass: bool = True):
+        """``conductivity_coefficients(lognormal_conductivity(self, sigma, seed, modes))``
+        formed on the device: the cross-section and extrusion factors are evaluated on
+        the host (``n_q1 n_q2`` and ``n_q3`` points), the ``N_q`` products, the Fourier
+        field (one complex matrix product) and its exponential on the device.
+        Returns float64 device tensors ``C (3, 3, N_q)`` and ``m (N_q,)`` (``None``
+        when ``mass`` is false)."""
+        import torch as t
+
+        from . import device as dv
+
+        pg = self.plane_geometry()
+        z, dz = self.line_geometry()
+        j = pg.jac
+        det2 = j[0, 0] * j[1, 1] - j[0, 1] * j[1, 0]
+        if np.any(det2 * dz[0] <= 0.0):
+            raise ValueError("Potential geometry issue: quadrature points with detJ <= 0")
+        g00 = j[0, 0] ** 2 + j[1, 0] ** 2
+        g11 = j[0, 1] ** 2 + j[1, 1] ** 2
+        g01 = j[0, 0] * j[0, 1] + j[1, 0] * j[1, 1]
+        dG = g00 * g11 - g01 ** 2
+        dev = dv.device()
+        f = lambda a: t.from_numpy(np.ascontiguousarray(a, dtype=np.float64).ravel()).to(dev)
+        det2_d, dz_d = f(det2), f(dz)
+        det = t.outer(dz_d, det2_d)  # (n3, nplane): q3 slowest
+        if sigma > 0.0:
+            kap = self._device_lognormal(sigma, seed, modes, pg, z)
+            kdet = det * kap
+        else:
+            kdet = det
+        nq = det.numel()
+        C = t.zeros((3, 3, nq), dtype=t.float64, device=dev)
+        inv = {(0, 0): g11 / dG, (1, 1): g00 / dG, (0, 1): -g01 / dG}
+        for (a, b), v in inv.items():
+            val = (kdet * f(v)[None, :]).reshape(-1)
+            C[a, b] = val
+            if a != b:
+                C[b, a] = val
+        C[2, 2] = (kdet / (dz_d * dz_d)[:, None]).reshape(-1)
+        m = det.reshape(-1).clone() if mass else None
+        return C, m
+
+    def _device_lognormal(self, sigma, seed, modes, pg, z):
+        """The field of ``lognormal_conductivity`` (same random numbers), on the device."""
+        import torch as t
+
+        from . import device as dv
+
+        rng = np.random.default_rng(seed)
+        k = np.arange(1, modes + 1)
+        K1, K2, K3 = np.meshgrid(k, k, k, indexing="ij")
+        amp = rng.standard_normal(K1.shape) / (1.0 + K1 ** 2 + K2 ** 2 + K3 ** 2)
+        phase = rng.uniform(0.0, 2 * np.pi, K1.shape)
+        scale = np.sqrt(np.sum(amp ** 2) / 2.0)
+        coef = amp * np.exp(1j * phase) / scale
+        cp = self.ctrlpts
+        lo, hi = cp.min(axis=1), cp.max(axis=1)
+        span = np.where(hi > lo, hi - lo, 1.0)
+        xh = ((pg.x[0] - lo[0]) / span[0]).ravel()
+        yh = ((pg.x[1] - lo[1]) / span[1]).ravel()
+        zh = (z - lo[2]) / span[2]
+        Ez = np.exp(1j * np.pi * zh[:, None] * k[None, :])
+        Zc = np.einsum("abc,zc->abz", coef, Ez).reshape(-1, zh.shape[0])  # (m*m, n3)
+        dev = dv.device()
+        kd = t.from_numpy(k.astype(np.float64)).to(dev)
+        Ex = t.exp(1j * np.pi * t.from_numpy(xh).to(dev)[:, None] * kd[None, :])
+        Ey = t.exp(1j * np.pi * t.from_numpy(yh).to(dev)[:, None] * kd[None, :])
+        Pxy = (Ex[:, :, None] * Ey[:, None, :]).reshape(xh.shape[0], -1)
+        g = t.real(t.from_numpy(Zc).to(dev).T.contiguous() @ Pxy.T.contiguous())  # (n3, nplane)
+        return t.exp(sigma * g)
+
+
 # ---------------------------------------------------------------------------
 # Patch builders
 # ---------------------------------------------------------------------------

@@ -449,7 +449,8 @@ class StiffnessMassOperator:
         self.dtype = dtype
         self.ndof = int(np.prod([q.nbctrlpts for q in quadrule_list]))
         self.nq = int(np.prod([q.nbquadpts for q in quadrule_list]))
-        self.is_nurbs = nurbs_weights is not None and np.size(nurbs_weights) > 0
+        self.is_nurbs = nurbs_weights is not None and int(
+            nurbs_weights.numel() if isinstance(nurbs_weights, t.Tensor) else np.size(nurbs_weights)) > 0
         Cd = None if C is None else dv.to_device(C, "f64").reshape(3, 3, -1)
         md = None if m is None else dv.to_device(m, "f64").reshape(-1)
         self.w64 = dv.to_device(nurbs_weights, "f64") if self.is_nurbs else None
@@ -483,6 +484,16 @@ class StiffnessMassOperator:
         else:
             self.terms = self._composed_terms(Cd, md)
         self.coef_count = FORM_NCOEF[self.form] if self.fused else len(self.terms)
+        self._work = {}
+
+    def workspace(self, nrhs: int):
+        """This operator's own partial-block buffer, so that operators sharing a plan
+        can run concurrently on different streams."""
+        w = self._work.get(nrhs)
+        if w is None:
+            w = dv.workspace(self.plan.workspace_bytes(nrhs))
+            self._work[nrhs] = w
+        return w
 
     def _composed_terms(self, Cd, md):
         t = dv.torch()
@@ -518,7 +529,8 @@ class StiffnessMassOperator:
         if out is None:
             out = t.empty_like(x)
         if self.fused:
-            self.plan.apply(self.packed, x, out, nrhs, self.w, alpha, accumulate, stream=stream)
+            self.plan.apply(self.packed, x, out, nrhs, self.w, alpha, accumulate, work=self.workspace(nrhs),
+                            stream=stream)
             return out
         xin = x
         if self.is_nurbs:

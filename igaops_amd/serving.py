@@ -1,0 +1,303 @@
+"""A persistent solve server on one GPU: continuous admission of Poisson/diffusion
+solves of mixed degree, mesh size, geometry and coefficient field.
+
+Each request is ``K(kappa) x = b`` on a unit cube or an extruded quarter annulus
+(degree ``p``, ``nel`` elements per direction, WQ type "2" rules), homogeneous
+Dirichlet on every face, ``kappa`` lognormal (``sigma``, ``seed``), solved to a true
+relative residual ``tol`` by FD-preconditioned GMRES, either in float64
+(``"fp64"``) or as float64 iterative refinement around float32 inner solves
+(``"mixed"``).
+
+How the loop works:
+
+* Requests become visible at their arrival time.  One is admitted when a stream
+  slot is free and its HBM footprint (packed coefficients, the set-up temporaries,
+  the Krylov bases of its solver) fits in what the active requests leave of the
+  budget; otherwise it waits.  Admission is in arrival order.
+* Everything a request touches lives in HBM for its lifetime: its coefficients, its
+  right-hand side and solution, its Krylov basis and least-squares state.  What
+  depends only on the shape (geometry, quadrature, FD eigenvectors, fused-operator
+  tables) is built once per shape and shared read-only.
+* Every active request owns a stream and private scratch buffers.  The loop visits
+  the active requests in turn and queues whatever each can do without waiting
+  (the next Arnoldi steps, a refinement residual, the solution update), so the
+  work of many requests is in flight on the device at once.  Same-shape requests
+  share plans and their launches overlap; nothing a request computes depends on
+  which others are present, so its result is the same alone and in the mix.
+* Latency is measured from arrival to completion (the true residual checked).
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Tuple
+
+import numpy as np
+
+from . import device as dv
+
+_RULES: Dict[Tuple[int, int], object] = {}
+
+
+def wq_rule(degree: int, nel: int):
+    """WQ type "2" rule on the open uniform knot vector, cached by ``(degree, nel)``."""
+    from .quadrature import WeightedQuadrature, make_open_knotvector
+
+    key = (int(degree), int(nel))
+    q = _RULES.get(key)
+    if q is None:
+        q = WeightedQuadrature(degree, make_open_knotvector(degree, nel), quadtype="2")
+        _RULES[key] = q
+    return q
+
+
+@dataclass
+class SolveRequest:
+    rid: int
+    geometry: str  # "cube" or "annulus"
+    degree: int
+    nel: int
+    sigma: float
+    seed: int
+    arrival: float = 0.0  # seconds after the start of the stream
+
+    @property
+    def shape(self):
+        return (self.geometry, self.degree, self.nel)
+
+
+@dataclass
+class SolveResult:
+    rid: int
+    shape: tuple
+    ndof: int
+    iterations: int
+    true_rel_residual: float
+    success: bool
+    arrival: float
+    admitted: float
+    completed: float
+    solution: object = None
+
+    @property
+    def latency(self) -> float:
+        return self.completed - self.arrival
+
+
+class ShapeResources:
+    """Everything a request of one (geometry, degree, nel) shares with its peers."""
+
+    def __init__(self, geometry: str, degree: int, nel: int):
+        from .fastdiag import SingleFD
+        from .geometry import quarter_annulus, unit_cube
+        from .solvers import Mask
+
+        q = wq_rule(degree, nel)
+        build = unit_cube if geometry == "cube" else quarter_annulus
+        self.patch = build(degree, nel, quadrule=q)
+        self.qs = self.patch.quadrule_list
+        self.ndof = self.patch.nbctrlpts_total
+        self.nq = self.patch.nbquadpts_total
+        fd = SingleFD("spatial")
+        fd.compute_space_eigendecomposition(self.qs, np.ones((1, 3, 2), bool))
+        fd.update_space_eigenvalues([0.0, 1.0])
+        self.space_fd = fd.space_preconditioner
+        self.mask = Mask(box=self.space_fd.box())
+        self.w64 = dv.to_device(self.patch.nurbs_weights, "f64") if self.patch.is_nurbs else None
+        self.fd_plans = {dt: self.space_fd.plan(dt) for dt in ("f64", "f32")}
+
+
+def footprint_bytes(ndof: int, nq: int, nurbs: bool, mode: str, maxiters: int, inner_maxiters: int) -> int:
+    """Upper estimate of the HBM a request holds at its peak."""
+    ncoef = 10 if nurbs else 6
+    setup = 10 * nq * 8  # C (3,3,N_q) and the field, released after packing
+    coef = ncoef * nq * 8 + (ncoef * nq * 4 if mode == "mixed" else 0)
+    if mode == "mixed":
+        krylov = 2 * (inner_maxiters + 1) * ndof * 4 + 6 * ndof * 8
+    else:
+        krylov = 2 * (maxiters + 1) * ndof * 8 + 4 * ndof * 8
+    scratch = 12 * ndof * 8
+    return int(setup + coef + krylov + scratch)
+
+
+class _Active:
+    """One admitted request: its stream, operators, scratch and solver run."""
+
+    def __init__(self, req: SolveRequest, shape: ShapeResources, mode: str, stream, tol: float, maxiters: int,
+                 inner_tol: float, inner_maxiters: int, lookahead: int, admitted: float):
+        from .operators import StiffnessMassOperator
+        from .runs import GmresRun, RefinementRun
+
+        t = dv.torch()
+        self.req, self.shape, self.mode, self.stream = req, shape, mode, stream
+        self.admitted = admitted
+        with t.cuda.stream(stream):
+            C, _ = shape.patch.device_coefficients(req.sigma, req.seed, mass=False)
+            w = shape.w64
+            self.A64 = StiffnessMassOperator(shape.qs, C=C, m=None, nurbs_weights=w, dtype="f64")
+            self.A32 = (StiffnessMassOperator(shape.qs, C=C, m=None, nurbs_weights=w, dtype="f32")
+                        if mode == "mixed" else None)
+            del C
+            rng = np.random.default_rng(req.seed + 7919)
+            b = t.from_numpy(rng.standard_normal(shape.ndof)).to(dv.device(), non_blocking=False)
+            shape.mask(b)
+            self.b = b
+            dt = "f32" if mode == "mixed" else "f64"
+            plan = shape.fd_plans[dt]
+            fd_work = plan.workspace(1)
+            self._fd_work = fd_work
+            P = lambda x, o: plan.apply(x, o, 1, None, work=fd_work)
+            A64 = lambda x, o: self.A64.apply(x, out=o)
+            if mode == "mixed":
+                A32 = lambda x, o: self.A32.apply(x, out=o)
+                self.run = RefinementRun(A64, A32, P, b, tol=tol, inner_tol=inner_tol,
+                                         inner_maxiters=inner_maxiters, mask=shape.mask, stream=stream,
+                                         lookahead=lookahead)
+            else:
+                self.run = GmresRun(A64, P, b, tol, maxiters, mask=shape.mask, stream=stream, lookahead=lookahead)
+        self.t0 = time.perf_counter()
+
+    def advance(self) -> bool:
+        return self.run.advance()
+
+    @property
+    def done(self) -> bool:
+        return self.run.done
+
+    def result(self, completed: float, keep_solution: bool) -> SolveResult:
+        t = dv.torch()
+        out = self.run.result
+        with t.cuda.stream(self.stream):
+            x = out.solution
+            r = self.b - self.A64.apply(x)
+            self.shape.mask(r)
+            rel = float(t.linalg.vector_norm(r) / t.linalg.vector_norm(self.b))
+        return SolveResult(self.req.rid, self.req.shape, self.shape.ndof, int(out.iterations), rel,
+                           bool(out.success), self.req.arrival, self.admitted, completed,
+                           x.clone() if keep_solution else None)
+
+
+class SolveServer:
+    """Continuous admission over a stream of requests (see the module docstring)."""
+
+    def __init__(self, mode: str = "mixed", hbm_budget_gb: float = 200.0, max_active: int = 16, tol: float = 1e-10,
+                 maxiters: int = 800, inner_tol: float = 1e-5, inner_maxiters: int = 300, lookahead: int = 2,
+                 keep_solutions: bool = False):
+        if mode not in ("mixed", "fp64"):
+            raise ValueError("mode must be 'mixed' or 'fp64'")
+        self.mode = mode
+        self.budget = int(hbm_budget_gb * 1e9)
+        self.max_active = int(max_active)
+        self.tol, self.maxiters = float(tol), int(maxiters)
+        self.inner_tol, self.inner_maxiters = float(inner_tol), int(inner_maxiters)
+        self.lookahead = int(lookahead)
+        self.keep_solutions = keep_solutions
+        self.shapes: Dict[tuple, ShapeResources] = {}
+        self.stats = {"max_active": 0, "max_bytes": 0, "deferred_for_memory": 0}
+
+    def shape(self, key) -> ShapeResources:
+        s = self.shapes.get(key)
+        if s is None:
+            s = ShapeResources(*key)
+            self.shapes[key] = s
+        return s
+
+    def footprint(self, req: SolveRequest) -> int:
+        n = (req.nel + req.degree) ** 3
+        q = wq_rule(req.degree, req.nel)
+        nq = q.nbquadpts ** 3
+        return footprint_bytes(n, nq, req.geometry == "annulus", self.mode, self.maxiters, self.inner_maxiters)
+
+    def serve(self, requests: List[SolveRequest], realtime: bool = True) -> List[SolveResult]:
+        """Run the stream to completion.  With ``realtime`` the requests arrive at their
+        ``arrival`` times (wall clock from the call); otherwise all are present at once."""
+        t = dv.torch()
+        dv.require_gpu()
+        pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
+        streams = [t.cuda.Stream() for _ in range(self.max_active)]
+        free_streams = list(range(self.max_active))
+        active: List[Tuple[_Active, int, int]] = []  # (state, stream index, bytes)
+        used = 0
+        results: List[SolveResult] = []
+        start = time.perf_counter()
+        now = lambda: time.perf_counter() - start
+        head = 0
+        while head < len(pending) or active:
+            clock = now()
+            # admission in arrival order
+            while head < len(pending) and free_streams:
+                req = pending[head]
+                if realtime and req.arrival > clock:
+                    break
+                fp = self.footprint(req)
+                if active and used + fp > self.budget:
+                    self.stats["deferred_for_memory"] += 1
+                    break
+                si = free_streams.pop()
+                shape = self.shape(req.shape)
+                st = _Active(req, shape, self.mode, streams[si], self.tol, self.maxiters, self.inner_tol,
+                             self.inner_maxiters, self.lookahead, admitted=now())
+                active.append((st, si, fp))
+                used += fp
+                head += 1
+                if not realtime:
+                    req.arrival = 0.0
+            self.stats["max_active"] = max(self.stats["max_active"], len(active))
+            self.stats["max_bytes"] = max(self.stats["max_bytes"], used)
+            progressed = False
+            for st, _, _ in active:
+                progressed |= st.advance()
+            still = []
+            for st, si, fp in active:
+                if st.done:
+                    results.append(st.result(now(), self.keep_solutions))
+                    free_streams.append(si)
+                    used -= fp
+                else:
+                    still.append((st, si, fp))
+            if len(still) != len(active):
+                progressed = True
+            active = still
+            if not progressed:
+                if active:
+                    # nothing to queue: wait briefly for the device
+                    time.sleep(20e-6)
+                elif head < len(pending):
+                    time.sleep(max(0.0, min(1e-3, pending[head].arrival - now())))
+        t.cuda.synchronize()
+        return results
+
+
+def poisson_stream(n: int, qps: float, seed: int = 0, nel_range=(16, 96), degree_range=(2, 5),
+                   sigma_range=(0.2, 1.0), geometries=("cube", "annulus")) -> List[SolveRequest]:
+    """Seeded request stream with exponential inter-arrival times at ``qps``."""
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / qps, size=n) if qps > 0 else np.zeros(n)
+    arr = np.cumsum(gaps) - gaps[0]
+    out = []
+    for i in range(n):
+        out.append(SolveRequest(
+            rid=i,
+            geometry=str(geometries[int(rng.integers(len(geometries)))]),
+            degree=int(rng.integers(degree_range[0], degree_range[1] + 1)),
+            nel=int(rng.integers(nel_range[0], nel_range[1] + 1)),
+            sigma=float(rng.uniform(*sigma_range)),
+            seed=int(rng.integers(1 << 30)),
+            arrival=float(arr[i]),
+        ))
+    return out
+
+
+def summarize(results: List[SolveResult], tol: float, wall: float) -> dict:
+    lat = np.array([r.latency for r in results]) if results else np.zeros(1)
+    ok = sum(1 for r in results if r.success and r.true_rel_residual <= tol * 1.0001)
+    return {
+        "requests": len(results),
+        "reached_tol": ok,
+        "solves_per_s": ok / wall if wall > 0 else 0.0,
+        "p50_ms": float(np.percentile(lat, 50) * 1e3),
+        "p99_ms": float(np.percentile(lat, 99) * 1e3),
+        "mean_iterations": float(np.mean([r.iterations for r in results])) if results else 0.0,
+        "max_true_rel_residual": float(max((r.true_rel_residual for r in results), default=0.0)),
+        "wall_s": wall,
+    }

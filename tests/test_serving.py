@@ -1,0 +1,89 @@
+"""Solve server: request stream and admission logic (CPU), results alone vs in the mix (GPU)."""
+import numpy as np
+import pytest
+
+from igaops_amd.serving import SolveRequest, SolveResult, footprint_bytes, poisson_stream, summarize
+
+
+def test_poisson_stream_is_seeded_and_in_range():
+    a = poisson_stream(200, qps=10.0, seed=3)
+    b = poisson_stream(200, qps=10.0, seed=3)
+    assert [(r.geometry, r.degree, r.nel, r.sigma, r.seed, r.arrival) for r in a] == \
+        [(r.geometry, r.degree, r.nel, r.sigma, r.seed, r.arrival) for r in b]
+    arr = np.array([r.arrival for r in a])
+    assert arr[0] == 0.0 and np.all(np.diff(arr) >= 0)
+    # mean inter-arrival close to 1/qps
+    assert abs(np.mean(np.diff(arr)) - 0.1) < 0.03
+    assert {r.geometry for r in a} == {"cube", "annulus"}
+    assert all(2 <= r.degree <= 5 and 16 <= r.nel <= 96 and 0.2 <= r.sigma <= 1.0 for r in a)
+
+
+def test_footprint_grows_with_size_and_precision():
+    small = footprint_bytes(20 ** 3, 60 ** 3, False, "fp64", 300, 200)
+    big = footprint_bytes(60 ** 3, 180 ** 3, False, "fp64", 300, 200)
+    assert big > 20 * small
+    # float32 inner bases are half the size of float64 ones
+    assert footprint_bytes(60 ** 3, 180 ** 3, True, "mixed", 300, 200) < footprint_bytes(60 ** 3, 180 ** 3, True,
+                                                                                         "fp64", 300, 200)
+
+
+def test_summarize_counts_only_converged():
+    rs = [SolveResult(i, ("cube", 2, 4), 100, 10, 1e-11 if i % 2 else 1e-8, True, 0.0, 0.0, 0.1 * (i + 1))
+          for i in range(10)]
+    s = summarize(rs, 1e-10, 2.0)
+    assert s["requests"] == 10 and s["reached_tol"] == 5 and s["solves_per_s"] == 2.5
+    assert abs(s["p50_ms"] - 550.0) < 1e-6
+
+
+def _mix():
+    shapes = [("cube", 2, 6), ("annulus", 3, 5), ("cube", 3, 4), ("annulus", 2, 6), ("cube", 2, 6), ("annulus", 4, 4)]
+    return [SolveRequest(i, g, p, n, 0.3 + 0.1 * i, 100 + i) for i, (g, p, n) in enumerate(shapes)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_results_same_alone_and_in_mix(mode):
+    """Every request of a mixed batch (two of them of the same shape, overlapping on
+    the device) returns the same solution as when it is served alone."""
+    from igaops_amd.serving import SolveServer
+
+    print("case", mode, flush=True)
+    srv = SolveServer(mode=mode, max_active=6, keep_solutions=True, maxiters=200)
+    mix = {r.rid: r for r in srv.serve(_mix(), realtime=False)}
+    assert srv.stats["max_active"] >= 4
+    for req in _mix():
+        alone = SolveServer(mode=mode, max_active=1, keep_solutions=True, maxiters=200).serve([req],
+                                                                                            realtime=False)[0]
+        got = mix[req.rid]
+        assert got.success and alone.success
+        assert got.true_rel_residual <= 1e-10 and alone.true_rel_residual <= 1e-10
+        assert got.iterations == alone.iterations
+        a, b = alone.solution.cpu().numpy(), got.solution.cpu().numpy()
+        assert np.linalg.norm(a - b) <= 1e-12 * np.linalg.norm(a)
+
+
+@pytest.mark.gpu
+def test_device_coefficients_match_host():
+    import torch
+
+    from igaops_amd.geometry import lognormal_conductivity, quarter_annulus
+
+    patch = quarter_annulus(3, 5)
+    C, m = patch.conductivity_coefficients(lognormal_conductivity(patch, 0.7, seed=5))
+    Cd, md = patch.device_coefficients(0.7, 5)
+    np.testing.assert_allclose(Cd.cpu().numpy(), C, rtol=1e-12, atol=1e-14 * np.abs(C).max())
+    np.testing.assert_allclose(md.cpu().numpy(), m, rtol=1e-13)
+
+
+@pytest.mark.gpu
+def test_admission_respects_memory_budget():
+    from igaops_amd.serving import SolveServer
+
+    reqs = _mix()
+    srv = SolveServer(mode="fp64", max_active=6, maxiters=200)
+    one = max(srv.footprint(r) for r in reqs)
+    srv.budget = int(2.5 * one)
+    res = srv.serve(reqs, realtime=False)
+    assert len(res) == len(reqs) and all(r.true_rel_residual <= 1e-10 for r in res)
+    assert srv.stats["max_bytes"] <= srv.budget
+    assert srv.stats["deferred_for_memory"] > 0
