@@ -366,25 +366,35 @@ __device__ __attribute__((always_inline)) inline void fused_apply_body(const Fus
   for (int b = 0; b < PD; ++b)
     if (q3s + b < q3e) load_step(q3s + b, cbuf[b], fbuf[b]);
 
+  // the oldest test plane leaves the window, the next trial plane enters it
+  auto advance = [&]() __attribute__((always_inline)) {
+    const Trial t = plane_phase(true, cur, true, xnext);
+#pragma unroll
+    for (int j = 0; j < P - 1; ++j) {
+#pragma unroll
+      for (int g = 0; g < 3; ++g) {
+        te[g][j] = te[g][j + 1];
+        tr[g][j] = tr[g][j + 1];
+      }
+    }
+    te[0][P - 1] = te[1][P - 1] = te[2][P - 1] = T(0);
+    tr[0][P - 1] = t.v00;
+    tr[1][P - 1] = t.v10;
+    tr[2][P - 1] = t.v01;
+    ++cur;
+    // unconditional (clamped plane) so that the load queue has the same shape on every path
+    xnext = load_x(cur + P <= c3hi ? cur + P : c3hi);
+  };
   auto step = [&](int q3, T (&c)[NCF], int& fb) __attribute__((always_inline)) {
     const int f = fb;
-    while (cur < f) {
-      // the oldest test plane leaves the window, the next trial plane enters it
-      const Trial t = plane_phase(true, cur, true, xnext);
-#pragma unroll
-      for (int j = 0; j < P - 1; ++j) {
-#pragma unroll
-        for (int g = 0; g < 3; ++g) {
-          te[g][j] = te[g][j + 1];
-          tr[g][j] = tr[g][j + 1];
-        }
-      }
-      te[0][P - 1] = te[1][P - 1] = te[2][P - 1] = T(0);
-      tr[0][P - 1] = t.v00;
-      tr[1][P - 1] = t.v10;
-      tr[2][P - 1] = t.v01;
-      ++cur;
-      if (cur + P <= c3hi) xnext = load_x(cur + P);
+    // Band starts grow by at most one plane per q3 point for the rules in use, so the
+    // first advance is peeled: between two advances of that path at least one
+    // coefficient refill is issued, and the wait for the x block leaves the newest
+    // coefficient loads in flight.  Further advances of the same step (the loop) are
+    // rare and may drain the load queue.
+    if (cur < f) {
+      advance();
+      while (cur < f) advance();
     }
     if (active) {
       const T (&tb)[4][P] = b3s[q3 - q3s];
@@ -405,8 +415,9 @@ __device__ __attribute__((always_inline)) inline void fused_apply_body(const Fus
         te[2][j] += tb[2][j] * fv[2];
       }
     }
-    // refill this buffer with the plane PD steps ahead
-    if (q3 + PD < q3e) load_step(q3 + PD, c, fb);
+    // refill this buffer with the plane PD steps ahead (past the slab end: a valid plane,
+    // unused, so that every path issues the same loads)
+    load_step(q3 + PD < q3e ? q3 + PD : q3e - 1, c, fb);
   };
   for (int q3 = q3s; q3 < q3e; q3 += PD) {
 #pragma unroll
