@@ -112,6 +112,11 @@ template <typename T, int P, int FORM>
 __device__ __attribute__((always_inline)) inline void fused_apply_body(const FusedArgs<T>& a) {
   // prefetch depth (q3 planes) of the coefficient stream
   constexpr int PD = sizeof(T) == 8 ? 2 : 3;
+  // Unrolling the band-restricted contraction loops lets their LDS reads overlap; it
+  // pays where registers allow (measured: FP32, and FP64 from P = 6, whose instances
+  // run at two waves per SIMD anyway), and costs the FP64 P <= 5 instances their
+  // third wave.
+  constexpr bool kUnrollBand = sizeof(T) == 4 || P >= 6;
   __shared__ T xs[kDMax][kDMax + 1];
   __shared__ T t1s[2][kDMax][kTQ1];
   __shared__ T s2s[3][kTQ2][kTQ1];
@@ -270,10 +275,16 @@ __device__ __attribute__((always_inline)) inline void fused_apply_body(const Fus
         const int c2l = it / kTQ1, l = it % kTQ1;
         T r0 = T(0), r1 = T(0);
         const int mhi = m2r[c2l][1];
-        for (int m = m2r[c2l][0]; m < mhi; ++m) {
+        auto r1_term = [&](int m) __attribute__((always_inline)) {
           const int j = c2l - f2s[m];
           r0 += b2s[2][m][j] * s2s[0][m][l] + b2s[3][m][j] * s2s[2][m][l];
           r1 += b2s[2][m][j] * s2s[1][m][l];
+        };
+        if constexpr (kUnrollBand) {
+#pragma unroll 4
+          for (int m = m2r[c2l][0]; m < mhi; ++m) r1_term(m);
+        } else {
+          for (int m = m2r[c2l][0]; m < mhi; ++m) r1_term(m);
         }
         r1s[0][c2l][l] = r0;
         r1s[1][c2l][l] = r1;
@@ -304,9 +315,15 @@ __device__ __attribute__((always_inline)) inline void fused_apply_body(const Fus
           const int c2l = xij[e] >> 8, c1l = xij[e] & 255;
           T acc = T(0);
           const int lhi = l1r[c1l][1];
-          for (int l = l1r[c1l][0]; l < lhi; ++l) {
+          auto flush_term = [&](int l) __attribute__((always_inline)) {
             const int j = c1l - f1s[l];
             acc += b1s[2][l][j] * r1s[0][c2l][l] + b1s[3][l][j] * r1s[1][c2l][l];
+          };
+          if constexpr (kUnrollBand) {
+#pragma unroll 4
+            for (int l = l1r[c1l][0]; l < lhi; ++l) flush_term(l);
+          } else {
+            for (int l = l1r[c1l][0]; l < lhi; ++l) flush_term(l);
           }
           blk[tid + e * kThreads] = acc;
         }
