@@ -222,7 +222,8 @@ def spacetime_section(steps=5, nrhs_seq=10, tol=1e-10, space=(3, 32), time_=(2, 
         "nq": nq,
         "matvec_ms": mv,
         "matvec_gdofs": n / mv / 1e6,
-        "matvec_resident_fields": len(op.terms),
+        "matvec_path": "fused, sliced in time" if op.fused else "composed",
+        "matvec_coefficient_fields": op.ncoef_fields,
         "matvec_pct_hbm_roofline": op.algorithmic_bytes(1) / (mv * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "fd_apply_ms": pc,
         "gcrodr_iterations": its,

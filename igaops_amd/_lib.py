@@ -57,6 +57,9 @@ SIGNATURES = {
     "iga_fd_workspace_bytes": (_i64, [_vp, _i32]),
     "iga_fd_apply": (_i32, [_vp, _vp, _vp, _i32, _vp, _vp]),
     "iga_fd_destroy": (_i32, [_vp]),
+    "iga_fused_apply_sliced": (_i32, [_vp, _vp, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _i32, _vp, C.c_double, _i32,
+                                      _vp, _vp]),
+    "iga_fused_pack_strided": (_i32, [_vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp]),
     "iga_gmres_state_doubles": (_i64, [_i32]),
     "iga_gmres_start": (_i32, [_i32, _i64, _i32, _vp, _vp, _vp, _vp, _vp]),
     "iga_gmres_step": (_i32, [_i32, _i64, _i32, _i32, _vp, _i64, _vp, _vp, _vp, _vp]),

@@ -55,6 +55,9 @@ struct FusedArgs {
   const int64_t* coef_base;           // [nt1*nt2]
   const int64_t* part_base;           // [nt1*nt2*nslab]
   int64_t part_stride;                // elements per right-hand side
+  // right-hand side k reads x[dof * x_dstride + k * x_rstride] and the coefficients
+  // at coef + k * coef_rstride (0: one coefficient set shared by all of them)
+  int64_t x_dstride, x_rstride, coef_rstride;
   const T* coef;
   const T* x;
   const T* w;  // NURBS weights or nullptr
@@ -150,7 +153,6 @@ __device__ __attribute__((always_inline)) inline void fused_apply_body(const Fus
   const int c3lo = a.s_c0[s];
   const bool active = (l1 < q1n) && (l2 < q2n);
   const int n1 = a.nc[0], n2 = a.nc[1];
-  const int nr = a.nrhs;
 
   // stage the direction-1 and -2 tables of this tile
   for (int i = tid; i < 4 * kTQ1 * P; i += kThreads) {
@@ -212,7 +214,8 @@ __device__ __attribute__((always_inline)) inline void fused_apply_body(const Fus
   // compiler does not wait on them at the point of issue; NURBS weights are kept
   // apart and applied when the block is used.
   constexpr int XR = (kDMax * kDMax + kThreads - 1) / kThreads;
-  const T* xk = a.x + k;
+  const T* xk = a.x + (int64_t)k * a.x_rstride;
+  const int64_t xds = a.x_dstride;
   const bool nurbs = a.w != nullptr;
   // Per-thread offsets of its x-block elements within a DOF plane and their LDS
   // slots, computed once (no division in the sweep).  Lanes without an element
@@ -241,7 +244,7 @@ __device__ __attribute__((always_inline)) inline void fused_apply_body(const Fus
 #pragma unroll
     for (int e = 0; e < XR; ++e) {
       const int64_t dof = xoff[e] + plane * c3;
-      b.x[e] = xk[dof * nr];
+      b.x[e] = xk[dof * xds];
       b.w[e] = nurbs ? a.w[dof] : T(1);
     }
     return b;
@@ -365,7 +368,7 @@ __device__ __attribute__((always_inline)) inline void fused_apply_body(const Fus
 
   const int npts = q1n * q2n;
   // inactive lanes read the tile's first point (a valid address) and discard it
-  const T* cptr = a.coef + a.coef_base[tile] + (active ? (l1 + q1n * l2) : 0);
+  const T* cptr = a.coef + (int64_t)k * a.coef_rstride + a.coef_base[tile] + (active ? (l1 + q1n * l2) : 0);
   constexpr int NCF = form_ncoef(FORM);
   const int64_t step_stride = (int64_t)NCF * npts;
   // Coefficients of plane q3 are loaded PD planes ahead into a ring of buffers;
@@ -480,6 +483,7 @@ struct GatherArgs {
   const T* partial;
   const T* w;
   T* y;
+  int64_t y_dstride, y_rstride;  // y[dof * y_dstride + k * y_rstride]
   T alpha;
   int accumulate;
 };
@@ -490,8 +494,11 @@ __global__ void __launch_bounds__(256) fused_gather_kernel(GatherArgs<T> g) {
   const int64_t total = ndof * g.nrhs;
   for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
        idx += (int64_t)gridDim.x * blockDim.x) {
-    const int k = (int)(idx % g.nrhs);
-    const int64_t dof = idx / g.nrhs;
+    // right-hand side fastest for interleaved y, DOF fastest for sliced y (coalesced stores)
+    const bool interleaved = g.y_rstride == 1;
+    const int k = interleaved ? (int)(idx % g.nrhs) : (int)(idx / ndof);
+    const int64_t dof = interleaved ? idx / g.nrhs : idx % ndof;
+    const int64_t yi = dof * g.y_dstride + (int64_t)k * g.y_rstride;
     const int c1 = (int)(dof % g.nc[0]);
     const int c2 = (int)((dof / g.nc[0]) % g.nc[1]);
     const int c3 = (int)(dof / ((int64_t)g.nc[0] * g.nc[1]));
@@ -512,8 +519,8 @@ __global__ void __launch_bounds__(256) fused_gather_kernel(GatherArgs<T> g) {
     }
     T v = g.alpha * acc;
     if (g.w) v *= g.w[dof];
-    if (g.accumulate) v += g.y[idx];
-    g.y[idx] = v;
+    if (g.accumulate) v += g.y[yi];
+    g.y[yi] = v;
   }
 }
 
@@ -533,6 +540,7 @@ struct PackArgs {
   const double* m;
   const double* W;
   const double* dW;
+  int64_t comp_stride;  // elements between the components of C and of dW
   T* out;
 };
 
@@ -553,12 +561,12 @@ __global__ void fused_pack_kernel(PackArgs<T> p) {
     double c[3][3] = {{0}};
     if (p.C)
       for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) c[i][j] = p.C[(int64_t)(i * 3 + j) * nqt + q];
+        for (int j = 0; j < 3; ++j) c[i][j] = p.C[(int64_t)(i * 3 + j) * p.comp_stride + q];
     const double mm = p.m ? p.m[q] : 0.0;
     if (p.W) {
       const double wi = 1.0 / p.W[q];
       double z[3];
-      for (int i = 0; i < 3; ++i) z[i] = p.dW[(int64_t)i * nqt + q] * wi * wi;
+      for (int i = 0; i < 3; ++i) z[i] = p.dW[(int64_t)i * p.comp_stride + q] * wi * wi;
       double cz[3] = {0, 0, 0}, ctz[3] = {0, 0, 0}, zcz = 0.0;
       for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) {
@@ -757,10 +765,19 @@ static void launch_apply(const iga_fused_s* f, const FusedArgs<T>& a, int nblk, 
   }
 }
 
+// Where right-hand side k lives: x[dof * xd + k * xr], y[dof * yd + k * yr], its
+// coefficients at coef + k * cr.
+struct Layout {
+  int64_t xd, xr, yd, yr, cr;
+  static Layout interleaved(int nrhs) { return Layout{nrhs, 1, nrhs, 1, 0}; }
+};
+
 // stages: bit 1 = main sweep (partial blocks), bit 2 = gather into y
 template <typename T>
 static void run_apply(const iga_fused_s* f, const void* coef, const void* x, void* y, int nrhs, const void* w,
-                      double alpha, int accumulate, void* work, hipStream_t s, int stages = 3) {
+                      double alpha, int accumulate, void* work, hipStream_t s, int stages = 3,
+                      Layout lay = Layout{0, 0, 0, 0, 0}) {
+  if (lay.xd == 0) lay = Layout::interleaved(nrhs);
   if (stages & 1) {
   FusedArgs<T> a{};
   for (int d = 0; d < 3; ++d) {
@@ -779,6 +796,9 @@ static void run_apply(const iga_fused_s* f, const void* coef, const void* x, voi
   a.coef_base = f->d_coef_base;
   a.part_base = f->d_part_base;
   a.part_stride = f->part_stride;
+  a.x_dstride = lay.xd;
+  a.x_rstride = lay.xr;
+  a.coef_rstride = lay.cr;
   a.coef = (const T*)coef;
   a.x = (const T*)x;
   a.w = (const T*)w;
@@ -808,6 +828,8 @@ static void run_apply(const iga_fused_s* f, const void* coef, const void* x, voi
   g.partial = (const T*)work;
   g.w = (const T*)w;
   g.y = (T*)y;
+  g.y_dstride = lay.yd;
+  g.y_rstride = lay.yr;
   g.alpha = (T)alpha;
   g.accumulate = accumulate;
   const int64_t total = (int64_t)f->nc[0] * f->nc[1] * f->nc[2] * nrhs;
@@ -817,7 +839,7 @@ static void run_apply(const iga_fused_s* f, const void* coef, const void* x, voi
 
 template <typename T>
 static void run_pack(const iga_fused_s* f, const double* C, const double* m, const double* W, const double* dW,
-                     void* out, hipStream_t s) {
+                     void* out, hipStream_t s, int64_t comp_stride = 0) {
   PackArgs<T> p{};
   for (int d = 0; d < 3; ++d) p.nq[d] = f->nq[d];
   p.nt1 = f->nt1;
@@ -834,6 +856,7 @@ static void run_pack(const iga_fused_s* f, const double* C, const double* m, con
   p.dW = dW;
   p.out = (T*)out;
   const int64_t nqt = (int64_t)f->nq[0] * f->nq[1] * f->nq[2];
+  p.comp_stride = comp_stride > 0 ? comp_stride : nqt;
   fused_pack_kernel<T><<<grid_for(nqt, 256), 256, 0, s>>>(p);
   IGA_LAUNCH_CHECK();
 }
@@ -1011,6 +1034,38 @@ int iga_fused_apply_stage(iga_fused_t f, int stage, const void* packed, const vo
       run_apply<double>(f, packed, x, y, nrhs, nurbs_w, alpha, accumulate, work, as_stream(stream), stage);
     else
       run_apply<float>(f, packed, x, y, nrhs, nurbs_w, alpha, accumulate, work, as_stream(stream), stage);
+  });
+}
+
+int iga_fused_pack_strided(iga_fused_t f, const double* C, const double* m, const double* W, const double* dW,
+                           int64_t comp_stride, void* packed, void* stream) {
+  return guard([&] {
+    IGA_CHECK_ARG(f && packed, "NULL argument");
+    IGA_CHECK_ARG((W == nullptr) == (dW == nullptr), "W and dW go together");
+    if (f->form != FORM_M1) IGA_CHECK_ARG(C, "stiffness forms need C");
+    if (f->form == FORM_M1 || f->form == FORM_KM7) IGA_CHECK_ARG(m, "mass forms need m");
+    const int64_t nqt = (int64_t)f->nq[0] * f->nq[1] * f->nq[2];
+    IGA_CHECK_ARG(comp_stride >= nqt, "component stride below the number of points");
+    if (f->dtype == IGA_F64)
+      run_pack<double>(f, C, m, W, dW, packed, as_stream(stream), comp_stride);
+    else
+      run_pack<float>(f, C, m, W, dW, packed, as_stream(stream), comp_stride);
+  });
+}
+
+int iga_fused_apply_sliced(iga_fused_t f, const void* packed, int64_t coef_slice_stride, const void* x,
+                           int64_t x_dstride, int64_t x_sstride, void* y, int64_t y_dstride, int64_t y_sstride,
+                           int nslices, const void* nurbs_w, double alpha, int accumulate, void* work, void* stream) {
+  return guard([&] {
+    IGA_CHECK_ARG(f && packed && x && y && work, "NULL argument");
+    IGA_CHECK_ARG(nslices >= 1, "nslices must be >= 1");
+    IGA_CHECK_ARG(x_dstride >= 1 && y_dstride >= 1 && x_sstride >= 0 && y_sstride >= 0 && coef_slice_stride >= 0,
+                  "bad strides");
+    const Layout lay{x_dstride, x_sstride, y_dstride, y_sstride, coef_slice_stride};
+    if (f->dtype == IGA_F64)
+      run_apply<double>(f, packed, x, y, nslices, nurbs_w, alpha, accumulate, work, as_stream(stream), 3, lay);
+    else
+      run_apply<float>(f, packed, x, y, nslices, nurbs_w, alpha, accumulate, work, as_stream(stream), 3, lay);
   });
 }
 

@@ -385,6 +385,24 @@ class FusedPlan:
                   dv.ptr(nurbs_w), float(alpha), int(bool(accumulate)), dv.ptr(work), dv.stream_ptr(stream))
         return y
 
+    def pack_strided(self, C_ptr: int, m_ptr: int, comp_stride: int, out, out_offset: int = 0, stream=None):
+        """Pack one slice of larger coefficient arrays (raw device addresses of its
+        first component; components ``comp_stride`` elements apart) into
+        ``out[out_offset:]``."""
+        es = out.element_size()
+        _lib.call("iga_fused_pack_strided", self._h, int(C_ptr), int(m_ptr), 0, 0, int(comp_stride),
+                  dv.ptr(out) + int(out_offset) * es, dv.stream_ptr(stream))
+        return out
+
+    def apply_sliced(self, packed, coef_slice_stride: int, x_ptr: int, x_dstride: int, x_sstride: int, y_ptr: int,
+                     y_dstride: int, y_sstride: int, nslices: int, alpha=1.0, accumulate=False, work=None,
+                     stream=None):
+        """``nslices`` 3-D applies with per-slice coefficients and strided vectors."""
+        work = work if work is not None else self.workspace(nslices)
+        _lib.call("iga_fused_apply_sliced", self._h, dv.ptr(packed), int(coef_slice_stride), int(x_ptr),
+                  int(x_dstride), int(x_sstride), int(y_ptr), int(y_dstride), int(y_sstride), int(nslices), 0,
+                  float(alpha), int(bool(accumulate)), dv.ptr(work), dv.stream_ptr(stream))
+
     def apply(self, packed, x, y, nrhs: int = 1, nurbs_w=None, alpha=1.0, accumulate=False, work=None, stream=None):
         work = work if work is not None else self.workspace(nrhs)
         _lib.call(

@@ -191,24 +191,49 @@ class SpaceTimeOperator:
     device, coefficients resident in HBM.
 
     ``c`` is an ``(N_q,)`` array or a scalar; ``C`` a ``(d, d, N_q)`` array or a
-    ``(d, d)`` matrix of constants (``d`` space directions).  Coefficient fields that
-    are identically zero are left out of the sum (they contribute nothing)."""
+    ``(d, d)`` matrix of constants (``d`` space directions).
 
-    def __init__(self, quadrule_list, c=1.0, C=None, dtype: str = "f64"):
-        from .operators import Term, terms_gradu_gradv, terms_u_v
+    With three space directions whose rules suit the fused sweep, the operator is
+    applied slice by slice in time: the time trial bases ``B0_t``, ``B1_t`` act first
+    (one banded mode product), then, for every time quadrature point, the fused 3-D
+    sweep with that point's coefficients (stiffness on the ``B0_t`` slice, mass on
+    the ``B1_t`` slice; both terms test with ``W0_t``), and finally the time test
+    weights ``W0_t``.  Otherwise (and with ``fused=False``) the composed device path
+    evaluates the sum of terms; there, identically zero fields are left out."""
 
-        t = dv.torch()
+    def __init__(self, quadrule_list, c=1.0, C=None, dtype: str = "f64", fused: Optional[bool] = None):
+        from .operators import fused_applicable
+
         self.quadrule_list = list(quadrule_list)
         nd = len(self.quadrule_list)
-        sd = nd - 1
+        self.sd = nd - 1
         self.nq = int(np.prod([q.nbquadpts for q in self.quadrule_list], dtype=np.int64))
         self.ndof = int(np.prod([q.nbctrlpts for q in self.quadrule_list], dtype=np.int64))
         self.dtype = dtype
-        tdt = t.float64 if dtype == "f64" else t.float32
         if C is None:
-            C = np.eye(sd)
+            C = np.eye(self.sd)
+        C = C if isinstance(C, dv.torch().Tensor) else np.asarray(C, dtype=np.float64)
+        if tuple(C.shape[:2]) != (self.sd, self.sd):
+            raise ValueError("C must be (d, d) or (d, d, N_q) over the space directions")
+        can_fuse = nd == 4 and fused_applicable(self.quadrule_list[:3])
+        self.fused = can_fuse if fused is None else (bool(fused) and can_fuse)
+        if fused and not can_fuse:
+            raise NotImplementedError("the sliced fused path needs three space directions with fused-capable rules")
+        if self.fused:
+            self._build_fused(c, C)
+        else:
+            self._build_composed(c, C)
+
+    # -- composed ---------------------------------------------------------------
+    def _build_composed(self, c, C):
+        from .operators import terms_gradu_gradv, terms_u_v
+
+        t = dv.torch()
+        tdt = dv.torch_dtype(self.dtype)
 
         def field(v):
+            if isinstance(v, t.Tensor):
+                return v.to(device=dv.device(), dtype=tdt).contiguous()
             v = np.asarray(v, dtype=np.float64)
             if v.ndim == 0:
                 if float(v) == 0.0:
@@ -218,27 +243,102 @@ class SpaceTimeOperator:
                 raise ValueError("coefficient field does not match the quadrature points")
             if not np.any(v != 0.0):
                 return None
-            return dv.to_device(v, dtype)
+            return dv.to_device(v, self.dtype)
 
-        C = np.asarray(C, dtype=np.float64)
-        if C.shape[:2] != (sd, sd):
-            raise ValueError("C must be (d, d) or (d, d, N_q) over the space directions")
+        sd = self.sd
         cfields = [[field(C[i, j]) for j in range(sd)] for i in range(sd)]
         terms = [tm for tm in terms_u_v(self.quadrule_list, field(c), (0, 1)) if tm.coef is not None]
         terms += [tm for tm in terms_gradu_gradv(self.quadrule_list, cfields, (0, 0)) if tm.coef is not None]
-        self.terms: List[Term] = terms
+        self.terms = terms
+        self.ncoef_fields = len(terms)
+
+    # -- fused, sliced in time -----------------------------------------------------
+    def _build_fused(self, c, C):
+        from .operators import FORM_GEN16, FORM_K6, FORM_M1, band_of, fused_plan
+
+        t = dv.torch()
+        qs, qt = self.quadrule_list[:3], self.quadrule_list[3]
+        self.nqs = int(np.prod([q.nbquadpts for q in qs], dtype=np.int64))
+        self.ncs = int(np.prod([q.nbctrlpts for q in qs], dtype=np.int64))
+        self.nqt, self.nct = int(qt.nbquadpts), int(qt.nbctrlpts)
+        dev = dv.device()
+        # coefficient sources: per-slice device arrays (components comp_stride apart)
+        if isinstance(C, t.Tensor) or np.ndim(C) == 3:
+            Cd = dv.to_device(C, "f64").reshape(3, 3, -1)
+            if Cd.shape[2] != self.nq:
+                raise ValueError("coefficient field does not match the quadrature points")
+            c_stride, c_slice = self.nq, self.nqs
+            sym = bool(t.equal(Cd, Cd.transpose(0, 1)))
+        else:
+            Cd = t.from_numpy(np.ascontiguousarray(np.broadcast_to(np.asarray(C, float)[:, :, None],
+                                                                   (3, 3, self.nqs)))).to(dev)
+            c_stride, c_slice = self.nqs, 0
+            sym = bool(np.allclose(C, np.asarray(C).T, rtol=0, atol=0))
+        if isinstance(c, t.Tensor) or np.ndim(c) == 1:
+            md = dv.to_device(c, "f64").reshape(-1)
+            if md.numel() != self.nq:
+                raise ValueError("coefficient field does not match the quadrature points")
+            m_slice = self.nqs
+        else:
+            md = t.full((self.nqs,), float(c), dtype=t.float64, device=dev)
+            m_slice = 0
+        nslab = -(-int(qs[2].nbquadpts) // 64)  # fewest slabs: many slices fill the device
+        self.planK = fused_plan(qs, FORM_K6 if sym else FORM_GEN16, self.dtype, nslab)
+        self.planM = fused_plan(qs, FORM_M1, self.dtype, nslab)
+        self.packK = dv.empty(self.planK.coef_elems * self.nqt, self.dtype)
+        self.packM = dv.empty(self.planM.coef_elems * self.nqt, self.dtype)
+        zero_m = t.zeros(self.nqs, dtype=t.float64, device=dev) if not sym else None
+        for s in range(self.nqt):
+            cp = dv.ptr(Cd) + 8 * s * c_slice
+            mp = dv.ptr(md) + 8 * s * m_slice
+            self.planK.pack_strided(cp, dv.ptr(zero_m) if zero_m is not None else 0, c_stride, self.packK,
+                                    s * self.planK.coef_elems)
+            self.planM.pack_strided(0, mp, self.nqs, self.packM, s * self.planM.coef_elems)
+        self.time_trial = band_of(qt.basis[0], qt.basis[1])
+        self.time_test = band_of(qt.weights[0])
+        self.ncoef_fields = (6 if sym else 16) + 1
+        self._work = {}
+
+    def _buffers(self, nrhs: int):
+        buf = self._work.get(nrhs)
+        if buf is None:
+            t = dv.torch()
+            tdt = dv.torch_dtype(self.dtype)
+            n = self.nqt * self.ncs * nrhs
+            buf = (t.empty(n, dtype=tdt, device=dv.device()), t.empty(n, dtype=tdt, device=dv.device()),
+                   t.empty(n, dtype=tdt, device=dv.device()), self.planK.workspace(self.nqt),
+                   self.planM.workspace(self.nqt))
+            self._work[nrhs] = buf
+        return buf
 
     def apply_device(self, x, out=None, nrhs: int = 1, stream=None):
-        from .operators import composed_apply
+        if not self.fused:
+            from .operators import composed_apply
 
-        return composed_apply(self.quadrule_list, self.terms, x, nrhs, out=out, stream=stream)
+            return composed_apply(self.quadrule_list, self.terms, x, nrhs, out=out, stream=stream)
+        t = dv.torch()
+        X0, X1, Y, wK, wM = self._buffers(nrhs)
+        # time trial bases: (n_c,t, N_s, r) -> (n_q,t, N_s, r) twice
+        self.time_trial.apply(x, [X0, X1], 1, self.ncs * nrhs, stream=stream)
+        es = X0.element_size()
+        for r in range(nrhs):
+            xd, xs = nrhs, self.ncs * nrhs
+            off = r * es
+            self.planK.apply_sliced(self.packK, self.planK.coef_elems, dv.ptr(X0) + off, xd, xs, dv.ptr(Y) + off,
+                                    xd, xs, self.nqt, work=wK, stream=stream)
+            self.planM.apply_sliced(self.packM, self.planM.coef_elems, dv.ptr(X1) + off, xd, xs, dv.ptr(Y) + off,
+                                    xd, xs, self.nqt, accumulate=True, work=wM, stream=stream)
+        if out is None:
+            out = t.empty(self.ndof * nrhs, dtype=x.dtype, device=x.device)
+        self.time_test.apply(Y, [out], 1, self.ncs * nrhs, stream=stream)
+        return out
 
     apply = apply_device
 
     def algorithmic_bytes(self, nrhs: int = 1) -> int:
-        """``b (2 N_dof r + c N_q)`` with ``c`` = the resident coefficient fields."""
+        """``b (2 N_dof r + c N_q)``, c = 7 (6 symmetric stiffness + 1 mass fields)."""
         b = 8 if self.dtype == "f64" else 4
-        return b * (2 * self.ndof * nrhs + len(self.terms) * self.nq)
+        return b * (2 * self.ndof * nrhs + 7 * self.nq)
 
 
 class BoxMasked:

@@ -205,6 +205,26 @@ int iga_fused_apply_stage(iga_fused_t plan, int stage, const void* packed, const
                           int nrhs, const void* nurbs_w, double alpha, int accumulate, void* work,
                           void* stream);
 
+/* Space-time form of the fused operator: nslices independent 3-D applies, slice s
+ * reading x[dof * x_dstride + s * x_sstride] and its own packed coefficients at
+ * packed + s * coef_slice_stride (elements; 0 = shared), writing
+ * y[dof * y_dstride + s * y_sstride].  With time the slowest direction, the slices
+ * are the time quadrature points of the space-time terms of
+ * BsplineOperations.compute_mf_scalar_{u_v,gradu_gradv}(..., time_ders=...)
+ * (operators/bsplines_ops.py:290-355): the time trial basis is applied before and
+ * the time test weights after (iga_band_mode_product).  work: bytes of
+ * iga_fused_workspace_bytes(plan, nslices). */
+int iga_fused_apply_sliced(iga_fused_t plan, const void* packed, int64_t coef_slice_stride, const void* x,
+                           int64_t x_dstride, int64_t x_sstride, void* y, int64_t y_dstride, int64_t y_sstride,
+                           int nslices, const void* nurbs_w, double alpha, int accumulate, void* work,
+                           void* stream);
+
+/* iga_fused_pack with the components of C (and dW) comp_stride elements apart
+ * (>= the plan's number of points): packs one time slice of space-time coefficient
+ * arrays (3, 3, N_q_space * n_q_time) without a copy. */
+int iga_fused_pack_strided(iga_fused_t plan, const double* C, const double* m, const double* W,
+                           const double* dW, int64_t comp_stride, void* packed, void* stream);
+
 /* ---- fast-diagonalization apply --------------------------------------------
  * Replaces SpaceFD.apply (fastdiag/utils/space_precond.py:273-320) for one DOF per
  * node: y[free] = U (Lambda^{-1} o U^T x[free]), y = 0 outside the free box, where

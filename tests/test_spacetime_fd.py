@@ -126,9 +126,11 @@ def test_spacetime_fd_inverts_separable_operator_on_device():
 
 
 @pytest.mark.gpu
-def test_spacetime_operator_matches_oracle():
+@pytest.mark.parametrize("fused", [True, False])
+def test_spacetime_operator_matches_oracle(fused):
     """Heat operator u_v(time_ders=(0,1)) + gradu_gradv(time_ders=(0,0)): device
-    coefficient fields and constant coefficients against the oracle, two right-hand sides."""
+    coefficient fields (symmetric and not) and constant coefficients against the
+    oracle, two right-hand sides, sliced fused path and composed path."""
     import torch
 
     from igaops_amd.spacetime import SpaceTimeOperator
@@ -143,8 +145,12 @@ def test_spacetime_operator_matches_oracle():
     G = rng.standard_normal((3, 3, nq)) * 0.2
     Cf = np.einsum("ikq,jkq->ijq", G, G) + np.eye(3)[:, :, None]
     X = rng.standard_normal((n, 2))
-    for cc, CC in ((c, Cf), (1.0, np.eye(3))):
-        op = SpaceTimeOperator(ql, cc, CC)
+    A = rng.standard_normal((3, 3, nq)) * 0.1
+    Cn = Cf + (A - A.transpose(1, 0, 2))
+    for cc, CC in ((c, Cf), (c, Cn), (1.0, np.eye(3))):
+        print("case", fused, np.ndim(cc), CC.ndim, flush=True)
+        op = SpaceTimeOperator(ql, cc, CC, fused=fused)
+        assert op.fused == fused
         y = op.apply_device(torch.as_tensor(X, device="cuda").reshape(-1), nrhs=2).reshape(n, 2).cpu().numpy()
         ca = np.broadcast_to(np.asarray(cc, float), (nq,)).copy()
         Ca = CC if CC.ndim == 3 else np.broadcast_to(CC[:, :, None], (3, 3, nq)).copy()
