@@ -118,7 +118,7 @@ def solve_section(patch, C, m, nrhs, tol=1e-10):
                                    tol, 400, mask=mask, restart=None, use_increment=False)
             else:
                 out = mixed_solve(lambda x, o: A64.apply(x, out=o), lambda x, o: A32.apply(x, out=o),
-                                  lambda x, o: fd.apply_device(x, out=o), b, tol=tol, inner_tol=1e-5, mask=mask)
+                                  lambda x, o: fd.apply_device(x, out=o), b, tol=tol, inner_tol=1e-6, mask=mask)
             torch.cuda.synchronize()
             lat.append(time.perf_counter() - t0)
             its.append(out.iterations)

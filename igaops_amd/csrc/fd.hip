@@ -72,39 +72,70 @@ __global__ void __launch_bounds__(256) mfma_gemm_kernel(GemmDesc g, const T* __r
 #pragma unroll
     for (int b = 0; b < 2; ++b) acc[a][b] = V{0, 0, 0, 0};
 
-  for (int k0 = 0; k0 < g.K; k0 += kKStep) {
-    __syncthreads();
-    // L tile: 64 rows x 16 k  (k fastest among the threads when lk == 1)
-    for (int e = tid; e < kTile * kKStep; e += 256) {
-      const int mm = e / kKStep, kk = e % kKStep;
-      const int m = m0 + mm, k = k0 + kk;
-      ls[mm][kk] = (m < g.M && k < g.K) ? Lb[m * g.lm + k * g.lk] : T(0);
+  // Each thread stages 4 elements of the L tile (64 x 16) and 4 of the R tile
+  // (16 x 64) per K step.  Their loads are issued kPre steps ahead into a register
+  // ring, from clamped (always valid) addresses on every path, and zeroed outside
+  // the matrix when written to LDS; so the K loop waits on one load latency, not
+  // on one per step.
+  constexpr int kPre = 4;
+  const int nsteps = (g.K + kKStep - 1) / kKStep;
+  int l_mm[4], l_kk[4], r_kk[4], r_jj[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const int idx = tid + e * 256;
+    l_mm[e] = idx / kKStep;
+    l_kk[e] = idx % kKStep;
+    if (g.rj == 1) {
+      r_kk[e] = idx / kTile;
+      r_jj[e] = idx % kTile;
+    } else {
+      r_jj[e] = idx / kKStep;
+      r_kk[e] = idx % kKStep;
     }
-    // R tile: 16 k x 64 columns (columns fastest among the threads when rj == 1)
-    for (int e = tid; e < kTile * kKStep; e += 256) {
-      int kk, jj;
-      if (g.rj == 1) {
-        kk = e / kTile;
-        jj = e % kTile;
-      } else {
-        jj = e / kKStep;
-        kk = e % kKStep;
+  }
+  T lreg[kPre][4], rreg[kPre][4];
+  auto load_step = [&](int s, T (&lr)[4], T (&rr)[4]) __attribute__((always_inline)) {
+    const int k0 = (s < nsteps ? s : nsteps - 1) * kKStep;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int m = min(m0 + l_mm[e], g.M - 1), k = min(k0 + l_kk[e], g.K - 1);
+      lr[e] = Lb[(int64_t)m * g.lm + (int64_t)k * g.lk];
+      const int kr = min(k0 + r_kk[e], g.K - 1), j = min(j0 + r_jj[e], g.N - 1);
+      rr[e] = Rb[(int64_t)kr * g.rk + (int64_t)j * g.rj];
+    }
+  };
+#pragma unroll
+  for (int b = 0; b < kPre; ++b) load_step(b, lreg[b], rreg[b]);
+
+  for (int s0 = 0; s0 < nsteps; s0 += kPre) {
+#pragma unroll
+    for (int b = 0; b < kPre; ++b) {
+      const int st = s0 + b;
+      if (st < nsteps) {
+        const int k0 = st * kKStep;
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const bool lok = (m0 + l_mm[e] < g.M) && (k0 + l_kk[e] < g.K);
+          ls[l_mm[e]][l_kk[e]] = lok ? lreg[b][e] : T(0);
+          const bool rok = (k0 + r_kk[e] < g.K) && (j0 + r_jj[e] < g.N);
+          rs[r_kk[e]][r_jj[e]] = rok ? rreg[b][e] : T(0);
+        }
+        __syncthreads();
+        load_step(st + kPre, lreg[b], rreg[b]);
+#pragma unroll
+        for (int k4 = 0; k4 < kKStep; k4 += 4) {
+          T av[2], bv[2];
+#pragma unroll
+          for (int a = 0; a < 2; ++a) av[a] = ls[wm + a * 16 + (lane & 15)][k4 + (lane >> 4)];
+#pragma unroll
+          for (int c = 0; c < 2; ++c) bv[c] = rs[k4 + (lane >> 4)][wn + c * 16 + (lane & 15)];
+#pragma unroll
+          for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) acc[a][c] = Acc<T>::mfma(av[a], bv[c], acc[a][c]);
+        }
       }
-      const int k = k0 + kk, j = j0 + jj;
-      rs[kk][jj] = (k < g.K && j < g.N) ? Rb[k * g.rk + j * g.rj] : T(0);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k4 = 0; k4 < kKStep; k4 += 4) {
-      T av[2], bv[2];
-#pragma unroll
-      for (int a = 0; a < 2; ++a) av[a] = ls[wm + a * 16 + (lane & 15)][k4 + (lane >> 4)];
-#pragma unroll
-      for (int b = 0; b < 2; ++b) bv[b] = rs[k4 + (lane >> 4)][wn + b * 16 + (lane & 15)];
-#pragma unroll
-      for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b) acc[a][b] = Acc<T>::mfma(av[a], bv[b], acc[a][b]);
     }
   }
 #pragma unroll

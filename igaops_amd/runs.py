@@ -277,7 +277,7 @@ class RefinementRun:
     ``||r|| / ||b|| <= tol``; the inner solves use float32 vectors and operators."""
 
     def __init__(self, A64: Callable, A_lo: Callable, P_lo: Callable, b, tol: float = 1e-10,
-                 inner_tol: float = 1e-5, inner_maxiters: int = 200, max_refinements: int = 30, mask=None,
+                 inner_tol: float = 1e-6, inner_maxiters: int = 200, max_refinements: int = 30, mask=None,
                  stream=None, lookahead: int = 2, lo_dtype: str = "f32"):
         t = dv.torch()
         self.A64, self.A_lo, self.P_lo, self.mask = A64, A_lo, P_lo, mask
@@ -320,7 +320,12 @@ class RefinementRun:
         with self._on_stream():
             _lib.call("iga_cast", _lib.IGA_F64, dv.DTYPES[self.lo_dtype], self.n, dv.ptr(self.r), dv.ptr(self.r_lo),
                       dv.stream_ptr(self.stream))
-        self.inner = GmresRun(self.A_lo, self.P_lo, self.r_lo, self.inner_tol, self.inner_maxiters, mask=self.mask,
+        # each inner solve only needs to shrink the residual by what is left to reach
+        # tol (with a factor 2 of margin), but not below inner_tol, the accuracy a
+        # float32 solve can deliver
+        rel = self.hist[-1] if self.hist else 1.0
+        itol = min(1e-1, max(self.inner_tol, 0.5 * self.tol / max(rel, 1e-300)))
+        self.inner = GmresRun(self.A_lo, self.P_lo, self.r_lo, itol, self.inner_maxiters, mask=self.mask,
                               stream=self.stream, lookahead=self.lookahead, dtype=self.lo_dtype)
         self.phase = "inner"
 
