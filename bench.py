@@ -85,6 +85,28 @@ def cpu_baseline(patch, C, m, x, budget_s=20.0):
     return y, float(np.median(times)), reps
 
 
+def fd_error_vs_oracle(patch, dtype, x):
+    """Relative error of the FD apply (Dirichlet on all faces, ``Lambda = M + K``
+    eigenvalues) in ``dtype`` against the oracle's float64 apply on ``x``."""
+    import torch
+
+    from igaops_amd.fastdiag import SingleFD
+    from igaops_amd.quadrature import StandardGauss
+    from oracle import fastdiag as ofd
+
+    qs = patch.quadrule_list
+    table = np.ones((1, 3, 2), bool)
+    fd = SingleFD("spatial")
+    fd.compute_space_eigendecomposition(qs, table)
+    fd.update_space_eigenvalues([1.0, 1.0])
+    xd = torch.as_tensor(x, device="cuda").to(torch.float64 if dtype == "f64" else torch.float32)
+    y = fd.apply(xd).double().cpu().numpy()
+    ref_fd = ofd.SpaceFD([StandardGauss(q.degree, q.knotvector, "legendre") for q in qs], table)
+    ref_fd.scalar_coefs = (1.0, 1.0)
+    ref = ref_fd.apply(x)
+    return float(np.linalg.norm(y - ref) / np.linalg.norm(ref))
+
+
 def solve_section(patch, C, m, nrhs, tol=1e-10):
     """FD-preconditioned GMRES on ``(K + M) x = b`` for each right-hand side: plain
     FP64 and FP32 operators inside FP64 iterative refinement."""
@@ -405,6 +427,7 @@ def main():
             "sample": f"one right-hand side of the same workload, oracle matvec (numpy/scipy), median of {reps}",
             "seconds_per_matvec": t_cpu,
         }
+        fd_err = fd_error_vs_oracle(patch, args.dtype, x0)
     out = {
         "metric": "matvec GDOF/s (% HBM roofline); solves/s + p50/p99 latency at 1e-10 residual",
         "value": value,
@@ -444,6 +467,7 @@ def main():
         "pct_hbm_roofline_matvec": alg_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "batched": batched,
         "rel_err_vs_oracle": relerr,
+        "fd_rel_err_vs_oracle": None if args.no_cpu else fd_err,
         "cpu_baseline": cpu,
         "setup_s": setup_s,
     }

@@ -174,3 +174,44 @@ def test_zero_and_repeat_calls_are_deterministic():
     y1 = op.apply(x).clone()
     y2 = op.apply(x).clone()
     assert torch.equal(y1, y2)
+
+
+def test_unit_cube_p3_n32_full_size():
+    """The CPU-runnable configuration at its full size: unit cube, degree 3, 32
+    elements per direction (42,875 DOF, 101^3 WQ points), unit conductivity: the
+    float64 K+M matvec against the oracle, and FD-preconditioned GMRES on K x = b to
+    1e-10 against the oracle's GMRES (the FD is the exact inverse of this separable
+    operator, so both stop within two iterations)."""
+    import torch
+
+    from igaops_amd.fastdiag import SingleFD
+    from igaops_amd.geometry import unit_cube
+    from igaops_amd.operators import StiffnessMassOperator
+    from igaops_amd.solvers import LinearSolver
+    from oracle import fastdiag as ofd
+    from oracle import solvers as osv
+    from igaops_amd.quadrature import StandardGauss
+
+    patch = unit_cube(3, 32)
+    qs = patch.quadrule_list
+    n = patch.nbctrlpts_total
+    assert n == 42875 and patch.nbquadpts_total == 1030301
+    C, m = patch.conductivity_coefficients()
+    x = np.random.default_rng(0).standard_normal(n)
+    op = StiffnessMassOperator(qs, C=C, m=m)
+    assert op.fused
+    y = op.apply(torch.as_tensor(x, device="cuda")).cpu().numpy()
+    assert rel(y, oo.stiffness_plus_mass(qs, C, m, x)) <= 1e-12
+    K = StiffnessMassOperator(qs, C=C)
+    fd = SingleFD("spatial")
+    fd.compute_space_eigendecomposition(qs, np.ones((1, 3, 2), bool))
+    fd.update_space_eigenvalues([0.0, 1.0])
+    free = np.asarray(fd.space_preconditioner.space_free_nodes[0])
+    dod = np.setdiff1d(np.arange(n), free).tolist()
+    b = np.random.default_rng(1).standard_normal(n)
+    out = LinearSolver(1e-10, 50, cleandod=dod).solve(K, b, Pfun=fd)
+    gq = [StandardGauss(q.degree, q.knotvector, "legendre") for q in qs]
+    ref = osv.gmres(lambda v: oo.bspline_gradu_gradv(qs, C, v), b, ofd.SpaceFD(gq, np.ones((1, 3, 2), bool)).apply,
+                    tol=1e-10, maxiters=50, cleandod=dod)
+    assert out.success and ref.success and out.iterations <= 2 and abs(out.iterations - ref.iterations) <= 1
+    assert rel(out.solution, ref.solution) <= 1e-9
