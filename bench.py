@@ -176,14 +176,22 @@ def serve_section(nreq=100, qps=20.0, seed=0, max_active=16, budget_gb=200.0):
         wq_rule(r.degree, r.nel)
     warm = time.perf_counter() - t0
     out = {"requests": nreq, "qps": qps, "seed": seed, "max_active": max_active, "rule_cache_warmup_s": warm}
+    import torch
+
     for mode in ("fp64", "mixed"):
+        torch.cuda.empty_cache()  # each mode starts from an empty allocator cache
         srv = SolveServer(mode=mode, max_active=max_active, hbm_budget_gb=budget_gb)
         t1 = time.perf_counter()
         res = srv.serve(poisson_stream(nreq, qps, seed=seed))
         wall = time.perf_counter() - t1
         s = summarize(res, 1e-10, wall)
         s.update({"max_concurrent": srv.stats["max_active"], "peak_admitted_gb": srv.stats["max_bytes"] / 1e9,
-                  "deferred_for_memory": srv.stats["deferred_for_memory"], "shapes": len(srv.shapes)})
+                  "deferred_for_memory": srv.stats["deferred_for_memory"], "shapes": len(srv.shapes),
+                  "host_admit_s": srv.stats["admit_s"], "host_advance_s": srv.stats["advance_s"],
+                  "host_complete_s": srv.stats["complete_s"],
+                  "host_setup_s": {k[6:-2]: v for k, v in srv.stats.items() if k.startswith("setup_")},
+                  "allocator": {k: srv.stats[k] for k in ("alloc_retries", "device_frees", "device_allocs",
+                                                           "peak_reserved_gb")}})
         out[mode] = s
         print(f"serve {mode}: {json.dumps(s)}", file=sys.stderr, flush=True)
     out["speedup_mixed_vs_fp64"] = out["mixed"]["solves_per_s"] / max(out["fp64"]["solves_per_s"], 1e-30)

@@ -166,6 +166,7 @@ struct iga_fd_s {
   void* d_Ut[4] = {nullptr, nullptr, nullptr, nullptr};  // U_d^T, row-major len x len
   void* d_U[4] = {nullptr, nullptr, nullptr, nullptr};
   void* d_inv = nullptr;               // Lambda^{-1} over the free box
+  void* d_base = nullptr;              // one block holding every U_d and U_d^T
   int64_t nsub = 0;
 };
 
@@ -267,36 +268,44 @@ int iga_fd_create(iga_fd_t* out, int ndim, const int64_t* n, const int64_t* lo, 
     f->ndim = ndim;
     f->nsub = 1;
     try {
+      const size_t es = dtype == IGA_F64 ? 8 : 4;
+      size_t total = 0;
       for (int d = 0; d < ndim; ++d) {
         IGA_CHECK_ARG(len[d] >= 1 && lo[d] >= 0 && lo[d] + len[d] <= n[d], "free box does not fit");
         f->n[d] = n[d];
         f->lo[d] = lo[d];
         f->len[d] = len[d];
         f->nsub *= len[d];
+        total += 2 * (((size_t)len[d] * len[d] * es + 255) & ~size_t(255));
+      }
+      // every U_d, U_d^T in one block and one transfer
+      std::vector<char> host(total);
+      size_t off = 0;
+      size_t offs[4][2];
+      for (int d = 0; d < ndim; ++d) {
         const int64_t m = len[d];
-        std::vector<double> ut((size_t)m * m), u((size_t)m * m);
-        for (int64_t i = 0; i < m; ++i)
-          for (int64_t k = 0; k < m; ++k) {
-            u[i * m + k] = U[d][i * m + k];   // U[i][k]  (row-major, as numpy)
-            ut[i * m + k] = U[d][k * m + i];  // U^T[i][k]
-          }
-        const size_t es = dtype == IGA_F64 ? 8 : 4;
-        IGA_HIP(hipMalloc(&f->d_U[d], es * m * m));
-        IGA_HIP(hipMalloc(&f->d_Ut[d], es * m * m));
-        if (dtype == IGA_F64) {
-          IGA_HIP(hipMemcpy(f->d_U[d], u.data(), 8 * m * m, hipMemcpyHostToDevice));
-          IGA_HIP(hipMemcpy(f->d_Ut[d], ut.data(), 8 * m * m, hipMemcpyHostToDevice));
-        } else {
-          std::vector<float> u32(u.begin(), u.end()), ut32(ut.begin(), ut.end());
-          IGA_HIP(hipMemcpy(f->d_U[d], u32.data(), 4 * m * m, hipMemcpyHostToDevice));
-          IGA_HIP(hipMemcpy(f->d_Ut[d], ut32.data(), 4 * m * m, hipMemcpyHostToDevice));
+        for (int which = 0; which < 2; ++which) {
+          offs[d][which] = off;
+          for (int64_t i = 0; i < m; ++i)
+            for (int64_t k = 0; k < m; ++k) {
+              // which 0: U[i][k] (row-major, as numpy); 1: U^T[i][k]
+              const double v = which == 0 ? U[d][i * m + k] : U[d][k * m + i];
+              if (dtype == IGA_F64)
+                reinterpret_cast<double*>(host.data() + off)[i * m + k] = v;
+              else
+                reinterpret_cast<float*>(host.data() + off)[i * m + k] = (float)v;
+            }
+          off += ((size_t)m * m * es + 255) & ~size_t(255);
         }
       }
-    } catch (...) {
-      for (int d = 0; d < 4; ++d) {
-        (void)hipFree(f->d_U[d]);
-        (void)hipFree(f->d_Ut[d]);
+      IGA_HIP(hipMalloc(&f->d_base, std::max<size_t>(total, 256)));
+      IGA_HIP(hipMemcpy(f->d_base, host.data(), total, hipMemcpyHostToDevice));
+      for (int d = 0; d < ndim; ++d) {
+        f->d_U[d] = (char*)f->d_base + offs[d][0];
+        f->d_Ut[d] = (char*)f->d_base + offs[d][1];
       }
+    } catch (...) {
+      (void)hipFree(f->d_base);
       delete f;
       throw;
     }
@@ -346,10 +355,7 @@ int iga_fd_apply(iga_fd_t f, const void* x, void* y, int nrhs, void* work, void*
 int iga_fd_destroy(iga_fd_t f) {
   return guard([&] {
     if (!f) return;
-    for (int d = 0; d < 4; ++d) {
-      IGA_HIP(hipFree(f->d_U[d]));
-      IGA_HIP(hipFree(f->d_Ut[d]));
-    }
+    IGA_HIP(hipFree(f->d_base));
     IGA_HIP(hipFree(f->d_inv));
     delete f;
   });

@@ -20,6 +20,8 @@
 
 #include <algorithm>
 #include <atomic>
+#include <cstring>
+#include <functional>
 #include <vector>
 
 #include "common.hpp"
@@ -633,14 +635,27 @@ struct iga_fused_s {
 
 namespace iga {
 
-template <typename V>
-static V* upload(iga_fused_s* f, const std::vector<V>& h) {
-  V* d = nullptr;
-  IGA_HIP(hipMalloc(&d, sizeof(V) * std::max<size_t>(h.size(), 1)));
-  f->owned.push_back(d);
-  if (!h.empty()) IGA_HIP(hipMemcpy(d, h.data(), sizeof(V) * h.size(), hipMemcpyHostToDevice));
-  return d;
-}
+// All device tables of a plan go up in one allocation and one copy: the host
+// vectors are staged with 256-byte alignment and each destination pointer is set
+// once the block exists (plan creation then costs one synchronous transfer).
+struct Staging {
+  std::vector<char> host;
+  std::vector<std::function<void(char*)>> fix;
+  template <typename Ptr, typename V>
+  void put(Ptr* dst, const std::vector<V>& h) {
+    const size_t off = (host.size() + 255) & ~size_t(255);
+    host.resize(off + std::max<size_t>(h.size(), 1) * sizeof(V));
+    if (!h.empty()) std::memcpy(host.data() + off, h.data(), h.size() * sizeof(V));
+    fix.emplace_back([dst, off](char* base) { *dst = reinterpret_cast<Ptr>(base + off); });
+  }
+  void commit(iga_fused_s* f) {
+    char* d = nullptr;
+    IGA_HIP(hipMalloc(&d, std::max<size_t>(host.size(), 256)));
+    f->owned.push_back(d);
+    IGA_HIP(hipMemcpy(d, host.data(), host.size(), hipMemcpyHostToDevice));
+    for (auto& set : fix) set(d);
+  }
+};
 
 static void free_fused(iga_fused_s* f) {
   for (void* p : f->owned) (void)hipFree(p);
@@ -941,28 +956,30 @@ int iga_fused_create(iga_fused_t* out, const int32_t* nc, const int32_t* nq, int
       f->coef_elems = cb;
       f->part_stride = pb;
       // band tables in the apply precision
+      Staging st;
       for (int d = 0; d < 3; ++d) {
-        f->d_first[d] = upload(f, std::vector<int32_t>(first[d], first[d] + nq[d]));
+        st.put(&f->d_first[d], std::vector<int32_t>(first[d], first[d] + nq[d]));
         const size_t nt = (size_t)4 * nq[d] * P;
         if (dtype == IGA_F64) {
-          f->d_tab[d] = upload(f, std::vector<double>(tabs[d], tabs[d] + nt));
+          st.put(&f->d_tab[d], std::vector<double>(tabs[d], tabs[d] + nt));
         } else {
           std::vector<float> t32(nt);
           for (size_t i = 0; i < nt; ++i) t32[i] = (float)tabs[d][i];
-          f->d_tab[d] = upload(f, t32);
+          st.put(&f->d_tab[d], t32);
         }
       }
-      f->d_t1_q0 = upload(f, d1.q0); f->d_t1_c0 = upload(f, d1.c0); f->d_t1_nc = upload(f, d1.nc);
-      f->d_t2_q0 = upload(f, d2.q0); f->d_t2_c0 = upload(f, d2.c0); f->d_t2_nc = upload(f, d2.nc);
-      f->d_s_q0 = upload(f, d3.q0); f->d_s_c0 = upload(f, d3.c0); f->d_s_nc = upload(f, d3.nc);
-      f->d_coef_base = upload(f, coef_base);
-      f->d_part_base = upload(f, part_base);
+      st.put(&f->d_t1_q0, d1.q0); st.put(&f->d_t1_c0, d1.c0); st.put(&f->d_t1_nc, d1.nc);
+      st.put(&f->d_t2_q0, d2.q0); st.put(&f->d_t2_c0, d2.c0); st.put(&f->d_t2_nc, d2.nc);
+      st.put(&f->d_s_q0, d3.q0); st.put(&f->d_s_c0, d3.c0); st.put(&f->d_s_nc, d3.nc);
+      st.put(&f->d_coef_base, coef_base);
+      st.put(&f->d_part_base, part_base);
       const DirTiling* dt[3] = {&d1, &d2, &d3};
       for (int d = 0; d < 3; ++d) {
-        f->d_list[d][0] = upload(f, dt[d]->cnt);
-        f->d_list[d][1] = upload(f, dt[d]->t);
-        f->d_list[d][2] = upload(f, dt[d]->off);
+        st.put(&f->d_list[d][0], dt[d]->cnt);
+        st.put(&f->d_list[d][1], dt[d]->t);
+        st.put(&f->d_list[d][2], dt[d]->off);
       }
+      st.commit(f);
     } catch (...) {
       free_fused(f);
       delete f;

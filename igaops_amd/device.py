@@ -29,10 +29,22 @@ def device():
     return torch().device("cuda", torch().cuda.current_device())
 
 
+_DEV_INDEX = None
+
+
 def stream_ptr(stream=None) -> int:
+    """The hipStream_t of ``stream`` (a torch stream) or of the current stream."""
+    global _DEV_INDEX
+    if stream is not None:
+        return int(stream.cuda_stream)
     t = torch()
-    s = stream if stream is not None else t.cuda.current_stream()
-    return int(s.cuda_stream)
+    raw = getattr(t._C, "_cuda_getCurrentRawStream", None)
+    if raw is not None:
+        # the current device does not change within a process of this library
+        if _DEV_INDEX is None:
+            _DEV_INDEX = t.cuda.current_device()
+        return int(raw(_DEV_INDEX))
+    return int(t.cuda.current_stream().cuda_stream)
 
 
 def torch_dtype(dtype: str):

@@ -229,8 +229,15 @@ class ExtrudedPatch:
         g01 = j[0, 0] * j[0, 1] + j[1, 0] * j[1, 1]
         dG = g00 * g11 - g01 ** 2
         dev = dv.device()
-        f = lambda a: t.from_numpy(np.ascontiguousarray(a, dtype=np.float64).ravel()).to(dev)
-        det2_d, dz_d = f(det2), f(dz)
+        inv_h = {(0, 0): g11 / dG, (1, 1): g00 / dG, (0, 1): -g01 / dG}
+        # one upload of all cross-section / extrusion factors
+        parts = [det2, inv_h[(0, 0)], inv_h[(1, 1)], inv_h[(0, 1)]]
+        npl = det2.size
+        host = np.concatenate([np.ravel(a) for a in parts] + [np.ravel(dz)])
+        allv = t.from_numpy(host).to(dev)
+        det2_d = allv[:npl]
+        invd = {k: allv[(i + 1) * npl:(i + 2) * npl] for i, k in enumerate(((0, 0), (1, 1), (0, 1)))}
+        dz_d = allv[4 * npl:]
         det = t.outer(dz_d, det2_d)  # (n3, nplane): q3 slowest
         if sigma > 0.0:
             kap = self._device_lognormal(sigma, seed, modes, pg, z)
@@ -239,9 +246,8 @@ class ExtrudedPatch:
             kdet = det
         nq = det.numel()
         C = t.zeros((3, 3, nq), dtype=t.float64, device=dev)
-        inv = {(0, 0): g11 / dG, (1, 1): g00 / dG, (0, 1): -g01 / dG}
-        for (a, b), v in inv.items():
-            val = (kdet * f(v)[None, :]).reshape(-1)
+        for (a, b), v in invd.items():
+            val = (kdet * v[None, :]).reshape(-1)
             C[a, b] = val
             if a != b:
                 C[b, a] = val
@@ -271,9 +277,11 @@ class ExtrudedPatch:
         Ez = np.exp(1j * np.pi * zh[:, None] * k[None, :])
         Zc = np.einsum("abc,zc->abz", coef, Ez).reshape(-1, zh.shape[0])  # (m*m, n3)
         dev = dv.device()
-        kd = t.from_numpy(k.astype(np.float64)).to(dev)
-        Ex = t.exp(1j * np.pi * t.from_numpy(xh).to(dev)[:, None] * kd[None, :])
-        Ey = t.exp(1j * np.pi * t.from_numpy(yh).to(dev)[:, None] * kd[None, :])
+        npl = xh.shape[0]
+        allv = t.from_numpy(np.concatenate([xh, yh, k.astype(np.float64)])).to(dev)
+        xd, yd, kd = allv[:npl], allv[npl:2 * npl], allv[2 * npl:]
+        Ex = t.exp(1j * np.pi * xd[:, None] * kd[None, :])
+        Ey = t.exp(1j * np.pi * yd[:, None] * kd[None, :])
         Pxy = (Ex[:, :, None] * Ey[:, None, :]).reshape(xh.shape[0], -1)
         g = t.real(t.from_numpy(Zc).to(dev).T.contiguous() @ Pxy.T.contiguous())  # (n3, nplane)
         return t.exp(sigma * g)

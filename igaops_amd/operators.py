@@ -460,7 +460,8 @@ class StiffnessMassOperator:
     be numpy or device tensors; they are uploaded / packed once.  ``dtype`` is the
     arithmetic of the apply ("f64" or "f32")."""
 
-    def __init__(self, quadrule_list, C=None, m=None, nurbs_weights=None, dtype="f64", force_composed=False, nslab=0):
+    def __init__(self, quadrule_list, C=None, m=None, nurbs_weights=None, dtype="f64", force_composed=False, nslab=0,
+                 symmetric=None, nurbs_fields=None):
         dv.require_gpu()
         t = dv.torch()
         self.quadrule_list = list(quadrule_list)
@@ -475,7 +476,8 @@ class StiffnessMassOperator:
         self.w = None if not self.is_nurbs else self.w64.to(dv.torch_dtype(dtype))
         self.fused = (not force_composed) and fused_applicable(self.quadrule_list)
         if self.fused:
-            sym = Cd is None or _is_symmetric(Cd)
+            # `symmetric` skips the check (a device comparison the host waits for)
+            sym = Cd is None or (bool(symmetric) if symmetric is not None else _is_symmetric(Cd))
             if self.is_nurbs:
                 form = FORM_SYM10 if sym else FORM_GEN16
             elif Cd is None:
@@ -491,7 +493,8 @@ class StiffnessMassOperator:
             self.plan = fused_plan(self.quadrule_list, form, dtype, nslab)
             W = dW = None
             if self.is_nurbs:
-                Wf, dWf = nurbs_weight_fields(self.quadrule_list, self.w64)
+                Wf, dWf = nurbs_fields if nurbs_fields is not None else nurbs_weight_fields(self.quadrule_list,
+                                                                                            self.w64)
                 W, dW = Wf, t.cat(dWf)
                 if Cd is None:
                     Cd = t.zeros((3, 3, self.nq), dtype=t.float64, device=W.device)

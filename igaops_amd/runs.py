@@ -55,7 +55,7 @@ class GmresRun:
     also the Krylov dimension (the reference does not restart)."""
 
     def __init__(self, A: Callable, P: Optional[Callable], b, tol: float, maxiters: int, mask=None, x0=None,
-                 stream=None, lookahead: int = 2, dtype: Optional[str] = None):
+                 stream=None, lookahead: int = 2, dtype: Optional[str] = None, host_buf=None):
         t = dv.torch()
         self.A, self.P, self.mask = A, P, mask
         self.tol, self.m = float(tol), int(maxiters)
@@ -69,8 +69,11 @@ class GmresRun:
         lib = _lib.load()
         with self._on_stream():
             self.b = b
-            self.V = t.empty((self.m + 1, n), dtype=tdt, device=b.device)
-            self.Z = t.empty((self.m, n), dtype=tdt, device=b.device)
+            # the bases grow by doubling as steps are queued (most solves stop far below
+            # maxiters, and the allocator keeps what a solve has held)
+            self.cap = min(self.m, 64)
+            self.V = t.empty((self.cap + 1, n), dtype=tdt, device=b.device)
+            self.Z = t.empty((self.cap, n), dtype=tdt, device=b.device)
             self.w = t.empty(n, dtype=tdt, device=b.device)
             self.x = t.zeros(n, dtype=tdt, device=b.device)
             self.x0 = x0
@@ -78,7 +81,16 @@ class GmresRun:
             self.ls = t.zeros(self.nls, dtype=t.float64, device=b.device)
             self.work = dv.workspace(lib.iga_reduce_workspace_bytes(n, self.m + 1))
             self.nb = t.zeros(1, dtype=t.float64, device=b.device)
-        self.hist_host = t.zeros(2 * (self.m + 2), dtype=t.float64, pin_memory=True)
+        # one pinned host buffer per run (or lent by the caller, at least
+        # host_doubles(maxiters) values): polled residuals, then the least-squares
+        # factors and the solution coefficients
+        need = self.host_doubles(self.m)
+        if host_buf is None or host_buf.numel() < need:
+            host_buf = t.zeros(need, dtype=t.float64, pin_memory=True)
+        self.host = host_buf
+        self.hist_host = host_buf[:2 * (self.m + 2)]
+        self._sol_region = host_buf[2 * (self.m + 2):2 * (self.m + 2) + self.m * self.m + self.m]
+        self._y_region = host_buf[2 * (self.m + 2) + self.m * self.m + self.m:need]
         self._hist_off = self.nls - 2 * (self.m + 1)
         self.launched = 0      # steps queued
         self.known = -1        # last step whose residual estimate has been read
@@ -91,6 +103,10 @@ class GmresRun:
         self._sol_event = None
         self._sol_host = None
         self.result = None
+
+    @staticmethod
+    def host_doubles(m: int) -> int:
+        return 2 * (m + 2) + m * m + 2 * m
 
     # -- plumbing --
     def _on_stream(self):
@@ -132,11 +148,24 @@ class GmresRun:
     def can_launch(self) -> bool:
         return self.kfinal is None and self.launched < self.m and self.launched - self.known < self.lookahead
 
+    def _grow(self, need: int):
+        t = dv.torch()
+        cap = self.cap
+        while cap < need:
+            cap = min(self.m, 2 * cap)
+        V = t.empty((cap + 1, self.n), dtype=self.V.dtype, device=self.V.device)
+        Z = t.empty((cap, self.n), dtype=self.Z.dtype, device=self.Z.device)
+        V[:self.cap + 1].copy_(self.V)
+        Z[:self.cap].copy_(self.Z)
+        self.V, self.Z, self.cap = V, Z, cap
+
     def launch(self):
         """Queue Arnoldi step k = launched + 1."""
         k = self.launched + 1
         dt = dv.DTYPES[self.dtype]
         with self._on_stream():
+            if k > self.cap:
+                self._grow(k)
             z = self.Z[k - 1]
             if self.P is not None:
                 self.P(self.V[k - 1], z)
@@ -197,7 +226,7 @@ class GmresRun:
             return
         with self._on_stream():
             off_r, off_g = 2 * m + 2, 2 * m + 2 + m * m + 2 * m
-            self._sol_host = t.empty(k * m + k, dtype=t.float64, pin_memory=True)
+            self._sol_host = self._sol_region[:k * m + k]
             self._sol_host[:k * m].copy_(self.ls[off_r:off_r + k * m], non_blocking=True)
             self._sol_host[k * m:].copy_(self.ls[off_g:off_g + k], non_blocking=True)
             self._sol_event = t.cuda.Event()
@@ -224,8 +253,9 @@ class GmresRun:
             y = _back_substitute(R, h[k * self.m:k * self.m + k])
             with self._on_stream():
                 # the pinned source and the device copy are kept with the run until it is dropped
-                self._y_host = t.from_numpy(y).pin_memory()
-                self._y_dev = self._y_host.to(self.x.device, non_blocking=True)
+                yh = self._y_region[:k]
+                yh.copy_(t.from_numpy(y))
+                self._y_dev = yh.to(self.x.device, non_blocking=True)
                 _lib.call("iga_axpy_many", dv.DTYPES[self.dtype], self.n, k, dv.ptr(self.Z), self.n,
                           dv.ptr(self._y_dev), 1.0, dv.ptr(self.x), self._sp())
         success = self.reason in ("relative", "absolute", "breakdown")
@@ -278,7 +308,7 @@ class RefinementRun:
 
     def __init__(self, A64: Callable, A_lo: Callable, P_lo: Callable, b, tol: float = 1e-10,
                  inner_tol: float = 1e-6, inner_maxiters: int = 200, max_refinements: int = 30, mask=None,
-                 stream=None, lookahead: int = 2, lo_dtype: str = "f32"):
+                 stream=None, lookahead: int = 2, lo_dtype: str = "f32", host_buf=None):
         t = dv.torch()
         self.A64, self.A_lo, self.P_lo, self.mask = A64, A_lo, P_lo, mask
         self.tol, self.inner_tol = float(tol), float(inner_tol)
@@ -295,7 +325,11 @@ class RefinementRun:
             self.w = t.empty_like(self.x)
             self.nrm = t.zeros(2, dtype=t.float64, device=b.device)
             self.work = dv.workspace(_lib.load().iga_reduce_workspace_bytes(self.n, 1))
-        self.nrm_host = t.zeros(2, dtype=t.float64, pin_memory=True)
+        need = GmresRun.host_doubles(self.inner_maxiters) + 2
+        if host_buf is None or host_buf.numel() < need:
+            host_buf = t.zeros(need, dtype=t.float64, pin_memory=True)
+        self.nrm_host = host_buf[:2]
+        self._inner_host = host_buf[2:need]
         self.phase = "norm"
         self.inner: Optional[GmresRun] = None
         self.inner_its = []
@@ -326,7 +360,8 @@ class RefinementRun:
         rel = self.hist[-1] if self.hist else 1.0
         itol = min(1e-1, max(self.inner_tol, 0.5 * self.tol / max(rel, 1e-300)))
         self.inner = GmresRun(self.A_lo, self.P_lo, self.r_lo, itol, self.inner_maxiters, mask=self.mask,
-                              stream=self.stream, lookahead=self.lookahead, dtype=self.lo_dtype)
+                              stream=self.stream, lookahead=self.lookahead, dtype=self.lo_dtype,
+                              host_buf=self._inner_host)
         self.phase = "inner"
 
     def _finish(self, success: bool):

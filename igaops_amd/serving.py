@@ -104,13 +104,20 @@ class ShapeResources:
         self.space_fd = fd.space_preconditioner
         self.mask = Mask(box=self.space_fd.box())
         self.w64 = dv.to_device(self.patch.nurbs_weights, "f64") if self.patch.is_nurbs else None
+        self.nurbs_fields = None
+        if self.w64 is not None:
+            from .operators import nurbs_weight_fields
+
+            self.nurbs_fields = nurbs_weight_fields(self.qs, self.w64)
+        # requests read these from their own streams: finish the set-up work first
+        dv.torch().cuda.current_stream().synchronize()
         self.fd_plans = {dt: self.space_fd.plan(dt) for dt in ("f64", "f32")}
 
 
 def footprint_bytes(ndof: int, nq: int, nurbs: bool, mode: str, maxiters: int, inner_maxiters: int) -> int:
     """Upper estimate of the HBM a request holds at its peak."""
     ncoef = 10 if nurbs else 6
-    setup = 10 * nq * 8  # C (3,3,N_q) and the field, released after packing
+    setup = 18 * nq * 8  # C (3,3,N_q), the field, NURBS weight fields: released after packing
     coef = ncoef * nq * 8 + (ncoef * nq * 4 if mode == "mixed" else 0)
     if mode == "mixed":
         krylov = 2 * (inner_maxiters + 1) * ndof * 4 + 6 * ndof * 8
@@ -124,37 +131,57 @@ class _Active:
     """One admitted request: its stream, operators, scratch and solver run."""
 
     def __init__(self, req: SolveRequest, shape: ShapeResources, mode: str, stream, tol: float, maxiters: int,
-                 inner_tol: float, inner_maxiters: int, lookahead: int, admitted: float):
+                 inner_tol: float, inner_maxiters: int, lookahead: int, admitted: float, host_buf=None):
         from .operators import StiffnessMassOperator
         from .runs import GmresRun, RefinementRun
 
         t = dv.torch()
         self.req, self.shape, self.mode, self.stream = req, shape, mode, stream
         self.admitted = admitted
+        self.setup_times = {}
+        tic = time.perf_counter()
+
+        def lap(name):
+            nonlocal tic
+            now = time.perf_counter()
+            self.setup_times[name] = now - tic
+            tic = now
+
         with t.cuda.stream(stream):
             C, _ = shape.patch.device_coefficients(req.sigma, req.seed, mass=False)
+            lap("coefficients")
             w = shape.w64
-            self.A64 = StiffnessMassOperator(shape.qs, C=C, m=None, nurbs_weights=w, dtype="f64")
-            self.A32 = (StiffnessMassOperator(shape.qs, C=C, m=None, nurbs_weights=w, dtype="f32")
+            # C is symmetric by construction; the NURBS weight fields belong to the shape
+            self.A64 = StiffnessMassOperator(shape.qs, C=C, m=None, nurbs_weights=w, dtype="f64", symmetric=True,
+                                             nurbs_fields=shape.nurbs_fields)
+            self.A32 = (StiffnessMassOperator(shape.qs, C=C, m=None, nurbs_weights=w, dtype="f32", symmetric=True,
+                                              nurbs_fields=shape.nurbs_fields)
                         if mode == "mixed" else None)
+            lap("operators")
             del C
-            rng = np.random.default_rng(req.seed + 7919)
-            b = t.from_numpy(rng.standard_normal(shape.ndof)).to(dv.device(), non_blocking=False)
+            gen = t.Generator(device=dv.device())
+            gen.manual_seed(int(req.seed) + 7919)
+            b = t.randn(shape.ndof, generator=gen, dtype=t.float64, device=dv.device())
             shape.mask(b)
             self.b = b
             dt = "f32" if mode == "mixed" else "f64"
             plan = shape.fd_plans[dt]
             fd_work = plan.workspace(1)
             self._fd_work = fd_work
+            # the closures capture the operators, not this object, so that a finished
+            # request holds no reference cycle and its HBM is released at once
+            op64, op32 = self.A64, self.A32
             P = lambda x, o: plan.apply(x, o, 1, None, work=fd_work)
-            A64 = lambda x, o: self.A64.apply(x, out=o)
+            A64 = lambda x, o: op64.apply(x, out=o)
             if mode == "mixed":
-                A32 = lambda x, o: self.A32.apply(x, out=o)
+                A32 = lambda x, o: op32.apply(x, out=o)
                 self.run = RefinementRun(A64, A32, P, b, tol=tol, inner_tol=inner_tol,
                                          inner_maxiters=inner_maxiters, mask=shape.mask, stream=stream,
-                                         lookahead=lookahead)
+                                         lookahead=lookahead, host_buf=host_buf)
             else:
-                self.run = GmresRun(A64, P, b, tol, maxiters, mask=shape.mask, stream=stream, lookahead=lookahead)
+                self.run = GmresRun(A64, P, b, tol, maxiters, mask=shape.mask, stream=stream, lookahead=lookahead,
+                                    host_buf=host_buf)
+            lap("solver")
         self.t0 = time.perf_counter()
 
     def advance(self) -> bool:
@@ -172,9 +199,12 @@ class _Active:
             r = self.b - self.A64.apply(x)
             self.shape.mask(r)
             rel = float(t.linalg.vector_norm(r) / t.linalg.vector_norm(self.b))
-        return SolveResult(self.req.rid, self.req.shape, self.shape.ndof, int(out.iterations), rel,
-                           bool(out.success), self.req.arrival, self.admitted, completed,
-                           x.clone() if keep_solution else None)
+        res = SolveResult(self.req.rid, self.req.shape, self.shape.ndof, int(out.iterations), rel,
+                          bool(out.success), self.req.arrival, self.admitted, completed,
+                          x.clone() if keep_solution else None)
+        # release everything this request holds on the device
+        self.run = self.A64 = self.A32 = self.b = self._fd_work = None
+        return res
 
 
 class SolveServer:
@@ -193,7 +223,8 @@ class SolveServer:
         self.lookahead = int(lookahead)
         self.keep_solutions = keep_solutions
         self.shapes: Dict[tuple, ShapeResources] = {}
-        self.stats = {"max_active": 0, "max_bytes": 0, "deferred_for_memory": 0}
+        self.stats = {"max_active": 0, "max_bytes": 0, "deferred_for_memory": 0, "admit_s": 0.0, "advance_s": 0.0,
+                      "complete_s": 0.0}
 
     def shape(self, key) -> ShapeResources:
         s = self.shapes.get(key)
@@ -215,6 +246,12 @@ class SolveServer:
         dv.require_gpu()
         pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
         streams = [t.cuda.Stream() for _ in range(self.max_active)]
+        # one pinned host buffer per stream slot, lent to the solver runs of that slot
+        from .runs import GmresRun
+
+        hd = (GmresRun.host_doubles(self.inner_maxiters) + 2 if self.mode == "mixed"
+              else GmresRun.host_doubles(self.maxiters))
+        host_bufs = [t.zeros(hd, dtype=t.float64, pin_memory=True) for _ in range(self.max_active)]
         free_streams = list(range(self.max_active))
         active: List[Tuple[_Active, int, int]] = []  # (state, stream index, bytes)
         used = 0
@@ -234,9 +271,13 @@ class SolveServer:
                     self.stats["deferred_for_memory"] += 1
                     break
                 si = free_streams.pop()
+                ta = time.perf_counter()
                 shape = self.shape(req.shape)
                 st = _Active(req, shape, self.mode, streams[si], self.tol, self.maxiters, self.inner_tol,
-                             self.inner_maxiters, self.lookahead, admitted=now())
+                             self.inner_maxiters, self.lookahead, admitted=now(), host_buf=host_bufs[si])
+                self.stats["admit_s"] += time.perf_counter() - ta
+                for k_, v_ in st.setup_times.items():
+                    self.stats["setup_" + k_ + "_s"] = self.stats.get("setup_" + k_ + "_s", 0.0) + v_
                 active.append((st, si, fp))
                 used += fp
                 head += 1
@@ -245,12 +286,17 @@ class SolveServer:
             self.stats["max_active"] = max(self.stats["max_active"], len(active))
             self.stats["max_bytes"] = max(self.stats["max_bytes"], used)
             progressed = False
+            ta = time.perf_counter()
             for st, _, _ in active:
                 progressed |= st.advance()
+            tb = time.perf_counter()
+            self.stats["advance_s"] += tb - ta
             still = []
             for st, si, fp in active:
                 if st.done:
                     results.append(st.result(now(), self.keep_solutions))
+                    self.stats["complete_s"] += time.perf_counter() - tb
+                    tb = time.perf_counter()
                     free_streams.append(si)
                     used -= fp
                 else:
@@ -265,6 +311,11 @@ class SolveServer:
                 elif head < len(pending):
                     time.sleep(max(0.0, min(1e-3, pending[head].arrival - now())))
         t.cuda.synchronize()
+        ms = t.cuda.memory_stats()
+        self.stats["alloc_retries"] = int(ms.get("num_alloc_retries", 0))
+        self.stats["device_frees"] = int(ms.get("num_device_free", 0))
+        self.stats["device_allocs"] = int(ms.get("num_device_alloc", 0))
+        self.stats["peak_reserved_gb"] = ms.get("reserved_bytes.all.peak", 0) / 1e9
         return results
 
 

@@ -127,17 +127,17 @@ class Mask:
         if cleandod is not None and len(cleandod) > 0:
             self.idx = dv.torch().as_tensor(np.asarray(cleandod, dtype=np.int64), device=dv.device())
             self._keep = self.idx
+        if box is not None:
+            # ctypes views of the box, built once (the mask runs twice per iteration)
+            self._box_arrays = [_lib.i64_array(a) for a in box]
 
     def __call__(self, v, stream=None):
         if self.idx is not None:
             _lib.call("iga_index_zero", dv.iga_dtype(v), int(self.idx.numel()), dv.ptr(self.idx), 1, dv.ptr(v),
                       dv.stream_ptr(stream))
         if self.box is not None:
-            n, lo, ln = self.box
-            na, npt = _lib.i64_array(n)
-            la, lpt = _lib.i64_array(lo)
-            lna, lnpt = _lib.i64_array(ln)
-            _lib.call("iga_box_mask", dv.iga_dtype(v), len(n), npt, lpt, lnpt, 1, dv.ptr(v), dv.stream_ptr(stream))
+            (na, npt), (la, lpt), (lna, lnpt) = self._box_arrays
+            _lib.call("iga_box_mask", dv.iga_dtype(v), len(na), npt, lpt, lnpt, 1, dv.ptr(v), dv.stream_ptr(stream))
 
 
 def device_callable(op, numpy_mode: bool, dtype: str = "f64"):
