@@ -505,17 +505,30 @@ __global__ void __launch_bounds__(256) fused_gather_kernel(GatherArgs<T> g) {
     const int c2 = (int)((dof / g.nc[0]) % g.nc[1]);
     const int c3 = (int)(dof / ((int64_t)g.nc[0] * g.nc[1]));
     const T* pk = g.partial + (int64_t)k * g.part_stride;
+    // All index loads first (fixed-size, predicated), then all partial loads: the
+    // dependent chain is three loads deep instead of one chain per contribution.
+    const int n1 = g.l1_cnt[c1], n2 = g.l2_cnt[c2], n3 = g.l3_cnt[c3];
+    int t1v[kKMax], o1v[kKMax], D1v[kKMax], t2v[kKMax], o2v[kKMax], D2v[kKMax], sv[kKMax], o3v[kKMax];
+#pragma unroll
+    for (int i = 0; i < kKMax; ++i) {
+      t1v[i] = i < n1 ? g.l1_t[c1 * kKMax + i] : 0;
+      o1v[i] = i < n1 ? g.l1_off[c1 * kKMax + i] : 0;
+      t2v[i] = i < n2 ? g.l2_t[c2 * kKMax + i] : 0;
+      o2v[i] = i < n2 ? g.l2_off[c2 * kKMax + i] : 0;
+      sv[i] = i < n3 ? g.l3_t[c3 * kKMax + i] : 0;
+      o3v[i] = i < n3 ? g.l3_off[c3 * kKMax + i] : 0;
+    }
+#pragma unroll
+    for (int i = 0; i < kKMax; ++i) {
+      D1v[i] = g.t1_nc[t1v[i]];
+      D2v[i] = g.t2_nc[t2v[i]];
+    }
     T acc = T(0);
-    for (int i3 = 0; i3 < g.l3_cnt[c3]; ++i3) {
-      const int s = g.l3_t[c3 * kKMax + i3], o3 = g.l3_off[c3 * kKMax + i3];
-      for (int i2 = 0; i2 < g.l2_cnt[c2]; ++i2) {
-        const int t2 = g.l2_t[c2 * kKMax + i2], o2 = g.l2_off[c2 * kKMax + i2];
-        const int D2 = g.t2_nc[t2];
-        for (int i1 = 0; i1 < g.l1_cnt[c1]; ++i1) {
-          const int t1 = g.l1_t[c1 * kKMax + i1], o1 = g.l1_off[c1 * kKMax + i1];
-          const int D1 = g.t1_nc[t1];
-          const int64_t base = g.part_base[((int64_t)t2 * g.nt1 + t1) * g.nslab + s];
-          acc += pk[base + ((int64_t)o3 * D2 + o2) * D1 + o1];
+    for (int i3 = 0; i3 < n3; ++i3) {
+      for (int i2 = 0; i2 < n2; ++i2) {
+        for (int i1 = 0; i1 < n1; ++i1) {
+          const int64_t base = g.part_base[((int64_t)t2v[i2] * g.nt1 + t1v[i1]) * g.nslab + sv[i3]];
+          acc += pk[base + ((int64_t)o3v[i3] * D2v[i2] + o2v[i2]) * D1v[i1] + o1v[i1]];
         }
       }
     }
