@@ -316,3 +316,74 @@ class SingleFastDiagonalization:
 
 
 SingleFD = SingleFastDiagonalization
+
+
+class MultiFastDiagonalization:
+    """``fastdiag/multi_preconditioner.py:13-103``: block-diagonal FD over several
+    patches, patch ``i`` owning the contiguous slice of its DOFs (space DOFs, times
+    the time DOFs for the space-time type) in list order.  Each block runs the
+    patch's device apply on its slice; numpy in, numpy out (device tensors stay on
+    the device)."""
+
+    def __init__(self, list_preconditioner):
+        if len(list_preconditioner) < 1:
+            raise RuntimeError("Empty list of preconditioners")
+        if not all(isinstance(m, SingleFastDiagonalization) for m in list_preconditioner):
+            raise RuntimeError("All elements should be preconditioners")
+        self._list_fd = list(list_preconditioner)
+        self._preconditioner_type = self._list_fd[0].preconditioner_type
+        for fd in self._list_fd:
+            if fd.preconditioner_type != self._preconditioner_type:
+                raise RuntimeError("Preconditioners should be of the same type")
+        self._slices = self._set_slices()
+
+    @property
+    def preconditioner_type(self):
+        return self._preconditioner_type
+
+    def _set_slices(self):
+        sizes = [0]
+        for fd in self._list_fd:
+            sp = fd.space_preconditioner
+            n = max(1, sp.nbDoFsPerNode) * sp.sp_nnz
+            if self._preconditioner_type == "spacetime":
+                n *= fd.sptm_preconditioner.box()[0][-1]
+            sizes.append(int(n))
+        c = np.cumsum(sizes)
+        return [slice(int(c[i]), int(c[i + 1])) for i in range(len(c) - 1)]
+
+    def apply_device(self, x, out=None, nrhs: int = 1, stream=None, transpose: bool = False):
+        t = dv.torch()
+        if out is None:
+            out = t.empty_like(x)
+        X = x.reshape(-1, nrhs)
+        Y = out.reshape(-1, nrhs)
+        if X.shape[0] != self._slices[-1].stop:
+            raise ValueError("array does not match the patches")
+        for fd, slc in zip(self._list_fd, self._slices):
+            xi = X[slc].contiguous().reshape(-1)
+            fn = fd.apply_transpose_device if transpose else fd.apply_device
+            Y[slc] = fn(xi, None, nrhs, stream).reshape(-1, nrhs)
+        return out
+
+    def _host_call(self, array_in, transpose: bool):
+        dv.require_gpu()
+        t = dv.torch()
+        was_np = not isinstance(array_in, t.Tensor)
+        shape = tuple(array_in.shape)
+        r = int(np.prod(shape[1:])) if len(shape) > 1 else 1
+        dtype = "f32" if (not was_np and array_in.dtype == t.float32) else "f64"
+        x = dv.to_device(array_in, dtype).reshape(-1)
+        y = self.apply_device(x, None, r, None, transpose).reshape(shape)
+        return y.double().cpu().numpy() if was_np else y
+
+    def apply(self, array_in):
+        """``multi_preconditioner.py:61-81``."""
+        return self._host_call(array_in, False)
+
+    def apply_transpose(self, array_in):
+        """``multi_preconditioner.py:83-103``."""
+        return self._host_call(array_in, True)
+
+
+MultiFD = MultiFastDiagonalization

@@ -187,3 +187,29 @@ def test_refinement_run_float32_vectors_reaches_1e10():
     free[np.asarray(fd.space_preconditioner.space_free_nodes[0])] = True
     Ax[~free] = 0
     assert np.linalg.norm(bh - Ax) / np.linalg.norm(bh) <= 1e-10
+
+
+def test_multi_patch_fd_matches_per_patch_oracle():
+    """Block-diagonal FD over two patches of different sizes: each slice is the
+    single-patch oracle apply of that patch (``multi_preconditioner.py:55-59``)."""
+    from igaops_amd.fastdiag import MultiFD, SingleFD
+
+    blocks, refs = [], []
+    for patch, coefs in ((unit_cube(2, 4), (0.0, 1.0)), (quarter_annulus(3, 3), (0.5, 1.0))):
+        qs = patch.quadrule_list
+        tab = np.ones((1, 3, 2), bool)
+        fd = SingleFD("spatial")
+        fd.compute_space_eigendecomposition(qs, tab)
+        fd.update_space_eigenvalues(list(coefs))
+        blocks.append(fd)
+        o = ofd.SpaceFD([StandardGauss(q.degree, q.knotvector, "legendre") for q in qs], tab)
+        o.scalar_coefs = coefs
+        refs.append((o, patch.nbctrlpts_total))
+    multi = MultiFD(blocks)
+    n = sum(nb for _, nb in refs)
+    X = np.random.default_rng(9).standard_normal((n, 2))
+    ref = np.concatenate([o.apply(X[a:a + nb]) for (o, nb), a in zip(refs, np.cumsum([0] + [nb for _, nb in refs]))])
+    assert rel(multi.apply(X), ref) <= 1e-12
+    assert rel(multi.apply_transpose(X[:, 0]), ref[:, 0]) <= 1e-12  # symmetric spatial FD
+    with pytest.raises(ValueError):
+        multi.apply(X[:-1])
