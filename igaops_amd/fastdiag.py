@@ -99,7 +99,6 @@ class SpaceFD:
         cur = np.maximum(cur, Constants.SAFEGUARD * np.max(cur))
         self.space_scalar_coefs = (float(scalar_coefs[0]), float(scalar_coefs[1]))
         self._eigvals_cache = [cur]
-        self._dev.pop("inv_eig", None)
         return [cur]
 
     @property
@@ -113,11 +112,9 @@ class SpaceFD:
         if mass_corrector is not None and ok(mass_corrector):
             self.mass_space_corrector = list(mass_corrector)
             self._eigvals_cache = None
-            self._dev.pop("inv_eig", None)
         if stiffness_corrector is not None and ok(stiffness_corrector):
             self.stiff_space_corrector = [np.asarray(s, dtype=float) for s in stiffness_corrector]
             self._eigvals_cache = None
-            self._dev.pop("inv_eig", None)
 
     # -- device apply ---------------------------------------------------------
     def plan(self, dtype: str) -> "FDPlan":
@@ -136,19 +133,6 @@ class SpaceFD:
         ln = np.array([len(idx) for idx in self.indices], dtype=np.int64)
         return np.array(self.nnz, dtype=np.int64), lo, ln
 
-    def device_factors(self, dtype: str):
-        from .operators import Band
-
-        key = ("U", dtype)
-        if key not in self._dev:
-            Ut = [Band([U.T.copy()]) for U in self.eigenvec_by_dir_space]
-            Un = [Band([U]) for U in self.eigenvec_by_dir_space]
-            self._dev[key] = (Ut, Un)
-        if "inv_eig" not in self._dev or self._dev["inv_eig"][0] != dtype:
-            inv = 1.0 / self.space_eigenvalues[0]
-            self._dev["inv_eig"] = (dtype, dv.to_device(inv, dtype))
-        return self._dev[key][0], self._dev[key][1], self._dev["inv_eig"][1]
-
     def apply_device(self, x, out=None, nrhs: int = 1, stream=None):
         """``out = P x`` for device arrays ((N, nrhs) C order), MFMA mode products."""
         t = dv.torch()
@@ -156,30 +140,6 @@ class SpaceFD:
         if out is None:
             out = t.empty_like(x)
         return self.plan(dtype).apply(x, out, nrhs, stream)
-
-    def apply_device_bands(self, x, out=None, nrhs: int = 1, stream=None):
-        """``out = P x`` through the generic banded mode-product kernel (dense bands)."""
-        from .operators import mf_apply_device
-
-        t = dv.torch()
-        dtype = "f32" if x.dtype == t.float32 else "f64"
-        Ut, Un, inv = self.device_factors(dtype)
-        n, lo, ln = self.box()
-        nsub = int(np.prod(ln))
-        na, npt = _lib.i64_array(n)
-        la, lpt = _lib.i64_array(lo)
-        lna, lnpt = _lib.i64_array(ln)
-        dt = dv.iga_dtype(x)
-        s = dv.stream_ptr(stream)
-        sub = t.empty(nsub * nrhs, dtype=x.dtype, device=x.device)
-        _lib.call("iga_box_gather", dt, len(n), npt, lpt, lnpt, nrhs, dv.ptr(x), dv.ptr(sub), s)
-        tmp = mf_apply_device(Ut, sub, nrhs, stream=stream)
-        _lib.call("iga_qp_scale", dt, nsub, nrhs, dv.ptr(inv), dv.ptr(tmp), dv.ptr(tmp), 0, 1.0, s)
-        res = mf_apply_device(Un, tmp, nrhs, stream=stream)
-        if out is None:
-            out = t.empty_like(x)
-        _lib.call("iga_box_scatter", dt, len(n), npt, lpt, lnpt, nrhs, dv.ptr(res), dv.ptr(out), 1, s)
-        return out
 
 
 class FDPlan:
