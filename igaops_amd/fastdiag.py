@@ -44,6 +44,8 @@ class SpaceFD:
         self.space_scalar_coefs = (0.0, 1.0)
         self._eigvals_cache = None
         self._dev = {}
+        # bumped whenever the eigenvalues change; device plans compare it
+        self.eig_version = 0
 
     def compute_space_eigendecomposition(self, space_quadrule_list, space_table_dirichlet):
         table = np.asarray(space_table_dirichlet, dtype=bool)
@@ -99,6 +101,7 @@ class SpaceFD:
         cur = np.maximum(cur, Constants.SAFEGUARD * np.max(cur))
         self.space_scalar_coefs = (float(scalar_coefs[0]), float(scalar_coefs[1]))
         self._eigvals_cache = [cur]
+        self.eig_version += 1
         return [cur]
 
     @property
@@ -124,8 +127,9 @@ class SpaceFD:
             n, lo, ln = self.box()
             p = FDPlan(n, lo, ln, self.eigenvec_by_dir_space, dtype)
             self._dev[key] = p
-        if p.eig_version != id(self.space_eigenvalues[0]):
-            p.set_eigenvalues(self.space_eigenvalues[0])
+        lam = self.space_eigenvalues[0]
+        if p.eig_version != self.eig_version:
+            p.set_eigenvalues(lam, self.eig_version)
         return p
 
     def box(self):
@@ -167,12 +171,12 @@ class FDPlan:
             _lib.load().iga_fd_destroy(h)
             self._h = None
 
-    def set_eigenvalues(self, lam: np.ndarray):
+    def set_eigenvalues(self, lam: np.ndarray, version=None):
         la, lp = _lib.f64_array(lam)
         if la.size != self.nsub:
             raise ValueError("eigenvalues do not match the free box")
         _lib.call("iga_fd_set_eigenvalues", self._h, lp)
-        self.eig_version = id(lam)
+        self.eig_version = version
 
     def workspace(self, nrhs: int = 1):
         return dv.workspace(_lib.load().iga_fd_workspace_bytes(self._h, int(nrhs)))

@@ -167,7 +167,7 @@ class SpaceTimeFD:
             self._plans[dtype] = p
         lam = self.space_fd.space_eigenvalues[0]
         adv = float(self._advection_time_corrector[0])
-        version = (id(lam), adv)
+        version = (self.space_fd.eig_version, adv)
         if p.version != version:
             D, g = self._eigenblocks_time
             p.set_blocks(lam, D, g, adv, version)

@@ -213,3 +213,24 @@ def test_multi_patch_fd_matches_per_patch_oracle():
     assert rel(multi.apply_transpose(X[:, 0]), ref[:, 0]) <= 1e-12  # symmetric spatial FD
     with pytest.raises(ValueError):
         multi.apply(X[:-1])
+
+
+def test_fd_eigenvalue_updates_reach_the_device_plan():
+    """Changing the scalar coefficients or correctors after a first apply updates the
+    device plan (``space_precond.py:195-236`` recomputes Lambda each time)."""
+    from igaops_amd.fastdiag import SingleFD
+
+    patch = unit_cube(2, 5)
+    qs = patch.quadrule_list
+    tab = np.ones((1, 3, 2), bool)
+    fd = SingleFD("spatial")
+    fd.compute_space_eigendecomposition(qs, tab)
+    o = ofd.SpaceFD([StandardGauss(q.degree, q.knotvector, "legendre") for q in qs], tab)
+    x = np.random.default_rng(10).standard_normal(patch.nbctrlpts_total)
+    for coefs in ((0.0, 1.0), (2.0, 1.0), (0.5, 3.0)):
+        fd.update_space_eigenvalues(list(coefs))
+        o.scalar_coefs = coefs
+        assert rel(fd.apply(x), o.apply(x)) <= 1e-12
+    fd.add_scalar_space_time_correctors(stiffness_corrector=[np.array([1.0, 2.0, 0.5])])
+    o.stiff_corr = np.array([1.0, 2.0, 0.5])
+    assert rel(fd.apply(x), o.apply(x)) <= 1e-12
