@@ -39,7 +39,7 @@ constexpr int kTQ1 = 16, kTQ2 = 16, kThreads = kTQ1 * kTQ2;
 // elements for rules with >= 2 points per element, so p + 10 <= 17 DOFs for p <= 7.
 constexpr int kDMax = 20;
 constexpr int kKMax = 4;   // max tiles/slabs whose footprint contains one DOF index
-constexpr int kSlabMax = 64;  // max q3 planes of one slab (direction-3 table staged in LDS)
+constexpr int kSlabMax = 128;  // max q3 planes of one slab (direction-3 table staged in LDS)
 
 template <typename T>
 struct DirTab {
