@@ -15,7 +15,8 @@ the 16 right-hand sides of the workload batched in one launch; the relative erro
 against the CPU oracle; the oracle's own single-thread throughput on the host
 (cpu_baseline); and FD-preconditioned GMRES solves of the 16 right-hand sides to
 1e-10 (plain FP64, and FP32 operators inside FP64 iterative refinement) with
-solves/s and p50/p99 latency.
+solves/s and p50/p99 latency; and a short saturated stream of 40 mixed-shape
+requests through the solve server, per mode.
 """
 from __future__ import annotations
 
@@ -189,6 +190,7 @@ def serve_section(nreq=100, qps=20.0, seed=0, max_active=16, budget_gb=200.0):
                   "deferred_for_memory": srv.stats["deferred_for_memory"], "shapes": len(srv.shapes),
                   "host_admit_s": srv.stats["admit_s"], "host_advance_s": srv.stats["advance_s"],
                   "host_complete_s": srv.stats["complete_s"],
+                  "batched_fd_steps": srv.stats.get("batched_fd_steps", 0),
                   "host_setup_s": {k[6:-2]: v for k, v in srv.stats.items() if k.startswith("setup_")},
                   "allocator": {k: srv.stats[k] for k in ("alloc_retries", "device_frees", "device_allocs",
                                                            "peak_reserved_gb")}})
@@ -346,6 +348,7 @@ def main():
     ap.add_argument("--spacetime", action="store_true", help="only the space-time GCRODR section")
     ap.add_argument("--serve", type=int, default=0, help="only the serving section, with this many requests")
     ap.add_argument("--qps", type=float, default=20.0)
+    ap.add_argument("--no-serve", action="store_true", help="skip the short serving stream of the default run")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -495,6 +498,10 @@ def main():
             out["roofline"]["traffic_note"] = why
     if not args.no_solve:
         out["solve"] = solve_section(patch, C, m, r_batch)
+    if not args.no_serve:
+        # a short saturated stream (all requests present at once): throughput of the
+        # server per mode; `--serve N --qps Q` runs longer streams at a set rate
+        out["serve"] = serve_section(40, 1e6)
     print(json.dumps(out))
 
 

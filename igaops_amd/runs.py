@@ -159,15 +159,28 @@ class GmresRun:
         Z[:self.cap].copy_(self.Z)
         self.V, self.Z, self.cap = V, Z, cap
 
-    def launch(self):
-        """Queue Arnoldi step k = launched + 1."""
+    def prepare_step(self):
+        """Capacity for step k = launched + 1; returns (k, V[k-1], Z[k-1]) so that a
+        caller can fill Z[k-1] = P V[k-1] itself (batched with other runs) before
+        ``launch(preconditioned=True)``."""
+        k = self.launched + 1
+        with self._on_stream():
+            if k > self.cap:
+                self._grow(k)
+        return k, self.V[k - 1], self.Z[k - 1]
+
+    def launch(self, preconditioned: bool = False):
+        """Queue Arnoldi step k = launched + 1 (with ``preconditioned``, Z[k-1]
+        already holds P V[k-1])."""
         k = self.launched + 1
         dt = dv.DTYPES[self.dtype]
         with self._on_stream():
             if k > self.cap:
                 self._grow(k)
             z = self.Z[k - 1]
-            if self.P is not None:
+            if preconditioned:
+                pass
+            elif self.P is not None:
                 self.P(self.V[k - 1], z)
             else:
                 z.copy_(self.V[k - 1])
@@ -267,7 +280,9 @@ class GmresRun:
         self.V = self.Z = None
 
     # -- driver --
-    def advance(self) -> bool:
+    def advance(self, launch: bool = True) -> bool:
+        """Queue what can be queued without waiting; with ``launch=False`` no
+        Arnoldi step is queued (the caller may batch them)."""
         progressed = False
         if self.launched == 0 and not self.events and self.known < 0:
             self.start()
@@ -275,7 +290,7 @@ class GmresRun:
         if self.result is not None:
             return False
         progressed |= self.poll()
-        while self.can_launch():
+        while launch and self.can_launch():
             self.launch()
             progressed = True
         if self.kfinal is not None:
@@ -286,6 +301,12 @@ class GmresRun:
                 self.complete()
                 progressed = True
         return progressed
+
+    def step_run(self) -> Optional["GmresRun"]:
+        """This solve if it can take an Arnoldi step now."""
+        if self.result is None and self.known >= 0 and self.can_launch():
+            return self
+        return None
 
     @property
     def done(self) -> bool:
@@ -371,7 +392,14 @@ class RefinementRun:
                                  "true-residual" if success else "max_refinements",
                                  history={"inner_iterations": list(self.inner_its)})
 
-    def advance(self) -> bool:
+    def step_run(self) -> Optional[GmresRun]:
+        """The inner solve if it can take an Arnoldi step now."""
+        if self.phase == "inner" and self.inner is not None and self.inner.result is None and self.inner.known >= 0 \
+                and self.inner.can_launch():
+            return self.inner
+        return None
+
+    def advance(self, launch: bool = True) -> bool:
         if self.result is not None:
             return False
         if self.phase == "norm":  # ||b||
@@ -389,7 +417,7 @@ class RefinementRun:
             self._start_inner()
             return True
         if self.phase == "inner":
-            progressed = self.inner.advance()
+            progressed = self.inner.advance(launch)
             if self.inner.done:
                 out = self.inner.result
                 self.inner_its.append(out.iterations)

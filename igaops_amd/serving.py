@@ -22,8 +22,12 @@ How the loop works:
   the active requests in turn and queues whatever each can do without waiting
   (the next Arnoldi steps, a refinement residual, the solution update), so the
   work of many requests is in flight on the device at once.  Same-shape requests
-  share plans and their launches overlap; nothing a request computes depends on
-  which others are present, so its result is the same alone and in the mix.
+  share plans; when several of them are ready for an Arnoldi step in the same
+  pass, their FD preconditioner applications go out as one multi-right-hand-side
+  launch (the FD depends on the shape only) on a batch stream, ordered by events.
+  Nothing a request computes depends on which others are present, so its result
+  is the same alone and in the mix (to rounding: the batched FD sums in the same
+  order per column).
 * Latency is measured from arrival to completion (the true residual checked).
 """
 from __future__ import annotations
@@ -212,7 +216,7 @@ class SolveServer:
 
     def __init__(self, mode: str = "mixed", hbm_budget_gb: float = 200.0, max_active: int = 16, tol: float = 1e-10,
                  maxiters: int = 800, inner_tol: float = 1e-6, inner_maxiters: int = 300, lookahead: int = 2,
-                 keep_solutions: bool = False):
+                 keep_solutions: bool = False, batch_fd: bool = True):
         if mode not in ("mixed", "fp64"):
             raise ValueError("mode must be 'mixed' or 'fp64'")
         self.mode = mode
@@ -222,9 +226,58 @@ class SolveServer:
         self.inner_tol, self.inner_maxiters = float(inner_tol), int(inner_maxiters)
         self.lookahead = int(lookahead)
         self.keep_solutions = keep_solutions
+        self.batch_fd = bool(batch_fd)
+        self._batch_stream = None
+        self._batch_work: Dict[tuple, object] = {}
         self.shapes: Dict[tuple, ShapeResources] = {}
         self.stats = {"max_active": 0, "max_bytes": 0, "deferred_for_memory": 0, "admit_s": 0.0, "advance_s": 0.0,
                       "complete_s": 0.0}
+
+    def _batched_steps(self, states) -> bool:
+        """One Arnoldi step for every group of same-shape requests ready for it, with
+        their preconditioner applications in one multi-right-hand-side FD launch on a
+        shared stream (the FD plan depends on the shape only).  Events order the
+        batch after each request's previous work and each request after the batch."""
+        t = dv.torch()
+        groups: Dict[tuple, list] = {}
+        for st in states:
+            run = st.run.step_run() if st.run is not None else None
+            if run is not None:
+                groups.setdefault((st.req.shape, run.dtype), []).append(run)
+        done = False
+        for (shape_key, dtype), runs in groups.items():
+            if len(runs) < 2:
+                continue
+            g = len(runs)
+            plan = self.shapes[shape_key].fd_plans[dtype]
+            steps = [r.prepare_step() for r in runs]
+            for r in runs:
+                ev = t.cuda.Event()
+                ev.record(r.stream)
+                self._batch_stream.wait_event(ev)
+            wkey = (shape_key, dtype, g)
+            work = self._batch_work.get(wkey)
+            if work is None:
+                with t.cuda.stream(self._batch_stream):
+                    work = plan.workspace(g)
+                self._batch_work[wkey] = work
+            with t.cuda.stream(self._batch_stream):
+                n = runs[0].n
+                X = t.empty((n, g), dtype=runs[0].V.dtype, device=runs[0].V.device)
+                Y = t.empty_like(X)
+                for j, (_, v, _) in enumerate(steps):
+                    X[:, j].copy_(v)
+                plan.apply(X.view(-1), Y.view(-1), g, None, work=work)
+                for j, (_, _, z) in enumerate(steps):
+                    z.copy_(Y[:, j])
+                done_ev = t.cuda.Event()
+                done_ev.record(self._batch_stream)
+            for r in runs:
+                r.stream.wait_event(done_ev)
+                r.launch(preconditioned=True)
+            self.stats["batched_fd_steps"] = self.stats.get("batched_fd_steps", 0) + g
+            done = True
+        return done
 
     def shape(self, key) -> ShapeResources:
         s = self.shapes.get(key)
@@ -246,6 +299,7 @@ class SolveServer:
         dv.require_gpu()
         pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
         streams = [t.cuda.Stream() for _ in range(self.max_active)]
+        self._batch_stream = t.cuda.Stream()
         # one pinned host buffer per stream slot, lent to the solver runs of that slot
         from .runs import GmresRun
 
@@ -287,6 +341,10 @@ class SolveServer:
             self.stats["max_bytes"] = max(self.stats["max_bytes"], used)
             progressed = False
             ta = time.perf_counter()
+            if self.batch_fd:
+                for st, _, _ in active:
+                    progressed |= st.run.advance(launch=False)
+                progressed |= self._batched_steps([st for st, _, _ in active])
             for st, _, _ in active:
                 progressed |= st.advance()
             tb = time.perf_counter()

@@ -44,13 +44,15 @@ def _mix():
 @pytest.mark.parametrize("mode", ["fp64", "mixed"])
 def test_results_same_alone_and_in_mix(mode):
     """Every request of a mixed batch (two of them of the same shape, overlapping on
-    the device) returns the same solution as when it is served alone."""
+    the device, their FD applications batched into shared launches) returns the
+    same solution as when it is served alone."""
     from igaops_amd.serving import SolveServer
 
     print("case", mode, flush=True)
     srv = SolveServer(mode=mode, max_active=6, keep_solutions=True, maxiters=200)
     mix = {r.rid: r for r in srv.serve(_mix(), realtime=False)}
     assert srv.stats["max_active"] >= 4
+    print("batched FD steps", srv.stats.get("batched_fd_steps", 0), flush=True)
     for req in _mix():
         alone = SolveServer(mode=mode, max_active=1, keep_solutions=True, maxiters=200).serve([req],
                                                                                             realtime=False)[0]
