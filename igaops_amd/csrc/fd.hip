@@ -50,54 +50,59 @@ struct GemmDesc {
 
 constexpr int kTile = 64, kKStep = 16;
 
-// C = L R (+ epilogue scale).  Lane layout of the 16x16x4 MFMA (measured on gfx950):
-// A operand: lane l -> A[l % 16][l / 16]; B operand: lane l -> B[l / 16][l % 16];
-// D register r of lane l -> D[Acc<T>::row(l, r)][l % 16].
-template <typename T>
+// C = L R (+ epilogue scale) on TILE x TILE output tiles (64 or 32): four waves in
+// 2 x 2, each owning (TILE/32) x (TILE/32) blocks of the 16x16 MFMA.  Lane layout
+// of the 16x16x4 MFMA (measured on gfx950): A operand: lane l -> A[l % 16][l / 16];
+// B operand: lane l -> B[l / 16][l % 16]; D register r of lane l ->
+// D[Acc<T>::row(l, r)][l % 16].  The small tile serves the short FD factors
+// (n <= 96): more work-groups and less padding for n just above a multiple of 64.
+template <typename T, int TILE>
 __global__ void __launch_bounds__(256) mfma_gemm_kernel(GemmDesc g, const T* __restrict__ L, const T* __restrict__ R,
                                                         T* __restrict__ Cout, const T* __restrict__ scale) {
   using V = typename Acc<T>::V;
-  __shared__ T ls[kTile][kKStep + 1];
-  __shared__ T rs[kKStep][kTile + 1];
+  constexpr int NB = TILE / 32;            // 16x16 blocks per wave and direction
+  constexpr int NE = TILE * kKStep / 256;  // tile elements each thread stages per K step
+  __shared__ T ls[TILE][kKStep + 1];
+  __shared__ T rs[kKStep][TILE + 1];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int m0 = blockIdx.x * kTile, j0 = blockIdx.y * kTile;
+  const int m0 = blockIdx.x * TILE, j0 = blockIdx.y * TILE;
   const int64_t bz = blockIdx.z;
   const T* Lb = L + bz * g.l_batch;
   const T* Rb = R + bz * g.r_batch;
   T* Cb = Cout + bz * g.c_batch;
-  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-  V acc[2][2];
+  const int wm = (wave >> 1) * (TILE / 2), wn = (wave & 1) * (TILE / 2);
+  V acc[NB][NB];
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < NB; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b) acc[a][b] = V{0, 0, 0, 0};
+    for (int b = 0; b < NB; ++b) acc[a][b] = V{0, 0, 0, 0};
 
-  // Each thread stages 4 elements of the L tile (64 x 16) and 4 of the R tile
-  // (16 x 64) per K step.  Their loads are issued kPre steps ahead into a register
-  // ring, from clamped (always valid) addresses on every path, and zeroed outside
-  // the matrix when written to LDS; so the K loop waits on one load latency, not
-  // on one per step.
+  // Each thread stages NE elements of the L tile (TILE x 16) and NE of the R tile
+  // (16 x TILE) per K step.  Their loads are issued kPre steps ahead into a
+  // register ring, from clamped (always valid) addresses on every path, and zeroed
+  // outside the matrix when written to LDS; so the K loop waits on one load
+  // latency, not on one per step.
   constexpr int kPre = 4;
   const int nsteps = (g.K + kKStep - 1) / kKStep;
-  int l_mm[4], l_kk[4], r_kk[4], r_jj[4];
+  int l_mm[NE], l_kk[NE], r_kk[NE], r_jj[NE];
 #pragma unroll
-  for (int e = 0; e < 4; ++e) {
+  for (int e = 0; e < NE; ++e) {
     const int idx = tid + e * 256;
     l_mm[e] = idx / kKStep;
     l_kk[e] = idx % kKStep;
     if (g.rj == 1) {
-      r_kk[e] = idx / kTile;
-      r_jj[e] = idx % kTile;
+      r_kk[e] = idx / TILE;
+      r_jj[e] = idx % TILE;
     } else {
       r_jj[e] = idx / kKStep;
       r_kk[e] = idx % kKStep;
     }
   }
-  T lreg[kPre][4], rreg[kPre][4];
-  auto load_step = [&](int s, T (&lr)[4], T (&rr)[4]) __attribute__((always_inline)) {
+  T lreg[kPre][NE], rreg[kPre][NE];
+  auto load_step = [&](int s, T (&lr)[NE], T (&rr)[NE]) __attribute__((always_inline)) {
     const int k0 = (s < nsteps ? s : nsteps - 1) * kKStep;
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
+    for (int e = 0; e < NE; ++e) {
       const int m = min(m0 + l_mm[e], g.M - 1), k = min(k0 + l_kk[e], g.K - 1);
       lr[e] = Lb[(int64_t)m * g.lm + (int64_t)k * g.lk];
       const int kr = min(k0 + r_kk[e], g.K - 1), j = min(j0 + r_jj[e], g.N - 1);
@@ -115,7 +120,7 @@ __global__ void __launch_bounds__(256) mfma_gemm_kernel(GemmDesc g, const T* __r
         const int k0 = st * kKStep;
         __syncthreads();
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
+        for (int e = 0; e < NE; ++e) {
           const bool lok = (m0 + l_mm[e] < g.M) && (k0 + l_kk[e] < g.K);
           ls[l_mm[e]][l_kk[e]] = lok ? lreg[b][e] : T(0);
           const bool rok = (k0 + r_kk[e] < g.K) && (j0 + r_jj[e] < g.N);
@@ -125,23 +130,23 @@ __global__ void __launch_bounds__(256) mfma_gemm_kernel(GemmDesc g, const T* __r
         load_step(st + kPre, lreg[b], rreg[b]);
 #pragma unroll
         for (int k4 = 0; k4 < kKStep; k4 += 4) {
-          T av[2], bv[2];
+          T av[NB], bv[NB];
 #pragma unroll
-          for (int a = 0; a < 2; ++a) av[a] = ls[wm + a * 16 + (lane & 15)][k4 + (lane >> 4)];
+          for (int a = 0; a < NB; ++a) av[a] = ls[wm + a * 16 + (lane & 15)][k4 + (lane >> 4)];
 #pragma unroll
-          for (int c = 0; c < 2; ++c) bv[c] = rs[k4 + (lane >> 4)][wn + c * 16 + (lane & 15)];
+          for (int c = 0; c < NB; ++c) bv[c] = rs[k4 + (lane >> 4)][wn + c * 16 + (lane & 15)];
 #pragma unroll
-          for (int a = 0; a < 2; ++a)
+          for (int a = 0; a < NB; ++a)
 #pragma unroll
-            for (int c = 0; c < 2; ++c) acc[a][c] = Acc<T>::mfma(av[a], bv[c], acc[a][c]);
+            for (int c = 0; c < NB; ++c) acc[a][c] = Acc<T>::mfma(av[a], bv[c], acc[a][c]);
         }
       }
     }
   }
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < NB; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b)
+    for (int b = 0; b < NB; ++b)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int m = m0 + wm + a * 16 + Acc<T>::row(lane, r);
@@ -178,6 +183,15 @@ static void mode_gemm_rect(const T* A, const T* X, T* Y, int64_t outer, int64_t 
                            const T* scale, hipStream_t s) {
   GemmDesc g{};
   dim3 grid;
+  // 32 x 32 tiles for short factors (more work-groups, less padding), else 64 x 64
+  const int tile = n_out <= 96 ? 32 : kTile;
+  auto launch = [&](dim3 gr, const T* Lp, const T* Rp, T* Yp, const T* sc) {
+    if (tile == 32)
+      mfma_gemm_kernel<T, 32><<<gr, 256, 0, s>>>(g, Lp, Rp, Yp, sc);
+    else
+      mfma_gemm_kernel<T, kTile><<<gr, 256, 0, s>>>(g, Lp, Rp, Yp, sc);
+    IGA_LAUNCH_CHECK();
+  };
   if (inner > 1) {
     g.M = (int)n_out;
     g.N = (int)inner;
@@ -192,10 +206,8 @@ static void mode_gemm_rect(const T* A, const T* X, T* Y, int64_t outer, int64_t 
     constexpr int64_t kZMax = 65535;
     for (int64_t o0 = 0; o0 < outer; o0 += kZMax) {
       const int64_t nb = std::min(kZMax, outer - o0);
-      grid = dim3((unsigned)((n_out + kTile - 1) / kTile), (unsigned)((inner + kTile - 1) / kTile), (unsigned)nb);
-      mfma_gemm_kernel<T><<<grid, 256, 0, s>>>(g, A, X + o0 * g.r_batch, Y + o0 * g.c_batch,
-                                               scale ? scale + o0 * g.scale_batch : nullptr);
-      IGA_LAUNCH_CHECK();
+      grid = dim3((unsigned)((n_out + tile - 1) / tile), (unsigned)((inner + tile - 1) / tile), (unsigned)nb);
+      launch(grid, A, X + o0 * g.r_batch, Y + o0 * g.c_batch, scale ? scale + o0 * g.scale_batch : nullptr);
     }
     return;
   } else {
@@ -207,10 +219,9 @@ static void mode_gemm_rect(const T* A, const T* X, T* Y, int64_t outer, int64_t 
     g.cm = n_out; g.cj = 1;
     g.scale_m = n_out; g.scale_j = 1;
     IGA_CHECK_ARG(outer < (int64_t)1 << 30, "mode product too large for the grid");
-    grid = dim3((unsigned)((outer + kTile - 1) / kTile), (unsigned)((n_out + kTile - 1) / kTile), 1);
+    grid = dim3((unsigned)((outer + tile - 1) / tile), (unsigned)((n_out + tile - 1) / tile), 1);
   }
-  mfma_gemm_kernel<T><<<grid, 256, 0, s>>>(g, X, A, Y, scale);
-  IGA_LAUNCH_CHECK();
+  launch(grid, X, A, Y, scale);
 }
 
 template <typename T>
