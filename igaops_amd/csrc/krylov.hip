@@ -39,61 +39,176 @@ static int reduce_blocks(int64_t n) {
   return (int)g;
 }
 
-// Many-dot products V w: blocks tile (element range) x (group of kDotVec vectors);
-// every thread keeps one partial sum per vector of its group over its elements, the
-// 64 lanes of a wave are combined by shuffles, and each wave writes its partials
-// (deterministic: fixed ranges, fixed combination order).
-constexpr int kDotVec = 16;
-constexpr int kDotChunk = 1024;  // elements per block when n is small enough
-constexpr int kDotMaxChunks = 4096;
+// Many-dot products of the rows of V with one or two vectors x0, x1.  Stage 1:
+// blocks tile (element range) x (group of rows); each thread keeps kDotAcc
+// partial sums (one per row of its group and vector), reads 16-byte vectors where
+// the layout allows it, the 64 lanes of a wave are combined by a transposed
+// butterfly, the waves of a block through LDS, and each block writes one partial
+// per (vector, row).  Stage 2 sums the partials of every block in a fixed order
+// (one wave per output), so the result does not depend on the scheduling of the
+// blocks.
+constexpr int kDotAcc = 16;
+constexpr int kDotBlocksTarget = 4096;
+constexpr int kDotMaxVectors = 16384;
+constexpr int kDotElemsPerThread = 2;  // 16-byte vectors per thread and chunk (measured best of 1, 2, 4, 8)
 
-static int dot_chunks(int64_t n) {
-  int64_t g = (n + kDotChunk - 1) / kDotChunk;
+template <typename T> struct Vec16;
+template <> struct Vec16<double> {
+  using type = double2;
+  static constexpr int width = 2;
+  __device__ static double dot(const type& a, const type& b) { return a.x * b.x + a.y * b.y; }
+};
+template <> struct Vec16<float> {
+  using type = float4;
+  static constexpr int width = 4;
+  __device__ static double dot(const type& a, const type& b) {
+    return (double)a.x * (double)b.x + (double)a.y * (double)b.y + (double)a.z * (double)b.z +
+           (double)a.w * (double)b.w;
+  }
+};
+
+// element chunks (blockIdx.x) for `groups` row groups (blockIdx.y)
+static int dot_chunks(int64_t n, int groups, int esize) {
+  int64_t cap = kDotBlocksTarget / (groups > 0 ? groups : 1);
+  if (cap < 64) cap = 64;
+  const int64_t chunk = (int64_t)kRedThreads * (16 / esize) * kDotElemsPerThread;
+  int64_t g = (n + chunk - 1) / chunk;
   if (g < 1) g = 1;
-  if (g > kDotMaxChunks) g = kDotMaxChunks;
+  if (g > cap) g = cap;
   return (int)g;
 }
 
-// partial[(b * 4 + wave) * k + j] for chunk b
-template <typename T>
+// partial[(x * k + j) * nb + b] = sum over chunk b of V_j x_x
+template <typename T, bool VECTOR, int NX>
 __global__ void __launch_bounds__(kRedThreads) dot_many_stage1(int64_t n, int k, const T* __restrict__ V,
-                                                               int64_t ldv, const T* __restrict__ w,
+                                                               int64_t ldv, const T* __restrict__ x0,
+                                                               const T* __restrict__ x1,
                                                                double* __restrict__ partial) {
+  constexpr int G = kDotAcc / NX;  // rows per group
+  __shared__ double red[kRedThreads / 64][kDotAcc];
   const int nb = gridDim.x;
-  const int64_t per = ((n + nb - 1) / nb + kRedThreads - 1) / kRedThreads * kRedThreads;
-  const int64_t lo = (int64_t)blockIdx.x * per;
-  const int64_t hi = lo + per < n ? lo + per : n;
-  const int j0 = blockIdx.y * kDotVec;
-  const int kb = k - j0 < kDotVec ? k - j0 : kDotVec;
-  double acc[kDotVec];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int j0 = blockIdx.y * G;
+  const int kb = k - j0 < G ? k - j0 : G;
+  double acc[kDotAcc];  // acc[x * G + jj]
 #pragma unroll
-  for (int jj = 0; jj < kDotVec; ++jj) acc[jj] = 0.0;
+  for (int q = 0; q < kDotAcc; ++q) acc[q] = 0.0;
   const T* Vb = V + (int64_t)j0 * ldv;
-  for (int64_t i = lo + threadIdx.x; i < hi; i += kRedThreads) {
-    const double wv = (double)w[i];
+  int64_t body = 0;
+  if (VECTOR) {
+    using VT = typename Vec16<T>::type;
+    constexpr int W = Vec16<T>::width;
+    const int64_t nv = n / W;
+    body = nv * W;
+    const int64_t per = (nv + nb - 1) / nb;
+    const int64_t lo = (int64_t)blockIdx.x * per;
+    const int64_t hi = lo + per < nv ? lo + per : nv;
+    for (int64_t i = lo + tid; i < hi; i += kRedThreads) {
+      const VT a = reinterpret_cast<const VT*>(x0)[i];
+      VT b{};
+      if (NX == 2) b = reinterpret_cast<const VT*>(x1)[i];
 #pragma unroll
-    for (int jj = 0; jj < kDotVec; ++jj)
-      if (jj < kb) acc[jj] += (double)Vb[(int64_t)jj * ldv + i] * wv;
-  }
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  double* out = partial + ((int64_t)blockIdx.x * (kRedThreads / 64) + wave) * k + j0;
-#pragma unroll
-  for (int jj = 0; jj < kDotVec; ++jj) {
-    if (jj < kb) {
-      const double v = wave_sum(acc[jj]);
-      if (lane == 0) out[jj] = v;
+      for (int jj = 0; jj < G; ++jj) {
+        if (jj < kb) {
+          const VT v = reinterpret_cast<const VT*>(Vb + (int64_t)jj * ldv)[i];
+          acc[jj] += Vec16<T>::dot(v, a);
+          if (NX == 2) acc[G + jj] += Vec16<T>::dot(v, b);
+        }
+      }
     }
+  }
+  {
+    // scalar elements: all of them (no vector layout) or the tail beyond the last whole vector
+    const int64_t m = n - body;
+    const int64_t per = VECTOR ? m : (m + nb - 1) / nb;
+    const int64_t lo = VECTOR ? (blockIdx.x == nb - 1 ? 0 : m) : (int64_t)blockIdx.x * per;
+    const int64_t hi = lo + per < m ? lo + per : m;
+    for (int64_t i = body + lo + tid; i < body + hi; i += kRedThreads) {
+      const double a = (double)x0[i];
+      const double b = NX == 2 ? (double)x1[i] : 0.0;
+#pragma unroll
+      for (int jj = 0; jj < G; ++jj) {
+        if (jj < kb) {
+          const double v = (double)Vb[(int64_t)jj * ldv + i];
+          acc[jj] += v * a;
+          if (NX == 2) acc[G + jj] += v * b;
+        }
+      }
+    }
+  }
+  // transposed butterfly over the 64 lanes: each exchange halves the values a lane
+  // holds, so lane 4 q ends with the wave's sum of accumulator q (17 exchanges, not 96)
+#pragma unroll
+  for (int st = 0; st < 4; ++st) {
+    const int off = 32 >> st, c = kDotAcc >> (st + 1);
+    const bool upper = (lane & off) != 0;
+#pragma unroll
+    for (int q = 0; q < c; ++q) {
+      const double give = upper ? acc[q] : acc[q + c];
+      const double keep = upper ? acc[q + c] : acc[q];
+      acc[q] = keep + __shfl_xor(give, off, 64);
+    }
+  }
+  acc[0] += __shfl_xor(acc[0], 2, 64);
+  acc[0] += __shfl_xor(acc[0], 1, 64);
+  if ((lane & 3) == 0) red[wave][lane >> 2] = acc[0];
+  __syncthreads();
+  if (tid < kDotAcc && tid % G < kb) {
+    double sum = 0.0;
+#pragma unroll
+    for (int q = 0; q < kRedThreads / 64; ++q) sum += red[q][tid];
+    const int x = tid / G, jj = tid % G;
+    partial[((int64_t)x * k + j0 + jj) * nb + blockIdx.x] = sum;
   }
 }
 
-__global__ void __launch_bounds__(kRedThreads) dot_many_stage2(int nb, int k, const double* __restrict__ partial,
-                                                               double* __restrict__ out) {
-  __shared__ double lds[kRedThreads / 64];
-  const int j = blockIdx.x;
-  double acc = 0.0;
-  for (int b = threadIdx.x; b < nb; b += blockDim.x) acc += partial[(int64_t)b * k + j];
-  const double s = block_sum(acc, lds);
-  if (threadIdx.x == 0) out[j] = s;
+// out[x * ostride + j] = sum over b of partial[(x * k + j) * nb + b]: one wave per output
+__global__ void __launch_bounds__(kRedThreads) dot_many_stage2(int nb, int k, int nx,
+                                                               const double* __restrict__ partial,
+                                                               double* __restrict__ out, int64_t ostride) {
+  const int o = blockIdx.x * (kRedThreads / 64) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (o >= nx * k) return;
+  const double* po = partial + (int64_t)o * nb;
+  double s = 0.0;
+  for (int b = lane; b < nb; b += 64) s += po[b];
+  s = wave_sum(s);
+  if (lane == 0) out[(int64_t)(o / k) * ostride + o % k] = s;
+}
+
+// Both stages; nx = 1 (x1 unused) or 2.
+static void dot_many_launch(int dtype, int64_t n, int k, const void* V, int64_t ldv, const void* x0,
+                            const void* x1, int nx, double* out, int64_t ostride, void* work, hipStream_t s) {
+  const int G = kDotAcc / nx;
+  const int groups = (k + G - 1) / G;
+  const int nb = dot_chunks(n, groups, dtype == IGA_F32 ? 4 : 8);
+  double* partial = (double*)work;
+  const dim3 grid(nb, groups);
+  auto aligned = [](const void* p) { return p == nullptr || ((uintptr_t)p & 15) == 0; };
+  const int width = dtype == IGA_F32 ? 4 : 2;
+  const bool vec = ldv % width == 0 && aligned(V) && aligned(x0) && aligned(x1);
+#define IGA_DOT1(T, VEC, NX) \
+  dot_many_stage1<T, VEC, NX><<<grid, kRedThreads, 0, s>>>(n, k, (const T*)V, ldv, (const T*)x0, (const T*)x1, partial)
+#define IGA_DOT2(T, VEC) \
+  do {                    \
+    if (nx == 2)          \
+      IGA_DOT1(T, VEC, 2); \
+    else                  \
+      IGA_DOT1(T, VEC, 1); \
+  } while (0)
+  if (dtype == IGA_F64) {
+    if (vec) IGA_DOT2(double, true); else IGA_DOT2(double, false);
+  } else if (dtype == IGA_F32) {
+    if (vec) IGA_DOT2(float, true); else IGA_DOT2(float, false);
+  } else {
+    throw Error(IGA_ERR_INVALID, "unknown dtype");
+  }
+#undef IGA_DOT2
+#undef IGA_DOT1
+  IGA_LAUNCH_CHECK();
+  const int outs = nx * k;
+  dot_many_stage2<<<(outs + kRedThreads / 64 - 1) / (kRedThreads / 64), kRedThreads, 0, s>>>(nb, k, nx, partial,
+                                                                                            out, ostride);
+  IGA_LAUNCH_CHECK();
 }
 
 template <typename T>
@@ -182,22 +297,47 @@ __global__ void cast_kernel(int64_t n, const S* __restrict__ x, D* __restrict__ 
 }
 
 // ---- device-side GMRES step -------------------------------------------------
+// Arnoldi with delayed classical Gram-Schmidt reorthogonalisation (DCGS2:
+// Bielich, Langou, Thomas, Swirydowicz, Yamazaki, Boman, "Low-synch Gram-Schmidt
+// with delayed reorthogonalization for Krylov solvers", Parallel Computing 2022).
+// Step k (1-based) finds the basis q_0..q_{j-1} (j = k - 1) final in V[0..j) and
+// in V[j] the tentative vector u_j = A' q_{j-1} - Q hcol_{j-1}, projected once and
+// not normalised; w = A' u_j.  One pass over V[0..j] and w gives
+//   s = Q^T u_j, a = u_j.u_j, t = Q^T w, c = u_j.w,
+// from which, on the device: rho = sqrt(a - s.s) and q_j = (u_j - Q s) / rho; the
+// Hessenberg column j-1 becomes final (hcol_{j-1} + s, rho) and its Givens
+// rotation yields the residual estimate of Krylov dimension j; with
+// y = H[0:j+1, 0:j] s, A' q_j = (w - Q_{j+1} y) / rho is projected once:
+//   hcol_j = (Q_{j+1}^T w - y) / rho,   u_{j+1} = w / rho - Q_{j+1} (y / rho + hcol_j).
+// A second pass over V[0..j] and w writes q_j into V[j] and u_{j+1} into V[j+1].
+// Two passes over the basis per step instead of the four of two-pass classical
+// Gram-Schmidt; the residual estimate of dimension j arrives with step j + 1.
+//
 // Per-solve state (float64) for Krylov dimension m, at fixed offsets:
-//   [0, m)       first Gram-Schmidt pass, [m, 2m) second pass,
-//   2m           ||w||^2, 2m+1 the scale of the new basis vector (1/||w||, or 1/beta),
-//   R            m x m, column-major, the rotated Hessenberg columns,
-//   cs, sn       the Givens rotations, g (m+1) the rotated right-hand side beta e_1,
-//   hist         2 (m+1): (g[k], ||w_k||) per step, the values a host polls.
-constexpr int kGmMaxDim = 2048;  // the rotation kernel stages 3 k doubles in LDS
+//   R (m x m, column-major, the rotated Hessenberg columns), cs, sn (m): the
+//   Givens rotations, g (m+1): the rotated beta e_1, hist 2 (m+1): (g[d], rho_d)
+//   for dimension d, the values a host polls; U, W (m+1 each): the dots of the
+//   first pass with u and w; cq, cw (m+1 each) and scal: the second pass's
+//   coefficients; H ((m+1) x (m+1), column-major): the Hessenberg matrix, its
+//   last column pending.
+constexpr int kGmMaxDim = 2000;  // the step kernel stages 4 m + 1 doubles in LDS (64 KiB)
+constexpr int kGmThreads = 256;
 
 struct GmState {
   int m;
-  __host__ __device__ int64_t R() const { return 2 * (int64_t)m + 2; }
-  __host__ __device__ int64_t cs() const { return R() + (int64_t)m * m; }
+  __host__ __device__ int64_t R() const { return 0; }
+  __host__ __device__ int64_t cs() const { return (int64_t)m * m; }
   __host__ __device__ int64_t sn() const { return cs() + m; }
   __host__ __device__ int64_t g() const { return sn() + m; }
   __host__ __device__ int64_t hist() const { return g() + m + 1; }
-  __host__ __device__ int64_t size() const { return hist() + 2 * ((int64_t)m + 1); }
+  __host__ __device__ int64_t U() const { return hist() + 2 * ((int64_t)m + 1); }
+  __host__ __device__ int64_t W() const { return U() + m + 1; }
+  __host__ __device__ int64_t cq() const { return W() + m + 1; }
+  __host__ __device__ int64_t cw() const { return cq() + m + 1; }
+  __host__ __device__ int64_t scal() const { return cw() + m + 1; }  // 1/rho, z_j/rho, beta^2, 1/beta
+  __host__ __device__ int64_t H() const { return scal() + 8; }
+  __host__ __device__ int64_t ldh() const { return (int64_t)m + 1; }
+  __host__ __device__ int64_t size() const { return H() + ldh() * ldh(); }
 };
 
 __global__ void gm_zero_kernel(int64_t n, double* __restrict__ p) {
@@ -205,62 +345,159 @@ __global__ void gm_zero_kernel(int64_t n, double* __restrict__ p) {
     p[i] = 0.0;
 }
 
-// beta = ||r|| from ls[2m]; g[0] = beta, scale = 1/beta (0 for r = 0)
+// beta = ||r|| from scal[2]; g[0] = beta, scal[3] = 1/beta (0 for r = 0), hist[0] = (beta, beta)
 __global__ void gm_start_kernel(GmState st, double* ls) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  const double beta = sqrt(fmax(ls[2 * st.m], 0.0));
+  double* sc = ls + st.scal();
+  const double beta = sqrt(fmax(sc[2], 0.0));
   ls[st.g()] = beta;
-  ls[2 * st.m + 1] = beta > 0.0 ? 1.0 / beta : 0.0;
+  sc[3] = beta > 0.0 ? 1.0 / beta : 0.0;
   ls[st.hist()] = beta;
   ls[st.hist() + 1] = beta;
 }
 
-// Step k (1-based): Hessenberg column h = pass1 + pass2, h[k] = ||w||, rotated by the
-// k-1 previous rotations; a new rotation annihilates h[k] and updates g.  The column
-// and the rotations are staged through LDS by one wave, the sequential rotation
-// sweep runs on one lane.
-__global__ void __launch_bounds__(64) gm_rotate_kernel(GmState st, int k, double* ls) {
-  extern __shared__ double sh[];  // h[k], c[k], s[k]
-  double* h = sh;
-  double* c = sh + k;
-  double* sv = sh + 2 * k;
-  const int m = st.m;
-  double* Rk = ls + st.R() + (int64_t)(k - 1) * m;
-  double* cs = ls + st.cs();
-  double* sn = ls + st.sn();
-  double* g = ls + st.g();
-  for (int i = threadIdx.x; i < k; i += blockDim.x) {
-    h[i] = ls[i] + ls[m + i];
-    c[i] = cs[i];
-    sv[i] = sn[i];
+// Block-wide sum in a fixed order; valid in every thread.
+__device__ inline double gm_block_sum(double v, double* red) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int q = 0; q < kGmThreads / 64; ++q) s += red[q];
+  __syncthreads();
+  return s;
+}
+
+// The small dense part of step j + 1 (one block): finalise column j - 1, rotate
+// it, form hcol_j and the second pass's coefficients.
+__global__ void __launch_bounds__(kGmThreads) gm_dcgs2_kernel(GmState st, int j, double* ls) {
+  extern __shared__ double sh[];  // s[j] | rotation stage: col[j+1], c[j], sn[j]  (y[j+1] reuses col)
+  __shared__ double red[kGmThreads / 64];
+  const int tid = threadIdx.x;
+  const int64_t ldh = st.ldh();
+  double* H = ls + st.H();
+  double* s = sh;
+  double* col = sh + j;
+  double* cr = col + j + 1;
+  double* sr = cr + j;
+  const double* U = ls + st.U();
+  const double* Wd = ls + st.W();
+  double* sc = ls + st.scal();
+  double ss = 0.0, stw = 0.0;
+  for (int l = tid; l < j; l += kGmThreads) {
+    const double sl = U[l];
+    s[l] = sl;
+    ss += sl * sl;
+    stw += sl * Wd[l];
+  }
+  ss = gm_block_sum(ss, red);
+  stw = gm_block_sum(stw, red);
+  const double a = U[j], c = Wd[j];
+  double rho = 1.0;  // step 1: V[0] = r / beta is final already
+  if (j > 0) {
+    const double r2 = a - ss;
+    rho = r2 > 1e-14 * a ? sqrt(r2) : 0.0;  // breakdown: u_j lies in the span of Q
+  }
+  const double inv = rho > 0.0 ? 1.0 / rho : 0.0;
+  if (j > 0) {
+    // final column j - 1: hcol_{j-1} + s, rho; staged with the rotations
+    double* Hc = H + (int64_t)(j - 1) * ldh;
+    for (int l = tid; l < j; l += kGmThreads) {
+      const double v = Hc[l] + s[l];
+      Hc[l] = v;
+      col[l] = v;
+    }
+    if (tid == 0) {
+      Hc[j] = rho;
+      col[j] = rho;
+    }
+    for (int l = tid; l + 1 < j; l += kGmThreads) {
+      cr[l] = ls[st.cs() + l];
+      sr[l] = ls[st.sn() + l];
+    }
+    __syncthreads();
+    if (tid == 0) {
+      for (int i = 0; i + 1 < j; ++i) {
+        const double x = col[i], z = col[i + 1];
+        col[i] = cr[i] * x + sr[i] * z;
+        col[i + 1] = -sr[i] * x + cr[i] * z;
+      }
+      const double x = col[j - 1];
+      double cc = 1.0, sv = 0.0;
+      if (rho != 0.0) {
+        const double r = hypot(x, rho);
+        cc = x / r;
+        sv = rho / r;
+      }
+      col[j - 1] = cc * x + sv * rho;
+      ls[st.cs() + j - 1] = cc;
+      ls[st.sn() + j - 1] = sv;
+      double* g = ls + st.g();
+      const double gj1 = g[j - 1];
+      g[j] = -sv * gj1;
+      g[j - 1] = cc * gj1;
+      ls[st.hist() + 2 * j] = -sv * gj1;
+      ls[st.hist() + 2 * j + 1] = rho;
+    }
+    __syncthreads();
+    double* Rc = ls + st.R() + (int64_t)(j - 1) * st.m;
+    for (int l = tid; l < j; l += kGmThreads) Rc[l] = col[l];
+    __syncthreads();
+  }
+  // y = H[0:j+1, 0:j] s (Hessenberg: row i starts at column i - 1), into col
+  double* y = col;
+  for (int i = tid; i <= j; i += kGmThreads) {
+    double acc = 0.0;
+    for (int l = i > 0 ? i - 1 : 0; l < j; ++l) acc += H[(int64_t)l * ldh + i] * s[l];
+    y[i] = acc;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    const double nw = sqrt(fmax(ls[2 * m], 0.0));
-    for (int i = 0; i + 1 < k; ++i) {
-      const double a = h[i], b = h[i + 1];
-      h[i] = c[i] * a + sv[i] * b;
-      h[i + 1] = -sv[i] * a + c[i] * b;
-    }
-    const double a = h[k - 1];
-    double cc = 1.0, ss = 0.0;
-    if (nw != 0.0) {
-      const double r = hypot(a, nw);
-      cc = a / r;
-      ss = nw / r;
-    }
-    cs[k - 1] = cc;
-    sn[k - 1] = ss;
-    h[k - 1] = cc * a + ss * nw;
-    const double gk1 = g[k - 1];
-    g[k] = -ss * gk1;
-    g[k - 1] = cc * gk1;
-    ls[2 * m + 1] = nw > 0.0 ? 1.0 / nw : 0.0;
-    ls[st.hist() + 2 * k] = -ss * gk1;
-    ls[st.hist() + 2 * k + 1] = nw;
+  // hcol_j = (Q_{j+1}^T w - y) / rho with q_j.w = (c - s.t) / rho; z = y / rho + hcol_j
+  const double qw = (c - stw) * inv;
+  const double hj = (qw - y[j]) * inv;
+  const double zj = y[j] * inv + hj;
+  double* Hj = H + (int64_t)j * ldh;
+  double* cq = ls + st.cq();
+  double* cw = ls + st.cw();
+  for (int l = tid; l < j; l += kGmThreads) {
+    const double h = (Wd[l] - y[l]) * inv;
+    const double z = y[l] * inv + h;
+    Hj[l] = h;
+    cq[l] = s[l] * inv;           // q_j = u_j / rho - Q (s / rho)
+    cw[l] = z - zj * s[l] * inv;  // u_{j+1} = w / rho - (z_j / rho) u_j - Q (z - z_j s / rho)
+  }
+  if (tid == 0) {
+    Hj[j] = hj;
+    sc[0] = inv;
+    sc[1] = zj * inv;
+  }
+}
+
+// q_j = inv u_j - Q cq -> V[j];  u_{j+1} = inv w - zinv u_j - Q cw -> V[j+1]
+template <typename T>
+__global__ void gm_update_kernel(int64_t n, int j, T* __restrict__ V, int64_t ldv, const T* __restrict__ w,
+                                 const double* __restrict__ cq, const double* __restrict__ cw,
+                                 const double* __restrict__ scal) {
+  extern __shared__ double hs[];  // cq[j], cw[j]
+  for (int l = threadIdx.x; l < j; l += blockDim.x) {
+    hs[l] = cq[l];
+    hs[j + l] = cw[l];
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < k; i += blockDim.x) Rk[i] = h[i];
+  const double inv = scal[0], zinv = scal[1];
+  T* Vj = V + (int64_t)j * ldv;
+  T* Vn = Vj + ldv;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double aq = 0.0, aw = 0.0;
+    for (int l = 0; l < j; ++l) {
+      const double v = (double)V[(int64_t)l * ldv + i];
+      aq += hs[l] * v;
+      aw += hs[j + l] * v;
+    }
+    const double u = (double)Vj[i];
+    Vj[i] = (T)(inv * u - aq);
+    Vn[i] = (T)(inv * (double)w[i] - zinv * u - aw);
+  }
 }
 
 }  // namespace iga
@@ -272,8 +509,17 @@ extern "C" {
 int64_t iga_reduce_workspace_bytes(int64_t n, int k) {
   if (n < 0 || k < 0) return 0;
   const int64_t kk = k > 0 ? k : 1;
-  const int64_t dots = (int64_t)dot_chunks(n) * (kRedThreads / 64) * kk;  // iga_dot_many
-  const int64_t zdots = (int64_t)reduce_blocks(n) * 2 * kk;              // iga_zdot_many
+  // many-dots with one or two vectors: one partial per (vector, row, chunk), for every k' <= k
+  int64_t dots = 0;
+  for (int nx = 1; nx <= 2; ++nx) {
+    const int G = kDotAcc / nx;
+    for (int64_t g = 1; g <= (kk + G - 1) / G; ++g) {
+      const int64_t kg = g * G < kk ? g * G : kk;
+      const int64_t b = (int64_t)dot_chunks(n, (int)g, 8) * kg * nx;
+      if (b > dots) dots = b;
+    }
+  }
+  const int64_t zdots = (int64_t)reduce_blocks(n) * 2 * kk;  // iga_zdot_many
   return (dots > zdots ? dots : zdots) * (int64_t)sizeof(double);
 }
 
@@ -281,20 +527,9 @@ int iga_dot_many(int dtype, int64_t n, int k, const void* V, int64_t ldv, const 
                  void* work, void* stream) {
   return guard([&] {
     IGA_CHECK_ARG(V && w && out && work && n >= 0 && k >= 0 && ldv >= n, "bad argument");
+    IGA_CHECK_ARG(k <= kDotMaxVectors, "too many vectors");
     if (k == 0) return;
-    hipStream_t s = as_stream(stream);
-    const int nb = dot_chunks(n);
-    double* partial = (double*)work;
-    const dim3 grid(nb, (k + kDotVec - 1) / kDotVec);
-    if (dtype == IGA_F64)
-      dot_many_stage1<double><<<grid, kRedThreads, 0, s>>>(n, k, (const double*)V, ldv, (const double*)w, partial);
-    else if (dtype == IGA_F32)
-      dot_many_stage1<float><<<grid, kRedThreads, 0, s>>>(n, k, (const float*)V, ldv, (const float*)w, partial);
-    else
-      throw Error(IGA_ERR_INVALID, "unknown dtype");
-    IGA_LAUNCH_CHECK();
-    dot_many_stage2<<<k, kRedThreads, 0, s>>>(nb * (kRedThreads / 64), k, partial, out);
-    IGA_LAUNCH_CHECK();
+    dot_many_launch(dtype, n, k, V, ldv, w, nullptr, 1, out, k, work, as_stream(stream));
   });
 }
 
@@ -386,18 +621,28 @@ int64_t iga_gmres_state_doubles(int m) {
   return GmState{m}.size();
 }
 
+int iga_gmres_layout(int m, int64_t* offsets) {
+  return guard([&] {
+    IGA_CHECK_ARG(offsets && m >= 1, "bad argument");
+    const GmState st{m};
+    offsets[0] = st.R();
+    offsets[1] = st.g();
+    offsets[2] = st.hist();
+  });
+}
+
 int iga_gmres_start(int dtype, int64_t n, int m, const void* r, void* V0, double* ls, void* work, void* stream) {
   return guard([&] {
     IGA_CHECK_ARG(r && V0 && ls && work && n >= 1 && m >= 1, "bad argument");
-    IGA_CHECK_ARG(m <= kGmMaxDim, "Krylov dimension above 2048");
+    IGA_CHECK_ARG(m <= kGmMaxDim, "Krylov dimension above 2000");
     hipStream_t s = as_stream(stream);
     const GmState st{m};
     gm_zero_kernel<<<grid_for(st.size(), 256, 1024), 256, 0, s>>>(st.size(), ls);
     IGA_LAUNCH_CHECK();
-    IGA_CHECK(iga_dot_many(dtype, n, 1, r, n, r, ls + 2 * m, work, stream));
+    IGA_CHECK(iga_dot_many(dtype, n, 1, r, n, r, ls + st.scal() + 2, work, stream));
     gm_start_kernel<<<1, 64, 0, s>>>(st, ls);
     IGA_LAUNCH_CHECK();
-    IGA_CHECK(iga_axpby(dtype, n, 0.0, ls + 2 * m + 1, 0, r, 0.0, V0, stream));
+    IGA_CHECK(iga_axpby(dtype, n, 0.0, ls + st.scal() + 3, 0, r, 0.0, V0, stream));
   });
 }
 
@@ -405,21 +650,28 @@ int iga_gmres_step(int dtype, int64_t n, int k, int m, void* V, int64_t ldv, voi
                    void* stream) {
   return guard([&] {
     IGA_CHECK_ARG(V && w && ls && work && n >= 1 && ldv >= n, "bad argument");
-    IGA_CHECK_ARG(k >= 1 && k <= m, "step outside the Krylov dimension");
-    IGA_CHECK_ARG(m <= kGmMaxDim, "Krylov dimension above 2048");
+    IGA_CHECK_ARG(k >= 1 && k <= m + 1, "step outside the Krylov dimension");
+    IGA_CHECK_ARG(m <= kGmMaxDim, "Krylov dimension above 2000");
+    IGA_CHECK_ARG(dtype == IGA_F64 || dtype == IGA_F32, "unknown dtype");
     hipStream_t s = as_stream(stream);
     const GmState st{m};
-    // classical Gram-Schmidt, two passes
-    IGA_CHECK(iga_dot_many(dtype, n, k, V, ldv, w, ls, work, stream));
-    IGA_CHECK(iga_axpy_many(dtype, n, k, V, ldv, ls, -1.0, w, stream));
-    IGA_CHECK(iga_dot_many(dtype, n, k, V, ldv, w, ls + m, work, stream));
-    IGA_CHECK(iga_axpy_many(dtype, n, k, V, ldv, ls + m, -1.0, w, stream));
-    IGA_CHECK(iga_dot_many(dtype, n, 1, w, n, w, ls + 2 * m, work, stream));
-    gm_rotate_kernel<<<1, 64, 3 * sizeof(double) * (size_t)k, s>>>(st, k, ls);
-    IGA_LAUNCH_CHECK();
+    const int j = k - 1;
     const size_t es = dtype == IGA_F64 ? 8 : 4;
-    void* vk = (char*)V + (size_t)k * ldv * es;
-    IGA_CHECK(iga_axpby(dtype, n, 0.0, ls + 2 * m + 1, 0, w, 0.0, vk, stream));
+    const void* u = (const char*)V + (size_t)j * ldv * es;
+    // first pass: V[0..j] against u = V[j] and w
+    dot_many_launch(dtype, n, j + 1, V, ldv, u, w, 2, ls + st.U(), st.W() - st.U(), work, s);
+    gm_dcgs2_kernel<<<1, kGmThreads, sizeof(double) * (4 * (size_t)j + 1), s>>>(st, j, ls);
+    IGA_LAUNCH_CHECK();
+    // second pass
+    const dim3 grid(grid_for(n, 256, 8192)), block(256);
+    const size_t shm = sizeof(double) * 2 * (size_t)(j > 0 ? j : 1);
+    if (dtype == IGA_F64)
+      gm_update_kernel<double><<<grid, block, shm, s>>>(n, j, (double*)V, ldv, (const double*)w, ls + st.cq(),
+                                                        ls + st.cw(), ls + st.scal());
+    else
+      gm_update_kernel<float><<<grid, block, shm, s>>>(n, j, (float*)V, ldv, (const float*)w, ls + st.cq(),
+                                                       ls + st.cw(), ls + st.scal());
+    IGA_LAUNCH_CHECK();
   });
 }
 

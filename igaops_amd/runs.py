@@ -4,12 +4,15 @@ per iteration.
 ``GmresRun`` is the reference's un-restarted, right-preconditioned GMRES
 (``solvers/linear_solver.py:438-525``) with its relative and absolute residual
 criteria (``solvers/utils/convergence_manager.py:156-327``).  Each Arnoldi step is
-queued on the run's stream: preconditioner, operator, both Gram-Schmidt passes,
-the Givens update of the least-squares problem and the new basis vector all stay
-on the device (``iga_gmres_step``).  The host learns each step's residual estimate
-``|g_k|`` from a two-value asynchronous copy and an event, so it can queue the next
-steps before the previous ones finish; steps queued past convergence do not change
-the solution, which is formed from the first ``k`` columns only.
+queued on the run's stream: preconditioner, operator, the orthogonalisation
+(classical Gram-Schmidt with the reorthogonalisation delayed by one step: two
+passes over the basis per step), the Givens update of the least-squares problem
+and the next basis vector all stay on the device (``iga_gmres_step``).  Step ``k``
+yields the residual estimate ``|g|`` of Krylov dimension ``k - 1``; the host reads
+it from a two-value asynchronous copy and an event, so it can queue the next
+steps before the previous ones finish.  Steps queued past convergence do not
+change the solution ``x0 + P (V[:k] y)``, formed from the first ``k`` basis
+vectors only.
 
 ``RefinementRun`` is mixed-precision iterative refinement: inner ``GmresRun`` solves
 of ``A_lo d = r`` in float32 vectors and operators, the residual ``r = b - A x`` and
@@ -22,6 +25,7 @@ separate streams (the serving loop does).
 from __future__ import annotations
 
 import contextlib
+import ctypes as C
 from typing import Callable, Optional
 
 import numpy as np
@@ -69,16 +73,18 @@ class GmresRun:
         lib = _lib.load()
         with self._on_stream():
             self.b = b
-            # the bases grow by doubling as steps are queued (most solves stop far below
-            # maxiters, and the allocator keeps what a solve has held)
-            self.cap = min(self.m, 64)
-            self.V = t.empty((self.cap + 1, n), dtype=tdt, device=b.device)
-            self.Z = t.empty((self.cap, n), dtype=tdt, device=b.device)
+            # the basis grows by doubling as steps are queued (most solves stop far below
+            # maxiters, and the allocator keeps what a solve has held); step k writes V[k]
+            self.V = t.empty((min(self.m, 64) + 2, n), dtype=tdt, device=b.device)
+            self.z = t.empty(n, dtype=tdt, device=b.device)
             self.w = t.empty(n, dtype=tdt, device=b.device)
             self.x = t.zeros(n, dtype=tdt, device=b.device)
             self.x0 = x0
             self.nls = int(lib.iga_gmres_state_doubles(self.m))
             self.ls = t.zeros(self.nls, dtype=t.float64, device=b.device)
+            offs = (C.c_int64 * 3)()
+            _lib.check(lib.iga_gmres_layout(self.m, offs), "iga_gmres_layout")
+            self._off_r, self._off_g, self._hist_off = int(offs[0]), int(offs[1]), int(offs[2])
             self.work = dv.workspace(lib.iga_reduce_workspace_bytes(n, self.m + 1))
             self.nb = t.zeros(1, dtype=t.float64, device=b.device)
         # one pinned host buffer per run (or lent by the caller, at least
@@ -91,7 +97,6 @@ class GmresRun:
         self.hist_host = host_buf[:2 * (self.m + 2)]
         self._sol_region = host_buf[2 * (self.m + 2):2 * (self.m + 2) + self.m * self.m + self.m]
         self._y_region = host_buf[2 * (self.m + 2) + self.m * self.m + self.m:need]
-        self._hist_off = self.nls - 2 * (self.m + 1)
         self.launched = 0      # steps queued
         self.known = -1        # last step whose residual estimate has been read
         self.events = {}
@@ -146,38 +151,38 @@ class GmresRun:
         self.launched = 0
 
     def can_launch(self) -> bool:
-        return self.kfinal is None and self.launched < self.m and self.launched - self.known < self.lookahead
+        # step k reports dimension k - 1: one more step in flight than the lookahead
+        return self.kfinal is None and self.launched <= self.m and self.launched - self.known <= self.lookahead
 
-    def _grow(self, need: int):
+    def _ensure_rows(self, k: int):
+        """Rows V[0..k] for step k."""
+        rows = self.V.shape[0]
+        if k < rows:
+            return
         t = dv.torch()
-        cap = self.cap
-        while cap < need:
-            cap = min(self.m, 2 * cap)
-        V = t.empty((cap + 1, self.n), dtype=self.V.dtype, device=self.V.device)
-        Z = t.empty((cap, self.n), dtype=self.Z.dtype, device=self.Z.device)
-        V[:self.cap + 1].copy_(self.V)
-        Z[:self.cap].copy_(self.Z)
-        self.V, self.Z, self.cap = V, Z, cap
+        while rows <= k:
+            rows = min(self.m + 2, 2 * rows)
+        V = t.empty((rows, self.n), dtype=self.V.dtype, device=self.V.device)
+        V[:self.V.shape[0]].copy_(self.V)
+        self.V = V
 
     def prepare_step(self):
-        """Capacity for step k = launched + 1; returns (k, V[k-1], Z[k-1]) so that a
-        caller can fill Z[k-1] = P V[k-1] itself (batched with other runs) before
+        """Capacity for step k = launched + 1; returns (k, V[k-1], z) so that a caller
+        can fill z = P V[k-1] itself (batched with other runs) before
         ``launch(preconditioned=True)``."""
         k = self.launched + 1
         with self._on_stream():
-            if k > self.cap:
-                self._grow(k)
-        return k, self.V[k - 1], self.Z[k - 1]
+            self._ensure_rows(k)
+        return k, self.V[k - 1], self.z
 
     def launch(self, preconditioned: bool = False):
-        """Queue Arnoldi step k = launched + 1 (with ``preconditioned``, Z[k-1]
-        already holds P V[k-1])."""
+        """Queue Arnoldi step k = launched + 1 (with ``preconditioned``, ``z`` already
+        holds P V[k-1])."""
         k = self.launched + 1
         dt = dv.DTYPES[self.dtype]
         with self._on_stream():
-            if k > self.cap:
-                self._grow(k)
-            z = self.Z[k - 1]
+            self._ensure_rows(k)
+            z = self.z
             if preconditioned:
                 pass
             elif self.P is not None:
@@ -191,7 +196,8 @@ class GmresRun:
                 self.mask(self.w)
             _lib.call("iga_gmres_step", dt, self.n, k, self.m, dv.ptr(self.V), self.n, dv.ptr(self.w),
                       dv.ptr(self.ls), dv.ptr(self.work), self._sp())
-            self._record(k, self._hist_off + 2 * k, 2 * k, 2)
+            if k >= 2:
+                self._record(k - 1, self._hist_off + 2 * (k - 1), 2 * (k - 1), 2)
         self.launched = k
 
     def poll(self, block: bool = False) -> bool:
@@ -238,7 +244,7 @@ class GmresRun:
             self._sol_host = None
             return
         with self._on_stream():
-            off_r, off_g = 2 * m + 2, 2 * m + 2 + m * m + 2 * m
+            off_r, off_g = self._off_r, self._off_g
             self._sol_host = self._sol_region[:k * m + k]
             self._sol_host[:k * m].copy_(self.ls[off_r:off_r + k * m], non_blocking=True)
             self._sol_host[k * m:].copy_(self.ls[off_g:off_g + k], non_blocking=True)
@@ -254,7 +260,7 @@ class GmresRun:
         return self._sol_event.query()
 
     def complete(self):
-        """x += Z[:k]^T y (queued); returns the solve's OutputArgs."""
+        """x += P (V[:k]^T y) (queued); returns the solve's OutputArgs."""
         from .solvers import OutputArgs
 
         t = dv.torch()
@@ -269,15 +275,25 @@ class GmresRun:
                 yh = self._y_region[:k]
                 yh.copy_(t.from_numpy(y))
                 self._y_dev = yh.to(self.x.device, non_blocking=True)
-                _lib.call("iga_axpy_many", dv.DTYPES[self.dtype], self.n, k, dv.ptr(self.Z), self.n,
-                          dv.ptr(self._y_dev), 1.0, dv.ptr(self.x), self._sp())
+                v = self.w
+                v.zero_()
+                _lib.call("iga_axpy_many", dv.DTYPES[self.dtype], self.n, k, dv.ptr(self.V), self.n,
+                          dv.ptr(self._y_dev), 1.0, dv.ptr(v), self._sp())
+                z = self.z
+                if self.P is not None:
+                    self.P(v, z)
+                else:
+                    z.copy_(v)
+                if self.mask:
+                    self.mask(z)
+                self.x.add_(z)
         success = self.reason in ("relative", "absolute", "breakdown")
         self.result = OutputArgs(self.x, np.asarray(self.res_hist), success, int(k or 0), self.reason)
         return self.result
 
     def release(self):
         """Drop the Krylov basis (the solution tensor is kept)."""
-        self.V = self.Z = None
+        self.V = None
 
     # -- driver --
     def advance(self, launch: bool = True) -> bool:

@@ -123,10 +123,11 @@ def footprint_bytes(ndof: int, nq: int, nurbs: bool, mode: str, maxiters: int, i
     ncoef = 10 if nurbs else 6
     setup = 18 * nq * 8  # C (3,3,N_q), the field, NURBS weight fields: released after packing
     coef = ncoef * nq * 8 + (ncoef * nq * 4 if mode == "mixed" else 0)
+    # one basis of m + 2 vectors and the dense GMRES state (two (m+1)^2 matrices)
     if mode == "mixed":
-        krylov = 2 * (inner_maxiters + 1) * ndof * 4 + 6 * ndof * 8
+        krylov = (inner_maxiters + 2) * ndof * 4 + 2 * (inner_maxiters + 1) ** 2 * 8 + 6 * ndof * 8
     else:
-        krylov = 2 * (maxiters + 1) * ndof * 8 + 4 * ndof * 8
+        krylov = (maxiters + 2) * ndof * 8 + 2 * (maxiters + 1) ** 2 * 8 + 4 * ndof * 8
     scratch = 12 * ndof * 8
     return int(setup + coef + krylov + scratch)
 

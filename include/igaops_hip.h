@@ -82,8 +82,9 @@ int iga_box_mask(int dtype, int ndim, const int64_t* n, const int64_t* lo, const
 int iga_index_zero(int dtype, int64_t nidx, const int64_t* idx, int64_t nrhs, void* x, void* stream);
 
 /* ---- Krylov reductions (FP64 or FP32 vectors, FP64 results) ----------------
- * Deterministic two-stage wavefront-shuffle reductions.  `work` is caller-owned
- * device scratch of iga_reduce_workspace_bytes(n, k) bytes.  They replace the
+ * Deterministic two-stage wavefront-shuffle reductions (fixed chunks, fixed
+ * combination order).  `work` is caller-owned device scratch of
+ * iga_reduce_workspace_bytes(n, k) bytes.  iga_dot_many takes k <= 16384.  They replace the
  * numpy dot / norm / axpy loops of solvers/linear_solver.py:481-495 and
  * solvers/deflated/helpers.py:44-58,116-133. */
 int64_t iga_reduce_workspace_bytes(int64_t n, int k);
@@ -127,19 +128,25 @@ int iga_axpby(int dtype, int64_t n, double a, const double* a_dev, int invert_a,
 int iga_cast(int dtype_x, int dtype_y, int64_t n, const void* x, void* y, void* stream);
 
 /* ---- device-side GMRES Arnoldi step ------------------------------------------
- * Replaces the inner loop of _GMRES (solvers/linear_solver.py:473-508): the two
- * Gram-Schmidt passes (481-490), the Hessenberg column and its least-squares
- * residual, with the least-squares problem kept in Givens-rotated form instead of
- * lstsq (501; the residual estimate |g_k| equals its residual norm).  `ls` is a
- * float64 device state of iga_gmres_state_doubles(m) values:
- *   [0,m) and [m,2m) the two projection passes, 2m ||w||^2, 2m+1 the next basis scale,
- *   R = ls + 2m+2 (m x m column-major, the rotated Hessenberg), cs, sn (m each),
- *   g (m+1), hist (2(m+1): g[k] and ||w_k|| after step k, for the host to poll).
+ * Replaces the inner loop of _GMRES (solvers/linear_solver.py:473-508): the
+ * orthogonalisation of the new Krylov vector (481-490, two classical Gram-Schmidt
+ * passes there), the Hessenberg column and its least-squares residual, with the
+ * least-squares problem kept in Givens-rotated form instead of lstsq (501; the
+ * residual estimate |g| equals its residual norm).  The reorthogonalisation is
+ * delayed by one step (DCGS2), so a step makes two passes over the basis and step
+ * k finishes basis vector k-1 and the Hessenberg column k-2.
+ * `ls` is a float64 device state of iga_gmres_state_doubles(m) values;
+ * iga_gmres_layout gives the offsets of R (m x m column-major, the rotated
+ * Hessenberg), g (m+1, the rotated beta e_1) and hist (2(m+1): g[d] and the
+ * subdiagonal H[d, d-1] of Krylov dimension d, for the host to poll).
  * iga_gmres_start: beta = ||r||, g[0] = beta, V0 = r / beta (state cleared first).
- * iga_gmres_step (k = 1..m, m <= 2048): orthogonalises w against V[0..k) in place, updates
- * R, cs, sn, g and writes V[k] = w / ||w||.  `work` holds
- * iga_reduce_workspace_bytes(n, m) bytes; vectors are `dtype`, the state float64. */
+ * iga_gmres_step (k = 1..m+1, m <= 2000): V[0..k-1) hold the final basis, V[k-1]
+ * the tentative vector u and w = A' u; writes the final V[k-1], the next tentative
+ * V[k], and for k >= 2 the column k-2 of R, g[k-1] and hist entry k-1.  V holds
+ * k+1 rows of stride ldv.  `work` holds iga_reduce_workspace_bytes(n, m+1) bytes;
+ * vectors are `dtype`, the state float64. */
 int64_t iga_gmres_state_doubles(int m);
+int iga_gmres_layout(int m, int64_t* offsets);  /* offsets[3] = R, g, hist */
 int iga_gmres_start(int dtype, int64_t n, int m, const void* r, void* V0, double* ls, void* work, void* stream);
 int iga_gmres_step(int dtype, int64_t n, int k, int m, void* V, int64_t ldv, void* w, double* ls, void* work,
                    void* stream);

@@ -168,6 +168,37 @@ def test_device_step_gmres_matches_host_step_gmres():
     assert float(torch.linalg.vector_norm(r) / torch.linalg.vector_norm(b)) <= 1e-9
 
 
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_device_step_gmres_at_maxiters_and_orthogonality(dtype):
+    """Stopped at the Krylov dimension (the last step finishes the last basis
+    vector): the same minimiser as the host-step solve over the same space, and a
+    basis orthonormal to working precision (delayed reorthogonalisation)."""
+    import torch
+
+    from igaops_amd.runs import GmresRun
+    from igaops_amd.solvers import gmres_device
+
+    patch, C, m, fd, mask, ops = _annulus_fd_problem()
+    n = patch.nbctrlpts_total
+    tdt = torch.float64 if dtype == "f64" else torch.float32
+    A = lambda x, o: ops[dtype].apply(x, out=o)
+    P = lambda x, o: fd.apply_device(x, out=o)
+    b = torch.as_tensor(np.random.default_rng(12).standard_normal(n), device="cuda", dtype=tdt)
+    mask(b)
+    kmax = 9
+    run = GmresRun(A, P, b, 1e-14, kmax, mask=mask)
+    out = run.run()
+    assert out.iterations == kmax and out.reason == "maxiters"
+    Q = run.V[:kmax].double()
+    orth = float((Q @ Q.T - torch.eye(kmax, dtype=torch.float64, device="cuda")).abs().max())
+    assert orth <= (1e-13 if dtype == "f64" else 1e-5)
+    if dtype == "f64":
+        ref = gmres_device(A, b, P, 1e-14, kmax, mask=mask, restart=400)
+        assert ref.iterations == kmax
+        np.testing.assert_allclose(out.residual, ref.residual, rtol=1e-8)
+        assert rel(out.solution.cpu().numpy(), ref.solution.cpu().numpy()) <= 1e-10
+
+
 def test_refinement_run_float32_vectors_reaches_1e10():
     import torch
 
