@@ -41,7 +41,8 @@ static int reduce_blocks(int64_t n) {
 
 // Many-dot products of the rows of V with one or two vectors x0, x1.  Stage 1:
 // blocks tile (element range) x (group of rows); each thread keeps kDotAcc
-// partial sums (one per row of its group and vector), reads 16-byte vectors where
+// partial sums (one per row of its group and vector: 16 rows for one vector, 8
+// for two; more accumulators cost occupancy and measured slower), reads 16-byte vectors where
 // the layout allows it, the 64 lanes of a wave are combined by a transposed
 // butterfly, the waves of a block through LDS, and each block writes one partial
 // per (vector, row).  Stage 2 sums the partials of every block in a fixed order
@@ -78,69 +79,17 @@ static int dot_chunks(int64_t n, int groups, int esize) {
   return (int)g;
 }
 
-// partial[(x * k + j) * nb + b] = sum over chunk b of V_j x_x
-template <typename T, bool VECTOR, int NX>
-__global__ void __launch_bounds__(kRedThreads) dot_many_stage1(int64_t n, int k, const T* __restrict__ V,
-                                                               int64_t ldv, const T* __restrict__ x0,
-                                                               const T* __restrict__ x1,
-                                                               double* __restrict__ partial) {
-  constexpr int G = kDotAcc / NX;  // rows per group
-  __shared__ double red[kRedThreads / 64][kDotAcc];
-  const int nb = gridDim.x;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int j0 = blockIdx.y * G;
-  const int kb = k - j0 < G ? k - j0 : G;
-  double acc[kDotAcc];  // acc[x * G + jj]
+// Transposed butterfly over the 64 lanes of a wave for N = 16 or 32 values per
+// lane: each exchange halves the values a lane holds, so lane q * (64 / N) ends
+// with the wave's sum of acc[q] (N + log2(64 / N) - 1 exchanges instead of N
+// separate 6-step trees).  Returns the lane's value.
+template <int N>
+__device__ inline double wave_sum_t(double (&acc)[N], int lane) {
+  static_assert(N == 16 || N == 32, "16 or 32 values per lane");
+  constexpr int kLevels = N == 16 ? 4 : 5;
 #pragma unroll
-  for (int q = 0; q < kDotAcc; ++q) acc[q] = 0.0;
-  const T* Vb = V + (int64_t)j0 * ldv;
-  int64_t body = 0;
-  if (VECTOR) {
-    using VT = typename Vec16<T>::type;
-    constexpr int W = Vec16<T>::width;
-    const int64_t nv = n / W;
-    body = nv * W;
-    const int64_t per = (nv + nb - 1) / nb;
-    const int64_t lo = (int64_t)blockIdx.x * per;
-    const int64_t hi = lo + per < nv ? lo + per : nv;
-    for (int64_t i = lo + tid; i < hi; i += kRedThreads) {
-      const VT a = reinterpret_cast<const VT*>(x0)[i];
-      VT b{};
-      if (NX == 2) b = reinterpret_cast<const VT*>(x1)[i];
-#pragma unroll
-      for (int jj = 0; jj < G; ++jj) {
-        if (jj < kb) {
-          const VT v = reinterpret_cast<const VT*>(Vb + (int64_t)jj * ldv)[i];
-          acc[jj] += Vec16<T>::dot(v, a);
-          if (NX == 2) acc[G + jj] += Vec16<T>::dot(v, b);
-        }
-      }
-    }
-  }
-  {
-    // scalar elements: all of them (no vector layout) or the tail beyond the last whole vector
-    const int64_t m = n - body;
-    const int64_t per = VECTOR ? m : (m + nb - 1) / nb;
-    const int64_t lo = VECTOR ? (blockIdx.x == nb - 1 ? 0 : m) : (int64_t)blockIdx.x * per;
-    const int64_t hi = lo + per < m ? lo + per : m;
-    for (int64_t i = body + lo + tid; i < body + hi; i += kRedThreads) {
-      const double a = (double)x0[i];
-      const double b = NX == 2 ? (double)x1[i] : 0.0;
-#pragma unroll
-      for (int jj = 0; jj < G; ++jj) {
-        if (jj < kb) {
-          const double v = (double)Vb[(int64_t)jj * ldv + i];
-          acc[jj] += v * a;
-          if (NX == 2) acc[G + jj] += v * b;
-        }
-      }
-    }
-  }
-  // transposed butterfly over the 64 lanes: each exchange halves the values a lane
-  // holds, so lane 4 q ends with the wave's sum of accumulator q (17 exchanges, not 96)
-#pragma unroll
-  for (int st = 0; st < 4; ++st) {
-    const int off = 32 >> st, c = kDotAcc >> (st + 1);
+  for (int st = 0; st < kLevels; ++st) {
+    const int off = 32 >> st, c = N >> (st + 1);
     const bool upper = (lane & off) != 0;
 #pragma unroll
     for (int q = 0; q < c; ++q) {
@@ -149,16 +98,91 @@ __global__ void __launch_bounds__(kRedThreads) dot_many_stage1(int64_t n, int k,
       acc[q] = keep + __shfl_xor(give, off, 64);
     }
   }
-  acc[0] += __shfl_xor(acc[0], 2, 64);
-  acc[0] += __shfl_xor(acc[0], 1, 64);
-  if ((lane & 3) == 0) red[wave][lane >> 2] = acc[0];
+  double v = acc[0];
+#pragma unroll
+  for (int off = 32 >> kLevels; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// partial[(x * k + j) * nb + b] = sum over chunk b of V_j x_x.
+// One-dimensional grid: block L runs chunk b = (L / 8 / groups) * 8 + L % 8 and
+// row group (L / 8) % groups, so that the row groups of one chunk run on one XCD
+// (work-groups go round-robin to the eight XCDs) one after the other and read
+// the chunk of x0, x1 from that XCD's L2.
+template <typename T, bool VECTOR, int NX>
+__global__ void __launch_bounds__(kRedThreads) __attribute__((amdgpu_waves_per_eu(1, 4))) dot_many_stage1(int64_t n, int k, int nb, int groups,
+                                                               const T* __restrict__ V, int64_t ldv,
+                                                               const T* __restrict__ x0,
+                                                               const T* __restrict__ x1,
+                                                               double* __restrict__ partial) {
+  constexpr int NA = kDotAcc;  // accumulators per thread
+  constexpr int G = NA / NX;    // rows per group
+  __shared__ double red[kRedThreads / 64][NA];
+  const int L = blockIdx.x;
+  const int bx = (L / 8 / groups) * 8 + L % 8;
+  if (bx >= nb) return;
+  const int by = (L / 8) % groups;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int j0 = by * G;
+  const int kb = k - j0 < G ? k - j0 : G;
+  double acc[NA];  // acc[x * G + jj]
+#pragma unroll
+  for (int q = 0; q < NA; ++q) acc[q] = 0.0;
+  const T* Vb = V + (int64_t)j0 * ldv;
+  int64_t body = 0;
+  if (VECTOR) {
+    using VT = typename Vec16<T>::type;
+    constexpr int W = Vec16<T>::width;
+    const int64_t nv = n / W;
+    body = nv * W;
+    const int64_t per = (nv + nb - 1) / nb;
+    const int64_t lo = (int64_t)bx * per;
+    const int64_t hi = lo + per < nv ? lo + per : nv;
+    for (int64_t i = lo + tid; i < hi; i += kRedThreads) {
+      const VT a = reinterpret_cast<const VT*>(x0)[i];
+      VT b{};
+      if (NX == 2) b = reinterpret_cast<const VT*>(x1)[i];
+      // every load is issued unconditionally (rows past the group's end re-read its
+      // last row, their sums are never written): a guarded load would be waited for
+      // before the next one is issued
+      VT v[G];
+#pragma unroll
+      for (int jj = 0; jj < G; ++jj) v[jj] = reinterpret_cast<const VT*>(Vb + (int64_t)(jj < kb ? jj : kb - 1) * ldv)[i];
+#pragma unroll
+      for (int jj = 0; jj < G; ++jj) {
+        acc[jj] += Vec16<T>::dot(v[jj], a);
+        if (NX == 2) acc[G + jj] += Vec16<T>::dot(v[jj], b);
+      }
+    }
+  }
+  {
+    // scalar elements: all of them (no vector layout) or the tail beyond the last whole vector
+    const int64_t m = n - body;
+    const int64_t per = VECTOR ? m : (m + nb - 1) / nb;
+    const int64_t lo = VECTOR ? (bx == nb - 1 ? 0 : m) : (int64_t)bx * per;
+    const int64_t hi = lo + per < m ? lo + per : m;
+    for (int64_t i = body + lo + tid; i < body + hi; i += kRedThreads) {
+      const double a = (double)x0[i];
+      const double b = NX == 2 ? (double)x1[i] : 0.0;
+#pragma unroll
+      for (int jj = 0; jj < G; ++jj) {
+        const int r = jj < kb ? jj : kb - 1;
+        const double v = (double)Vb[(int64_t)r * ldv + i];
+        acc[jj] += v * a;
+        if (NX == 2) acc[G + jj] += v * b;
+      }
+    }
+  }
+  const double wsum = wave_sum_t<NA>(acc, lane);
+  constexpr int kSpread = 64 / NA;  // lanes per accumulator after the butterfly
+  if (lane % kSpread == 0) red[wave][lane / kSpread] = wsum;
   __syncthreads();
-  if (tid < kDotAcc && tid % G < kb) {
+  if (tid < NA && tid % G < kb) {
     double sum = 0.0;
 #pragma unroll
     for (int q = 0; q < kRedThreads / 64; ++q) sum += red[q][tid];
     const int x = tid / G, jj = tid % G;
-    partial[((int64_t)x * k + j0 + jj) * nb + blockIdx.x] = sum;
+    partial[((int64_t)x * k + j0 + jj) * nb + bx] = sum;
   }
 }
 
@@ -182,12 +206,13 @@ static void dot_many_launch(int dtype, int64_t n, int k, const void* V, int64_t 
   const int groups = (k + G - 1) / G;
   const int nb = dot_chunks(n, groups, dtype == IGA_F32 ? 4 : 8);
   double* partial = (double*)work;
-  const dim3 grid(nb, groups);
+  const dim3 grid((unsigned)(((nb + 7) / 8) * 8 * groups));
   auto aligned = [](const void* p) { return p == nullptr || ((uintptr_t)p & 15) == 0; };
   const int width = dtype == IGA_F32 ? 4 : 2;
   const bool vec = ldv % width == 0 && aligned(V) && aligned(x0) && aligned(x1);
 #define IGA_DOT1(T, VEC, NX) \
-  dot_many_stage1<T, VEC, NX><<<grid, kRedThreads, 0, s>>>(n, k, (const T*)V, ldv, (const T*)x0, (const T*)x1, partial)
+  dot_many_stage1<T, VEC, NX><<<grid, kRedThreads, 0, s>>>(n, k, nb, groups, (const T*)V, ldv, (const T*)x0, \
+                                                           (const T*)x1, partial)
 #define IGA_DOT2(T, VEC) \
   do {                    \
     if (nx == 2)          \
@@ -219,6 +244,7 @@ __global__ void axpy_many_kernel(int64_t n, int k, const T* __restrict__ V, int6
   __syncthreads();
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     double acc = 0.0;
+#pragma unroll 8
     for (int j = 0; j < k; ++j) acc += hs[j] * (double)V[(int64_t)j * ldv + i];
     w[i] = (T)((double)w[i] + alpha * acc);
   }
@@ -318,7 +344,7 @@ __global__ void cast_kernel(int64_t n, const S* __restrict__ x, D* __restrict__ 
 //   Givens rotations, g (m+1): the rotated beta e_1, hist 2 (m+1): (g[d], rho_d)
 //   for dimension d, the values a host polls; U, W (m+1 each): the dots of the
 //   first pass with u and w; cq, cw (m+1 each) and scal: the second pass's
-//   coefficients; H ((m+1) x (m+1), column-major): the Hessenberg matrix, its
+//   coefficients; H ((m+1) x (m+1), row-major): the Hessenberg matrix, its
 //   last column pending.
 constexpr int kGmMaxDim = 2000;  // the step kernel stages 4 m + 1 doubles in LDS (64 KiB)
 constexpr int kGmThreads = 256;
@@ -356,24 +382,17 @@ __global__ void gm_start_kernel(GmState st, double* ls) {
   ls[st.hist() + 1] = beta;
 }
 
-// Block-wide sum in a fixed order; valid in every thread.
-__device__ inline double gm_block_sum(double v, double* red) {
-  v = wave_sum(v);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (lane == 0) red[wave] = v;
-  __syncthreads();
-  double s = 0.0;
-  for (int q = 0; q < kGmThreads / 64; ++q) s += red[q];
-  __syncthreads();
-  return s;
-}
-
 // The small dense part of step j + 1 (one block): finalise column j - 1, rotate
-// it, form hcol_j and the second pass's coefficients.
-__global__ void __launch_bounds__(kGmThreads) gm_dcgs2_kernel(GmState st, int j, double* ls) {
-  extern __shared__ double sh[];  // s[j] | rotation stage: col[j+1], c[j], sn[j]  (y[j+1] reuses col)
-  __shared__ double red[kGmThreads / 64];
-  const int tid = threadIdx.x;
+// it, form hcol_j and the second pass's coefficients.  Every global value the
+// kernel needs before the product y = H s is loaded up front, unconditionally
+// (clamped indices), so that they arrive in one memory round trip.
+constexpr int kGmPer = (kGmMaxDim + kGmThreads - 1) / kGmThreads;  // indices l per thread
+
+__global__ void __launch_bounds__(kGmThreads) __attribute__((amdgpu_waves_per_eu(1, 1)))
+gm_dcgs2_kernel(GmState st, int j, double* ls) {
+  extern __shared__ double sh[];  // s[j] | col[j+1] | cr[j] | sr[j]   (y[j+1] reuses col)
+  __shared__ double red[2][kGmThreads / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int64_t ldh = st.ldh();
   double* H = ls + st.H();
   double* s = sh;
@@ -383,16 +402,46 @@ __global__ void __launch_bounds__(kGmThreads) gm_dcgs2_kernel(GmState st, int j,
   const double* U = ls + st.U();
   const double* Wd = ls + st.W();
   double* sc = ls + st.scal();
-  double ss = 0.0, stw = 0.0;
-  for (int l = tid; l < j; l += kGmThreads) {
-    const double sl = U[l];
-    s[l] = sl;
-    ss += sl * sl;
-    stw += sl * Wd[l];
+  const int jm = j > 0 ? j - 1 : 0;
+  // l = tid + c * kGmThreads: s_l, t_l, H[l, j-1] (hcol_{j-1}), cs_l, sn_l
+  double sl[kGmPer], tl[kGmPer], hl[kGmPer], cl[kGmPer], snl[kGmPer];
+#pragma unroll
+  for (int c = 0; c < kGmPer; ++c) {
+    const int l = tid + c * kGmThreads;
+    const int lc = l < j ? l : 0;
+    const int lr = l < jm ? l : 0;
+    sl[c] = U[lc];
+    tl[c] = Wd[lc];
+    hl[c] = H[(int64_t)lc * ldh + jm];
+    cl[c] = ls[st.cs() + lr];
+    snl[c] = ls[st.sn() + lr];
   }
-  ss = gm_block_sum(ss, red);
-  stw = gm_block_sum(stw, red);
-  const double a = U[j], c = Wd[j];
+  const double a = U[j], cw_j = Wd[j];
+  const double gj1 = ls[st.g() + jm];
+  double ss = 0.0, stw = 0.0;
+#pragma unroll
+  for (int c = 0; c < kGmPer; ++c) {
+    const int l = tid + c * kGmThreads;
+    if (l < j) {
+      s[l] = sl[c];
+      ss += sl[c] * sl[c];
+      stw += sl[c] * tl[c];
+    }
+  }
+  ss = wave_sum(ss);
+  stw = wave_sum(stw);
+  if (lane == 0) {
+    red[0][wave] = ss;
+    red[1][wave] = stw;
+  }
+  __syncthreads();
+  ss = 0.0;
+  stw = 0.0;
+#pragma unroll
+  for (int q = 0; q < kGmThreads / 64; ++q) {
+    ss += red[0][q];
+    stw += red[1][q];
+  }
   double rho = 1.0;  // step 1: V[0] = r / beta is final already
   if (j > 0) {
     const double r2 = a - ss;
@@ -401,28 +450,47 @@ __global__ void __launch_bounds__(kGmThreads) gm_dcgs2_kernel(GmState st, int j,
   const double inv = rho > 0.0 ? 1.0 / rho : 0.0;
   if (j > 0) {
     // final column j - 1: hcol_{j-1} + s, rho; staged with the rotations
-    double* Hc = H + (int64_t)(j - 1) * ldh;
-    for (int l = tid; l < j; l += kGmThreads) {
-      const double v = Hc[l] + s[l];
-      Hc[l] = v;
-      col[l] = v;
+#pragma unroll
+    for (int c = 0; c < kGmPer; ++c) {
+      const int l = tid + c * kGmThreads;
+      if (l < j) {
+        const double v = hl[c] + sl[c];
+        H[(int64_t)l * ldh + jm] = v;
+        col[l] = v;
+        if (l < jm) {
+          cr[l] = cl[c];
+          sr[l] = snl[c];
+        }
+      }
     }
     if (tid == 0) {
-      Hc[j] = rho;
+      H[(int64_t)j * ldh + jm] = rho;
       col[j] = rho;
-    }
-    for (int l = tid; l + 1 < j; l += kGmThreads) {
-      cr[l] = ls[st.cs() + l];
-      sr[l] = ls[st.sn() + l];
     }
     __syncthreads();
     if (tid == 0) {
-      for (int i = 0; i + 1 < j; ++i) {
-        const double x = col[i], z = col[i + 1];
-        col[i] = cr[i] * x + sr[i] * z;
-        col[i + 1] = -sr[i] * x + cr[i] * z;
+      // previous rotations, the carried entry in a register and the LDS reads of
+      // four rotations issued ahead of their use
+      double x = col[0];
+      int i = 0;
+      for (; i + 4 < j; i += 4) {
+        const double b0 = col[i + 1], b1 = col[i + 2], b2 = col[i + 3], b3 = col[i + 4];
+        const double c0 = cr[i], c1 = cr[i + 1], c2 = cr[i + 2], c3 = cr[i + 3];
+        const double s0 = sr[i], s1 = sr[i + 1], s2 = sr[i + 2], s3 = sr[i + 3];
+        col[i] = c0 * x + s0 * b0;
+        x = -s0 * x + c0 * b0;
+        col[i + 1] = c1 * x + s1 * b1;
+        x = -s1 * x + c1 * b1;
+        col[i + 2] = c2 * x + s2 * b2;
+        x = -s2 * x + c2 * b2;
+        col[i + 3] = c3 * x + s3 * b3;
+        x = -s3 * x + c3 * b3;
       }
-      const double x = col[j - 1];
+      for (; i + 1 < j; ++i) {
+        const double b = col[i + 1];
+        col[i] = cr[i] * x + sr[i] * b;
+        x = -sr[i] * x + cr[i] * b;
+      }
       double cc = 1.0, sv = 0.0;
       if (rho != 0.0) {
         const double r = hypot(x, rho);
@@ -432,10 +500,8 @@ __global__ void __launch_bounds__(kGmThreads) gm_dcgs2_kernel(GmState st, int j,
       col[j - 1] = cc * x + sv * rho;
       ls[st.cs() + j - 1] = cc;
       ls[st.sn() + j - 1] = sv;
-      double* g = ls + st.g();
-      const double gj1 = g[j - 1];
-      g[j] = -sv * gj1;
-      g[j - 1] = cc * gj1;
+      ls[st.g() + j] = -sv * gj1;
+      ls[st.g() + j - 1] = cc * gj1;
       ls[st.hist() + 2 * j] = -sv * gj1;
       ls[st.hist() + 2 * j + 1] = rho;
     }
@@ -444,30 +510,47 @@ __global__ void __launch_bounds__(kGmThreads) gm_dcgs2_kernel(GmState st, int j,
     for (int l = tid; l < j; l += kGmThreads) Rc[l] = col[l];
     __syncthreads();
   }
-  // y = H[0:j+1, 0:j] s (Hessenberg: row i starts at column i - 1), into col
+  // y = H[0:j+1, 0:j] s (Hessenberg: row i starts at column i - 1), into col: the
+  // rows of the row-major H dotted with s, 16 rows per wave at a time, lanes along
+  // the row (the entries below the band are zero)
   double* y = col;
-  for (int i = tid; i <= j; i += kGmThreads) {
-    double acc = 0.0;
-    for (int l = i > 0 ? i - 1 : 0; l < j; ++l) acc += H[(int64_t)l * ldh + i] * s[l];
-    y[i] = acc;
+  for (int r0 = wave * 16; r0 <= j; r0 += 16 * (kGmThreads / 64)) {
+    double acc[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) acc[q] = 0.0;
+    for (int l = (r0 > 0 ? r0 - 1 : 0) + lane; l < j; l += 64) {
+      const double slv = s[l];
+      double h[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q)  // unconditional loads (rows past j re-read row j)
+        h[q] = H[(int64_t)(r0 + q <= j ? r0 + q : j) * ldh + l];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) acc[q] += h[q] * slv;
+    }
+    const double v = wave_sum_t<16>(acc, lane);
+    if ((lane & 3) == 0 && r0 + (lane >> 2) <= j) y[r0 + (lane >> 2)] = v;
   }
   __syncthreads();
   // hcol_j = (Q_{j+1}^T w - y) / rho with q_j.w = (c - s.t) / rho; z = y / rho + hcol_j
-  const double qw = (c - stw) * inv;
+  const double qw = (cw_j - stw) * inv;
   const double hj = (qw - y[j]) * inv;
   const double zj = y[j] * inv + hj;
-  double* Hj = H + (int64_t)j * ldh;
+  double* Hj = H + j;  // column j
   double* cq = ls + st.cq();
   double* cw = ls + st.cw();
-  for (int l = tid; l < j; l += kGmThreads) {
-    const double h = (Wd[l] - y[l]) * inv;
-    const double z = y[l] * inv + h;
-    Hj[l] = h;
-    cq[l] = s[l] * inv;           // q_j = u_j / rho - Q (s / rho)
-    cw[l] = z - zj * s[l] * inv;  // u_{j+1} = w / rho - (z_j / rho) u_j - Q (z - z_j s / rho)
+#pragma unroll
+  for (int c = 0; c < kGmPer; ++c) {
+    const int l = tid + c * kGmThreads;
+    if (l < j) {
+      const double h = (tl[c] - y[l]) * inv;
+      const double z = y[l] * inv + h;
+      Hj[(int64_t)l * ldh] = h;
+      cq[l] = sl[c] * inv;           // q_j = u_j / rho - Q (s / rho)
+      cw[l] = z - zj * sl[c] * inv;  // u_{j+1} = w / rho - (z_j / rho) u_j - Q (z - z_j s / rho)
+    }
   }
   if (tid == 0) {
-    Hj[j] = hj;
+    Hj[(int64_t)j * ldh] = hj;
     sc[0] = inv;
     sc[1] = zj * inv;
   }
@@ -489,6 +572,7 @@ __global__ void gm_update_kernel(int64_t n, int j, T* __restrict__ V, int64_t ld
   T* Vn = Vj + ldv;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     double aq = 0.0, aw = 0.0;
+#pragma unroll 8
     for (int l = 0; l < j; ++l) {
       const double v = (double)V[(int64_t)l * ldv + i];
       aq += hs[l] * v;
