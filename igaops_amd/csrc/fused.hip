@@ -505,18 +505,22 @@ __global__ void __launch_bounds__(256) fused_gather_kernel(GatherArgs<T> g) {
     const int c2 = (int)((dof / g.nc[0]) % g.nc[1]);
     const int c3 = (int)(dof / ((int64_t)g.nc[0] * g.nc[1]));
     const T* pk = g.partial + (int64_t)k * g.part_stride;
-    // All index loads first (fixed-size, predicated), then all partial loads: the
-    // dependent chain is three loads deep instead of one chain per contribution.
+    // All index loads first, then all partial loads, every load unconditional (the
+    // tables hold kKMax slots per DOF, zero past the count): the dependent chain is
+    // three loads deep, and no load waits for the one before it.
     const int n1 = g.l1_cnt[c1], n2 = g.l2_cnt[c2], n3 = g.l3_cnt[c3];
     int t1v[kKMax], o1v[kKMax], D1v[kKMax], t2v[kKMax], o2v[kKMax], D2v[kKMax], sv[kKMax], o3v[kKMax];
 #pragma unroll
     for (int i = 0; i < kKMax; ++i) {
-      t1v[i] = i < n1 ? g.l1_t[c1 * kKMax + i] : 0;
-      o1v[i] = i < n1 ? g.l1_off[c1 * kKMax + i] : 0;
-      t2v[i] = i < n2 ? g.l2_t[c2 * kKMax + i] : 0;
-      o2v[i] = i < n2 ? g.l2_off[c2 * kKMax + i] : 0;
-      sv[i] = i < n3 ? g.l3_t[c3 * kKMax + i] : 0;
-      o3v[i] = i < n3 ? g.l3_off[c3 * kKMax + i] : 0;
+      const int a1 = g.l1_t[c1 * kKMax + i], b1 = g.l1_off[c1 * kKMax + i];
+      const int a2 = g.l2_t[c2 * kKMax + i], b2 = g.l2_off[c2 * kKMax + i];
+      const int a3 = g.l3_t[c3 * kKMax + i], b3 = g.l3_off[c3 * kKMax + i];
+      t1v[i] = i < n1 ? a1 : 0;
+      o1v[i] = i < n1 ? b1 : 0;
+      t2v[i] = i < n2 ? a2 : 0;
+      o2v[i] = i < n2 ? b2 : 0;
+      sv[i] = i < n3 ? a3 : 0;
+      o3v[i] = i < n3 ? b3 : 0;
     }
 #pragma unroll
     for (int i = 0; i < kKMax; ++i) {
@@ -524,11 +528,32 @@ __global__ void __launch_bounds__(256) fused_gather_kernel(GatherArgs<T> g) {
       D2v[i] = g.t2_nc[t2v[i]];
     }
     T acc = T(0);
-    for (int i3 = 0; i3 < n3; ++i3) {
-      for (int i2 = 0; i2 < n2; ++i2) {
-        for (int i1 = 0; i1 < n1; ++i1) {
-          const int64_t base = g.part_base[((int64_t)t2v[i2] * g.nt1 + t1v[i1]) * g.nslab + sv[i3]];
-          acc += pk[base + ((int64_t)o3v[i3] * D2v[i2] + o2v[i2]) * D1v[i1] + o1v[i1]];
+    if (n1 <= 2 && n2 <= 2 && n3 <= 2) {
+      // up to two covering tiles per direction (the common case): the eight
+      // candidate contributions, absent ones read at slot 0 and skipped in the sum
+      int64_t addr[8];
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const int i1 = c & 1, i2 = (c >> 1) & 1, i3 = c >> 2;
+        const int64_t base = g.part_base[((int64_t)t2v[i2] * g.nt1 + t1v[i1]) * g.nslab + sv[i3]];
+        addr[c] = base + ((int64_t)o3v[i3] * D2v[i2] + o2v[i2]) * D1v[i1] + o1v[i1];
+      }
+      T val[8];
+#pragma unroll
+      for (int c = 0; c < 8; ++c) val[c] = pk[addr[c]];
+      // the order of the general loop below: i3, then i2, then i1
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const int i1 = c & 1, i2 = (c >> 1) & 1, i3 = c >> 2;
+        if (i1 < n1 && i2 < n2 && i3 < n3) acc += val[c];
+      }
+    } else {
+      for (int i3 = 0; i3 < n3; ++i3) {
+        for (int i2 = 0; i2 < n2; ++i2) {
+          for (int i1 = 0; i1 < n1; ++i1) {
+            const int64_t base = g.part_base[((int64_t)t2v[i2] * g.nt1 + t1v[i1]) * g.nslab + sv[i3]];
+            acc += pk[base + ((int64_t)o3v[i3] * D2v[i2] + o2v[i2]) * D1v[i1] + o1v[i1]];
+          }
         }
       }
     }
