@@ -200,6 +200,41 @@ def serve_section(nreq=100, qps=20.0, seed=0, max_active=16, budget_gb=200.0):
     return out
 
 
+def krylov_section(n=68 ** 3, m=200, sizes=(16, 64, 128), reps=20):
+    """One device GMRES Arnoldi orthogonalisation step (``iga_gmres_step``: two-vector
+    many-dot pass, small dense kernel, update pass) at the annulus p=4 n=64 vector
+    length, on random bases of j vectors, HIP-event timed; microseconds per step."""
+    import torch
+
+    from igaops_amd import _lib
+    from igaops_amd import device as dv
+
+    lib = _lib.load()
+    out = {}
+    for name, tdt, code in (("f64", torch.float64, _lib.IGA_F64), ("f32", torch.float32, _lib.IGA_F32)):
+        g = torch.Generator(device="cuda").manual_seed(5)
+        V = torch.randn(m + 2, n, dtype=tdt, device="cuda", generator=g) / n ** 0.5
+        w = torch.randn(n, dtype=tdt, device="cuda", generator=g) / n ** 0.5
+        ls = torch.zeros(int(lib.iga_gmres_state_doubles(m)), dtype=torch.float64, device="cuda")
+        work = dv.workspace(lib.iga_reduce_workspace_bytes(n, m + 1))
+        sp = dv.stream_ptr()
+        res = {}
+        for j in sizes:
+            def step():
+                _lib.call("iga_gmres_step", code, n, j + 1, m, dv.ptr(V), n, dv.ptr(w), dv.ptr(ls), dv.ptr(work), sp)
+            for _ in range(3):
+                step()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                step()
+            e1.record()
+            torch.cuda.synchronize()
+            res[str(j)] = e0.elapsed_time(e1) / reps * 1e3
+        out[name] = res
+    return {"n": n, "step_us_by_basis_size": out}
+
+
 def spacetime_section(steps=5, nrhs_seq=10, tol=1e-10, space=(3, 32), time_=(2, 64)):
     """Space-time heat equation: 3-D unit cube (degree 3, 32 elements/dir) x time
     (degree 2, 64 elements), WQ type "2" rules, unit diffusivity,
@@ -346,6 +381,7 @@ def main():
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-traffic", action="store_true")
     ap.add_argument("--spacetime", action="store_true", help="only the space-time GCRODR section")
+    ap.add_argument("--krylov", action="store_true", help="only the GMRES orthogonalisation step timings")
     ap.add_argument("--serve", type=int, default=0, help="only the serving section, with this many requests")
     ap.add_argument("--qps", type=float, default=20.0)
     ap.add_argument("--no-serve", action="store_true", help="skip the short serving stream of the default run")
@@ -359,6 +395,9 @@ def main():
         return
     if args.spacetime:
         print(json.dumps({"spacetime": spacetime_section()}))
+        return
+    if args.krylov:
+        print(json.dumps({"krylov": krylov_section()}))
         return
     if args.serve:
         print(json.dumps({"serve": serve_section(args.serve, args.qps)}))
