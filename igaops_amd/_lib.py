@@ -62,6 +62,12 @@ SIGNATURES = {
     "iga_fused_pack_strided": (_i32, [_vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp]),
     "iga_gmres_state_doubles": (_i64, [_i32]),
     "iga_gmres_layout": (_i32, [_i32, _i64p]),
+    "iga_sumfact_nnz": (_i64, [_i32, _i64p, C.POINTER(_i32p)]),
+    "iga_sumfact_assemble": (
+        _i32,
+        [_i32, _i32, _i64p, _i64p, _i64p, C.POINTER(_i32p), C.POINTER(_i32p), _i64p, C.POINTER(_i32p),
+         C.POINTER(_i32p), C.POINTER(_f64p), _i64p, C.POINTER(_vp), _f64p, _vp, _vp, _vp, _vp, _vp],
+    ),
     "iga_gmres_start": (_i32, [_i32, _i64, _i32, _vp, _vp, _vp, _vp, _vp]),
     "iga_gmres_step": (_i32, [_i32, _i64, _i32, _i32, _vp, _i64, _vp, _vp, _vp, _vp]),
     "iga_stfd_create": (_i32, [C.POINTER(_vp), _i64p, _i64p, _i64p, C.POINTER(_f64p), _f64p, _f64p, _i32]),

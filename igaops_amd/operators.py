@@ -652,8 +652,27 @@ def _operator_call(kind, quadrule_list, coefficients, array_in, time_ders, nurbs
     return _from_dev(y, was_np, shape)
 
 
+def _asm(kind: str, name: str):
+    def call(quadrule_list, coefficients, *args, **kwargs):
+        from . import assembly
+
+        return getattr(getattr(assembly, kind), name)(quadrule_list, coefficients, *args, **kwargs)
+
+    call.__name__ = name
+    call.__doc__ = f"``{name}``: sparse assembly / load vector on the device (``igaops_amd.assembly``)."
+    return staticmethod(call)
+
+
 class BsplineOperations:
-    """``operators/bsplines_ops.py:56-429`` (matrix-free part) on the device."""
+    """``operators/bsplines_ops.py:56-429`` on the device: matrix-free applies and the
+    sum-factorised assembly (``assemble_scalar_*``)."""
+
+    assemble_scalar_u_v = _asm("BsplineAssembly", "assemble_scalar_u_v")
+    assemble_scalar_gradu_gradv = _asm("BsplineAssembly", "assemble_scalar_gradu_gradv")
+    assemble_scalar_gradu_v = _asm("BsplineAssembly", "assemble_scalar_gradu_v")
+    assemble_scalar_u_gradv = _asm("BsplineAssembly", "assemble_scalar_u_gradv")
+    assemble_scalar_u_force = _asm("BsplineAssembly", "assemble_scalar_u_force")
+    assemble_scalar_gradu_force = _asm("BsplineAssembly", "assemble_scalar_gradu_force")
 
     @staticmethod
     def compute_mf_scalar_u_v(quadrule_list, coefficients, array_in, time_ders=None, nurbs_weights=np.array([])):
@@ -673,7 +692,15 @@ class BsplineOperations:
 
 
 class NurbsOperations:
-    """``operators/nurbs_ops.py:80-463`` (matrix-free part) on the device."""
+    """``operators/nurbs_ops.py:80-463`` on the device: matrix-free applies and the
+    sum-factorised assembly (``assemble_scalar_*``)."""
+
+    assemble_scalar_u_v = _asm("NurbsAssembly", "assemble_scalar_u_v")
+    assemble_scalar_gradu_gradv = _asm("NurbsAssembly", "assemble_scalar_gradu_gradv")
+    assemble_scalar_gradu_v = _asm("NurbsAssembly", "assemble_scalar_gradu_v")
+    assemble_scalar_u_gradv = _asm("NurbsAssembly", "assemble_scalar_u_gradv")
+    assemble_scalar_u_force = _asm("NurbsAssembly", "assemble_scalar_u_force")
+    assemble_scalar_gradu_force = _asm("NurbsAssembly", "assemble_scalar_gradu_force")
 
     @staticmethod
     def compute_mf_scalar_u_v(quadrule_list, coefficients, array_in, time_ders=None, nurbs_weights=np.array([])):

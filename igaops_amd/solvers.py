@@ -454,6 +454,46 @@ class LinearSolver:
             return OutputArgs(solution=np.array([]), residual=np.array([]), success=False)
         return self._last_simulation
 
+    # dense float64 LU on the device: N^2 doubles of HBM
+    DIRECT_MAX_N = 40000
+
+    @staticmethod
+    def direct(A, b, verbose: bool = True) -> OutputArgs:
+        """``linear_solver.py:595-630``: ``A x = b`` by LU.  The reference factors the
+        sparse matrix with SuperLU on the host; here the matrix (scipy sparse, dense
+        array or ``LinearOperator``) is expanded to a dense float64 matrix in HBM and
+        factored on the device (rocSOLVER LU through ``torch.linalg.solve``), for
+        N <= 40000."""
+        import time as _time
+
+        from scipy import sparse as sp
+        from scipy.sparse.linalg import LinearOperator
+
+        dv.require_gpu()
+        t = dv.torch()
+        start = _time.time()
+        if isinstance(A, LinearOperator):
+            A = np.asarray(A @ np.eye(A.shape[1]))
+        n = A.shape[0]
+        if A.shape[0] != A.shape[1]:
+            raise ValueError("the matrix must be square")
+        if n > LinearSolver.DIRECT_MAX_N:
+            raise ValueError(f"direct solve on the device is limited to N <= {LinearSolver.DIRECT_MAX_N}")
+        dense = t.zeros((n, n), dtype=t.float64, device=dv.device())
+        if sp.issparse(A):
+            coo = sp.coo_array(A)
+            r = t.as_tensor(coo.row.astype(np.int64), device=dv.device())
+            c = t.as_tensor(coo.col.astype(np.int64), device=dv.device())
+            dense.index_put_((r, c), t.as_tensor(coo.data.astype(np.float64), device=dv.device()), accumulate=True)
+        else:
+            dense.copy_(dv.to_device(np.asarray(A, dtype=np.float64), "f64").reshape(n, n))
+        was_np = not isinstance(b, t.Tensor)
+        bd = dv.to_device(b, "f64")
+        x = t.linalg.solve(dense, bd.reshape(n, -1)).reshape(bd.shape)
+        if verbose:
+            logger.info(f"Direct solver took {_time.time() - start:.2e} seconds.")
+        return OutputArgs(solution=x.cpu().numpy() if was_np else x, residual=np.array([]), success=True)
+
     def set_cleandod(self, cleandod):
         if cleandod is not None and not isinstance(cleandod, list):
             raise TypeError("Cleandod should be list")

@@ -151,6 +151,27 @@ int iga_gmres_start(int dtype, int64_t n, int m, const void* r, void* V0, double
 int iga_gmres_step(int dtype, int64_t n, int k, int m, void* V, int64_t ldv, void* w, double* ls, void* work,
                    void* stream);
 
+/* ---- Sum-factorised sparse assembly ------------------------------------------
+ * Replaces Operations.sumfact_assembling (operators/utils/operations.py:59-115) and
+ * the sums of it in BsplineOperations / NurbsOperations.assemble_scalar_*
+ * (operators/bsplines_ops.py:138-246, operators/nurbs_ops.py:182-287):
+ *   A[i, j] = sum_t scale[t] sum_q prod_d V_{t,d}(i_d, j_d, q_d) coef[t][q]   (* w[i] w[j] if w)
+ * with V_{t,d}(i, j, q) = W_{t,d}[i, q] B_{t,d}[q, j], on the Kronecker product of
+ * the 1-D patterns.  Host tables per direction d: pat_ptr[d] (nr[d]+1) and pat_col[d]
+ * (npairs[d]) list the pairs (i_d, j_d); lst_ptr[t*ndim+d] (npairs[d]+1) indexes,
+ * for term t, each pair's entries of lst_q[d] (quadrature point q_d) and lst_v[d]
+ * (value), nlist[d] entries each.  Device: coef[t] (prod nq values), w (optional,
+ * square matrices), and the outputs row_ptr (N_r+1, int64), cols (int64) and vals
+ * (float64) of iga_sumfact_nnz entries; rows in C order (direction 1 fastest),
+ * columns ascending.  ndim <= 4, nterms <= 32; every table index is checked on the
+ * host; the call returns once the matrix is written. */
+int64_t iga_sumfact_nnz(int ndim, const int64_t* nr, const int32_t* const* pat_ptr);
+int iga_sumfact_assemble(int ndim, int nterms, const int64_t* nr, const int64_t* nq, const int64_t* nc,
+                         const int32_t* const* pat_ptr, const int32_t* const* pat_col, const int64_t* npairs,
+                         const int32_t* const* lst_ptr, const int32_t* const* lst_q, const double* const* lst_v,
+                         const int64_t* nlist, const double* const* coef, const double* scale,
+                         int64_t* row_ptr, const double* w, int64_t* cols, double* vals, void* stream);
+
 /* Gather / scatter of a box of a 3-D (or lower) tensor-product array (the free
  * nodes of the FD preconditioner, fastdiag/utils/space_precond.py:104-122,296,315):
  *   gather:  sub[s] = full[offset(s)]        (box lo[d]..lo[d]+len[d]-1, dims n[d])

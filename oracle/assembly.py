@@ -8,7 +8,10 @@ assembly of ``operators/utils/operations.py:59-115`` computes the same matrices)
 For Gauss rules the NURBS operators are also assembled from the rational basis
 ``R_A = w_A N_A / W`` and its gradient ``dR_A = w_A (dN_A W - N_A dW) / W^2`` evaluated
 directly, which checks the chain-rule algebra of ``operators/nurbs_ops.py:343-390``
-without reusing it.
+without reusing it.  ``sumfact_terms`` is the same Kronecker formula for any list of
+(weights, basis, coefficient, sign) terms (pinned by reproducing ``bspline_mass`` and
+``bspline_stiffness``); the load vectors ``nurbs_*force_gauss`` integrate the rational
+basis directly.  Test infrastructure only.
 """
 from __future__ import annotations
 
@@ -106,3 +109,31 @@ def nurbs_mass_gauss(quadrule_list, m, nurbs_weights) -> sp.csr_array:
     R = rational_basis(quadrule_list, nurbs_weights)
     om = gauss_point_weights(quadrule_list)
     return sp.csr_array(R[0].T @ sp.diags_array(om * m) @ R[0])
+
+
+def sumfact_terms(quadrule_list, terms) -> sp.csr_array:
+    """``sum_t s_t W^{zeta_t} diag(c_t) B^{beta_t}`` for ``terms`` of
+    ``(zeta, beta, c, s)`` tuples: ``Operations.sumfact_assembling``
+    (``operators/utils/operations.py:59-115``) summed over terms, written as the
+    explicit Kronecker products it factorises."""
+    A = None
+    for zeta, beta, c, s in terms:
+        term = s * (weights_kron(quadrule_list, zeta) @ sp.diags_array(np.asarray(c, dtype=float)) @
+                    basis_kron(quadrule_list, beta))
+        A = term if A is None else A + term
+    return sp.csr_array(A)
+
+
+def nurbs_force_gauss(quadrule_list, c, nurbs_weights) -> np.ndarray:
+    """``F_A = sum_q omega_q R_A c_q`` (Gauss rules), ``c`` of shape (nm, N_q)."""
+    R = rational_basis(quadrule_list, nurbs_weights)
+    om = gauss_point_weights(quadrule_list)
+    return np.stack([R[0].T @ (om * ci) for ci in np.atleast_2d(c)])
+
+
+def nurbs_grad_force_gauss(quadrule_list, c, nurbs_weights) -> np.ndarray:
+    """``F_A = sum_q omega_q grad R_A . c_q`` (Gauss rules), ``c`` of shape (nm, d, N_q)."""
+    R = rational_basis(quadrule_list, nurbs_weights)
+    om = gauss_point_weights(quadrule_list)
+    nd = len(quadrule_list)
+    return np.stack([sum(R[1 + i].T @ (om * ci[i]) for i in range(nd)) for ci in c])
