@@ -235,6 +235,38 @@ def krylov_section(n=68 ** 3, m=200, sizes=(16, 64, 128), reps=20):
     return {"n": n, "step_us_by_basis_size": out}
 
 
+def assembly_section(p=2, nel=32, reps=3):
+    """Sum-factorised sparse assembly of the stiffness + mass matrix of the unit cube
+    (degree p, nel elements/dir, WQ type "2", lognormal conductivity) on the device:
+    seconds per matrix (tables, kernel, copy to a scipy CSR), and the oracle's
+    explicit Kronecker assembly of the same matrix on one host core for scale."""
+    import torch
+
+    from igaops_amd.geometry import lognormal_conductivity, unit_cube
+    from igaops_amd.operators import BsplineOperations
+    from oracle import assembly as asm
+
+    patch = unit_cube(p, nel)
+    C, m = patch.conductivity_coefficients(lognormal_conductivity(patch, 0.5, seed=1))
+    qs = patch.quadrule_list
+    A = None
+    times = []
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        A = BsplineOperations.assemble_scalar_gradu_gradv(qs, C) + BsplineOperations.assemble_scalar_u_v(qs, m)
+        times.append(time.perf_counter() - t0)
+    # CPU leg: the oracle as baseline and checker
+    t0 = time.perf_counter()
+    ref = asm.bspline_stiffness(qs, C) + asm.bspline_mass(qs, m)
+    t_cpu = time.perf_counter() - t0
+    err = float(abs(A - ref).max() / abs(ref).max())
+    return {"workload": f"unit cube p={p} nel={nel}, stiffness + mass", "n": int(A.shape[0]), "nnz": int(A.nnz),
+            "seconds_per_matrix": min(times), "max_rel_diff_vs_oracle": err,
+            "cpu_baseline": {"value": t_cpu, "unit": "s per matrix", "cores": 1, "kind": "port",
+                             "sample": "the oracle's explicit Kronecker assembly of the same matrix"}}
+
+
 def spacetime_section(steps=5, nrhs_seq=10, tol=1e-10, space=(3, 32), time_=(2, 64)):
     """Space-time heat equation: 3-D unit cube (degree 3, 32 elements/dir) x time
     (degree 2, 64 elements), WQ type "2" rules, unit diffusivity,
@@ -382,6 +414,7 @@ def main():
     ap.add_argument("--no-traffic", action="store_true")
     ap.add_argument("--spacetime", action="store_true", help="only the space-time GCRODR section")
     ap.add_argument("--krylov", action="store_true", help="only the GMRES orthogonalisation step timings")
+    ap.add_argument("--assembly", action="store_true", help="only the sparse assembly timing")
     ap.add_argument("--serve", type=int, default=0, help="only the serving section, with this many requests")
     ap.add_argument("--qps", type=float, default=20.0)
     ap.add_argument("--no-serve", action="store_true", help="skip the short serving stream of the default run")
@@ -398,6 +431,9 @@ def main():
         return
     if args.krylov:
         print(json.dumps({"krylov": krylov_section()}))
+        return
+    if args.assembly:
+        print(json.dumps({"assembly": assembly_section()}))
         return
     if args.serve:
         print(json.dumps({"serve": serve_section(args.serve, args.qps)}))
