@@ -20,6 +20,8 @@ Modules and the reference code each one follows:
   (``solvers/linear_solver.py:149-257,438-525``,
   ``solvers/utils/convergence_manager.py:13-327``) and GCRODR
   (``solvers/deflated/gcrodr.py:188-373``, ``solvers/deflated/helpers.py:13-265``)
+* ``polynomial``   - the GMRES polynomial preconditioner
+  (``solvers/polynomial/preconditioner.py:7-366``)
 
 How each function is pinned (closed forms, reference known answers, brute force)
 is listed in the module docstrings and exercised by ``tests/test_oracle_*.py``.
