@@ -35,6 +35,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+MFMA_PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}  # dense MFMA peaks of the MI355X
 
 WORKLOADS = {
     "annulus_p4_n64": dict(geometry="quarter_annulus", degree=4, nel=64, nrhs=16, sigma=0.5),
@@ -86,9 +87,12 @@ def cpu_baseline(patch, C, m, x, budget_s=20.0):
     return y, float(np.median(times)), reps
 
 
-def fd_error_vs_oracle(patch, dtype, x):
-    """Relative error of the FD apply (Dirichlet on all faces, ``Lambda = M + K``
-    eigenvalues) in ``dtype`` against the oracle's float64 apply on ``x``."""
+def fd_error_vs_oracle(patch, dtype, x, reps=20):
+    """The FD apply (Dirichlet on all faces, ``Lambda = M + K`` eigenvalues) in
+    ``dtype``: relative error against the oracle's float64 apply on ``x``, device time
+    per apply (HIP events, ``reps`` applies), its MFMA flops ``4 N_sub sum_d n_d`` (six
+    mode products) against the dense peak, and the oracle's time per apply on one
+    host core."""
     import torch
 
     from igaops_amd.fastdiag import SingleFD
@@ -102,10 +106,34 @@ def fd_error_vs_oracle(patch, dtype, x):
     fd.update_space_eigenvalues([1.0, 1.0])
     xd = torch.as_tensor(x, device="cuda").to(torch.float64 if dtype == "f64" else torch.float32)
     y = fd.apply(xd).double().cpu().numpy()
+    out = torch.empty_like(xd)
+    for _ in range(3):
+        fd.apply_device(xd, out)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fd.apply_device(xd, out)
+    e1.record()
+    torch.cuda.synchronize()
+    t_dev = e0.elapsed_time(e1) / reps * 1e-3
     ref_fd = ofd.SpaceFD([StandardGauss(q.degree, q.knotvector, "legendre") for q in qs], table)
     ref_fd.scalar_coefs = (1.0, 1.0)
+    t0 = time.perf_counter()
     ref = ref_fd.apply(x)
-    return float(np.linalg.norm(y - ref) / np.linalg.norm(ref))
+    t_cpu = time.perf_counter() - t0
+    lens = [len(ix) for ix in ref_fd.indices]
+    nsub = int(np.prod(lens))
+    flops = 4.0 * nsub * sum(lens)
+    peak = MFMA_PEAK_TFLOPS[dtype]
+    return {
+        "rel_err_vs_oracle": float(np.linalg.norm(y - ref) / np.linalg.norm(ref)),
+        "free_box": lens,
+        "us_per_apply": t_dev * 1e6,
+        "mfma": {"achieved": flops / t_dev / 1e12, "peak": peak, "unit": "TFLOP/s",
+                 "frac": flops / t_dev / 1e12 / peak, "flops": flops},
+        "cpu_baseline": {"value": t_cpu * 1e6, "unit": "us per apply", "cores": 1, "kind": "port",
+                         "sample": "the oracle's FD apply (numpy / scipy dgemm) on the same vector"},
+    }
 
 
 def solve_section(patch, C, m, nrhs, tol=1e-10):
@@ -265,6 +293,16 @@ def assembly_section(p=2, nel=32, reps=3):
             "seconds_per_matrix": min(times), "max_rel_diff_vs_oracle": err,
             "cpu_baseline": {"value": t_cpu, "unit": "s per matrix", "cores": 1, "kind": "port",
                              "sample": "the oracle's explicit Kronecker assembly of the same matrix"}}
+
+
+def krylov_roofline(n, j=64):
+    """One FP64 GMRES orthogonalisation step at basis size ``j`` on vectors of the
+    workload's length: two passes over ``j + 2`` vectors, against the HBM peak."""
+    us = krylov_section(n=n, m=max(2 * j, 16), sizes=(j,))["step_us_by_basis_size"]["f64"][str(j)]
+    nbytes = 2.0 * (j + 2) * n * 8
+    return {"basis_size": j, "us_per_step": us,
+            "hbm": {"achieved": nbytes / (us * 1e-6) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": nbytes / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, "bytes": nbytes}}
 
 
 def spacetime_section(steps=5, nrhs_seq=10, tol=1e-10, space=(3, 32), time_=(2, 64)):
@@ -513,7 +551,8 @@ def main():
             "sample": f"one right-hand side of the same workload, oracle matvec (numpy/scipy), median of {reps}",
             "seconds_per_matvec": t_cpu,
         }
-        fd_err = fd_error_vs_oracle(patch, args.dtype, x0)
+        fd_info = fd_error_vs_oracle(patch, args.dtype, x0)
+        fd_err = fd_info["rel_err_vs_oracle"]
     out = {
         "metric": "matvec GDOF/s (% HBM roofline); solves/s + p50/p99 latency at 1e-10 residual",
         "value": value,
@@ -554,6 +593,8 @@ def main():
         "batched": batched,
         "rel_err_vs_oracle": relerr,
         "fd_rel_err_vs_oracle": None if args.no_cpu else fd_err,
+        "fd_apply": None if args.no_cpu else fd_info,
+        "gmres_step": krylov_roofline(n),
         "cpu_baseline": cpu,
         "setup_s": setup_s,
     }
