@@ -190,3 +190,36 @@ def test_assembly_rejects_bad_tables():
                   P(lst_ptr), P(lst_q), (C.POINTER(C.c_double) * 1)(lst_v.ctypes.data_as(C.POINTER(C.c_double))),
                   i64(1), coef, (C.c_double * 1)(1.0), dv.ptr(buf), None, dv.ptr(buf), dv.ptr(buf),
                   dv.stream_ptr())
+
+
+@pytest.mark.parametrize("kind", ["u_v", "gradu_gradv", "u_gradv", "gradu_v"])
+def test_two_dimensional_nurbs_assembly_matches_matrix_free(kind):
+    """``tests/test_assemble.py:93-134`` in its own dimension: a 2-D NURBS quarter
+    annulus (degree 3, 3 elements, Gauss-Legendre), coefficients ``det J`` (times
+    ones), assembled matrix times a random vector against the matrix-free operator."""
+    from igaops_amd.operators import NurbsOperations
+
+    patch = quarter_annulus(3, 3, quadclass="gauss", quadtype="legendre")
+    qs = patch.quadrule_list[:2]
+    n2d = qs[0].nbctrlpts * qs[1].nbctrlpts
+    w = np.asarray(patch.nurbs_weights)[:n2d]  # the extrusion leaves the weights unchanged along z
+    nq = qs[0].nbquadpts * qs[1].nbquadpts
+    detj = np.random.default_rng(3).uniform(0.5, 1.5, nq)
+    coef = {"u_v": detj, "gradu_gradv": np.ones((2, 2, nq)) * detj, "u_gradv": np.ones((2, nq)) * detj,
+            "gradu_v": np.ones((2, nq)) * detj}[kind]
+    v = np.random.default_rng(0).random(n2d)
+    A = getattr(NurbsOperations, f"assemble_scalar_{kind}")(qs, coef, nurbs_weights=w)
+    _canonical(A)
+    ref = getattr(oo, f"nurbs_{kind}")(qs, coef, v, nurbs_weights=w)
+    np.testing.assert_allclose(A @ v, ref, rtol=1e-10, atol=1e-12 * np.abs(ref).max())
+
+
+def test_one_dimensional_bspline_assembly_matches_oracle():
+    from igaops_amd.operators import BsplineOperations
+
+    q = WeightedQuadrature(4, make_open_knotvector(4, 7), "1")
+    c = np.random.default_rng(5).uniform(0.5, 1.5, q.nbquadpts)
+    K = BsplineOperations.assemble_scalar_gradu_gradv([q], c.reshape(1, 1, -1))
+    M = BsplineOperations.assemble_scalar_u_v([q], c)
+    _close(K, asm.bspline_stiffness([q], c.reshape(1, 1, -1)))
+    _close(M, asm.bspline_mass([q], c))
