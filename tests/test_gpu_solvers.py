@@ -265,3 +265,32 @@ def test_fd_eigenvalue_updates_reach_the_device_plan():
     fd.add_scalar_space_time_correctors(stiffness_corrector=[np.array([1.0, 2.0, 0.5])])
     o.stiff_corr = np.array([1.0, 2.0, 0.5])
     assert rel(fd.apply(x), o.apply(x)) <= 1e-12
+
+
+def test_device_step_gmres_with_initial_guess_and_exact_preconditioner():
+    """The device-step GMRES from a nonzero x0 reaches the same solution; with the
+    exact inverse as preconditioner it stops after one step (the next tentative
+    basis vector vanishes: the breakdown path of the delayed reorthogonalisation)."""
+    import torch
+
+    from igaops_amd.runs import GmresRun
+
+    rng = np.random.default_rng(21)
+    n = 300
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    Ah = (Q * np.geomspace(1.0, 200.0, n)) @ Q.T
+    A = torch.as_tensor(Ah, device="cuda")
+    Ainv = torch.as_tensor(np.linalg.inv(Ah), device="cuda")
+    mv = lambda x, o: o.copy_(A @ x)
+    b = torch.as_tensor(rng.standard_normal(n), device="cuda")
+    ref = np.linalg.solve(Ah, b.cpu().numpy())
+    out = GmresRun(mv, None, b, 1e-12, 400).run()
+    assert out.success
+    np.testing.assert_allclose(out.solution.cpu().numpy(), ref, rtol=1e-8, atol=1e-10 * np.abs(ref).max())
+    x0 = torch.as_tensor(ref + 1e-3 * rng.standard_normal(n), device="cuda")
+    out0 = GmresRun(mv, None, b, 1e-12, 400, x0=x0).run()
+    assert out0.success and out0.iterations <= out.iterations
+    np.testing.assert_allclose(out0.solution.cpu().numpy(), ref, rtol=1e-8, atol=1e-10 * np.abs(ref).max())
+    exact = GmresRun(mv, lambda x, o: o.copy_(Ainv @ x), b, 1e-12, 50).run()
+    assert exact.success and exact.iterations <= 2
+    np.testing.assert_allclose(exact.solution.cpu().numpy(), ref, rtol=1e-9, atol=1e-11 * np.abs(ref).max())
