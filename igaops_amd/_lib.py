@@ -59,6 +59,8 @@ SIGNATURES = {
     "iga_fd_destroy": (_i32, [_vp]),
     "iga_fused_apply_sliced": (_i32, [_vp, _vp, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _i32, _vp, C.c_double, _i32,
                                       _vp, _vp]),
+    "iga_fused_apply_sliced2": (_i32, [_vp, _vp, _i64, _vp, _vp, _i64, _i64, _vp, _i64, _i64, _i32, _vp, C.c_double,
+                                       _i32, _vp, _vp]),
     "iga_fused_pack_strided": (_i32, [_vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp]),
     "iga_gmres_state_doubles": (_i64, [_i32]),
     "iga_gmres_layout": (_i32, [_i32, _i64p]),

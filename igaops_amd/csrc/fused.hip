@@ -22,16 +22,20 @@
 #include <atomic>
 #include <cstring>
 #include <functional>
+#include <type_traits>
 #include <vector>
 
 #include "common.hpp"
 
 namespace iga {
 
-enum FusedForm { FORM_M1 = 0, FORM_K6 = 1, FORM_KM7 = 2, FORM_SYM10 = 3, FORM_GEN16 = 4 };
+// FORM_KM7X: the KM7 fields with the trial value taken from a second input x1 (the
+// gradient from x): the space-time operator's stiffness on the time-interpolated
+// vector and mass on the time-differentiated one in one sweep.
+enum FusedForm { FORM_M1 = 0, FORM_K6 = 1, FORM_KM7 = 2, FORM_SYM10 = 3, FORM_GEN16 = 4, FORM_KM7X = 5 };
 
 __host__ __device__ constexpr int form_ncoef(int f) {
-  return f == FORM_M1 ? 1 : f == FORM_K6 ? 6 : f == FORM_KM7 ? 7 : f == FORM_SYM10 ? 10 : 16;
+  return f == FORM_M1 ? 1 : f == FORM_K6 ? 6 : (f == FORM_KM7 || f == FORM_KM7X) ? 7 : f == FORM_SYM10 ? 10 : 16;
 }
 
 constexpr int kTQ1 = 16, kTQ2 = 16, kThreads = kTQ1 * kTQ2;
@@ -62,7 +66,8 @@ struct FusedArgs {
   int64_t x_dstride, x_rstride, coef_rstride;
   const T* coef;
   const T* x;
-  const T* w;  // NURBS weights or nullptr
+  const T* x1;  // FORM_KM7X: the input of the value (mass) term, strides of x
+  const T* w;   // NURBS weights or nullptr
   T* partial;
   DirTab<T> d[3];
 };
@@ -87,7 +92,7 @@ __device__ inline void point_contract(const T* __restrict__ c, int64_t stride, c
     f[1] = c11 * u[1] + c12 * u[2] + c13 * u[3];
     f[2] = c12 * u[1] + c22 * u[2] + c23 * u[3];
     f[3] = c13 * u[1] + c23 * u[2] + c33 * u[3];
-  } else if constexpr (FORM == FORM_KM7) {
+  } else if constexpr (FORM == FORM_KM7 || FORM == FORM_KM7X) {
     const T m = c[0], c11 = c[stride], c22 = c[2 * stride], c33 = c[3 * stride];
     const T c12 = c[4 * stride], c13 = c[5 * stride], c23 = c[6 * stride];
     f[0] = m * u[0];
@@ -122,8 +127,10 @@ __device__ __attribute__((always_inline)) inline void fused_apply_body(const Fus
   // run at two waves per SIMD anyway), and costs the FP64 P <= 5 instances their
   // third wave.
   constexpr bool kUnrollBand = sizeof(T) == 4 || P >= 6;
+  constexpr bool DUAL = FORM == FORM_KM7X;
   __shared__ T xs[kDMax][kDMax + 1];
-  __shared__ T t1s[2][kDMax][kTQ1];
+  __shared__ T xs1[DUAL ? kDMax : 1][kDMax + 1];
+  __shared__ T t1s[DUAL ? 3 : 2][kDMax][kTQ1];
   __shared__ T s2s[3][kTQ2][kTQ1];
   __shared__ T r1s[2][kDMax][kTQ1];
   __shared__ T b1s[4][kTQ1][P];
@@ -207,7 +214,8 @@ __device__ __attribute__((always_inline)) inline void fused_apply_body(const Fus
   }
   __syncthreads();
 
-  T tr[3][P];  // trial partials of the window planes: (val,val), (d1,val), (val,d2)
+  constexpr int NTR = DUAL ? 4 : 3;
+  T tr[NTR][P];  // trial partials of the window planes: (val,val), (d1,val), (val,d2) [, (val,val) of x1]
   T te[3][P];  // test partials: (val,val), (d1,val), (val,d2) in directions (1,2)
 
   // x block of one DOF plane, held in registers between its (early) load and its
@@ -217,6 +225,7 @@ __device__ __attribute__((always_inline)) inline void fused_apply_body(const Fus
   // apart and applied when the block is used.
   constexpr int XR = (kDMax * kDMax + kThreads - 1) / kThreads;
   const T* xk = a.x + (int64_t)k * a.x_rstride;
+  const T* xk1 = DUAL ? a.x1 + (int64_t)k * a.x_rstride : xk;
   const int64_t xds = a.x_dstride;
   const bool nurbs = a.w != nullptr;
   // Per-thread offsets of its x-block elements within a DOF plane and their LDS
@@ -235,18 +244,28 @@ __device__ __attribute__((always_inline)) inline void fused_apply_body(const Fus
     xij[e] = ok ? (i2 << 8) | i1 : -1;
   }
   const int64_t plane = (int64_t)n1 * n2;
-  struct XBlock {
+  // the second input's registers exist only in the two-input form
+  struct XBlock1 {
     T x[XR], w[XR];
   };
-  struct Trial {
+  struct XBlock2 {
+    T x[XR], w[XR], x1[XR];
+  };
+  using XBlock = std::conditional_t<DUAL, XBlock2, XBlock1>;
+  struct Trial1 {
     T v00, v10, v01;
   };
+  struct Trial2 {
+    T v00, v10, v01, vm;
+  };
+  using Trial = std::conditional_t<DUAL, Trial2, Trial1>;
   auto load_x = [&](int c3) __attribute__((always_inline)) {
     XBlock b;
 #pragma unroll
     for (int e = 0; e < XR; ++e) {
       const int64_t dof = xoff[e] + plane * c3;
       b.x[e] = xk[dof * xds];
+      if constexpr (DUAL) b.x1[e] = xk1[dof * xds];
       b.w[e] = nurbs ? a.w[dof] : T(1);
     }
     return b;
@@ -259,7 +278,7 @@ __device__ __attribute__((always_inline)) inline void fused_apply_body(const Fus
   T* part = a.partial + (int64_t)k * a.part_stride + a.part_base[(int64_t)tile * a.nslab + s];
   T* xsf = &xs[0][0];
   auto plane_phase = [&](bool do_flush, int c3f, bool do_trial, XBlock xb) __attribute__((always_inline)) {
-    Trial out{T(0), T(0), T(0)};
+    Trial out{};
     __syncthreads();  // LDS buffers are reused
     if (do_flush) {
       s2s[0][l2][l1] = active ? te[0][0] : T(0);
@@ -268,8 +287,13 @@ __device__ __attribute__((always_inline)) inline void fused_apply_body(const Fus
     }
     if (do_trial) {
 #pragma unroll
-      for (int e = 0; e < XR; ++e)
-        if (xij[e] >= 0) xsf[(xij[e] >> 8) * (kDMax + 1) + (xij[e] & 255)] = nurbs ? xb.x[e] * xb.w[e] : xb.x[e];
+      for (int e = 0; e < XR; ++e) {
+        if (xij[e] >= 0) {
+          const int slot = (xij[e] >> 8) * (kDMax + 1) + (xij[e] & 255);
+          xsf[slot] = nurbs ? xb.x[e] * xb.w[e] : xb.x[e];
+          if constexpr (DUAL) (&xs1[0][0])[slot] = nurbs ? xb.x1[e] * xb.w[e] : xb.x1[e];
+        }
+      }
     }
     __syncthreads();
     // stage A: direction-2 test contraction (R1) and direction-1 trial contraction (T1)
@@ -296,17 +320,19 @@ __device__ __attribute__((always_inline)) inline void fused_apply_body(const Fus
       } else {
         const int i2 = (it - na) / kTQ1, l = (it - na) % kTQ1;
         const int f = f1s[l];
-        T acc0 = T(0), acc1 = T(0);
+        T acc0 = T(0), acc1 = T(0), acc2 = T(0);
         if (l < q1n) {
 #pragma unroll
           for (int j = 0; j < P; ++j) {
             const T xv = xs[i2][f + j];
             acc0 += b1s[0][l][j] * xv;
             acc1 += b1s[1][l][j] * xv;
+            if constexpr (DUAL) acc2 += b1s[0][l][j] * xs1[i2][f + j];
           }
         }
         t1s[0][i2][l] = acc0;
         t1s[1][i2][l] = acc1;
+        if constexpr (DUAL) t1s[DUAL ? 2 : 0][i2][l] = acc2;
       }
     }
     __syncthreads();
@@ -343,6 +369,7 @@ __device__ __attribute__((always_inline)) inline void fused_apply_body(const Fus
         out.v00 += b0 * a0;
         out.v10 += b0 * a1;
         out.v01 += b1 * a0;
+        if constexpr (DUAL) out.vm += b0 * t1s[DUAL ? 2 : 0][f + j][l1];
       }
     }
     return out;
@@ -362,6 +389,7 @@ __device__ __attribute__((always_inline)) inline void fused_apply_body(const Fus
       tr[0][j] = t.v00;
       tr[1][j] = t.v10;
       tr[2][j] = t.v01;
+      if constexpr (DUAL) tr[NTR - 1][j] = t.vm;
       te[0][j] = te[1][j] = te[2][j] = T(0);
     }
   }
@@ -394,15 +422,15 @@ __device__ __attribute__((always_inline)) inline void fused_apply_body(const Fus
 #pragma unroll
     for (int j = 0; j < P - 1; ++j) {
 #pragma unroll
-      for (int g = 0; g < 3; ++g) {
-        te[g][j] = te[g][j + 1];
-        tr[g][j] = tr[g][j + 1];
-      }
+      for (int g = 0; g < 3; ++g) te[g][j] = te[g][j + 1];
+#pragma unroll
+      for (int g = 0; g < NTR; ++g) tr[g][j] = tr[g][j + 1];
     }
     te[0][P - 1] = te[1][P - 1] = te[2][P - 1] = T(0);
     tr[0][P - 1] = t.v00;
     tr[1][P - 1] = t.v10;
     tr[2][P - 1] = t.v01;
+    if constexpr (DUAL) tr[NTR - 1][P - 1] = t.vm;
     ++cur;
     // unconditional (clamped plane) so that the load queue has the same shape on every path
     xnext = load_x(cur + P <= c3hi ? cur + P : c3hi);
@@ -423,7 +451,7 @@ __device__ __attribute__((always_inline)) inline void fused_apply_body(const Fus
       T u[4] = {T(0), T(0), T(0), T(0)};
 #pragma unroll
       for (int j = 0; j < P; ++j) {
-        u[0] += tb[0][j] * tr[0][j];
+        u[0] += tb[0][j] * tr[DUAL ? NTR - 1 : 0][j];  // the value: from x1 for FORM_KM7X
         u[1] += tb[0][j] * tr[1][j];
         u[2] += tb[0][j] * tr[2][j];
         u[3] += tb[1][j] * tr[0][j];
@@ -636,6 +664,7 @@ __global__ void fused_pack_kernel(PackArgs<T> p) {
         dst[3 * st] = (T)D[1][2]; dst[4 * st] = (T)D[1][3]; dst[5 * st] = (T)D[2][3];
         break;
       case FORM_KM7:
+      case FORM_KM7X:
         dst[0] = (T)D[0][0]; dst[st] = (T)D[1][1]; dst[2 * st] = (T)D[2][2]; dst[3 * st] = (T)D[3][3];
         dst[4 * st] = (T)D[1][2]; dst[5 * st] = (T)D[1][3]; dst[6 * st] = (T)D[2][3];
         break;
@@ -778,6 +807,7 @@ static int resident_blocks_p(int form) {
     case FORM_K6: return resident_blocks_pf<T, P, FORM_K6>();
     case FORM_KM7: return resident_blocks_pf<T, P, FORM_KM7>();
     case FORM_SYM10: return resident_blocks_pf<T, P, FORM_SYM10>();
+    case FORM_KM7X: return resident_blocks_pf<T, P, FORM_KM7X>();
     default: return resident_blocks_pf<T, P, FORM_GEN16>();
   }
 }
@@ -801,6 +831,7 @@ static void launch_apply_p(const iga_fused_s* f, const FusedArgs<T>& a, int nblk
     case FORM_K6: launch_apply_pf<T, P, FORM_K6>(a, nblk, s); break;
     case FORM_KM7: launch_apply_pf<T, P, FORM_KM7>(a, nblk, s); break;
     case FORM_SYM10: launch_apply_pf<T, P, FORM_SYM10>(a, nblk, s); break;
+    case FORM_KM7X: launch_apply_pf<T, P, FORM_KM7X>(a, nblk, s); break;
     default: launch_apply_pf<T, P, FORM_GEN16>(a, nblk, s); break;
   }
 }
@@ -829,8 +860,10 @@ struct Layout {
 template <typename T>
 static void run_apply(const iga_fused_s* f, const void* coef, const void* x, void* y, int nrhs, const void* w,
                       double alpha, int accumulate, void* work, hipStream_t s, int stages = 3,
-                      Layout lay = Layout{0, 0, 0, 0, 0}) {
+                      Layout lay = Layout{0, 0, 0, 0, 0}, const void* x1 = nullptr) {
   if (lay.xd == 0) lay = Layout::interleaved(nrhs);
+  IGA_CHECK_ARG((f->form == FORM_KM7X) == (x1 != nullptr),
+                "the two-input form takes a second input (iga_fused_apply_sliced2), the others do not");
   if (stages & 1) {
   FusedArgs<T> a{};
   for (int d = 0; d < 3; ++d) {
@@ -854,6 +887,7 @@ static void run_apply(const iga_fused_s* f, const void* coef, const void* x, voi
   a.coef_rstride = lay.cr;
   a.coef = (const T*)coef;
   a.x = (const T*)x;
+  a.x1 = (const T*)x1;
   a.w = (const T*)w;
   a.partial = (T*)work;
   const int64_t nlog = (int64_t)f->nt1 * f->nt2 * f->nslab * nrhs;
@@ -923,7 +957,7 @@ int iga_fused_create(iga_fused_t* out, const int32_t* nc, const int32_t* nq, int
   return guard([&] {
     IGA_CHECK_ARG(out && nc && nq && first && tabs, "NULL argument");
     IGA_CHECK_ARG(P >= 3 && P <= 8, "band width p+1 must be in 3..8");
-    IGA_CHECK_ARG(form >= FORM_M1 && form <= FORM_GEN16, "unknown coefficient form");
+    IGA_CHECK_ARG(form >= FORM_M1 && form <= FORM_KM7X, "unknown coefficient form");
     IGA_CHECK_ARG(dtype == IGA_F64 || dtype == IGA_F32, "unknown dtype");
     for (int d = 0; d < 3; ++d) {
       IGA_CHECK_ARG(nc[d] >= P && nq[d] >= 1, "bad sizes");
@@ -1058,7 +1092,7 @@ int iga_fused_pack(iga_fused_t f, const double* C, const double* m, const double
     IGA_CHECK_ARG(f && packed, "NULL argument");
     IGA_CHECK_ARG((W == nullptr) == (dW == nullptr), "W and dW go together");
     if (f->form != FORM_M1) IGA_CHECK_ARG(C, "stiffness forms need C");
-    if (f->form == FORM_M1 || f->form == FORM_KM7) IGA_CHECK_ARG(m, "mass forms need m");
+    if (f->form == FORM_M1 || f->form == FORM_KM7 || f->form == FORM_KM7X) IGA_CHECK_ARG(m, "mass forms need m");
     if (f->dtype == IGA_F64)
       run_pack<double>(f, C, m, W, dW, packed, as_stream(stream));
     else
@@ -1098,7 +1132,7 @@ int iga_fused_pack_strided(iga_fused_t f, const double* C, const double* m, cons
     IGA_CHECK_ARG(f && packed, "NULL argument");
     IGA_CHECK_ARG((W == nullptr) == (dW == nullptr), "W and dW go together");
     if (f->form != FORM_M1) IGA_CHECK_ARG(C, "stiffness forms need C");
-    if (f->form == FORM_M1 || f->form == FORM_KM7) IGA_CHECK_ARG(m, "mass forms need m");
+    if (f->form == FORM_M1 || f->form == FORM_KM7 || f->form == FORM_KM7X) IGA_CHECK_ARG(m, "mass forms need m");
     const int64_t nqt = (int64_t)f->nq[0] * f->nq[1] * f->nq[2];
     IGA_CHECK_ARG(comp_stride >= nqt, "component stride below the number of points");
     if (f->dtype == IGA_F64)
@@ -1121,6 +1155,23 @@ int iga_fused_apply_sliced(iga_fused_t f, const void* packed, int64_t coef_slice
       run_apply<double>(f, packed, x, y, nslices, nurbs_w, alpha, accumulate, work, as_stream(stream), 3, lay);
     else
       run_apply<float>(f, packed, x, y, nslices, nurbs_w, alpha, accumulate, work, as_stream(stream), 3, lay);
+  });
+}
+
+int iga_fused_apply_sliced2(iga_fused_t f, const void* packed, int64_t coef_slice_stride, const void* x,
+                            const void* x1, int64_t x_dstride, int64_t x_sstride, void* y, int64_t y_dstride,
+                            int64_t y_sstride, int nslices, const void* nurbs_w, double alpha, int accumulate,
+                            void* work, void* stream) {
+  return guard([&] {
+    IGA_CHECK_ARG(f && packed && x && x1 && y && work, "NULL argument");
+    IGA_CHECK_ARG(nslices >= 1, "nslices must be >= 1");
+    IGA_CHECK_ARG(x_dstride >= 1 && y_dstride >= 1 && x_sstride >= 0 && y_sstride >= 0 && coef_slice_stride >= 0,
+                  "bad strides");
+    const Layout lay{x_dstride, x_sstride, y_dstride, y_sstride, coef_slice_stride};
+    if (f->dtype == IGA_F64)
+      run_apply<double>(f, packed, x, y, nslices, nurbs_w, alpha, accumulate, work, as_stream(stream), 3, lay, x1);
+    else
+      run_apply<float>(f, packed, x, y, nslices, nurbs_w, alpha, accumulate, work, as_stream(stream), 3, lay, x1);
   });
 }
 

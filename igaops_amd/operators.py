@@ -307,7 +307,8 @@ def nurbs_weight_fields(quadrule_list, w_dev):
 # ---------------------------------------------------------------------------
 
 FORM_M1, FORM_K6, FORM_KM7, FORM_SYM10, FORM_GEN16 = 0, 1, 2, 3, 4
-FORM_NCOEF = {0: 1, 1: 6, 2: 7, 3: 10, 4: 16}
+FORM_KM7X = 5  # the KM7 fields, value (mass) term on a second input
+FORM_NCOEF = {0: 1, 1: 6, 2: 7, 3: 10, 4: 16, 5: 7}
 
 
 class FusedPlan:
@@ -402,6 +403,16 @@ class FusedPlan:
         _lib.call("iga_fused_apply_sliced", self._h, dv.ptr(packed), int(coef_slice_stride), int(x_ptr),
                   int(x_dstride), int(x_sstride), int(y_ptr), int(y_dstride), int(y_sstride), int(nslices), 0,
                   float(alpha), int(bool(accumulate)), dv.ptr(work), dv.stream_ptr(stream))
+
+    def apply_sliced2(self, packed, coef_slice_stride: int, x_ptr: int, x1_ptr: int, x_dstride: int, x_sstride: int,
+                      y_ptr: int, y_dstride: int, y_sstride: int, nslices: int, alpha=1.0, accumulate=False,
+                      work=None, stream=None):
+        """``apply_sliced`` of a FORM_KM7X plan: the stiffness on ``x``, the mass on
+        ``x1`` (same strides), one sweep."""
+        work = work if work is not None else self.workspace(nslices)
+        _lib.call("iga_fused_apply_sliced2", self._h, dv.ptr(packed), int(coef_slice_stride), int(x_ptr),
+                  int(x1_ptr), int(x_dstride), int(x_sstride), int(y_ptr), int(y_dstride), int(y_sstride),
+                  int(nslices), 0, float(alpha), int(bool(accumulate)), dv.ptr(work), dv.stream_ptr(stream))
 
     def apply(self, packed, x, y, nrhs: int = 1, nurbs_w=None, alpha=1.0, accumulate=False, work=None, stream=None):
         work = work if work is not None else self.workspace(nrhs)

@@ -254,7 +254,7 @@ class SpaceTimeOperator:
 
     # -- fused, sliced in time -----------------------------------------------------
     def _build_fused(self, c, C):
-        from .operators import FORM_GEN16, FORM_K6, FORM_M1, band_of, fused_plan
+        from .operators import FORM_GEN16, FORM_K6, FORM_KM7X, FORM_M1, band_of, fused_plan
 
         t = dv.torch()
         qs, qt = self.quadrule_list[:3], self.quadrule_list[3]
@@ -283,17 +283,28 @@ class SpaceTimeOperator:
             md = t.full((self.nqs,), float(c), dtype=t.float64, device=dev)
             m_slice = 0
         nslab = -(-int(qs[2].nbquadpts) // 64)  # fewest slabs: many slices fill the device
-        self.planK = fused_plan(qs, FORM_K6 if sym else FORM_GEN16, self.dtype, nslab)
-        self.planM = fused_plan(qs, FORM_M1, self.dtype, nslab)
-        self.packK = dv.empty(self.planK.coef_elems * self.nqt, self.dtype)
-        self.packM = dv.empty(self.planM.coef_elems * self.nqt, self.dtype)
-        zero_m = t.zeros(self.nqs, dtype=t.float64, device=dev) if not sym else None
-        for s in range(self.nqt):
-            cp = dv.ptr(Cd) + 8 * s * c_slice
-            mp = dv.ptr(md) + 8 * s * m_slice
-            self.planK.pack_strided(cp, dv.ptr(zero_m) if zero_m is not None else 0, c_stride, self.packK,
-                                    s * self.planK.coef_elems)
-            self.planM.pack_strided(0, mp, self.nqs, self.packM, s * self.planM.coef_elems)
+        # symmetric C: one sweep with the stiffness on the time-interpolated vector and
+        # the mass on the time-differentiated one (form KM7X); otherwise two sweeps
+        self.dual = sym
+        if sym:
+            self.planK = fused_plan(qs, FORM_KM7X, self.dtype, nslab)
+            self.planM = None
+            self.packK = dv.empty(self.planK.coef_elems * self.nqt, self.dtype)
+            self.packM = None
+            for s in range(self.nqt):
+                self.planK.pack_strided(dv.ptr(Cd) + 8 * s * c_slice, dv.ptr(md) + 8 * s * m_slice, c_stride,
+                                        self.packK, s * self.planK.coef_elems)
+        else:
+            self.planK = fused_plan(qs, FORM_GEN16, self.dtype, nslab)
+            self.planM = fused_plan(qs, FORM_M1, self.dtype, nslab)
+            self.packK = dv.empty(self.planK.coef_elems * self.nqt, self.dtype)
+            self.packM = dv.empty(self.planM.coef_elems * self.nqt, self.dtype)
+            zero_m = t.zeros(self.nqs, dtype=t.float64, device=dev)
+            for s in range(self.nqt):
+                self.planK.pack_strided(dv.ptr(Cd) + 8 * s * c_slice, dv.ptr(zero_m), c_stride, self.packK,
+                                        s * self.planK.coef_elems)
+                self.planM.pack_strided(0, dv.ptr(md) + 8 * s * m_slice, self.nqs, self.packM,
+                                        s * self.planM.coef_elems)
         self.time_trial = band_of(qt.basis[0], qt.basis[1])
         self.time_test = band_of(qt.weights[0])
         self.ncoef_fields = (6 if sym else 16) + 1
@@ -307,7 +318,7 @@ class SpaceTimeOperator:
             n = self.nqt * self.ncs * nrhs
             buf = (t.empty(n, dtype=tdt, device=dv.device()), t.empty(n, dtype=tdt, device=dv.device()),
                    t.empty(n, dtype=tdt, device=dv.device()), self.planK.workspace(self.nqt),
-                   self.planM.workspace(self.nqt))
+                   self.planM.workspace(self.nqt) if self.planM is not None else None)
             self._work[nrhs] = buf
         return buf
 
@@ -324,10 +335,14 @@ class SpaceTimeOperator:
         for r in range(nrhs):
             xd, xs = nrhs, self.ncs * nrhs
             off = r * es
-            self.planK.apply_sliced(self.packK, self.planK.coef_elems, dv.ptr(X0) + off, xd, xs, dv.ptr(Y) + off,
-                                    xd, xs, self.nqt, work=wK, stream=stream)
-            self.planM.apply_sliced(self.packM, self.planM.coef_elems, dv.ptr(X1) + off, xd, xs, dv.ptr(Y) + off,
-                                    xd, xs, self.nqt, accumulate=True, work=wM, stream=stream)
+            if self.dual:
+                self.planK.apply_sliced2(self.packK, self.planK.coef_elems, dv.ptr(X0) + off, dv.ptr(X1) + off, xd,
+                                         xs, dv.ptr(Y) + off, xd, xs, self.nqt, work=wK, stream=stream)
+            else:
+                self.planK.apply_sliced(self.packK, self.planK.coef_elems, dv.ptr(X0) + off, xd, xs,
+                                        dv.ptr(Y) + off, xd, xs, self.nqt, work=wK, stream=stream)
+                self.planM.apply_sliced(self.packM, self.planM.coef_elems, dv.ptr(X1) + off, xd, xs,
+                                        dv.ptr(Y) + off, xd, xs, self.nqt, accumulate=True, work=wM, stream=stream)
         if out is None:
             out = t.empty(self.ndof * nrhs, dtype=x.dtype, device=x.device)
         self.time_test.apply(Y, [out], 1, self.ncs * nrhs, stream=stream)

@@ -197,7 +197,8 @@ int iga_box_scatter(int dtype, int ndim, const int64_t* n, const int64_t* lo, co
  *   W0[q][j] = weights[0][c, q], W1[q][j] = weights[2][c, q].
  * form: 0 mass (1 coefficient/point), 1 stiffness with symmetric C (6),
  *       2 stiffness + mass (7), 3 symmetric 4x4 (10: NURBS stiffness [+ mass]),
- *       4 general 4x4 (16).
+ *       4 general 4x4 (16), 5 the fields of 2 with the mass term on a second input
+ *       (iga_fused_apply_sliced2).
  * nslab: number of q3 slabs (0 = automatic). */
 typedef struct iga_fused_s* iga_fused_t;
 
@@ -246,6 +247,16 @@ int iga_fused_apply_sliced(iga_fused_t plan, const void* packed, int64_t coef_sl
                            int64_t x_dstride, int64_t x_sstride, void* y, int64_t y_dstride, int64_t y_sstride,
                            int nslices, const void* nurbs_w, double alpha, int accumulate, void* work,
                            void* stream);
+
+/* The space-time operator's two sliced sweeps in one (form 5, the KM7 fields): the
+ * stiffness term on x and the mass term on x1 (same strides), i.e.
+ * gradu_gradv(time_ders=(0,0)) + u_v(time_ders=(0,1)) of operators/bsplines_ops.py
+ * on the time-interpolated and time-differentiated vectors; the plan must have
+ * form 5 and the other apply entries reject it. */
+int iga_fused_apply_sliced2(iga_fused_t plan, const void* packed, int64_t coef_slice_stride, const void* x,
+                            const void* x1, int64_t x_dstride, int64_t x_sstride, void* y, int64_t y_dstride,
+                            int64_t y_sstride, int nslices, const void* nurbs_w, double alpha, int accumulate,
+                            void* work, void* stream);
 
 /* iga_fused_pack with the components of C (and dW) comp_stride elements apart
  * (>= the plan's number of points): packs one time slice of space-time coefficient
