@@ -198,7 +198,9 @@ class SpaceTimeOperator:
     (one banded mode product), then, for every time quadrature point, the fused 3-D
     sweep with that point's coefficients (stiffness on the ``B0_t`` slice, mass on
     the ``B1_t`` slice; both terms test with ``W0_t``), and finally the time test
-    weights ``W0_t``.  Otherwise (and with ``fused=False``) the composed device path
+    weights ``W0_t``.  With a symmetric ``C`` the sweep reads both slices in one
+    launch (seven coefficient fields); otherwise a general stiffness sweep and a mass
+    sweep follow each other.  Otherwise (and with ``fused=False``) the composed device path
     evaluates the sum of terms; there, identically zero fields are left out."""
 
     def __init__(self, quadrule_list, c=1.0, C=None, dtype: str = "f64", fused: Optional[bool] = None):
