@@ -102,55 +102,78 @@ struct Box {
   int64_t n[4], lo[4], len[4];
 };
 
+// Index arithmetic in I = int32_t when every index of the call, plus one grid
+// stride (at most 2^28 threads), stays below 2^31 -- the hosts use it below 2^30
+// elements -- else int64_t: a 32-bit division is a short sequence, a 64-bit one a
+// long one.
+
 // sub index s -> full index; the box is stored with direction 1 fastest.
-__device__ inline int64_t box_offset(const Box& b, int64_t s) {
-  int64_t off = 0, stride = 1;
+template <typename I>
+__device__ inline I box_offset(const Box& b, I s) {
+  I off = 0, stride = 1;
   for (int d = 0; d < b.ndim; ++d) {
-    const int64_t c = s % b.len[d];
-    s /= b.len[d];
-    off += (b.lo[d] + c) * stride;
-    stride *= b.n[d];
+    const I len = (I)b.len[d];
+    const I c = s % len;
+    s /= len;
+    off += ((I)b.lo[d] + c) * stride;
+    stride *= (I)b.n[d];
   }
   return off;
 }
 
-__device__ inline bool in_box(const Box& b, int64_t f) {
+// full index f -> whether it lies in the box, and its sub index when it does
+template <typename I>
+__device__ inline bool box_sub_index(const Box& b, I f, I& sub) {
+  bool in = true;
+  I s = 0, stride = 1;
   for (int d = 0; d < b.ndim; ++d) {
-    const int64_t c = f % b.n[d];
-    f /= b.n[d];
-    if (c < b.lo[d] || c >= b.lo[d] + b.len[d]) return false;
+    const I n = (I)b.n[d];
+    const I c = f % n - (I)b.lo[d];
+    f /= n;
+    in = in && c >= 0 && c < (I)b.len[d];
+    s += c * stride;
+    stride *= (I)b.len[d];
   }
-  return true;
+  sub = s;
+  return in;
 }
 
-template <typename T>
-__global__ void box_gather_kernel(Box b, int64_t nsub, int64_t r, const T* __restrict__ full,
-                                  T* __restrict__ sub) {
-  const int64_t total = nsub * r;
-  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
-       idx += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t s = idx / r, k = idx % r;
-    sub[idx] = full[box_offset(b, s) * r + k];
-  }
-}
-
-template <typename T>
-__global__ void box_scatter_kernel(Box b, int64_t nsub, int64_t r, const T* __restrict__ sub,
-                                   T* __restrict__ full) {
-  const int64_t total = nsub * r;
-  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
-       idx += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t s = idx / r, k = idx % r;
-    full[box_offset(b, s) * r + k] = sub[idx];
+template <typename T, typename I>
+__global__ void box_gather_kernel(Box b, I nsub, I r, const T* __restrict__ full, T* __restrict__ sub) {
+  const I total = nsub * r;
+  for (I idx = blockIdx.x * (I)blockDim.x + threadIdx.x; idx < total; idx += (I)gridDim.x * blockDim.x) {
+    const I s = idx / r, k = idx % r;
+    sub[idx] = full[box_offset<I>(b, s) * r + k];
   }
 }
 
-template <typename T>
-__global__ void box_zero_outside_kernel(Box b, int64_t nfull, int64_t r, T* __restrict__ full) {
-  const int64_t total = nfull * r;
-  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
-       idx += (int64_t)gridDim.x * blockDim.x) {
-    if (!in_box(b, idx / r)) full[idx] = T(0);
+template <typename T, typename I>
+__global__ void box_scatter_kernel(Box b, I nsub, I r, const T* __restrict__ sub, T* __restrict__ full) {
+  const I total = nsub * r;
+  for (I idx = blockIdx.x * (I)blockDim.x + threadIdx.x; idx < total; idx += (I)gridDim.x * blockDim.x) {
+    const I s = idx / r, k = idx % r;
+    full[box_offset<I>(b, s) * r + k] = sub[idx];
+  }
+}
+
+// every element of the full array: the box value inside, zero outside (one pass)
+template <typename T, typename I>
+__global__ void box_scatter_full_kernel(Box b, I nfull, I r, const T* __restrict__ sub, T* __restrict__ full) {
+  const I total = nfull * r;
+  for (I idx = blockIdx.x * (I)blockDim.x + threadIdx.x; idx < total; idx += (I)gridDim.x * blockDim.x) {
+    const I f = idx / r, k = idx % r;
+    I si;
+    const bool in = box_sub_index<I>(b, f, si);
+    full[idx] = in ? sub[si * r + k] : T(0);
+  }
+}
+
+template <typename T, typename I>
+__global__ void box_zero_outside_kernel(Box b, I nfull, I r, T* __restrict__ full) {
+  const I total = nfull * r;
+  for (I idx = blockIdx.x * (I)blockDim.x + threadIdx.x; idx < total; idx += (I)gridDim.x * blockDim.x) {
+    I si;
+    if (!box_sub_index<I>(b, idx / r, si)) full[idx] = T(0);
   }
 }
 
@@ -287,15 +310,30 @@ int iga_box_gather(int dtype, int ndim, const int64_t* n, const int64_t* lo, con
     int64_t nsub, nfull;
     Box b = make_box(ndim, n, lo, len, &nsub, &nfull);
     if (nsub == 0) return;
-    dim3 grid(grid_for(nsub * nrhs, 256)), block(256);
-    if (dtype == IGA_F64)
-      box_gather_kernel<double><<<grid, block, 0, as_stream(stream)>>>(b, nsub, nrhs,
-                                                                       (const double*)full, (double*)sub);
-    else if (dtype == IGA_F32)
-      box_gather_kernel<float><<<grid, block, 0, as_stream(stream)>>>(b, nsub, nrhs, (const float*)full,
-                                                                      (float*)sub);
-    else
-      throw Error(IGA_ERR_INVALID, "unknown dtype");
+    hipStream_t s = as_stream(stream);
+    // the index type is chosen by the largest index the kernel forms (full offsets)
+    const int64_t count = nfull * nrhs;
+    if (count < ((int64_t)1 << 30)) {
+      if (dtype == IGA_F64)
+        box_gather_kernel<double, int32_t><<<grid_for(nsub * nrhs, 256), 256, 0, s>>>(
+            b, (int32_t)nsub, (int32_t)nrhs, (const double*)full, (double*)sub);
+      else if (dtype == IGA_F32)
+        box_gather_kernel<float, int32_t><<<grid_for(nsub * nrhs, 256), 256, 0, s>>>(
+            b, (int32_t)nsub, (int32_t)nrhs, (const float*)full, (float*)sub);
+      else
+        throw Error(IGA_ERR_INVALID, "unknown dtype");
+    } else {
+      if (dtype == IGA_F64)
+        box_gather_kernel<double, int64_t><<<grid_for(nsub * nrhs, 256), 256, 0, s>>>(b, nsub, nrhs,
+                                                                                       (const double*)full,
+                                                                                       (double*)sub);
+      else if (dtype == IGA_F32)
+        box_gather_kernel<float, int64_t><<<grid_for(nsub * nrhs, 256), 256, 0, s>>>(b, nsub, nrhs,
+                                                                                      (const float*)full,
+                                                                                      (float*)sub);
+      else
+        throw Error(IGA_ERR_INVALID, "unknown dtype");
+    }
     IGA_LAUNCH_CHECK();
   });
 }
@@ -307,24 +345,52 @@ int iga_box_scatter(int dtype, int ndim, const int64_t* n, const int64_t* lo, co
     int64_t nsub, nfull;
     Box b = make_box(ndim, n, lo, len, &nsub, &nfull);
     hipStream_t s = as_stream(stream);
-    if (zero_outside && nfull > 0) {
-      dim3 g(grid_for(nfull * nrhs, 256)), blk(256);
-      if (dtype == IGA_F64)
-        box_zero_outside_kernel<double><<<g, blk, 0, s>>>(b, nfull, nrhs, (double*)full);
-      else if (dtype == IGA_F32)
-        box_zero_outside_kernel<float><<<g, blk, 0, s>>>(b, nfull, nrhs, (float*)full);
-      else
-        throw Error(IGA_ERR_INVALID, "unknown dtype");
+    const int64_t count = nfull * nrhs;
+    const bool small = count < ((int64_t)1 << 30);
+    if (zero_outside) {
+      if (nfull == 0) return;
+      if (small) {
+        if (dtype == IGA_F64)
+          box_scatter_full_kernel<double, int32_t><<<grid_for(count, 256), 256, 0, s>>>(
+              b, (int32_t)nfull, (int32_t)nrhs, (const double*)sub, (double*)full);
+        else if (dtype == IGA_F32)
+          box_scatter_full_kernel<float, int32_t><<<grid_for(count, 256), 256, 0, s>>>(
+              b, (int32_t)nfull, (int32_t)nrhs, (const float*)sub, (float*)full);
+        else
+          throw Error(IGA_ERR_INVALID, "unknown dtype");
+      } else {
+        if (dtype == IGA_F64)
+          box_scatter_full_kernel<double, int64_t><<<grid_for(count, 256), 256, 0, s>>>(
+              b, nfull, nrhs, (const double*)sub, (double*)full);
+        else if (dtype == IGA_F32)
+          box_scatter_full_kernel<float, int64_t><<<grid_for(count, 256), 256, 0, s>>>(
+              b, nfull, nrhs, (const float*)sub, (float*)full);
+        else
+          throw Error(IGA_ERR_INVALID, "unknown dtype");
+      }
       IGA_LAUNCH_CHECK();
+      return;
     }
     if (nsub == 0) return;
-    dim3 grid(grid_for(nsub * nrhs, 256)), block(256);
-    if (dtype == IGA_F64)
-      box_scatter_kernel<double><<<grid, block, 0, s>>>(b, nsub, nrhs, (const double*)sub, (double*)full);
-    else if (dtype == IGA_F32)
-      box_scatter_kernel<float><<<grid, block, 0, s>>>(b, nsub, nrhs, (const float*)sub, (float*)full);
-    else
-      throw Error(IGA_ERR_INVALID, "unknown dtype");
+    if (small) {
+      if (dtype == IGA_F64)
+        box_scatter_kernel<double, int32_t><<<grid_for(nsub * nrhs, 256), 256, 0, s>>>(
+            b, (int32_t)nsub, (int32_t)nrhs, (const double*)sub, (double*)full);
+      else if (dtype == IGA_F32)
+        box_scatter_kernel<float, int32_t><<<grid_for(nsub * nrhs, 256), 256, 0, s>>>(
+            b, (int32_t)nsub, (int32_t)nrhs, (const float*)sub, (float*)full);
+      else
+        throw Error(IGA_ERR_INVALID, "unknown dtype");
+    } else {
+      if (dtype == IGA_F64)
+        box_scatter_kernel<double, int64_t><<<grid_for(nsub * nrhs, 256), 256, 0, s>>>(
+            b, nsub, nrhs, (const double*)sub, (double*)full);
+      else if (dtype == IGA_F32)
+        box_scatter_kernel<float, int64_t><<<grid_for(nsub * nrhs, 256), 256, 0, s>>>(
+            b, nsub, nrhs, (const float*)sub, (float*)full);
+      else
+        throw Error(IGA_ERR_INVALID, "unknown dtype");
+    }
     IGA_LAUNCH_CHECK();
   });
 }
@@ -336,13 +402,25 @@ int iga_box_mask(int dtype, int ndim, const int64_t* n, const int64_t* lo, const
     int64_t nsub, nfull;
     Box b = make_box(ndim, n, lo, len, &nsub, &nfull);
     if (nfull == 0) return;
-    dim3 g(grid_for(nfull * nrhs, 256)), blk(256);
-    if (dtype == IGA_F64)
-      box_zero_outside_kernel<double><<<g, blk, 0, as_stream(stream)>>>(b, nfull, nrhs, (double*)full);
-    else if (dtype == IGA_F32)
-      box_zero_outside_kernel<float><<<g, blk, 0, as_stream(stream)>>>(b, nfull, nrhs, (float*)full);
-    else
-      throw Error(IGA_ERR_INVALID, "unknown dtype");
+    hipStream_t s = as_stream(stream);
+    const int64_t count = nfull * nrhs;
+    if (count < ((int64_t)1 << 30)) {
+      if (dtype == IGA_F64)
+        box_zero_outside_kernel<double, int32_t><<<grid_for(count, 256), 256, 0, s>>>(b, (int32_t)nfull,
+                                                                                       (int32_t)nrhs, (double*)full);
+      else if (dtype == IGA_F32)
+        box_zero_outside_kernel<float, int32_t><<<grid_for(count, 256), 256, 0, s>>>(b, (int32_t)nfull,
+                                                                                      (int32_t)nrhs, (float*)full);
+      else
+        throw Error(IGA_ERR_INVALID, "unknown dtype");
+    } else {
+      if (dtype == IGA_F64)
+        box_zero_outside_kernel<double, int64_t><<<grid_for(count, 256), 256, 0, s>>>(b, nfull, nrhs, (double*)full);
+      else if (dtype == IGA_F32)
+        box_zero_outside_kernel<float, int64_t><<<grid_for(count, 256), 256, 0, s>>>(b, nfull, nrhs, (float*)full);
+      else
+        throw Error(IGA_ERR_INVALID, "unknown dtype");
+    }
     IGA_LAUNCH_CHECK();
   });
 }
