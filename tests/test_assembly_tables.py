@@ -66,3 +66,40 @@ def test_oracle_sumfact_terms_reproduces_mass_and_stiffness():
                                rtol=1e-13, atol=1e-15)
     np.testing.assert_allclose(asm.sumfact_terms(qs, [([0] * 3, [0] * 3, m, 1.0)]).toarray(),
                                asm.bspline_mass(qs, m).toarray(), rtol=1e-13, atol=1e-15)
+
+
+def test_spacetime_gradu_gradv_term_list_decomposes_into_pinned_operators():
+    """The four-direction term list of the space-time ``assemble_scalar_gradu_gradv``
+    (``bsplines_ops.py:161-191``, time_ders (0, 0)) splits, for rules whose test
+    weights ignore the trial derivative (WQ type "2": weights[1] == weights[0],
+    weights[3] == weights[2] in every direction), into operators the oracle states
+    independently: the spatial stiffness with C[:3, :3], ``gradu_v`` (test gradient)
+    with C[i][3], ``u_gradv`` (trial gradient) with C[3][j] and the mass with C[3][3].
+    A swapped C index or a dropped time term changes the product."""
+    from scipy import sparse as sp
+
+    from oracle import assembly as asm
+    from oracle import operators as oo
+
+    qs = [WeightedQuadrature(2, make_open_knotvector(2, 2), "2")] * 3 + \
+        [WeightedQuadrature(1, make_open_knotvector(1, 3), "2")]
+    for q in qs:
+        for a, b in ((1, 0), (3, 2)):
+            assert abs(sp.csr_array(q.weights[a]) - sp.csr_array(q.weights[b])).sum() == 0
+    nq = int(np.prod([q.nbquadpts for q in qs]))
+    rng = np.random.default_rng(21)
+    C = rng.uniform(0.5, 1.5, (4, 4, nq))
+    td = (0, 0)
+    terms = [(t.weights, t.basis, np.asarray(t.coef), t.scale) for t in asm_terms_gradu_gradv(qs, C, td)]
+    K = asm.sumfact_terms(qs, terms)
+    v = rng.standard_normal(K.shape[1])
+    ref = (oo.bspline_gradu_gradv(qs, C[:3, :3], v, time_ders=td)
+           + oo.bspline_gradu_v(qs, C[:3, 3], v, time_ders=td)
+           + oo.bspline_u_gradv(qs, C[3, :3], v, time_ders=td)
+           + oo.bspline_u_v(qs, C[3, 3], v, time_ders=td))
+    np.testing.assert_allclose(K @ v, ref, rtol=1e-11, atol=1e-13 * np.abs(ref).max())
+    # the decomposition is not vacuous: the C[3][j] and C[i][3] blocks are not symmetric
+    Ct = C.copy()
+    Ct[3, :3], Ct[:3, 3] = C[:3, 3], C[3, :3]
+    terms_t = [(t.weights, t.basis, np.asarray(t.coef), t.scale) for t in asm_terms_gradu_gradv(qs, Ct, td)]
+    assert np.abs(asm.sumfact_terms(qs, terms_t) @ v - ref).max() > 1e-6 * np.abs(ref).max()

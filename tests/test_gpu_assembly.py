@@ -103,7 +103,8 @@ def test_nurbs_assembly_matches_rational_basis_and_matrix_free(kind):
 def test_spacetime_assembly_four_directions():
     """Space x time (4-D): the mass with time_ders (0, 1) against the oracle mass;
     gradu_gradv with time_ders (0, 0) against the Kronecker restatement of the
-    reference's term list (all four directions, bsplines_ops.py:160-191)."""
+    reference's term list (all four directions, bsplines_ops.py:161-191) and against
+    the oracle operators that list decomposes into."""
     from igaops_amd.assembly import asm_terms_gradu_gradv
     from igaops_amd.operators import BsplineOperations
 
@@ -120,6 +121,12 @@ def test_spacetime_assembly_four_directions():
     K = BsplineOperations.assemble_scalar_gradu_gradv(qs, C, time_ders=(0, 0))
     terms = [(t.weights, t.basis, np.asarray(t.coef), t.scale) for t in asm_terms_gradu_gradv(qs, C, (0, 0))]
     _close(K, asm.sumfact_terms(qs, terms))
+    # and against the pinned operators it decomposes into (WQ type 2, time_ders (0, 0))
+    v = rng.standard_normal(K.shape[1])
+    td = (0, 0)
+    ref = (oo.bspline_gradu_gradv(qs, C[:3, :3], v, time_ders=td) + oo.bspline_gradu_v(qs, C[:3, 3], v, time_ders=td)
+           + oo.bspline_u_gradv(qs, C[3, :3], v, time_ders=td) + oo.bspline_u_v(qs, C[3, 3], v, time_ders=td))
+    np.testing.assert_allclose(K @ v, ref, rtol=1e-11, atol=1e-13 * np.abs(ref).max())
 
 
 def test_load_vectors():
