@@ -74,16 +74,19 @@ def cpu_baseline(patch, C, m, x, budget_s=20.0):
     """The oracle's matvec (the reference algorithm in numpy/scipy) on the host."""
     from oracle import operators as oo
 
+    from threadpoolctl import threadpool_limits
+
     w = patch.nurbs_weights if patch.is_nurbs else None
-    t0 = time.perf_counter()
-    y = oo.stiffness_plus_mass(patch.quadrule_list, C, m, x, nurbs_weights=w)
-    first = time.perf_counter() - t0
-    reps = max(1, min(5, int(budget_s / max(first, 1e-3)) - 1))
-    times = []
-    for _ in range(reps):
+    with threadpool_limits(limits=1):  # one core, as reported
         t0 = time.perf_counter()
-        oo.stiffness_plus_mass(patch.quadrule_list, C, m, x, nurbs_weights=w)
-        times.append(time.perf_counter() - t0)
+        y = oo.stiffness_plus_mass(patch.quadrule_list, C, m, x, nurbs_weights=w)
+        first = time.perf_counter() - t0
+        reps = max(1, min(5, int(budget_s / max(first, 1e-3)) - 1))
+        times = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            oo.stiffness_plus_mass(patch.quadrule_list, C, m, x, nurbs_weights=w)
+            times.append(time.perf_counter() - t0)
     return y, float(np.median(times)), reps
 
 
@@ -118,9 +121,16 @@ def fd_error_vs_oracle(patch, dtype, x, reps=20):
     t_dev = e0.elapsed_time(e1) / reps * 1e-3
     ref_fd = ofd.SpaceFD([StandardGauss(q.degree, q.knotvector, "legendre") for q in qs], table)
     ref_fd.scalar_coefs = (1.0, 1.0)
-    t0 = time.perf_counter()
-    ref = ref_fd.apply(x)
-    t_cpu = time.perf_counter() - t0
+    from threadpoolctl import threadpool_limits
+
+    with threadpool_limits(limits=1):  # one core, as reported; median of 3 warm calls
+        ref = ref_fd.apply(x)
+        times = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            ref_fd.apply(x)
+            times.append(time.perf_counter() - t0)
+    t_cpu = float(np.median(times))
     lens = [len(ix) for ix in ref_fd.indices]
     nsub = int(np.prod(lens))
     flops = 4.0 * nsub * sum(lens)
@@ -132,7 +142,7 @@ def fd_error_vs_oracle(patch, dtype, x, reps=20):
         "mfma": {"achieved": flops / t_dev / 1e12, "peak": peak, "unit": "TFLOP/s",
                  "frac": flops / t_dev / 1e12 / peak, "flops": flops},
         "cpu_baseline": {"value": t_cpu * 1e6, "unit": "us per apply", "cores": 1, "kind": "port",
-                         "sample": "the oracle's FD apply (numpy / scipy dgemm) on the same vector"},
+                         "sample": "the oracle's FD apply (numpy / scipy dgemm, one BLAS thread) on the same vector, median of 3 after one warm-up"},
     }
 
 
@@ -548,7 +558,8 @@ def main():
             "unit": "GDOF/s",
             "cores": 1,
             "kind": "port",
-            "sample": f"one right-hand side of the same workload, oracle matvec (numpy/scipy), median of {reps}",
+            "sample": f"one right-hand side of the same workload, oracle matvec (numpy/scipy, one BLAS thread), "
+                      f"median of {reps} after one warm-up call",
             "seconds_per_matvec": t_cpu,
         }
         fd_info = fd_error_vs_oracle(patch, args.dtype, x0)
