@@ -504,11 +504,12 @@ __global__ void __launch_bounds__(kThreads) fused_apply_kernel_wide(FusedArgs<T>
 template <typename T>
 struct GatherArgs {
   int nc[3], nt1, nt2, nslab, nrhs;
-  const int32_t *l1_cnt, *l1_t, *l1_off;  // [nc1], [nc1][kKMax] x2
+  // [nc_d], [nc_d][kKMax] x2; in directions 1 and 2 a tile slot holds
+  // t | (DOF count of tile t) << 16, in direction 3 the slab index
+  const int32_t *l1_cnt, *l1_t, *l1_off;
   const int32_t *l2_cnt, *l2_t, *l2_off;
   const int32_t *l3_cnt, *l3_t, *l3_off;
-  const int32_t *t1_nc, *t2_nc;
-  const int64_t* part_base;
+  int64_t part_blk;     // elements of every (tile, slab) partial block: block b starts at b * part_blk
   int64_t part_stride;
   const T* partial;
   const T* w;
@@ -543,17 +544,14 @@ __global__ void __launch_bounds__(256) fused_gather_kernel(GatherArgs<T> g) {
       const int a1 = g.l1_t[c1 * kKMax + i], b1 = g.l1_off[c1 * kKMax + i];
       const int a2 = g.l2_t[c2 * kKMax + i], b2 = g.l2_off[c2 * kKMax + i];
       const int a3 = g.l3_t[c3 * kKMax + i], b3 = g.l3_off[c3 * kKMax + i];
-      t1v[i] = i < n1 ? a1 : 0;
+      t1v[i] = i < n1 ? (a1 & 0xffff) : 0;
+      D1v[i] = i < n1 ? (a1 >> 16) : 0;
       o1v[i] = i < n1 ? b1 : 0;
-      t2v[i] = i < n2 ? a2 : 0;
+      t2v[i] = i < n2 ? (a2 & 0xffff) : 0;
+      D2v[i] = i < n2 ? (a2 >> 16) : 0;
       o2v[i] = i < n2 ? b2 : 0;
       sv[i] = i < n3 ? a3 : 0;
       o3v[i] = i < n3 ? b3 : 0;
-    }
-#pragma unroll
-    for (int i = 0; i < kKMax; ++i) {
-      D1v[i] = g.t1_nc[t1v[i]];
-      D2v[i] = g.t2_nc[t2v[i]];
     }
     T acc = T(0);
     if (n1 <= 2 && n2 <= 2 && n3 <= 2) {
@@ -563,7 +561,7 @@ __global__ void __launch_bounds__(256) fused_gather_kernel(GatherArgs<T> g) {
 #pragma unroll
       for (int c = 0; c < 8; ++c) {
         const int i1 = c & 1, i2 = (c >> 1) & 1, i3 = c >> 2;
-        const int64_t base = g.part_base[((int64_t)t2v[i2] * g.nt1 + t1v[i1]) * g.nslab + sv[i3]];
+        const int64_t base = (((int64_t)t2v[i2] * g.nt1 + t1v[i1]) * g.nslab + sv[i3]) * g.part_blk;
         addr[c] = base + ((int64_t)o3v[i3] * D2v[i2] + o2v[i2]) * D1v[i1] + o1v[i1];
       }
       T val[8];
@@ -579,7 +577,7 @@ __global__ void __launch_bounds__(256) fused_gather_kernel(GatherArgs<T> g) {
       for (int i3 = 0; i3 < n3; ++i3) {
         for (int i2 = 0; i2 < n2; ++i2) {
           for (int i1 = 0; i1 < n1; ++i1) {
-            const int64_t base = g.part_base[((int64_t)t2v[i2] * g.nt1 + t1v[i1]) * g.nslab + sv[i3]];
+            const int64_t base = (((int64_t)t2v[i2] * g.nt1 + t1v[i1]) * g.nslab + sv[i3]) * g.part_blk;
             acc += pk[base + ((int64_t)o3v[i3] * D2v[i2] + o2v[i2]) * D1v[i1] + o1v[i1]];
           }
         }
@@ -688,7 +686,7 @@ struct iga_fused_s {
   int dtype = IGA_F64, form = FORM_KM7, P = 0;
   int nc[3] = {0, 0, 0}, nq[3] = {0, 0, 0};
   int nt1 = 0, nt2 = 0, nslab = 0;
-  int64_t coef_elems = 0, part_stride = 0;
+  int64_t coef_elems = 0, part_stride = 0, part_blk = 0;
   // device tables
   int32_t* d_first[3] = {nullptr, nullptr, nullptr};
   void* d_tab[3] = {nullptr, nullptr, nullptr};  // T[4][nq][P]
@@ -908,9 +906,7 @@ static void run_apply(const iga_fused_s* f, const void* coef, const void* x, voi
   g.l1_cnt = f->d_list[0][0]; g.l1_t = f->d_list[0][1]; g.l1_off = f->d_list[0][2];
   g.l2_cnt = f->d_list[1][0]; g.l2_t = f->d_list[1][1]; g.l2_off = f->d_list[1][2];
   g.l3_cnt = f->d_list[2][0]; g.l3_t = f->d_list[2][1]; g.l3_off = f->d_list[2][2];
-  g.t1_nc = f->d_t1_nc;
-  g.t2_nc = f->d_t2_nc;
-  g.part_base = f->d_part_base;
+  g.part_blk = f->part_blk;
   g.part_stride = f->part_stride;
   g.partial = (const T*)work;
   g.w = (const T*)w;
@@ -1013,20 +1009,26 @@ int iga_fused_create(iga_fused_t* out, const int32_t* nc, const int32_t* nq, int
       f->nslab = (int)d3.c0.size();
       // packed coefficient offsets per tile
       const int ncoef = form_ncoef(form);
+      // partial blocks all get the size of the largest one, so that the gather
+      // computes a block's start instead of loading it (one dependent load less)
+      const int64_t blk = (int64_t)*std::max_element(d1.nc.begin(), d1.nc.end()) *
+                          *std::max_element(d2.nc.begin(), d2.nc.end()) *
+                          *std::max_element(d3.nc.begin(), d3.nc.end());
       std::vector<int64_t> coef_base(f->nt1 * f->nt2), part_base((size_t)f->nt1 * f->nt2 * f->nslab);
-      int64_t cb = 0, pb = 0;
+      int64_t cb = 0;
       for (int t2 = 0; t2 < f->nt2; ++t2)
         for (int t1 = 0; t1 < f->nt1; ++t1) {
           const int64_t npts = (int64_t)(d1.q0[t1 + 1] - d1.q0[t1]) * (d2.q0[t2 + 1] - d2.q0[t2]);
           coef_base[t2 * f->nt1 + t1] = cb;
           cb += npts * ncoef * nq[2];
           for (int sl = 0; sl < f->nslab; ++sl) {
-            part_base[((size_t)t2 * f->nt1 + t1) * f->nslab + sl] = pb;
-            pb += (int64_t)d1.nc[t1] * d2.nc[t2] * d3.nc[sl];
+            const size_t b = ((size_t)t2 * f->nt1 + t1) * f->nslab + sl;
+            part_base[b] = (int64_t)b * blk;
           }
         }
       f->coef_elems = cb;
-      f->part_stride = pb;
+      f->part_blk = blk;
+      f->part_stride = (int64_t)part_base.size() * blk;
       // band tables in the apply precision
       Staging st;
       for (int d = 0; d < 3; ++d) {
@@ -1048,7 +1050,16 @@ int iga_fused_create(iga_fused_t* out, const int32_t* nc, const int32_t* nq, int
       const DirTiling* dt[3] = {&d1, &d2, &d3};
       for (int d = 0; d < 3; ++d) {
         st.put(&f->d_list[d][0], dt[d]->cnt);
-        st.put(&f->d_list[d][1], dt[d]->t);
+        std::vector<int32_t> tt = dt[d]->t;
+        if (d < 2) {
+          // tile index and that tile's DOF count in one word
+          for (int32_t& v : tt) {
+            if (v < 0 || v > 0xffff || dt[d]->nc[v] > 0x7fff)
+              throw Error(IGA_ERR_UNSUPPORTED, "too many tiles for the gather tables");
+            v |= dt[d]->nc[v] << 16;
+          }
+        }
+        st.put(&f->d_list[d][1], tt);
         st.put(&f->d_list[d][2], dt[d]->off);
       }
       st.commit(f);
