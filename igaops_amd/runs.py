@@ -74,8 +74,12 @@ class GmresRun:
         with self._on_stream():
             self.b = b
             # the basis grows by doubling as steps are queued (most solves stop far below
-            # maxiters, and the allocator keeps what a solve has held); step k writes V[k]
-            self.V = t.empty((min(self.m, 64) + 2, n), dtype=tdt, device=b.device)
+            # maxiters, and the allocator keeps what a solve has held); step k writes V[k].
+            # Rows start on 16-byte boundaries (leading dimension padded to 16 bytes) so
+            # that the reductions over the basis take their vector path for every n.
+            per16 = 16 // t.empty(0, dtype=tdt).element_size()
+            self.ldv = -(-n // per16) * per16
+            self.V = t.empty((min(self.m, 64) + 2, self.ldv), dtype=tdt, device=b.device)
             self.z = t.empty(n, dtype=tdt, device=b.device)
             self.w = t.empty(n, dtype=tdt, device=b.device)
             self.x = t.zeros(n, dtype=tdt, device=b.device)
@@ -143,7 +147,7 @@ class GmresRun:
                 r.sub_(tmp)
                 _lib.call("iga_dot_many", dt, self.n, 1, dv.ptr(self.b), self.n, dv.ptr(self.b), dv.ptr(self.nb),
                           dv.ptr(self.work), self._sp())
-            _lib.call("iga_gmres_start", dt, self.n, self.m, dv.ptr(r), dv.ptr(self.V[0]), dv.ptr(self.ls),
+            _lib.call("iga_gmres_start", dt, self.n, self.m, dv.ptr(r), dv.ptr(self.row(0)), dv.ptr(self.ls),
                       dv.ptr(self.work), self._sp())
             if self.x0 is not None:
                 self.hist_host[2 * (self.m + 1):2 * (self.m + 1) + 1].copy_(self.nb, non_blocking=True)
@@ -154,6 +158,10 @@ class GmresRun:
         # step k reports dimension k - 1: one more step in flight than the lookahead
         return self.kfinal is None and self.launched <= self.m and self.launched - self.known <= self.lookahead
 
+    def row(self, i: int):
+        """Basis vector i (a length-n view of row i)."""
+        return self.V[i, : self.n]
+
     def _ensure_rows(self, k: int):
         """Rows V[0..k] for step k."""
         rows = self.V.shape[0]
@@ -162,7 +170,7 @@ class GmresRun:
         t = dv.torch()
         while rows <= k:
             rows = min(self.m + 2, 2 * rows)
-        V = t.empty((rows, self.n), dtype=self.V.dtype, device=self.V.device)
+        V = t.empty((rows, self.ldv), dtype=self.V.dtype, device=self.V.device)
         V[:self.V.shape[0]].copy_(self.V)
         self.V = V
 
@@ -173,7 +181,7 @@ class GmresRun:
         k = self.launched + 1
         with self._on_stream():
             self._ensure_rows(k)
-        return k, self.V[k - 1], self.z
+        return k, self.row(k - 1), self.z
 
     def launch(self, preconditioned: bool = False):
         """Queue Arnoldi step k = launched + 1 (with ``preconditioned``, ``z`` already
@@ -186,15 +194,15 @@ class GmresRun:
             if preconditioned:
                 pass
             elif self.P is not None:
-                self.P(self.V[k - 1], z)
+                self.P(self.row(k - 1), z)
             else:
-                z.copy_(self.V[k - 1])
+                z.copy_(self.row(k - 1))
             if self.mask:
                 self.mask(z)
             self.A(z, self.w)
             if self.mask:
                 self.mask(self.w)
-            _lib.call("iga_gmres_step", dt, self.n, k, self.m, dv.ptr(self.V), self.n, dv.ptr(self.w),
+            _lib.call("iga_gmres_step", dt, self.n, k, self.m, dv.ptr(self.V), self.ldv, dv.ptr(self.w),
                       dv.ptr(self.ls), dv.ptr(self.work), self._sp())
             if k >= 2:
                 self._record(k - 1, self._hist_off + 2 * (k - 1), 2 * (k - 1), 2)
@@ -277,7 +285,7 @@ class GmresRun:
                 self._y_dev = yh.to(self.x.device, non_blocking=True)
                 v = self.w
                 v.zero_()
-                _lib.call("iga_axpy_many", dv.DTYPES[self.dtype], self.n, k, dv.ptr(self.V), self.n,
+                _lib.call("iga_axpy_many", dv.DTYPES[self.dtype], self.n, k, dv.ptr(self.V), self.ldv,
                           dv.ptr(self._y_dev), 1.0, dv.ptr(v), self._sp())
                 z = self.z
                 if self.P is not None:
