@@ -209,7 +209,8 @@ static void dot_many_launch(int dtype, int64_t n, int k, const void* V, int64_t 
   const dim3 grid((unsigned)(((nb + 7) / 8) * 8 * groups));
   auto aligned = [](const void* p) { return p == nullptr || ((uintptr_t)p & 15) == 0; };
   const int width = dtype == IGA_F32 ? 4 : 2;
-  const bool vec = ldv % width == 0 && aligned(V) && aligned(x0) && aligned(x1);
+  // with one row the leading dimension is never used
+  const bool vec = (k == 1 || ldv % width == 0) && aligned(V) && aligned(x0) && aligned(x1);
 #define IGA_DOT1(T, VEC, NX) \
   dot_many_stage1<T, VEC, NX><<<grid, kRedThreads, 0, s>>>(n, k, nb, groups, (const T*)V, ldv, (const T*)x0, \
                                                            (const T*)x1, partial)

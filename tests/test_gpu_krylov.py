@@ -19,11 +19,13 @@ def _setup():
 
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
 @pytest.mark.parametrize("n,k,pad,woff", [(0, 3, 2, 0), (1, 1, 0, 0), (3, 17, 0, 0), (1001, 5, 1, 1),
+                                          (1001, 1, 0, 0), (70001, 1, 2, 0), (4099, 1, 0, 1),
                                           (4099, 16, 1, 0), (70001, 64, 3, 0), (262147, 33, 0, 0),
                                           (314432, 130, 0, 0)])
 def test_dot_many_matches_float64_sums(dtype, n, k, pad, woff):
     """Vector path (16-byte loads, with a scalar tail) where ldv and the pointers allow
-    it, scalar path otherwise (odd ldv, w at an odd element offset)."""
+    it (any ldv for a single row), scalar path otherwise (odd ldv with several rows,
+    w at an odd element offset)."""
     torch, _lib, dv = _setup()
     print("case", dtype, n, k, pad, woff, flush=True)
     tdt = torch.float64 if dtype == "f64" else torch.float32
