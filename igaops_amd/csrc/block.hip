@@ -147,6 +147,68 @@ __global__ void __launch_bounds__(kGramThreads) zgram_mfma_kernel(int64_t n, int
   }
 }
 
+// The same Gram for at most 2 x 2 rows (norms, a row against one or two others):
+// the matrix-core tile would have its 16 lanes per operand row read the same
+// addresses, so here each thread takes one complex element per row and step
+// (16-byte loads) and keeps NA x NB complex sums; waves reduce by shuffles, the
+// block through LDS, and the partial layout is the one of zgram_mfma_kernel.
+template <int NA, int NB>
+__global__ void __launch_bounds__(kGramThreads) zgram_small_kernel(int64_t n, int na, int nb, int nchunks,
+                                                                   const double* const* __restrict__ X,
+                                                                   const double* const* __restrict__ Y,
+                                                                   double* __restrict__ partial) {
+  constexpr int NV = 2 * NA * NB;
+  __shared__ double red[kGramThreads / 64][NV];
+  const int chunk = blockIdx.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const double2* xp[NA];
+  const double2* yp[NB];
+#pragma unroll
+  for (int a = 0; a < NA; ++a) xp[a] = reinterpret_cast<const double2*>(X[a < na ? a : na - 1]);
+#pragma unroll
+  for (int b = 0; b < NB; ++b) yp[b] = reinterpret_cast<const double2*>(Y[b < nb ? b : nb - 1]);
+  const int64_t per = (n + nchunks - 1) / nchunks;
+  const int64_t lo = (int64_t)chunk * per;
+  const int64_t hi = lo + per < n ? lo + per : n;
+  double acc[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) acc[q] = 0.0;
+  for (int64_t i = lo + threadIdx.x; i < hi; i += kGramThreads) {
+    double2 xv[NA], yv[NB];
+#pragma unroll
+    for (int a = 0; a < NA; ++a) xv[a] = xp[a][i];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) yv[b] = yp[b][i];
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        acc[(a * NB + b) * 2] += xv[a].x * yv[b].x + xv[a].y * yv[b].y;
+        acc[(a * NB + b) * 2 + 1] += xv[a].x * yv[b].y - xv[a].y * yv[b].x;
+      }
+  }
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    double v = acc[q];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    acc[q] = v;
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) red[wave][q] = acc[q];
+  }
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    const int q = threadIdx.x;
+    double v = 0.0;
+#pragma unroll
+    for (int w = 0; w < kGramThreads / 64; ++w) v += red[w][q];
+    const int a = q / (2 * NB), b = (q / 2) % NB, part = q & 1;
+    if (a < na && b < nb) partial[((int64_t)a * 2 * nb + 2 * b + part) * nchunks + chunk] = v;
+  }
+}
+
 // out[o] = sum over chunks of partial[o * nchunks + chunk], in chunk order: one wave per output
 __global__ void __launch_bounds__(256) zgram_finish_kernel(int nchunks, int nout, const double* __restrict__ partial,
                                                            double* __restrict__ out) {
@@ -230,6 +292,18 @@ int iga_zgram(int64_t n, int na, const double* const* X, int nb, const double* c
     const int nout = 2 * na * nb;
     if (n == 0) {
       zero_kernel<<<(nout + 255) / 256, 256, 0, s>>>(nout, out);
+      IGA_LAUNCH_CHECK();
+      return;
+    }
+    const int nchunks_small = gram_chunks(n);
+    if (na <= 2 && nb <= 2) {
+      double* partial = (double*)work;
+      if (na == 1 && nb == 1)
+        zgram_small_kernel<1, 1><<<nchunks_small, kGramThreads, 0, s>>>(n, na, nb, nchunks_small, X, Y, partial);
+      else
+        zgram_small_kernel<2, 2><<<nchunks_small, kGramThreads, 0, s>>>(n, na, nb, nchunks_small, X, Y, partial);
+      IGA_LAUNCH_CHECK();
+      zgram_finish_kernel<<<(nout + 3) / 4, 256, 0, s>>>(nchunks_small, nout, partial, out);
       IGA_LAUNCH_CHECK();
       return;
     }

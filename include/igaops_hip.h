@@ -113,8 +113,8 @@ int iga_zaxpy_many(int64_t n, int k, const double* V, int64_t ldv, const double*
  *   iga_zcombine: Out_c = beta Out_c + sum_j coef[j*nout + c] V_j   (1 <= nout <= 24)
  * Rows are complex128 (interleaved re, im), each starting on a 16-byte boundary.
  * iga_zgram runs on the FP64 matrix cores (X^H Y as a real product over the 2n
- * interleaved doubles) with a fixed-order second stage: repeat calls give the same
- * bits.  work for iga_zgram: iga_zgram_workspace_bytes(n, na, nb) bytes. */
+ * interleaved doubles), or for at most 2 x 2 rows one complex element per thread,
+ * with a fixed-order second stage: repeat calls give the same bits.  work for iga_zgram: iga_zgram_workspace_bytes(n, na, nb) bytes. */
 int64_t iga_zgram_workspace_bytes(int64_t n, int na, int nb);
 int iga_zgram(int64_t n, int na, const double* const* X, int nb, const double* const* Y, double* out,
               void* work, void* stream);

@@ -39,11 +39,13 @@ def _gram_raw(torch, X, Y, n):
 
 @pytest.mark.parametrize("n,na,nb", [(0, 2, 3), (1, 1, 1), (7, 3, 2), (8, 1, 1), (9, 16, 8), (1000, 20, 20),
                                      (12345, 20, 1), (3001, 17, 9), (4097, 33, 40), (20011, 64, 64),
-                                     (600000, 1, 1), (300001, 32, 24)])
+                                     (600000, 1, 1), (300001, 32, 24), (100003, 2, 2), (50001, 1, 2),
+                                     (50001, 2, 1), (3, 2, 2)])
 def test_zgram_matches_numpy(n, na, nb):
-    """Every tile shape (1..2 x 1..4 fragments of 16 x 16), ragged n (partial last
-    step of 8 elements), rows from several allocations, and repeat calls giving the
-    same bits."""
+    """Every tile shape of the matrix-core kernel (1..2 x 1..4 fragments of 16 x 16),
+    the element-per-thread kernel for at most 2 x 2 rows, ragged n (partial last step
+    of 8 elements), rows from several allocations, and repeat calls giving the same
+    bits."""
     import torch
 
     print("case", n, na, nb, flush=True)
