@@ -90,7 +90,6 @@ class GmresRun:
             _lib.check(lib.iga_gmres_layout(self.m, offs), "iga_gmres_layout")
             self._off_r, self._off_g, self._hist_off = int(offs[0]), int(offs[1]), int(offs[2])
             self.work = dv.workspace(lib.iga_reduce_workspace_bytes(n, self.m + 1))
-            self.nb = t.zeros(1, dtype=t.float64, device=b.device)
         # one pinned host buffer per run (or lent by the caller, at least
         # host_doubles(maxiters) values): polled residuals, then the least-squares
         # factors and the solution coefficients
@@ -141,16 +140,12 @@ class GmresRun:
             if self.x0 is not None:
                 self.x.copy_(self.x0)
                 tmp = t.empty_like(self.w)
-                self.A(self.x, tmp)
+                self.A(self.x.clone(), tmp)
                 if self.mask:
                     self.mask(tmp)
                 r.sub_(tmp)
-                _lib.call("iga_dot_many", dt, self.n, 1, dv.ptr(self.b), self.n, dv.ptr(self.b), dv.ptr(self.nb),
-                          dv.ptr(self.work), self._sp())
             _lib.call("iga_gmres_start", dt, self.n, self.m, dv.ptr(r), dv.ptr(self.row(0)), dv.ptr(self.ls),
                       dv.ptr(self.work), self._sp())
-            if self.x0 is not None:
-                self.hist_host[2 * (self.m + 1):2 * (self.m + 1) + 1].copy_(self.nb, non_blocking=True)
             self._record(0, self._hist_off, 0, 2)
         self.launched = 0
 
@@ -224,9 +219,10 @@ class GmresRun:
             progressed = True
             g, nw = float(self.hist_host[2 * j]), float(self.hist_host[2 * j + 1])
             if j == 0:
+                # relative to ||b - A x0|| (the reference shifts b by A x0 and starts
+                # from zero, linear_solver.py:228-236)
                 self.beta = abs(g)
-                nb = float(self.hist_host[2 * (self.m + 1)]) ** 0.5 if self.x0 is not None else self.beta
-                self.ref = nb if nb > 0 else self.beta
+                self.ref = self.beta
                 self.res_hist.append(1.0)
                 if self.beta < Constants.SAFEGUARD:
                     self.kfinal, self.reason = 0, "absolute"
@@ -295,9 +291,25 @@ class GmresRun:
                 if self.mask:
                     self.mask(z)
                 self.x.add_(z)
-        success = self.reason in ("relative", "absolute", "breakdown")
+        success = self.reason in ("relative", "absolute")
+        if self.reason == "breakdown":
+            # a vanishing basis vector means the Krylov space is invariant, which in exact
+            # arithmetic makes the least-squares solution exact; the true residual decides
+            success = self._true_residual_meets_tol()
         self.result = OutputArgs(self.x, np.asarray(self.res_hist), success, int(k or 0), self.reason)
         return self.result
+
+    def _true_residual_meets_tol(self) -> bool:
+        """``||b - A x|| <= tol * ||b - A x0||`` or below the absolute floor (synchronising)."""
+        t = dv.torch()
+        with self._on_stream():
+            r = t.empty_like(self.w)
+            self.A(self.x.clone(), r)
+            r.sub_(self.b)
+            if self.mask:
+                self.mask(r)
+            res = float(t.linalg.vector_norm(r.double()))
+        return res <= self.tol * self.ref or res < Constants.SAFEGUARD
 
     def release(self):
         """Drop the Krylov basis (the solution tensor is kept)."""

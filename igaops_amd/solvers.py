@@ -238,7 +238,8 @@ def gmres_device(
         kr.dots(v.unsqueeze(0), 1, v, dbuf[:1])
         return float(dbuf[:1].cpu()[0]) ** 0.5
 
-    norm_b = norm_dev(r) if x0 is None else norm_dev(b)
+    # the relative criterion is measured against ||b - A x0||, as the reference solves
+    # the shifted system b - A x0 from zero (linear_solver.py:228-236)
     beta = norm_dev(r)
     res_hist = [1.0]
     total_it = 0
@@ -246,7 +247,7 @@ def gmres_device(
     success = False
     if beta < abs_floor:
         return OutputArgs(x, np.array([1.0]), True, 0, "absolute")
-    ref = norm_b if norm_b > 0 else beta
+    ref = beta
     while True:
         V[0].copy_(r)
         kr.axpby(1.0 / beta, r, 0.0, V[0])
@@ -529,15 +530,19 @@ class LinearSolver:
             mask(bd)
         x0d = None
         if x0 is not None:
+            # linear_solver.py:228-236: solve A d = b - A x0 from zero, then x = x0 + d
             x0d = dv.to_device(x0, "f64").reshape(-1).clone()
             if mask:
                 mask(x0d)
+            ax0 = t.empty_like(bd)
+            A(x0d.clone(), ax0)
+            if mask:
+                mask(ax0)
+            bd.sub_(ax0)
         start = time.time()
         if self._linear_type == "gmres":
             out = gmres_device(A, bd, P, self._config.tolerance, self._config.maxiters, mask=mask,
-                               restart=self._restart, x0=x0d)
-            if x0d is not None:
-                out.solution = out.solution + x0d
+                               restart=self._restart)
         else:
             from .gcrodr import GCRODR
 
@@ -549,6 +554,8 @@ class LinearSolver:
             Am = (lambda x, out: (A(x, out), mask(out) if mask else None)[0]) if mask else A
             Pm = (lambda x, out: (P(x, out), mask(out) if mask else None)[0]) if mask else P
             out = self._gcrodr_solver.solve_device(Am, bd, Pm)
+        if x0d is not None:
+            out.solution = out.solution + x0d
         if numpy_mode:
             out.solution = out.solution.cpu().numpy().reshape(np.shape(b))
         self._last_simulation = out

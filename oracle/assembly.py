@@ -137,3 +137,21 @@ def nurbs_grad_force_gauss(quadrule_list, c, nurbs_weights) -> np.ndarray:
     om = gauss_point_weights(quadrule_list)
     nd = len(quadrule_list)
     return np.stack([sum(R[1 + i].T @ (om * ci[i]) for i in range(nd)) for ci in c])
+
+
+def nurbs_gradu_v_gauss(quadrule_list, c, nurbs_weights) -> sp.csr_array:
+    """``G_AB = sum_q omega_q sum_i d_i R_A c_i R_B`` (test gradient, trial value;
+    the integrand of ``nurbs_ops.py:392-429`` with ``mf_type='gradu_v'``), Gauss rules."""
+    R = rational_basis(quadrule_list, nurbs_weights)
+    om = gauss_point_weights(quadrule_list)
+    nd = len(quadrule_list)
+    return sp.csr_array(sum(R[1 + i].T @ sp.diags_array(om * c[i]) @ R[0] for i in range(nd)))
+
+
+def nurbs_u_gradv_gauss(quadrule_list, c, nurbs_weights) -> sp.csr_array:
+    """``G_AB = sum_q omega_q sum_j R_A c_j d_j R_B`` (test value, trial gradient),
+    Gauss rules."""
+    R = rational_basis(quadrule_list, nurbs_weights)
+    om = gauss_point_weights(quadrule_list)
+    nd = len(quadrule_list)
+    return sp.csr_array(sum(R[0].T @ sp.diags_array(om * c[j]) @ R[1 + j] for j in range(nd)))

@@ -15,8 +15,14 @@ the quadrature points).  Index conventions kept from the reference:
 Pinned by (tests/test_oracle_operators.py): explicit sparse assembly of the same
 integrals (``oracle.assembly``), constant-coefficient Kronecker sums on the identity
 geometry (Gauss-exact), ``M 1 = int N_A``, ``K 1 = 0``, symmetry of K and M, NURBS
-with unit weights == B-spline, the quarter-annulus volume ``1^T M 1 = 3 pi / 4``, and
-the NURBS chain rule checked against a finite-dimensional brute force on a tiny patch.
+with unit weights == B-spline (all four operators), the quarter-annulus volume
+``1^T M 1 = 3 pi / 4``, all four NURBS operators against direct assembly from the
+rational basis ``R_A = w_A N_A / W`` and its gradient on the Gauss annulus (with planted
+chain-rule errors that must fail), and the space-time operator
+``u_v(time_ders=(0,1)) + gradu_gradv(time_ders=(0,0))`` against the closed form
+``kron(W_t, M_s^3) + kron(M_t, K_s)`` from 1-D matrices built with
+``scipy.interpolate.BSpline`` (plus the WQ type "1" time-weight convention of
+``bsplines_ops.py:309-316``, where a wrong weight index fails).
 """
 from __future__ import annotations
 

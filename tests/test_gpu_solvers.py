@@ -294,3 +294,86 @@ def test_device_step_gmres_with_initial_guess_and_exact_preconditioner():
     exact = GmresRun(mv, lambda x, o: o.copy_(Ainv @ x), b, 1e-12, 50).run()
     assert exact.success and exact.iterations <= 2
     np.testing.assert_allclose(exact.solution.cpu().numpy(), ref, rtol=1e-9, atol=1e-11 * np.abs(ref).max())
+
+
+def _annulus_stiffness_gmres_problem():
+    from igaops_amd.fastdiag import SingleFD
+    from igaops_amd.operators import StiffnessMassOperator
+
+    patch, C, m = build_problem("annulus", 3, 6)
+    qs = patch.quadrule_list
+    w = patch.nurbs_weights
+    n = patch.nbctrlpts_total
+    fd = SingleFD("spatial")
+    fd.compute_space_eigendecomposition(qs, np.ones((1, 3, 2), bool))
+    fd.update_space_eigenvalues([0.0, 1.0])
+    free = np.asarray(fd.space_preconditioner.space_free_nodes[0])
+    dod = np.setdiff1d(np.arange(n), free).tolist()
+    op = StiffnessMassOperator(qs, C=C, m=None, nurbs_weights=w)
+    gq = [StandardGauss(q.degree, q.knotvector, "legendre") for q in qs]
+    ofd_ = ofd.SpaceFD(gq, np.ones((1, 3, 2), bool))
+    oA = lambda v: oo.nurbs_gradu_gradv(qs, C, v, nurbs_weights=w)
+    return patch, op, fd, dod, oA, ofd_
+
+
+@pytest.mark.parametrize("guess,tol", [("near", 1e-10), ("near", 1e-6), ("random", 1e-10)])
+def test_linear_solver_warm_start_matches_oracle(guess, tol):
+    """``LinearSolver.solve(..., x0=x0)`` solves ``A d = b - A x0`` from zero, measures
+    the relative criterion against ``||b - A x0||`` and returns ``x0 + d``
+    (``linear_solver.py:228-236``): the oracle's iteration count within one and its
+    solution."""
+    from igaops_amd.solvers import LinearSolver
+
+    patch, op, fd, dod, oA, ofd_ = _annulus_stiffness_gmres_problem()
+    n = patch.nbctrlpts_total
+    rng = np.random.default_rng(3)
+    b = rng.standard_normal(n)
+    exact = osv.gmres(oA, b, ofd_.apply, tol=1e-12, maxiters=200, cleandod=dod).solution
+    x0 = exact + 1e-3 * rng.standard_normal(n) if guess == "near" else rng.standard_normal(n)
+    x0_in = x0.copy()
+    out = LinearSolver(tol, 200, cleandod=dod).solve(op, b, Pfun=fd, x0=x0)
+    ref = osv.gmres(oA, b, ofd_.apply, tol=tol, maxiters=200, cleandod=dod, x0=x0)
+    np.testing.assert_array_equal(x0, x0_in)  # x0 is copied, not masked in place
+    assert out.success and ref.success
+    assert abs(out.iterations - ref.iterations) <= 1
+    assert rel(out.solution, ref.solution) <= (1e-8 if tol <= 1e-10 else 1e-4)
+    # the true residual reduction is measured from the initial residual
+    r0 = b.copy()
+    r0[dod] = 0
+    ax0 = oA(np.where(np.isin(np.arange(n), dod), 0.0, x0))
+    ax0[dod] = 0
+    r0 -= ax0
+    Ax = oA(out.solution)
+    Ax[dod] = 0
+    r = b.copy()
+    r[dod] = 0
+    assert np.linalg.norm(r - Ax) <= 10 * tol * np.linalg.norm(r0)
+
+
+def test_device_step_gmres_warm_start_matches_oracle():
+    """``GmresRun(..., x0=x0)`` on the annulus: the oracle's iterations within one
+    (the oracle stops on its relative criterion here) and its solution."""
+    import torch
+
+    from igaops_amd.runs import GmresRun
+    from igaops_amd.solvers import Mask
+
+    patch, op, fd, dod, oA, ofd_ = _annulus_stiffness_gmres_problem()
+    n = patch.nbctrlpts_total
+    rng = np.random.default_rng(3)
+    b = rng.standard_normal(n)
+    exact = osv.gmres(oA, b, ofd_.apply, tol=1e-12, maxiters=200, cleandod=dod).solution
+    x0 = exact + 1e-3 * rng.standard_normal(n)
+    x0[dod] = 0
+    bm = b.copy()
+    bm[dod] = 0
+    ref = osv.gmres(oA, b, ofd_.apply, tol=1e-10, maxiters=200, cleandod=dod, x0=x0)
+    assert ref.reason == "relative"
+    mask = Mask(box=fd.space_preconditioner.box())
+    A = lambda x, o: op.apply(x, out=o)
+    P = lambda x, o: fd.apply_device(x, out=o)
+    out = GmresRun(A, P, torch.as_tensor(bm, device="cuda"), 1e-10, 200, mask=mask,
+                   x0=torch.as_tensor(x0, device="cuda")).run()
+    assert out.success and out.reason == "relative"
+    assert abs(out.iterations - ref.iterations) <= 1
+    assert rel(out.solution.cpu().numpy(), ref.solution) <= 1e-8

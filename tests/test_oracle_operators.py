@@ -187,3 +187,156 @@ def test_planted_index_error_is_detected():
     assert not np.allclose(oo.bspline_gradu_gradv(qs, C2, v), ref, rtol=1e-8)
     Ct = np.transpose(C, (1, 0, 2)).copy()
     assert not np.allclose(oo.bspline_gradu_gradv(qs, Ct, v), ref, rtol=1e-8)
+
+
+@pytest.mark.parametrize("kind", ["gradu_v", "u_gradv"])
+def test_nurbs_first_order_against_rational_basis(kind):
+    """``nurbs_ops.py:392-463``: the NURBS first-order operators (built from B-spline
+    ones on ``w o x`` with ``c W~^2`` and ``-(c.z) W~``) == direct assembly of
+    ``int d_i R_A c_i R_B`` / ``int R_A c_j d_j R_B`` from the rational basis, Gauss
+    rules, annulus geometry, random non-constant coefficients."""
+    patch = quarter_annulus(3, 2, quadclass="gauss", quadtype="legendre")
+    qs = patch.quadrule_list
+    w = patch.nurbs_weights
+    nq = int(np.prod([q.nbquadpts for q in qs]))
+    rng = np.random.default_rng(15)
+    c = rng.standard_normal((3, nq))
+    v = rng.standard_normal(patch.nbctrlpts_total)
+    if kind == "gradu_v":
+        ref = asm.nurbs_gradu_v_gauss(qs, c, w) @ v
+        got = oo.nurbs_gradu_v(qs, c, v, nurbs_weights=w)
+    else:
+        ref = asm.nurbs_u_gradv_gauss(qs, c, w) @ v
+        got = oo.nurbs_u_gradv(qs, c, v, nurbs_weights=w)
+    np.testing.assert_allclose(got, ref, rtol=1e-10, atol=1e-12 * np.abs(ref).max())
+    # the two are transposes of each other for the same coefficient field
+    G = asm.nurbs_gradu_v_gauss(qs, c, w).toarray()
+    np.testing.assert_allclose(G.T, asm.nurbs_u_gradv_gauss(qs, c, w).toarray(), atol=1e-13 * np.abs(G).max())
+
+
+@pytest.mark.parametrize("kind", ["gradu_v", "u_gradv"])
+def test_nurbs_first_order_unit_weights_equal_bspline(kind):
+    q = WeightedQuadrature(3, make_open_knotvector(3, 4), "1")
+    qs = [q] * 3
+    rng = np.random.default_rng(16)
+    nq, nc = q.nbquadpts ** 3, q.nbctrlpts ** 3
+    c = rng.standard_normal((3, nq))
+    v = rng.standard_normal(nc)
+    f_n = oo.nurbs_gradu_v if kind == "gradu_v" else oo.nurbs_u_gradv
+    f_b = oo.bspline_gradu_v if kind == "gradu_v" else oo.bspline_u_gradv
+    ref = f_b(qs, c, v)
+    np.testing.assert_allclose(f_n(qs, c, v, nurbs_weights=np.ones(nc)), ref, rtol=1e-12,
+                               atol=1e-13 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("kind", ["gradu_v", "u_gradv"])
+def test_planted_nurbs_chain_rule_errors_are_detected(kind, monkeypatch):
+    """Mistakes planted in the oracle's weight projection (the nonzero component of
+    ``z = grad W / W^2`` with a flipped sign, or ``W~`` replaced by ``W~^2``) make the
+    NURBS first-order operators fail the rational-basis comparison."""
+    patch = quarter_annulus(3, 2, quadclass="gauss", quadtype="legendre")
+    qs = patch.quadrule_list
+    w = patch.nurbs_weights
+    nq = int(np.prod([q.nbquadpts for q in qs]))
+    rng = np.random.default_rng(17)
+    c = rng.standard_normal((3, nq))
+    v = rng.standard_normal(patch.nbctrlpts_total)
+    A = asm.nurbs_gradu_v_gauss(qs, c, w) if kind == "gradu_v" else asm.nurbs_u_gradv_gauss(qs, c, w)
+    ref = A @ v
+    f = oo.nurbs_gradu_v if kind == "gradu_v" else oo.nurbs_u_gradv
+    original = oo.project_nurbs_weights
+
+    def flipped_z(quadrule_list, nurbs_weights, nders=1):
+        winv, z = original(quadrule_list, nurbs_weights, nders)
+        if z is not None:
+            z = z.copy()
+            k = int(np.argmax(np.abs(z).reshape(z.shape[0], -1).max(axis=1)))  # the direction W varies in
+            z[k] = -z[k]
+        return winv, z
+
+    def squared_winv(quadrule_list, nurbs_weights, nders=1):
+        winv, z = original(quadrule_list, nurbs_weights, nders)
+        return winv ** 2, z
+
+    for planted in (flipped_z, squared_winv):
+        monkeypatch.setattr(oo, "project_nurbs_weights", planted)
+        got = f(qs, c, v, nurbs_weights=w)
+        assert np.linalg.norm(got - ref) > 1e-3 * np.linalg.norm(ref), planted.__name__
+    monkeypatch.setattr(oo, "project_nurbs_weights", original)
+    np.testing.assert_allclose(f(qs, c, v, nurbs_weights=w), ref, rtol=1e-10, atol=1e-12 * np.abs(ref).max())
+
+
+def _bspline_1d_matrices(p, nel):
+    """Exact 1-D mass, stiffness and advection ``int N_A N_B'`` of the open uniform
+    B-spline basis, from ``scipy.interpolate.BSpline`` and Legendre-Gauss with p+2
+    points per element (exact for the polynomial degrees involved); independent of
+    the quadrature-rule classes."""
+    from scipy.interpolate import BSpline
+
+    kv = np.asarray(make_open_knotvector(p, nel), dtype=float)
+    n = len(kv) - p - 1
+    spl = BSpline(kv, np.eye(n), p)
+    dspl = spl.derivative()
+    xg, wg = np.polynomial.legendre.leggauss(p + 2)
+    brk = np.unique(kv)
+    x = np.concatenate([0.5 * (b - a) * xg + 0.5 * (a + b) for a, b in zip(brk[:-1], brk[1:])])
+    om = np.concatenate([0.5 * (b - a) * wg for a, b in zip(brk[:-1], brk[1:])])
+    N, dN = spl(x), dspl(x)
+    return (N.T * om) @ N, (dN.T * om) @ dN, (N.T * om) @ dN
+
+
+def test_spacetime_operator_closed_form_gauss():
+    """``u_v(time_ders=(0,1), c=1) + gradu_gradv(time_ders=(0,0), C=I)`` on the cube
+    (``bsplines_ops.py:290-355``, ``examples/performance_matvec_sptm.py:44-57``) equals
+    ``kron(W_t, M_s (x) M_s (x) M_s) + kron(M_t, K_s)`` with W_t the time advection
+    ``int N_A N_B'`` and K_s the spatial Kronecker-sum stiffness (Gauss rules, exact)."""
+    ps, ns, pt, nt = 2, 3, 2, 4
+    qs = [StandardGauss(ps, make_open_knotvector(ps, ns), "legendre")] * 3
+    qt = StandardGauss(pt, make_open_knotvector(pt, nt), "legendre")
+    ql = qs + [qt]
+    nq = int(np.prod([q.nbquadpts for q in ql]))
+    Ms, Ks, _ = _bspline_1d_matrices(ps, ns)
+    Mt, _, Wt = _bspline_1d_matrices(pt, nt)
+
+    def k3(a, b, c):
+        return np.kron(c, np.kron(b, a))
+
+    Mss = k3(Ms, Ms, Ms)
+    Kss = k3(Ks, Ms, Ms) + k3(Ms, Ks, Ms) + k3(Ms, Ms, Ks)
+    A_ref = np.kron(Wt, Mss) + np.kron(Mt, Kss)
+    C = np.zeros((4, 4, nq))
+    for d in range(3):
+        C[d, d] = 1.0
+    v = np.random.default_rng(18).standard_normal(A_ref.shape[0])
+    got = oo.bspline_u_v(ql, np.ones(nq), v, time_ders=(0, 1)) + oo.bspline_gradu_gradv(ql, C, v, time_ders=(0, 0))
+    ref = A_ref @ v
+    np.testing.assert_allclose(got, ref, rtol=1e-11, atol=1e-12 * np.abs(ref).max())
+    # planted: the time derivative on the test side instead of the trial side
+    wrong = oo.bspline_u_v(ql, np.ones(nq), v, time_ders=(1, 0)) + oo.bspline_gradu_gradv(ql, C, v, time_ders=(0, 0))
+    assert np.linalg.norm(wrong - ref) > 1e-2 * np.linalg.norm(ref)
+
+
+def test_spacetime_mass_uses_time_weights_of_the_test_derivative_wq1():
+    """With a WQ type "1" time rule, ``u_v(time_ders=(0,1))`` tests with
+    ``weights[time_ders[0]] = W00`` (``bsplines_ops.py:309-316``), not with the zeta
+    rule's ``weights[1] = W01`` that the other operators use: it equals
+    ``kron(W00_t B1_t, M_s)`` built from explicit 1-D matrices, and differs from
+    ``kron(W01_t B1_t, M_s)`` (a wrong weight index fails)."""
+    ps, ns, pt, nt = 2, 3, 2, 5
+    qs = [WeightedQuadrature(ps, make_open_knotvector(ps, ns), "1")] * 3
+    qt = WeightedQuadrature(pt, make_open_knotvector(pt, nt), "1")
+    ql = qs + [qt]
+    nq = int(np.prod([q.nbquadpts for q in ql]))
+    m1 = (qs[0].weights[0] @ qs[0].basis[0]).toarray()
+    Mss = np.kron(m1, np.kron(m1, m1))
+    T00 = (qt.weights[0] @ qt.basis[1]).toarray()
+    T01 = (qt.weights[1] @ qt.basis[1]).toarray()
+    assert np.abs(T00 - T01).max() > 1e-6 * np.abs(T01).max()  # the two rules differ for type 1
+    v = np.random.default_rng(19).standard_normal(Mss.shape[0] * qt.nbctrlpts)
+    got = oo.bspline_u_v(ql, np.ones(nq), v, time_ders=(0, 1))
+    ref = np.kron(T00, Mss) @ v
+    np.testing.assert_allclose(got, ref, rtol=1e-11, atol=1e-12 * np.abs(ref).max())
+    assert np.linalg.norm(np.kron(T01, Mss) @ v - ref) > 1e-8 * np.linalg.norm(ref)
+    # both time factors approximate the exact advection (WQ type 1 exactness)
+    _, _, Wt = _bspline_1d_matrices(pt, nt)
+    np.testing.assert_allclose(T01, Wt, atol=1e-8 * np.abs(Wt).max())
