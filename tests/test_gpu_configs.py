@@ -1,0 +1,174 @@
+"""The benchmark configurations at their full sizes on the GPU.
+
+* Curved FP64 drop-in: quarter annulus, degree 4, 64 elements per direction (68^3 =
+  314,432 DOF, 199^3 WQ points), lognormal conductivity (sigma 0.5): the K + M matvec
+  and the 16-right-hand-side launch against the oracle, and FD-preconditioned GMRES
+  of the 16 right-hand sides to a true relative residual of 1e-10.
+* Mixed precision: the same geometry at degree 2, 4 and 6 with 128 elements per
+  direction (up to 134^3 = 2.4 M DOF): the float32 operator and float32 FD against
+  the float64 oracle, and one float64 iterative-refinement solve per degree to a
+  true 1e-10.
+* Space-time heat: cube degree 3 x 32 elements by time degree 2 x 64 elements
+  (2.83 M DOF): the arrowhead FD inverts the Gauss-rule operator, which the WQ
+  operator equals for unit diffusivity, so ``A P b = b`` on the free nodes; GCRODR
+  (krylov 40, recycled 20) over 10 perturbed right-hand sides to a true 1e-10.
+
+Each step prints what it runs (``pytest -s`` shows it).
+"""
+import numpy as np
+import pytest
+
+from igaops_amd.geometry import lognormal_conductivity, quarter_annulus
+from igaops_amd.quadrature import StandardGauss
+from oracle import fastdiag as ofd
+from oracle import operators as oo
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def annulus(p, nel, sigma=0.5, seed=0):
+    patch = quarter_annulus(p, nel)
+    C, m = patch.conductivity_coefficients(lognormal_conductivity(patch, sigma, seed=seed))
+    return patch, C, m
+
+
+def fd_all_dirichlet(patch, coefs=(1.0, 1.0)):
+    from igaops_amd.fastdiag import SingleFD
+    from igaops_amd.solvers import Mask
+
+    fd = SingleFD("spatial")
+    fd.compute_space_eigendecomposition(patch.quadrule_list, np.ones((1, 3, 2), bool))
+    fd.update_space_eigenvalues(list(coefs))
+    return fd, Mask(box=fd.space_preconditioner.box())
+
+
+def true_residual(A64, mask, b, x):
+    import torch
+
+    r = b - A64.apply(x)
+    mask(r)
+    return float(torch.linalg.vector_norm(r) / torch.linalg.vector_norm(b))
+
+
+def test_curved_fp64_p4_n64_full_size():
+    import torch
+
+    from igaops_amd.operators import StiffnessMassOperator
+    from igaops_amd.solvers import gmres_device
+
+    say("annulus p=4 nel=64: set-up")
+    patch, C, m = annulus(4, 64)
+    qs, w, n = patch.quadrule_list, patch.nurbs_weights, patch.nbctrlpts_total
+    assert n == 68 ** 3 and patch.nbquadpts_total == 199 ** 3
+    A64 = StiffnessMassOperator(qs, C=C, m=m, nurbs_weights=w, dtype="f64")
+    assert A64.fused
+    X = np.random.default_rng(1).standard_normal((n, 16))
+    say("16 right-hand sides in one launch against the oracle")
+    Y = A64.apply(torch.as_tensor(X, device="cuda").reshape(-1), nrhs=16).reshape(n, 16).cpu().numpy()
+    ref = oo.stiffness_plus_mass(qs, C, m, X, nurbs_weights=w)
+    errs = [rel(Y[:, k], ref[:, k]) for k in range(16)]
+    say("max relative error of the 16 columns", max(errs))
+    assert max(errs) <= 1e-12
+    y1 = A64.apply(torch.as_tensor(X[:, 3].copy(), device="cuda")).cpu().numpy()
+    assert rel(y1, ref[:, 3]) <= 1e-12
+    say("16 FD-preconditioned GMRES solves to a true 1e-10")
+    fd, mask = fd_all_dirichlet(patch)
+    rng = np.random.default_rng(2)
+    for k in range(16):
+        b = torch.as_tensor(rng.standard_normal(n), device="cuda")
+        mask(b)
+        out = gmres_device(lambda x, o: A64.apply(x, out=o), b, lambda x, o: fd.apply_device(x, out=o), 1e-10, 400,
+                           mask=mask, use_increment=False)
+        tr = true_residual(A64, mask, b, out.solution)
+        say(f"  rhs {k}: {out.iterations} iterations, true relative residual {tr:.2e}")
+        assert out.success and tr <= 1e-10
+
+
+@pytest.mark.parametrize("p", [2, 4, 6])
+def test_mixed_precision_n128_full_size(p):
+    import torch
+
+    from igaops_amd.operators import StiffnessMassOperator
+    from igaops_amd.solvers import mixed_solve
+
+    say(f"annulus p={p} nel=128: set-up")
+    patch, C, m = annulus(p, 128)
+    qs, w, n = patch.quadrule_list, patch.nurbs_weights, patch.nbctrlpts_total
+    assert n == (128 + p) ** 3
+    A64 = StiffnessMassOperator(qs, C=C, m=m, nurbs_weights=w, dtype="f64")
+    A32 = StiffnessMassOperator(qs, C=C, m=m, nurbs_weights=w, dtype="f32")
+    x = np.random.default_rng(3).standard_normal(n)
+    say("float32 and float64 operators against the float64 oracle")
+    ref = oo.stiffness_plus_mass(qs, C, m, x, nurbs_weights=w)
+    xd = torch.as_tensor(x, device="cuda")
+    e64 = rel(A64.apply(xd).cpu().numpy(), ref)
+    e32 = rel(A32.apply(xd.float()).double().cpu().numpy(), ref)
+    say(f"  relative error float64 {e64:.2e}, float32 {e32:.2e}")
+    assert e64 <= 1e-12 and e32 <= 5e-6
+    del ref, C
+    fd, mask = fd_all_dirichlet(patch)
+    say("float32 FD against the float64 oracle FD")
+    ofd_ = ofd.SpaceFD([StandardGauss(q.degree, q.knotvector, "legendre") for q in qs], np.ones((1, 3, 2), bool))
+    ofd_.scalar_coefs = (1.0, 1.0)
+    pref = ofd_.apply(x)
+    f64 = rel(fd.apply(xd).cpu().numpy(), pref)
+    f32 = rel(fd.apply(xd.float()).double().cpu().numpy(), pref)
+    say(f"  FD relative error float64 {f64:.2e}, float32 {f32:.2e}")
+    assert f64 <= 1e-12 and f32 <= 1e-5
+    say("iterative refinement (float32 inner GMRES, float64 residual) to a true 1e-10")
+    b = torch.as_tensor(np.random.default_rng(4).standard_normal(n), device="cuda")
+    mask(b)
+    out = mixed_solve(lambda v, o: A64.apply(v, out=o), lambda v, o: A32.apply(v, out=o),
+                      lambda v, o: fd.apply_device(v, out=o), b, tol=1e-10, inner_tol=1e-6, mask=mask)
+    tr = true_residual(A64, mask, b, out.solution)
+    say(f"  {out.iterations} inner iterations in {len(out.history['inner_iterations'])} refinements, "
+        f"true relative residual {tr:.2e}")
+    assert out.success and tr <= 1e-10
+
+
+def test_spacetime_heat_full_size():
+    import torch
+
+    from igaops_amd.fastdiag import SingleFD
+    from igaops_amd.gcrodr import GCRODR
+    from igaops_amd.quadrature import WeightedQuadrature, make_open_knotvector
+    from igaops_amd.spacetime import BoxMasked, SpaceTimeOperator
+
+    say("space-time heat, space p=3 nel=32, time p=2 nel=64: set-up")
+    qs = [WeightedQuadrature(3, make_open_knotvector(3, 32), quadtype="2") for _ in range(3)]
+    qt = WeightedQuadrature(2, make_open_knotvector(2, 64), quadtype="2")
+    fd = SingleFD("spacetime")
+    fd.compute_space_eigendecomposition(qs, np.ones((1, 3, 2), bool))
+    fd.compute_time_schurdecomposition(qt)
+    fd.update_space_eigenvalues([0.0, 1.0])
+    op = SpaceTimeOperator(qs + [qt], 1.0, np.eye(3))
+    n = op.ndof
+    assert n == 35 ** 3 * 66
+    A = BoxMasked(op, fd.sptm_preconditioner.box())
+    rng = np.random.default_rng(5)
+    b = torch.as_tensor(rng.standard_normal(n), device="cuda")
+    A._mask(b, 1)
+    say("A P b = b on the free nodes")
+    APb = A.apply_device(fd.sptm_preconditioner.apply_device(b))
+    e = float(torch.linalg.vector_norm(APb - b) / torch.linalg.vector_norm(b))
+    say(f"  relative deviation {e:.2e}")
+    assert e <= 1e-9
+    say("GCRODR (krylov 40, recycled 20) over 10 perturbed right-hand sides")
+    g = GCRODR(1e-10, 2000, krylov_size=40, recycled_size=20)
+    for i in range(10):
+        out = g.solve_device(A, b, Pfun=fd.sptm_preconditioner)
+        r = b - A.apply_device(out.solution)
+        tr = float(torch.linalg.vector_norm(r) / torch.linalg.vector_norm(b))
+        say(f"  rhs {i}: {out.iterations} iterations, true relative residual {tr:.2e}")
+        assert out.success and tr <= 1e-10
+        xi = torch.as_tensor(rng.standard_normal(n), device="cuda")
+        A._mask(xi, 1)
+        b = b + 0.05 * torch.linalg.vector_norm(b) / n ** 0.5 * xi

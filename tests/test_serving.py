@@ -89,3 +89,36 @@ def test_admission_respects_memory_budget():
     assert len(res) == len(reqs) and all(r.true_rel_residual <= 1e-10 for r in res)
     assert srv.stats["max_bytes"] <= srv.budget
     assert srv.stats["deferred_for_memory"] > 0
+
+
+def _span_mix():
+    """Shapes at both ends of the served stream's range: nel 16 and 96, degree 2 and
+    5, both geometries; two requests share the largest shape."""
+    shapes = [("cube", 2, 16), ("annulus", 5, 96), ("annulus", 2, 16), ("cube", 5, 96), ("annulus", 5, 96),
+              ("cube", 3, 40)]
+    return [SolveRequest(i, g, p, n, 0.2 + 0.15 * i, 500 + i) for i, (g, p, n) in enumerate(shapes)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_results_same_alone_and_in_mix_across_the_stream_range(mode):
+    """Requests at the ends of the stream's size range (1.03 M DOF and 26 M points for
+    degree 5 and 96 elements) give the same solution alone and in the mix, each to a
+    true 1e-10."""
+    from igaops_amd.serving import SolveServer
+
+    print("case", mode, flush=True)
+    srv = SolveServer(mode=mode, max_active=6, keep_solutions=True)
+    mix = {r.rid: r for r in srv.serve(_span_mix(), realtime=False)}
+    assert srv.stats["max_active"] >= 4
+    for req in _span_mix():
+        alone = SolveServer(mode=mode, max_active=1, keep_solutions=True).serve([req], realtime=False)[0]
+        got = mix[req.rid]
+        print(req.shape, got.iterations, got.true_rel_residual, flush=True)
+        assert got.success and alone.success
+        assert got.true_rel_residual <= 1e-10 and alone.true_rel_residual <= 1e-10
+        assert got.iterations == alone.iterations
+        a, b = alone.solution.cpu().numpy(), got.solution.cpu().numpy()
+        assert np.linalg.norm(a - b) <= 1e-12 * np.linalg.norm(a)
+        del alone, got
+    mix.clear()

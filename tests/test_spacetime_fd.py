@@ -162,36 +162,63 @@ def test_spacetime_operator_matches_oracle(fused):
 
 @pytest.mark.gpu
 def test_gcrodr_spacetime_sequence():
-    """FD-preconditioned GCRODR on the space-time heat operator over a sequence of
-    perturbed right-hand sides: each solve reaches the preconditioned tolerance, the
-    true residual is small, and recycling does not increase the matvec count."""
+    """FD-preconditioned GCRODR (``solvers/deflated/gcrodr.py``) on a space-time heat
+    operator with a strongly varying diffusivity (the FD is far from exact) and short
+    cycles (krylov 12, recycled 6, so every solve cycles and recycles) over a sequence
+    of perturbed right-hand sides: the oracle's GCRODR matvec counts within two, each
+    solve at its tolerance on the preconditioned residual, and a true residual within
+    ten times the tolerance (the reference's GCRODR test bound)."""
     import torch
 
     from igaops_amd.fastdiag import SingleFD
     from igaops_amd.gcrodr import GCRODR
     from igaops_amd.spacetime import BoxMasked, SpaceTimeOperator
+    from oracle import operators as oo
+    from oracle import solvers as osv
 
     qs, qt = rules(2, 4, 2, 6)
     ql = qs + [qt]
+    nq = int(np.prod([q.nbquadpts for q in ql]))
+    rng = np.random.default_rng(7)
+    c = np.exp(2.0 * rng.standard_normal(nq))
+    k = np.exp(2.0 * rng.standard_normal(nq))
+    G = rng.standard_normal((3, 3, nq)) * 0.3
+    Cf = (np.einsum("ikq,jkq->ijq", G, G) + np.eye(3)[:, :, None]) * k
     fd = SingleFD("spacetime")
     fd.compute_space_eigendecomposition(qs, np.ones((1, 3, 2), bool))
     fd.compute_time_schurdecomposition(qt)
     fd.update_space_eigenvalues([0.0, 1.0])
     box = fd.sptm_preconditioner.box()
-    A = BoxMasked(SpaceTimeOperator(ql, 1.0, np.eye(3)), box)
+    A = BoxMasked(SpaceTimeOperator(ql, c, Cf), box)
     n = int(np.prod([q.nbctrlpts for q in ql]))
-    rng = np.random.default_rng(7)
-    b = torch.as_tensor(rng.standard_normal(n), device="cuda")
-    A._mask(b, 1)
-    g = GCRODR(1e-8, 400, krylov_size=30, recycled_size=10)
-    its = []
+    sfd = ofd.SpaceFD([gauss_of(q) for q in qs], np.ones((1, 3, 2), bool))
+    sfd.scalar_coefs = (0.0, 1.0)
+    ost = ofd.SpaceTimeFD(sfd, gauss_of(qt))
+    fixed = np.ones(n, bool)
+    fixed[ost.free_nodes()] = False
+
+    def oA(v):
+        v = v.copy()
+        v[fixed] = 0
+        y = oo.bspline_u_v(ql, c, v, time_ders=(0, 1)) + oo.bspline_gradu_gradv(ql, Cf, v, time_ders=(0, 0))
+        y[fixed] = 0
+        return y
+
+    bh = rng.standard_normal(n)
+    bh[fixed] = 0
+    tol = 1e-10
+    g = GCRODR(tol, 2000, krylov_size=12, recycled_size=6)
+    og = osv.GCRODR(tol, 2000, krylov_size=12, recycled_size=6)
     for i in range(3):
+        b = torch.as_tensor(bh, device="cuda")
         out = g.solve_device(A, b, Pfun=fd.sptm_preconditioner)
-        assert out.success
-        its.append(out.iterations)
+        ref = og.solve(oA, bh, ost.apply)
         r = b - A.apply_device(out.solution)
-        assert float(torch.linalg.vector_norm(r) / torch.linalg.vector_norm(b)) <= 1e-5
-        xi = torch.as_tensor(rng.standard_normal(n), device="cuda")
-        A._mask(xi, 1)
-        b = b + 0.05 * torch.linalg.vector_norm(b) / n ** 0.5 * xi
-    assert its[-1] <= its[0]
+        tr = float(torch.linalg.vector_norm(r) / torch.linalg.vector_norm(b))
+        print("rhs", i, out.iterations, "iterations (oracle", ref.iterations, "), true relative residual", tr,
+              flush=True)
+        assert out.success and ref.success and out.residual[-1] < tol
+        assert out.iterations > 12  # cycled
+        assert abs(out.iterations - ref.iterations) <= 2
+        assert tr <= 10 * tol
+        bh = bh + 0.05 * np.linalg.norm(bh) / n ** 0.5 * rng.standard_normal(n) * (~fixed)
