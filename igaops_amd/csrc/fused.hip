@@ -22,6 +22,8 @@
 #include <atomic>
 #include <cstring>
 #include <functional>
+#include <mutex>
+#include <string>
 #include <type_traits>
 #include <vector>
 
@@ -944,6 +946,32 @@ static void run_pack(const iga_fused_s* f, const double* C, const double* m, con
   IGA_LAUNCH_CHECK();
 }
 
+// Scratch of iga_fused_apply_noalloc: the sweep's partial blocks live in this
+// file-scope array instead of a caller workspace, so that the entry point allocates
+// nothing.  Every GPU the library runs on holds its own copy (kNoAllocBytes, until
+// the process ends).  Host code never passes the array's name to a kernel: it uses
+// the device address hipGetSymbolAddress returns, looked up once per device with
+// that device current and kept in g_scratch_ptr.
+constexpr int64_t kNoAllocBytes = int64_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+constexpr int kScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kScratchDevices] = {};
+
+static void* scratch_of_current_device() {
+  int dev = -1;
+  IGA_HIP(hipGetDevice(&dev));
+  IGA_CHECK_ARG(dev >= 0 && dev < kScratchDevices, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    IGA_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    IGA_CHECK_ARG(p != nullptr, "no device address for the scratch array");
+    g_scratch_ptr[dev] = p;
+  }
+  return g_scratch_ptr[dev];
+}
+
 }  // namespace iga
 
 extern "C" {
@@ -1116,6 +1144,25 @@ int iga_fused_apply(iga_fused_t f, const void* packed, const void* x, void* y, i
   return guard([&] {
     IGA_CHECK_ARG(f && packed && x && y && work, "NULL argument");
     IGA_CHECK_ARG(nrhs >= 1, "nrhs must be >= 1");
+    if (f->dtype == IGA_F64)
+      run_apply<double>(f, packed, x, y, nrhs, nurbs_w, alpha, accumulate, work, as_stream(stream));
+    else
+      run_apply<float>(f, packed, x, y, nrhs, nurbs_w, alpha, accumulate, work, as_stream(stream));
+  });
+}
+
+int iga_fused_apply_noalloc(iga_fused_t f, const void* packed, const void* x, void* y, int nrhs,
+                            const void* nurbs_w, double alpha, int accumulate, void* stream) {
+  return guard([&] {
+    IGA_CHECK_ARG(f && packed && x && y, "NULL argument");
+    IGA_CHECK_ARG(nrhs >= 1, "nrhs must be >= 1");
+    const int64_t need = iga_fused_workspace_bytes(f, nrhs);
+    if (need > kNoAllocBytes)
+      throw Error(IGA_ERR_INVALID, "the partial blocks of this call need " + std::to_string(need) +
+                                       " bytes, above the " + std::to_string(kNoAllocBytes) +
+                                       "-byte built-in scratch of iga_fused_apply_noalloc (use iga_fused_apply "
+                                       "with a workspace)");
+    void* work = scratch_of_current_device();
     if (f->dtype == IGA_F64)
       run_apply<double>(f, packed, x, y, nrhs, nurbs_w, alpha, accumulate, work, as_stream(stream));
     else

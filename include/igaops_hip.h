@@ -231,6 +231,19 @@ int iga_fused_pack(iga_fused_t plan, const double* C, const double* m, const dou
 int iga_fused_apply(iga_fused_t plan, const void* packed, const void* x, void* y, int nrhs,
                     const void* nurbs_w, double alpha, int accumulate, void* work, void* stream);
 
+/* iga_fused_apply without the workspace argument, for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops and harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  The partial blocks go to a built-in
+ * device array of 64 MiB (67,108,864 bytes) that each GPU running the library holds
+ * until the process ends; the call allocates nothing (no hipMalloc / hipFree, no
+ * allocating library routine) and rejects with IGA_ERR_INVALID any call whose
+ * iga_fused_workspace_bytes(plan, nrhs) exceeds 64 MiB.  Every call on one GPU shares
+ * that GPU's array and leaves its last contents there: calls that overlap on one GPU
+ * (other streams or host threads) race, so serialise them.  Results are those of
+ * iga_fused_apply with the same arguments (same kernels, same summation order). */
+int iga_fused_apply_noalloc(iga_fused_t plan, const void* packed, const void* x, void* y, int nrhs,
+                            const void* nurbs_w, double alpha, int accumulate, void* stream);
+
 /* The two launches of iga_fused_apply, separately (for timing): stage 1 runs the
  * coefficient sweep into the partial blocks of `work`, stage 2 adds them into y. */
 int iga_fused_apply_stage(iga_fused_t plan, int stage, const void* packed, const void* x, void* y,
