@@ -35,7 +35,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-MFMA_PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}  # dense MFMA peaks of the MI355X
+MFMA_PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3, "bf16": 2500.0}  # dense MFMA peaks of the MI355X
 
 WORKLOADS = {
     "annulus_p4_n64": dict(geometry="quarter_annulus", degree=4, nel=64, nrhs=16, sigma=0.5),
@@ -107,18 +107,26 @@ def fd_error_vs_oracle(patch, dtype, x, reps=20):
     fd = SingleFD("spatial")
     fd.compute_space_eigendecomposition(qs, table)
     fd.update_space_eigenvalues([1.0, 1.0])
-    xd = torch.as_tensor(x, device="cuda").to(torch.float64 if dtype == "f64" else torch.float32)
-    y = fd.apply(xd).double().cpu().numpy()
-    out = torch.empty_like(xd)
-    for _ in range(3):
-        fd.apply_device(xd, out)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(reps):
-        fd.apply_device(xd, out)
-    e1.record()
-    torch.cuda.synchronize()
-    t_dev = e0.elapsed_time(e1) / reps * 1e-3
+    sp = fd.space_preconditioner
+
+    def run(dt):
+        sp.float32_plan = "bf16" if dt == "bf16" else "f32"
+        xd = torch.as_tensor(x, device="cuda").to(torch.float64 if dt == "f64" else torch.float32)
+        yv = fd.apply(xd).double().cpu().numpy()
+        out = torch.empty_like(xd)
+        for _ in range(3):
+            fd.apply_device(xd, out)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fd.apply_device(xd, out)
+        e1.record()
+        torch.cuda.synchronize()
+        sp.float32_plan = "f32"
+        return yv, e0.elapsed_time(e1) / reps * 1e-3
+
+    runs = {dt: run(dt) for dt in ("f64", "f32", "bf16")}
+    y, t_dev = runs[dtype]
     ref_fd = ofd.SpaceFD([StandardGauss(q.degree, q.knotvector, "legendre") for q in qs], table)
     ref_fd.scalar_coefs = (1.0, 1.0)
     from threadpoolctl import threadpool_limits
@@ -135,12 +143,17 @@ def fd_error_vs_oracle(patch, dtype, x, reps=20):
     nsub = int(np.prod(lens))
     flops = 4.0 * nsub * sum(lens)
     peak = MFMA_PEAK_TFLOPS[dtype]
+    rel = lambda v: float(np.linalg.norm(v - ref) / np.linalg.norm(ref))
     return {
-        "rel_err_vs_oracle": float(np.linalg.norm(y - ref) / np.linalg.norm(ref)),
+        "rel_err_vs_oracle": rel(y),
         "free_box": lens,
         "us_per_apply": t_dev * 1e6,
+        "launches": 3,
         "mfma": {"achieved": flops / t_dev / 1e12, "peak": peak, "unit": "TFLOP/s",
                  "frac": flops / t_dev / 1e12 / peak, "flops": flops},
+        "by_precision": {dt: {"us_per_apply": tt * 1e6, "rel_err_vs_oracle": rel(yy),
+                              "mfma_frac": flops / tt / 1e12 / MFMA_PEAK_TFLOPS[dt]}
+                         for dt, (yy, tt) in runs.items()},
         "cpu_baseline": {"value": t_cpu * 1e6, "unit": "us per apply", "cores": 1, "kind": "port",
                          "sample": "the oracle's FD apply (numpy / scipy dgemm, one BLAS thread) on the same vector, median of 3 after one warm-up"},
     }

@@ -13,6 +13,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libigaops_hip.so"
 
 IGA_F64 = 0
 IGA_F32 = 1
+IGA_BF16 = 2  # FD plans: float vectors, bf16 matrix-core inputs, FP32 accumulation
 
 _i32, _i64, _f64, _vp = C.c_int, C.c_int64, C.c_double, C.c_void_p
 _i32p = C.POINTER(C.c_int32)

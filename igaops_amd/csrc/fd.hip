@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <vector>
 
 #include "common.hpp"
@@ -230,6 +231,544 @@ static void mode_gemm(const T* A, const T* X, T* Y, int64_t outer, int64_t n, in
   mode_gemm_rect<T>(A, X, Y, outer, n, n, inner, scale, s);
 }
 
+// ---------------------------------------------------------------------------
+// Three-launch FD apply of a three-direction free box (the spatial preconditioner):
+//   plane kernel, forward:  for box plane i3 and a block S of 16 rows i1, gather the
+//     plane of x, Y1(S, :) = U1^T X along direction 1, then W(S, :, i3) = Y1 U2 along
+//     direction 2 (box-dense W, direction 1 fastest)
+//   line kernel:            for 16 columns (i1, i2, rhs) of W: Z = U3^T W, Z *= Lambda^-1,
+//     W = U3 Z, in place
+//   plane kernel, backward: y(box) = U1 (W along 1), then U2 along 2, written into the
+//     full vector; every entry outside the box is set to zero
+// Each work-group stages its operands in LDS once (all loads issued together) and runs
+// both of its products on the matrix cores: FP64 (v_mfma_f64_16x16x4f64), FP32
+// (v_mfma_f32_16x16x4f32), or BF16 inputs with FP32 accumulation
+// (v_mfma_f32_16x16x16bf16_1k; float storage, operands rounded to bf16 as they are
+// read).  Three launches replace the gather, six GEMM launches and the scatter.
+// ---------------------------------------------------------------------------
+
+enum FdMode { FD_F64 = 0, FD_F32 = 1, FD_BF16 = 2 };
+
+typedef short v4s __attribute__((ext_vector_type(4)));
+
+__device__ inline short f32_to_bf16(float x) {
+  uint32_t u = __float_as_uint(x);
+  u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even (finite inputs)
+  return (short)(u >> 16);
+}
+
+// Matrix-core operands.  For FP64 / FP32 a lane feeds one element of A (row lane & 15,
+// k = k0 + lane / 16) and one of B (column lane & 15, same k) to the 16x16x4 MFMA; for
+// BF16 it feeds four consecutive k (k0 + 4 (lane / 16) + 0..3) to the 16x16x16 MFMA,
+// read as one 8-byte LDS word because BF16 operands are stored in LDS as bf16.  The
+// same k set reaches A and B, so the sum over k is the full product.  KS is the K step
+// whose operands are all read before its MFMAs.
+template <int MODE>
+struct FdMma;
+template <>
+struct FdMma<FD_F64> {
+  using T = double;   // storage of vectors and accumulators
+  using S = double;   // LDS operand storage
+  using Op = double;  // one MFMA operand of a lane
+  using V = v4d;
+  static constexpr int KI = 4, KS = 8;
+  __device__ static S to_s(T v) { return v; }
+  // operand of a row starting at `rowp` (k contiguous)
+  __device__ static Op op(const S* rowp, int k0, int lane) { return rowp[k0 + (lane >> 4)]; }
+  // operand of column `c` of an array stored k-major with pitch p
+  __device__ static Op op_t(const S* base, int p, int c, int k0, int lane) { return base[(k0 + (lane >> 4)) * p + c]; }
+  template <int TENSOR>
+  __device__ static V mma(Op a, Op b, V c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+  __device__ static int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+template <>
+struct FdMma<FD_F32> {
+  using T = float;
+  using S = float;
+  using Op = float;
+  using V = v4f;
+  static constexpr int KI = 4, KS = 8;
+  __device__ static S to_s(T v) { return v; }
+  __device__ static Op op(const S* rowp, int k0, int lane) { return rowp[k0 + (lane >> 4)]; }
+  __device__ static Op op_t(const S* base, int p, int c, int k0, int lane) { return base[(k0 + (lane >> 4)) * p + c]; }
+  template <int TENSOR>
+  __device__ static V mma(Op a, Op b, V c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+  __device__ static int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+// BF16: LDS holds float; an eigenvector-matrix operand is rounded to bf16 as it is
+// read, a vector (tensor) operand x is split into bf16 hi = bf16(x) and lo = bf16(x - hi)
+// and enters two MFMAs.  The products then carry the bf16 rounding of the eigenvectors
+// (a fixed linear map) and the tensor to about 16 significant bits, so the
+// preconditioner stays linear to ~1e-5 (a single bf16 rounding of the tensor makes it
+// nonlinear at the 1e-2 level, which stalls the Krylov solves it preconditions).
+template <>
+struct FdMma<FD_BF16> {
+  using T = float;
+  using S = float;
+  using Op = v4f;
+  using V = v4f;
+  static constexpr int KI = 16, KS = 16;
+  __device__ static S to_s(T v) { return v; }
+  // rows start on 16-byte boundaries (pitch a multiple of 4)
+  __device__ static Op op(const S* rowp, int k0, int lane) {
+    return *reinterpret_cast<const v4f*>(rowp + k0 + 4 * (lane >> 4));
+  }
+  __device__ static Op op_t(const S* base, int p, int c, int k0, int lane) {
+    const S* q = base + (k0 + 4 * (lane >> 4)) * p + c;
+    return v4f{q[0], q[p], q[2 * p], q[3 * p]};
+  }
+  __device__ static v4s round(Op x) {
+    return v4s{f32_to_bf16(x[0]), f32_to_bf16(x[1]), f32_to_bf16(x[2]), f32_to_bf16(x[3])};
+  }
+  __device__ static v4s low(Op x, v4s hi) {
+    v4s lo;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) lo[e] = f32_to_bf16(x[e] - __uint_as_float((uint32_t)(uint16_t)hi[e] << 16));
+    return lo;
+  }
+  // TENSOR = 0: A is the vector operand; 1: B is
+  template <int TENSOR>
+  __device__ static V mma(Op a, Op b, V c) {
+    if constexpr (TENSOR == 0) {
+      const v4s ah = round(a), bb = round(b);
+      c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bb, c, 0, 0, 0);
+      return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(low(a, ah), bb, c, 0, 0, 0);
+    } else {
+      const v4s aa = round(a), bh = round(b);
+      c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(aa, bh, c, 0, 0, 0);
+      return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(aa, low(b, bh), c, 0, 0, 0);
+    }
+  }
+  __device__ static int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
+constexpr int kFdMaxTiles = 3;  // 16-wide tiles per wave: box lengths up to 4 * 3 * 16 = 192
+
+// acc[u] (+)= A B_u over k < K (a multiple of KS) for the TPW tiles u of a wave; `fa(k0)`
+// gives the A operand, `fb(u, k0)` the B operand of tile u; TENSOR names the operand
+// holding vector data (0: A, 1: B) for the split BF16 products.  The LDS reads of the next K
+// step are issued before the MFMAs of the current one (two operand buffers), and tiles
+// alternate, so consecutive MFMAs are independent and LDS latency overlaps them.
+template <int MODE, int TPW, int TENSOR, class FA, class FB>
+__device__ inline void fd_mma_loop(typename FdMma<MODE>::V (&acc)[TPW], int K, const FA& fa, const FB& fb) {
+  using M = FdMma<MODE>;
+  constexpr int NS = M::KS / M::KI;
+  typename M::Op av[2][NS], bv[2][NS][TPW];
+  auto fetch = [&](int buf, int k0) __attribute__((always_inline)) {
+#pragma unroll
+    for (int s2 = 0; s2 < NS; ++s2) {
+      av[buf][s2] = fa(k0 + s2 * M::KI);
+#pragma unroll
+      for (int u = 0; u < TPW; ++u) bv[buf][s2][u] = fb(u, k0 + s2 * M::KI);
+    }
+  };
+  auto run = [&](int buf) __attribute__((always_inline)) {
+#pragma unroll
+    for (int s2 = 0; s2 < NS; ++s2)
+#pragma unroll
+      for (int u = 0; u < TPW; ++u) acc[u] = M::template mma<TENSOR>(av[buf][s2], bv[buf][s2][u], acc[u]);
+  };
+  fetch(0, 0);
+  int k0 = 0;
+  // two K steps per trip so that each buffer is addressed statically
+  for (; k0 + 2 * M::KS <= K; k0 += 2 * M::KS) {
+    fetch(1, k0 + M::KS);
+    run(0);
+    if (k0 + 2 * M::KS < K) fetch(0, k0 + 2 * M::KS);
+    run(1);
+  }
+  if (k0 < K) run(0);
+}
+
+// Stage a rows x cols block into LDS, dst[r * p + c] = src(r)[c * cstride]: rows split
+// over the four waves, columns over the 64 lanes (CC = ceil(cols / 64) <= 3 passes), RG
+// rows per wave per round with every load of the round issued before the stores.
+// Clamped indices keep every load in bounds; out-of-range elements are not stored.
+template <int MODE, int CC, class RowPtr>
+__device__ inline void fd_stage(typename FdMma<MODE>::S* dst, int p, int rows, int cols, const RowPtr& rowptr,
+                                int64_t cstride, int wave, int lane) {
+  using M = FdMma<MODE>;
+  using T = typename M::T;
+  constexpr int RG = CC == 1 ? 8 : CC == 2 ? 6 : 4;
+  for (int r0 = 0; r0 < rows; r0 += 4 * RG) {
+    T v[RG][CC];
+#pragma unroll
+    for (int i = 0; i < RG; ++i) {
+      const T* src = rowptr(min(r0 + wave + 4 * i, rows - 1));
+#pragma unroll
+      for (int cc = 0; cc < CC; ++cc) v[i][cc] = src[(int64_t)min(lane + 64 * cc, cols - 1) * cstride];
+    }
+#pragma unroll
+    for (int i = 0; i < RG; ++i) {
+      const int r = r0 + wave + 4 * i;
+#pragma unroll
+      for (int cc = 0; cc < CC; ++cc) {
+        const int c = lane + 64 * cc;
+        if (r < rows && c < cols) dst[r * p + c] = M::to_s(v[i][cc]);
+      }
+    }
+  }
+}
+
+// Zero columns [c0, c1) of rows [0, rows) (c1 - c0 < 64): the K padding of an operand.
+template <int MODE>
+__device__ inline void fd_zero_cols(typename FdMma<MODE>::S* dst, int p, int rows, int c0, int c1, int wave,
+                                    int lane) {
+  if (lane < c1 - c0)
+    for (int r = wave; r < rows; r += 4) dst[r * p + c0 + lane] = FdMma<MODE>::to_s(0);
+}
+
+template <typename T>
+struct FdPlaneArgs {
+  int n1, n2, n3, nrhs;
+  int64_t nf1, nf2, nf3;  // full lengths of the three directions
+  int64_t lo1, lo2, lo3;  // box start
+  const T* in;            // forward: full x; backward: box-dense W
+  T* out;                 // forward: box-dense W; backward: full y
+  const T* A1;            // n1 x n1 row-major: Y1(i1, .) = sum_k1 A1[i1][k1] X(k1, .)
+  const T* A2;            // n2 x n2 row-major: Y(., i2) = sum_k2 A2[i2][k2] Y1(., k2)
+  int p;                  // LDS row pitch (elements of the LDS operand type)
+};
+
+// LDS: P (n2 rows x p: the plane X[k2][k1], then A2[i2][k2]) and Q (16 rows x p: the A1
+// rows of S, then Y1[i1][k2]); then the output tile staging [i2][17] in T over P.  Rows
+// are zero-padded in k up to a multiple of KS.  Tiles past n2 read clamped (finite) rows:
+// their garbage stays in output columns that are never stored, and Y1's columns past n2
+// are stored as zeros.  TPW = ceil(ceil(n2 / 16) / 4) tiles per wave.
+template <int MODE, bool BWD, int TPW>
+__global__ void __launch_bounds__(256) fd_plane_kernel(FdPlaneArgs<typename FdMma<MODE>::T> a) {
+  using M = FdMma<MODE>;
+  using T = typename M::T;
+  using V = typename M::V;
+  using S = typename M::S;
+  extern __shared__ __align__(16) unsigned char fd_smem[];
+  S* P = reinterpret_cast<S*>(fd_smem);
+  const int p = a.p, n1 = a.n1, n2 = a.n2;
+  S* Q = P + (int64_t)n2 * p;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int k = blockIdx.z;
+  const int nr = a.nrhs;
+  if (BWD && (int)blockIdx.y >= a.n3) {
+    // a plane of y outside the box in direction 3: zeros, split over blockIdx.x
+    const int t = blockIdx.y - a.n3;
+    const int64_t i3f = t < a.lo3 ? t : a.lo3 + a.n3 + (t - a.lo3);
+    const int64_t np = a.nf1 * a.nf2;
+    for (int64_t q = (int64_t)blockIdx.x * 256 + tid; q < np; q += (int64_t)gridDim.x * 256)
+      a.out[(q + np * i3f) * nr + k] = T(0);
+    return;
+  }
+  const int i3 = blockIdx.y;
+  const int i1s = blockIdx.x * 16;
+  const int K1 = (n1 + M::KS - 1) / M::KS * M::KS;  // zero-padded inner lengths
+  const int K2 = (n2 + M::KS - 1) / M::KS * M::KS;
+  // the plane X[k2][k1] (rows k2, k1 contiguous in both layouts) and the A1 rows of S
+  const int64_t row0 = BWD ? (int64_t)n1 * n2 * i3 * nr + k
+                           : ((a.lo1) + a.nf1 * (a.lo2 + a.nf2 * (a.lo3 + i3))) * nr + k;
+  const int64_t rstep = BWD ? (int64_t)n1 * nr : a.nf1 * nr;
+  fd_stage<MODE, TPW>(P, p, n2, n1, [&](int r) { return a.in + row0 + rstep * r; }, nr, wave, lane);
+  const int qrows = min(16, n1 - i1s);  // rows of S inside the box; the rest are zero
+  fd_stage<MODE, TPW>(Q, p, qrows, n1, [&](int r) { return a.A1 + (int64_t)(i1s + r) * n1; }, 1, wave, lane);
+  for (int q = tid; q < (16 - qrows) * p; q += 256) Q[qrows * p + q] = M::to_s(T(0));
+  fd_zero_cols<MODE>(P, p, n2, n1, K1, wave, lane);
+  fd_zero_cols<MODE>(Q, p, qrows, n1, K1, wave, lane);
+  __syncthreads();
+  const int lr = lane & 15;
+  const S* brow[TPW];  // this lane's B row (its output column) in each tile, clamped
+#pragma unroll
+  for (int u = 0; u < TPW; ++u) brow[u] = P + min((wave + 4 * u) * 16 + lr, n2 - 1) * p;
+  const S* arow = Q + lr * p;
+  V acc[TPW];
+  // step 1: Y1(i1, k2) = sum_k1 A1[i1][k1] X(k1, k2); M = 16 rows of S, N = k2 tiles
+#pragma unroll
+  for (int u = 0; u < TPW; ++u) acc[u] = V{0, 0, 0, 0};
+  fd_mma_loop<MODE, TPW, 1>(
+      acc, K1, [&](int k0) { return M::op(arow, k0, lane); }, [&](int u, int k0) { return M::op(brow[u], k0, lane); });
+  __syncthreads();
+  // Y1 into Q (columns up to K2; past n2 zeros), A2 into P
+#pragma unroll
+  for (int u = 0; u < TPW; ++u) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int col = (wave + 4 * u) * 16 + lr;
+      if (col < K2) Q[M::row(lane, r) * p + col] = M::to_s(col < n2 ? acc[u][r] : T(0));
+    }
+  }
+  fd_stage<MODE, TPW>(P, p, n2, n2, [&](int r) { return a.A2 + (int64_t)r * n2; }, 1, wave, lane);
+  fd_zero_cols<MODE>(P, p, n2, n2, K2, wave, lane);
+  __syncthreads();
+  // step 2: Y(i1, i2) = sum_k2 Y1(i1, k2) A2[i2][k2]; B(k2, i2) = A2[i2][k2]
+#pragma unroll
+  for (int u = 0; u < TPW; ++u) acc[u] = V{0, 0, 0, 0};
+  fd_mma_loop<MODE, TPW, 0>(
+      acc, K2, [&](int k0) { return M::op(arow, k0, lane); }, [&](int u, int k0) { return M::op(brow[u], k0, lane); });
+  __syncthreads();
+  // stage the 16 x n2 result as [i2][i1] (pitch 17, type T) for writes along direction 1
+  constexpr int RP = 17;
+  T* R = reinterpret_cast<T*>(fd_smem);
+#pragma unroll
+  for (int u = 0; u < TPW; ++u) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int col = (wave + 4 * u) * 16 + lr;
+      if (col < n2) R[col * RP + M::row(lane, r)] = acc[u][r];
+    }
+  }
+  __syncthreads();
+  for (int q = tid; q < 16 * n2; q += 256) {
+    const int il = q & 15, i2 = q >> 4, i1 = i1s + il;
+    if (i1 < n1) {
+      const T v = R[i2 * RP + il];
+      if constexpr (BWD)
+        a.out[((a.lo1 + i1) + a.nf1 * ((a.lo2 + i2) + a.nf2 * (a.lo3 + i3))) * nr + k] = v;
+      else
+        a.out[((int64_t)i1 + (int64_t)n1 * ((int64_t)i2 + (int64_t)n2 * i3)) * nr + k] = v;
+    }
+  }
+  if (BWD && blockIdx.x == 0) {
+    // entries of this plane of y outside the box in directions 1 and 2: rows over the
+    // waves, columns over the lanes
+    const int nf1 = (int)a.nf1, nf2 = (int)a.nf2;
+    T* pl = a.out + (a.nf1 * a.nf2 * (a.lo3 + i3)) * nr + k;
+    for (int i2f = wave; i2f < nf2; i2f += 4) {
+      const bool row_in = i2f >= a.lo2 && i2f < a.lo2 + n2;
+      for (int i1f = lane; i1f < nf1; i1f += 64)
+        if (!row_in || i1f < a.lo1 || i1f >= a.lo1 + n1) pl[((int64_t)i2f * nf1 + i1f) * nr] = T(0);
+    }
+  }
+}
+
+template <typename T>
+struct FdLineArgs {
+  int n3, p, xp;
+  int nrhs;
+  int64_t ncol;    // n1 n2 nrhs columns of W
+  int64_t plane;   // n1 n2: stride of the eigen-index k3 in Lambda
+  T* w;            // box-dense (row i3 of ncol values), overwritten
+  const T* Ut;     // U3^T row-major n3 x n3: Ut[k3][i3]
+  const T* inv;    // Lambda^-1 over the box (direction 1 fastest)
+};
+
+// Both products are run transposed, with the 16 columns of the work-group as the shared
+// A operand and the n3 rows spread over the tiles of the four waves:
+//   Z^T(j, k3) = sum_i3 X^T(j, i3) U3^T[k3][i3],   Y^T(j, i3) = sum_k3 Z^T(j, k3) U3[i3][k3]
+// LDS: U (K3 rows x p: U3^T[k3][i3], rows past n3 zero; the second product reads it
+// transposed) and X (16 rows x xp: X^T[j][i3], then Z^T[j][k3]).  Tile rows past n3 are clamped: their garbage
+// stays in output columns that are never stored, and Z^T's columns past n3 are zeros.
+template <int MODE, int TPW>
+__global__ void __launch_bounds__(256) fd_line_kernel(FdLineArgs<typename FdMma<MODE>::T> a) {
+  using M = FdMma<MODE>;
+  using T = typename M::T;
+  using V = typename M::V;
+  using S = typename M::S;
+  constexpr bool kTwoCopies = false;  // one copy of U3^T; the second product reads it transposed
+  extern __shared__ __align__(16) unsigned char fd_smem[];
+  const int n3 = a.n3, p = a.p, xp = a.xp;
+  const int K3 = (n3 + M::KS - 1) / M::KS * M::KS;
+  S* U = reinterpret_cast<S*>(fd_smem);
+  S* Ub = U + (int64_t)K3 * p;
+  S* X = Ub + (kTwoCopies ? (int64_t)n3 * p : 0);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t j0 = (int64_t)blockIdx.x * 16;
+  // U3^T rows (and for BF16 its transpose), the 16 columns of W transposed into X
+  fd_stage<MODE, TPW>(U, p, n3, n3, [&](int r) { return a.Ut + (int64_t)r * n3; }, 1, wave, lane);
+  if constexpr (kTwoCopies) {
+    // Ub[i3][k3] = U3^T[k3][i3]: row i3 of Ub is column i3 of U3^T
+    fd_stage<MODE, TPW>(Ub, p, n3, n3, [&](int r) { return a.Ut + r; }, n3, wave, lane);
+    fd_zero_cols<MODE>(Ub, p, n3, n3, K3, wave, lane);
+  }
+  for (int q = tid; q < (K3 - n3) * p; q += 256) U[n3 * p + q] = M::to_s(T(0));  // rows past n3
+  fd_zero_cols<MODE>(U, p, n3, n3, K3, wave, lane);
+  {
+    // X[j][i3] = W(i3, j0 + j): 16 threads per W row, rows i3 = tid / 16 + 16 e
+    const int jc = tid & 15;
+    const int64_t jg = min(j0 + jc, a.ncol - 1);
+    for (int r0 = 0; r0 < n3; r0 += 16 * 8) {
+      T v[8];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) v[e] = a.w[(int64_t)min(r0 + (tid >> 4) + 16 * e, n3 - 1) * a.ncol + jg];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int r = r0 + (tid >> 4) + 16 * e;
+        if (r < n3) X[jc * xp + r] = M::to_s(j0 + jc < a.ncol ? v[e] : T(0));
+      }
+    }
+    for (int q = tid; q < (K3 - n3) * 16; q += 256) X[(q & 15) * xp + n3 + (q >> 4)] = M::to_s(T(0));
+  }
+  __syncthreads();
+  const int lr = lane & 15;
+  int mrow[TPW];  // this lane's row of U3 / U3^T in each of its tiles, clamped
+#pragma unroll
+  for (int u = 0; u < TPW; ++u) mrow[u] = min((wave + 4 * u) * 16 + lr, n3 - 1);
+  const S* arow = X + lr * xp;
+  V acc[TPW];
+  // step 1: Z^T(j, k3) = sum_i3 X^T(j, i3) U3^T[k3][i3]
+#pragma unroll
+  for (int u = 0; u < TPW; ++u) acc[u] = V{0, 0, 0, 0};
+  fd_mma_loop<MODE, TPW, 0>(
+      acc, K3, [&](int k0) { return M::op(arow, k0, lane); },
+      [&](int u, int k0) { return M::op(U + mrow[u] * p, k0, lane); });
+  __syncthreads();
+  // scale by Lambda^-1 and store Z^T over X (columns past n3 zero)
+#pragma unroll
+  for (int u = 0; u < TPW; ++u) {
+    const int m = (wave + 4 * u) * 16 + lr;  // k3
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int jr = M::row(lane, r);
+      const int64_t j = j0 + jr;
+      const int64_t c = (j < a.ncol ? j : a.ncol - 1) / a.nrhs;
+      if (m < K3)
+        X[jr * xp + m] = M::to_s((m < n3 && j < a.ncol) ? acc[u][r] * a.inv[c + a.plane * min(m, n3 - 1)] : T(0));
+    }
+  }
+  __syncthreads();
+  // step 2: Y^T(j, i3) = sum_k3 Z^T(j, k3) U3[i3][k3]  (U3[i3][k3] = U3^T[k3][i3])
+#pragma unroll
+  for (int u = 0; u < TPW; ++u) acc[u] = V{0, 0, 0, 0};
+  if constexpr (kTwoCopies) {
+    fd_mma_loop<MODE, TPW, 0>(
+        acc, K3, [&](int k0) { return M::op(arow, k0, lane); },
+        [&](int u, int k0) { return M::op(Ub + mrow[u] * p, k0, lane); });
+  } else {
+    fd_mma_loop<MODE, TPW, 0>(
+        acc, K3, [&](int k0) { return M::op(arow, k0, lane); },
+        [&](int u, int k0) { return M::op_t(U, p, mrow[u], k0, lane); });
+  }
+#pragma unroll
+  for (int u = 0; u < TPW; ++u) {
+    const int m = (wave + 4 * u) * 16 + lr;  // i3
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int64_t j = j0 + M::row(lane, r);
+      if (m < n3 && j < a.ncol) a.w[(int64_t)m * a.ncol + j] = acc[u][r];
+    }
+  }
+}
+
+// LDS sizes and row pitches of the two kernels for box lengths len[0..2].  Pitches: k
+// extent rounded to the K step, plus one element (FP64 / FP32, odd: conflict-free
+// strided reads) or four (BF16: rows on 8-byte boundaries).
+struct Fd3Shape {
+  int p_plane, p_line, xp_line;
+  size_t lds_plane, lds_line;
+};
+
+static Fd3Shape fd3_shape(const int64_t* len, int mode) {
+  const int ks = mode == FD_BF16 ? 16 : 8;
+  const size_t es = mode == FD_F64 ? 8 : 4;
+  const size_t et = mode == FD_F64 ? 8 : 4;
+  auto kpad = [&](int64_t n) { return (int)((n + ks - 1) / ks * ks); };
+  auto pitch = [&](int64_t n) { return kpad(n) + (mode == FD_BF16 ? 4 : 1); };
+  Fd3Shape s{};
+  s.p_plane = pitch(std::max(len[0], len[1]));
+  s.lds_plane = std::max(es * (size_t)(len[1] + 16) * s.p_plane, et * (size_t)len[1] * 17);
+  s.p_line = pitch(len[2]);
+  s.xp_line = pitch(len[2]);
+  const size_t urows = (size_t)kpad(len[2]);
+  s.lds_line = es * (urows * s.p_line + 16 * (size_t)s.xp_line);
+  return s;
+}
+
+constexpr size_t kFdLdsMax = 160 * 1024;
+
+static bool fd3_applicable(const iga_fd_s* f, int mode) {
+  if (f->ndim != 3) return false;
+  for (int d = 0; d < 3; ++d)
+    if (f->len[d] < 1 || f->len[d] > 4 * kFdMaxTiles * 16) return false;
+  const Fd3Shape s = fd3_shape(f->len, mode);
+  return s.lds_plane <= kFdLdsMax && s.lds_line <= kFdLdsMax;
+}
+
+// Dynamic LDS above 64 KiB must be granted per kernel; done once per kernel and size.
+template <typename Kern>
+static void fd3_allow_lds(Kern kern, size_t bytes, std::atomic<size_t>& granted) {
+  if (bytes <= 64 * 1024 || bytes <= granted.load(std::memory_order_relaxed)) return;
+  IGA_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)bytes));
+  granted.store(bytes, std::memory_order_relaxed);
+}
+
+template <int MODE, int TPW>
+static void fd3_launch(const iga_fd_s* f, FdPlaneArgs<typename FdMma<MODE>::T> pa,
+                       FdLineArgs<typename FdMma<MODE>::T> la, const Fd3Shape& sh, int nrhs, int which,
+                       hipStream_t s) {
+  static std::atomic<size_t> granted[3];
+  const unsigned nsb = (unsigned)((pa.n1 + 15) / 16);
+  if (which == 0) {
+    fd3_allow_lds(fd_plane_kernel<MODE, false, TPW>, sh.lds_plane, granted[0]);
+    fd_plane_kernel<MODE, false, TPW><<<dim3(nsb, (unsigned)pa.n3, (unsigned)nrhs), 256, sh.lds_plane, s>>>(pa);
+  } else if (which == 1) {
+    fd3_allow_lds(fd_line_kernel<MODE, TPW>, sh.lds_line, granted[1]);
+    fd_line_kernel<MODE, TPW><<<(unsigned)((la.ncol + 15) / 16), 256, sh.lds_line, s>>>(la);
+  } else {
+    const unsigned extra = (unsigned)(f->n[2] - pa.n3);
+    fd3_allow_lds(fd_plane_kernel<MODE, true, TPW>, sh.lds_plane, granted[2]);
+    fd_plane_kernel<MODE, true, TPW><<<dim3(nsb, (unsigned)pa.n3 + extra, (unsigned)nrhs), 256, sh.lds_plane, s>>>(
+        pa);
+  }
+  IGA_LAUNCH_CHECK();
+}
+
+template <int MODE>
+static void fd3_launch_tpw(const iga_fd_s* f, FdPlaneArgs<typename FdMma<MODE>::T> pa,
+                           FdLineArgs<typename FdMma<MODE>::T> la, const Fd3Shape& sh, int nrhs, int which,
+                           int tiles, hipStream_t s) {
+  const int tpw = (tiles + 3) / 4;
+  if (tpw <= 1)
+    fd3_launch<MODE, 1>(f, pa, la, sh, nrhs, which, s);
+  else if (tpw == 2)
+    fd3_launch<MODE, 2>(f, pa, la, sh, nrhs, which, s);
+  else
+    fd3_launch<MODE, 3>(f, pa, la, sh, nrhs, which, s);
+}
+
+template <int MODE>
+static void fd3_apply(const iga_fd_s* f, const typename FdMma<MODE>::T* x, typename FdMma<MODE>::T* y, int nrhs,
+                      typename FdMma<MODE>::T* w, hipStream_t s) {
+  using T = typename FdMma<MODE>::T;
+  IGA_CHECK_ARG(nrhs <= 65535, "too many right-hand sides for one FD launch");
+  const Fd3Shape sh = fd3_shape(f->len, MODE);
+  const int n1 = (int)f->len[0], n2 = (int)f->len[1], n3 = (int)f->len[2];
+  FdPlaneArgs<T> pa{};
+  pa.n1 = n1;
+  pa.n2 = n2;
+  pa.n3 = n3;
+  pa.nrhs = nrhs;
+  pa.nf1 = f->n[0];
+  pa.nf2 = f->n[1];
+  pa.nf3 = f->n[2];
+  pa.lo1 = f->lo[0];
+  pa.lo2 = f->lo[1];
+  pa.lo3 = f->lo[2];
+  pa.p = sh.p_plane;
+  FdLineArgs<T> la{};
+  la.n3 = n3;
+  la.p = sh.p_line;
+  la.xp = sh.xp_line;
+  la.nrhs = nrhs;
+  la.plane = (int64_t)n1 * n2;
+  la.ncol = la.plane * nrhs;
+  la.w = w;
+  la.Ut = (const T*)f->d_Ut[2];
+  la.inv = (const T*)f->d_inv;
+  // forward: x -> W (directions 1, 2)
+  pa.in = x;
+  pa.out = w;
+  pa.A1 = (const T*)f->d_Ut[0];
+  pa.A2 = (const T*)f->d_Ut[1];
+  // the plane kernels' tiles per wave also set their column passes: cover both lengths
+  const int plane_tiles = (std::max(n1, n2) + 15) / 16;
+  fd3_launch_tpw<MODE>(f, pa, la, sh, nrhs, 0, plane_tiles, s);
+  // direction 3 with the eigenvalue scaling, in place on W
+  fd3_launch_tpw<MODE>(f, pa, la, sh, nrhs, 1, (n3 + 15) / 16, s);
+  // backward: W -> y (directions 1, 2), zeros outside the box
+  pa.in = w;
+  pa.out = y;
+  pa.A1 = (const T*)f->d_U[0];
+  pa.A2 = (const T*)f->d_U[1];
+  fd3_launch_tpw<MODE>(f, pa, la, sh, nrhs, 2, plane_tiles, s);
+}
+
 template <typename T>
 static void fd_apply_t(const iga_fd_s* f, const T* x, T* y, int nrhs, T* w0, T* w1, hipStream_t s) {
   // gather the free box
@@ -273,7 +812,7 @@ int iga_fd_create(iga_fd_t* out, int ndim, const int64_t* n, const int64_t* lo, 
   return guard([&] {
     IGA_CHECK_ARG(out && n && lo && len && U, "NULL argument");
     IGA_CHECK_ARG(ndim >= 1 && ndim <= 4, "ndim must be in 1..4");
-    IGA_CHECK_ARG(dtype == IGA_F64 || dtype == IGA_F32, "unknown dtype");
+    IGA_CHECK_ARG(dtype == IGA_F64 || dtype == IGA_F32 || dtype == IGA_BF16, "unknown dtype");
     auto* f = new iga_fd_s();
     f->dtype = dtype;
     f->ndim = ndim;
@@ -353,6 +892,18 @@ int iga_fd_apply(iga_fd_t f, const void* x, void* y, int nrhs, void* work, void*
     IGA_CHECK_ARG(f && x && y && work && nrhs >= 1, "NULL argument");
     IGA_CHECK_ARG(f->d_inv, "eigenvalues are not set");
     const int64_t half = f->nsub * nrhs;
+    const int mode = f->dtype == IGA_F64 ? FD_F64 : f->dtype == IGA_F32 ? FD_F32 : FD_BF16;
+    if (fd3_applicable(f, mode)) {
+      if (mode == FD_F64)
+        fd3_apply<FD_F64>(f, (const double*)x, (double*)y, nrhs, (double*)work, as_stream(stream));
+      else if (mode == FD_F32)
+        fd3_apply<FD_F32>(f, (const float*)x, (float*)y, nrhs, (float*)work, as_stream(stream));
+      else
+        fd3_apply<FD_BF16>(f, (const float*)x, (float*)y, nrhs, (float*)work, as_stream(stream));
+      return;
+    }
+    IGA_CHECK_ARG(f->dtype != IGA_BF16, "the BF16 FD apply takes three directions whose box planes fit the "
+                                         "LDS (lengths up to about 190)");
     if (f->dtype == IGA_F64) {
       double* w0 = (double*)work;
       fd_apply_t<double>(f, (const double*)x, (double*)y, nrhs, w0, w0 + half, as_stream(stream));

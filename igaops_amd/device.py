@@ -7,7 +7,7 @@ import numpy as np
 
 from . import _lib
 
-DTYPES = {"f64": _lib.IGA_F64, "f32": _lib.IGA_F32}
+DTYPES = {"f64": _lib.IGA_F64, "f32": _lib.IGA_F32, "bf16": _lib.IGA_BF16}
 
 
 def torch():
@@ -49,7 +49,8 @@ def stream_ptr(stream=None) -> int:
 
 def torch_dtype(dtype: str):
     t = torch()
-    return {"f64": t.float64, "f32": t.float32}[dtype]
+    # "bf16" names the FD arithmetic; its vectors are float32
+    return {"f64": t.float64, "f32": t.float32, "bf16": t.float32}[dtype]
 
 
 def iga_dtype(t) -> int:

@@ -46,6 +46,8 @@ class SpaceFD:
         self._dev = {}
         # bumped whenever the eigenvalues change; device plans compare it
         self.eig_version = 0
+        # plan of float32 inputs: "f32" or "bf16" (bf16 matrix-core inputs, FP32 accumulation)
+        self.float32_plan = "f32"
 
     def compute_space_eigendecomposition(self, space_quadrule_list, space_table_dirichlet):
         table = np.asarray(space_table_dirichlet, dtype=bool)
@@ -138,9 +140,11 @@ class SpaceFD:
         return np.array(self.nnz, dtype=np.int64), lo, ln
 
     def apply_device(self, x, out=None, nrhs: int = 1, stream=None):
-        """``out = P x`` for device arrays ((N, nrhs) C order), MFMA mode products."""
+        """``out = P x`` for device arrays ((N, nrhs) C order), MFMA mode products;
+        float32 vectors use the plan named by ``float32_plan`` ("f32", or "bf16": bf16
+        matrix-core inputs with FP32 accumulation)."""
         t = dv.torch()
-        dtype = "f32" if x.dtype == t.float32 else "f64"
+        dtype = self.float32_plan if x.dtype == t.float32 else "f64"
         if out is None:
             out = t.empty_like(x)
         return self.plan(dtype).apply(x, out, nrhs, stream)

@@ -34,6 +34,7 @@ extern "C" {
 
 #define IGA_F64 0
 #define IGA_F32 1
+#define IGA_BF16 2 /* FD plans only: float vectors, BF16 matrix-core inputs, FP32 accumulation */
 
 /* ---- library ------------------------------------------------------------- */
 
@@ -286,7 +287,13 @@ int iga_fused_pack_strided(iga_fused_t plan, const double* C, const double* m, c
  * U = U_ndim (x) ... (x) U_1.  The free box of direction d is [lo[d], lo[d]+len[d])
  * of n[d] DOFs (the Dirichlet trimming of space_precond.py:160-171); U[d] is the
  * host float64 row-major len[d] x len[d] eigenvector matrix of eigh(K_d, M_d)
- * (space_precond.py:175).  Mode products run as MFMA GEMM tiles in the plan dtype. */
+ * (space_precond.py:175).  Mode products run on the matrix cores in the plan dtype:
+ * for three directions in three launches (gather and directions 1-2; direction 3
+ * with the Lambda^{-1} scaling; directions 1-2 back and the scatter) when the box
+ * lengths are at most 192 and a plane fits the 160 KiB of LDS (FP64: up to about
+ * 132), otherwise one GEMM launch per mode product.  dtype IGA_BF16 takes float
+ * vectors and rounds every matrix-core input to bf16, accumulating in FP32 (three
+ * directions only). */
 typedef struct iga_fd_s* iga_fd_t;
 
 int iga_fd_create(iga_fd_t* out, int ndim, const int64_t* n, const int64_t* lo, const int64_t* len,

@@ -5,9 +5,9 @@
   and the 16-right-hand-side launch against the oracle, and FD-preconditioned GMRES
   of the 16 right-hand sides to a true relative residual of 1e-10.
 * Mixed precision: the same geometry at degree 2, 4 and 6 with 128 elements per
-  direction (up to 134^3 = 2.4 M DOF): the float32 operator and float32 FD against
-  the float64 oracle, and one float64 iterative-refinement solve per degree to a
-  true 1e-10.
+  direction (up to 134^3 = 2.4 M DOF): the float32 operator and the float32 and
+  BF16-input FD against the float64 oracle, and float64 iterative-refinement solves per
+  degree (FP32 FD, then BF16 FD) to a true 1e-10.
 * Space-time heat: cube degree 3 x 32 elements by time degree 2 x 64 elements
   (2.83 M DOF): the arrowhead FD inverts the Gauss-rule operator, which the WQ
   operator equals for unit diffusivity, so ``A P b = b`` on the free nodes; GCRODR
@@ -132,6 +132,19 @@ def test_mixed_precision_n128_full_size(p):
     say(f"  {out.iterations} inner iterations in {len(out.history['inner_iterations'])} refinements, "
         f"true relative residual {tr:.2e}")
     assert out.success and tr <= 1e-10
+    say("BF16-input FD (bf16 eigenvectors, split vector operands, FP32 accumulation)")
+    sp = fd.space_preconditioner
+    sp.float32_plan = "bf16"
+    fb = rel(fd.apply(xd.float()).double().cpu().numpy(), pref)
+    say(f"  FD relative error bf16 {fb:.2e}")
+    assert fb <= 1e-2
+    out16 = mixed_solve(lambda v, o: A64.apply(v, out=o), lambda v, o: A32.apply(v, out=o),
+                        lambda v, o: fd.apply_device(v, out=o), b, tol=1e-10, inner_tol=1e-6, mask=mask)
+    tr16 = true_residual(A64, mask, b, out16.solution)
+    say(f"  with the BF16 FD: {out16.iterations} inner iterations in {len(out16.history['inner_iterations'])} "
+        f"refinements, true relative residual {tr16:.2e}")
+    assert out16.success and tr16 <= 1e-10
+    sp.float32_plan = "f32"
 
 
 def test_spacetime_heat_full_size():

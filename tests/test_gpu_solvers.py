@@ -377,3 +377,43 @@ def test_device_step_gmres_warm_start_matches_oracle():
     assert out.success and out.reason == "relative"
     assert abs(out.iterations - ref.iterations) <= 1
     assert rel(out.solution.cpu().numpy(), ref.solution) <= 1e-8
+
+
+@pytest.mark.parametrize("p,nel", [(3, 6), (2, 9), (4, 20)])
+def test_fd_bf16_apply_against_oracle(p, nel):
+    """BF16 matrix-core inputs with FP32 accumulation (eigenvectors rounded to bf16,
+    vector operands split into bf16 high and low parts): within 1e-2 of the float64
+    oracle apply (the eigenvector rounding, 2^-9 relative per entry), one and several
+    right-hand sides; FP32 and FP64 plans of the same three-launch path to their own
+    precision."""
+    import torch
+
+    from igaops_amd.fastdiag import SingleFD
+
+    patch = unit_cube(p, nel)
+    qs = patch.quadrule_list
+    tab = np.ones((1, 3, 2), bool)
+    fd = SingleFD("spatial")
+    fd.compute_space_eigendecomposition(qs, tab)
+    fd.update_space_eigenvalues([0.5, 1.0])
+    sp = fd.space_preconditioner
+    o = ofd.SpaceFD([StandardGauss(q.degree, q.knotvector, "legendre") for q in qs], tab)
+    o.scalar_coefs = (0.5, 1.0)
+    n = patch.nbctrlpts_total
+    X = np.random.default_rng(13).standard_normal((n, 3))
+    ref = o.apply(X)
+    Xd = torch.as_tensor(X, device="cuda")
+    sp.float32_plan = "bf16"
+    y = sp.apply_device(Xd.float().reshape(-1).contiguous(), nrhs=3).reshape(n, 3).double().cpu().numpy()
+    err = rel(y, ref)
+    print("bf16 FD relative error", err, flush=True)
+    assert err <= 1e-2
+    y1 = sp.apply_device(Xd[:, 1].float().contiguous()).double().cpu().numpy()
+    assert rel(y1, ref[:, 1]) <= 1e-2
+    free = np.zeros(n, bool)
+    free[o.free_nodes()] = True
+    assert np.all(y[~free] == 0)
+    sp.float32_plan = "f32"
+    assert rel(sp.apply_device(Xd.float().reshape(-1).contiguous(), nrhs=3).reshape(n, 3).double().cpu().numpy(),
+               ref) <= 1e-5
+    assert rel(sp.apply_device(Xd.reshape(-1).contiguous(), nrhs=3).reshape(n, 3).cpu().numpy(), ref) <= 1e-12
