@@ -41,9 +41,10 @@ __host__ __device__ constexpr int form_ncoef(int f) {
 }
 
 constexpr int kTQ1 = 16, kTQ2 = 16, kThreads = kTQ1 * kTQ2;
-// Max DOF footprint of a 16-point tile in directions 1, 2: 16 points span at most 9
-// elements for rules with >= 2 points per element, so p + 10 <= 17 DOFs for p <= 7.
-constexpr int kDMax = 20;
+// Max DOF footprint of a 16-point tile in directions 1, 2 (the plane products are
+// 16 x 16): 16 points of rules with >= 3 points per element (WQ type "2", Gauss with
+// p >= 2) span at most 7 elements, so at most p + 7 <= 14 DOFs for p <= 7.
+constexpr int kDMax = 16;
 constexpr int kKMax = 4;   // max tiles/slabs whose footprint contains one DOF index
 constexpr int kSlabMax = 128;  // max q3 planes of one slab (direction-3 table staged in LDS)
 
@@ -120,28 +121,68 @@ __device__ inline void point_contract(const T* __restrict__ c, int64_t stride, c
   }
 }
 
+// 16x16 products on the matrix cores for the plane contractions.  Lane layout of the
+// 16x16x4 MFMA (measured on gfx950): A operand A[l % 16][k0 + l / 16], B operand
+// B[k0 + l / 16][l % 16]; result register r of lane l holds row (l / 16) + 4r for FP64
+// and 4 (l / 16) + r for FP32, column l % 16.
+template <typename T>
+struct PlaneMma;
+template <>
+struct PlaneMma<double> {
+  typedef double V __attribute__((ext_vector_type(4)));
+  __device__ static V mma(double a, double b, V c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+  __device__ static int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+template <>
+struct PlaneMma<float> {
+  typedef float V __attribute__((ext_vector_type(4)));
+  __device__ static V mma(float a, float b, V c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+  __device__ static int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
+// acc += A B over K = 16: A row-major [16][lda] (rows M, columns K), B row-major
+// [16][ldb] (rows K, columns N); four dependent MFMAs.
+template <typename T>
+__device__ inline typename PlaneMma<T>::V mm16(const T* A, int lda, const T* B, int ldb, typename PlaneMma<T>::V acc,
+                                               int lane) {
+  const int i = lane & 15, g = lane >> 4;
+#pragma unroll
+  for (int k0 = 0; k0 < 16; k0 += 4) acc = PlaneMma<T>::mma(A[i * lda + k0 + g], B[(k0 + g) * ldb + i], acc);
+  return acc;
+}
+
+template <typename T>
+__device__ inline void st16(T* C, int ldc, typename PlaneMma<T>::V acc, int lane) {
+#pragma unroll
+  for (int r = 0; r < 4; ++r) C[PlaneMma<T>::row(lane, r) * ldc + (lane & 15)] = acc[r];
+}
+
 template <typename T, int P, int FORM>
 __device__ __attribute__((always_inline)) inline void fused_apply_body(const FusedArgs<T>& a) {
+  using MM = PlaneMma<T>;
+  using V = typename MM::V;
   // prefetch depth (q3 planes) of the coefficient stream
   constexpr int PD = sizeof(T) == 8 ? 2 : 3;
-  // Unrolling the band-restricted contraction loops lets their LDS reads overlap; it
-  // pays where registers allow (measured: FP32, and FP64 from P = 6, whose instances
-  // run at two waves per SIMD anyway), and costs the FP64 P <= 5 instances their
-  // third wave.
-  constexpr bool kUnrollBand = sizeof(T) == 4 || P >= 6;
   constexpr bool DUAL = FORM == FORM_KM7X;
-  __shared__ T xs[kDMax][kDMax + 1];
-  __shared__ T xs1[DUAL ? kDMax : 1][kDMax + 1];
-  __shared__ T t1s[DUAL ? 3 : 2][kDMax][kTQ1];
-  __shared__ T s2s[3][kTQ2][kTQ1];
-  __shared__ T r1s[2][kDMax][kTQ1];
-  __shared__ T b1s[4][kTQ1][P];
-  __shared__ T b2s[4][kTQ2][P];
-  __shared__ int f1s[kTQ1];
-  __shared__ int f2s[kTQ2];
-  __shared__ int l1r[kDMax][2];  // tile points l whose direction-1 band holds DOF c1l: [lo, hi)
-  __shared__ int m2r[kDMax][2];  // same for direction 2
-  __shared__ T b3s[kSlabMax][4][P];  // direction-3 band values of the slab's planes
+  constexpr int NT1 = DUAL ? 3 : 2;  // direction-1 trial partials: B0 x, B1 x (, B0 x1)
+  constexpr int NT2 = DUAL ? 4 : 3;  // trial plane fields
+  constexpr int AP = 17;             // pitch of arrays read as the A operand (rows over lanes)
+  // The footprint block of a DOF plane in LDS, zero outside D1 x D2 (set once)
+  __shared__ T xs[kTQ2][AP];
+  __shared__ T xs1[DUAL ? kTQ2 : 1][AP];
+  // dense band matrices of the tile (zero outside the band and past the tile):
+  //   bd1[f][c1][l1] = B_f of direction 1 at point l1, DOF c1   (B operand, K = c1)
+  //   bd2[f][l2][c2] = B_f of direction 2                        (A operand, K = c2)
+  //   wd2[h][c2][m]  = W_h of direction 2 at point m, DOF c2      (A operand, K = m)
+  //   wd1[h][l][c1]  = W_h of direction 1                        (B operand, K = l)
+  __shared__ T bd1[2][16][16];
+  __shared__ T bd2[2][16][AP];
+  __shared__ T wd2[2][16][AP];
+  __shared__ T wd1[2][16][16];
+  __shared__ T s2s[NT2 > 3 ? NT2 : 3][16][16];  // test values [m][l], then the trial plane [l2][l1]
+  __shared__ T t1s[NT1][16][16];                // direction-1 trial partials [c2][l1]
+  __shared__ T r1s[2][16][AP];                  // direction-2 test partials [c2][l]
+  __shared__ T b3s[kSlabMax][4][P];             // direction-3 band values of the slab's planes
 
   const int L = xcd_logical(blockIdx.x, gridDim.x);
   if (L >= a.nlogical) return;  // padding blocks of the XCD remap (whole block exits)
@@ -155,6 +196,7 @@ __device__ __attribute__((always_inline)) inline void fused_apply_body(const Fus
   const int tile = t2 * a.nt1 + t1;
 
   const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
   const int l1 = tid % kTQ1, l2 = tid / kTQ1;
   const int q1s = a.t1_q0[t1], q1n = a.t1_q0[t1 + 1] - q1s;
   const int q2s = a.t2_q0[t2], q2n = a.t2_q0[t2 + 1] - q2s;
@@ -165,50 +207,49 @@ __device__ __attribute__((always_inline)) inline void fused_apply_body(const Fus
   const bool active = (l1 < q1n) && (l2 < q2n);
   const int n1 = a.nc[0], n2 = a.nc[1];
 
-  // stage the direction-1 and -2 tables of this tile
-  for (int i = tid; i < 4 * kTQ1 * P; i += kThreads) {
-    const int m = i / (kTQ1 * P), l = (i / P) % kTQ1, j = i % P;
-    b1s[m][l][j] = (l < q1n) ? a.d[0].tab[((int64_t)m * a.nq[0] + q1s + l) * P + j] : T(0);
-  }
-  for (int i = tid; i < 4 * kTQ2 * P; i += kThreads) {
-    const int m = i / (kTQ2 * P), l = (i / P) % kTQ2, j = i % P;
-    b2s[m][l][j] = (l < q2n) ? a.d[1].tab[((int64_t)m * a.nq[1] + q2s + l) * P + j] : T(0);
-  }
-  if (tid < kTQ1) f1s[tid] = (tid < q1n) ? a.d[0].first[q1s + tid] - c1lo : 0;
-  if (tid < kTQ2) f2s[tid] = (tid < q2n) ? a.d[1].first[q2s + tid] - c2lo : 0;
-  // per DOF index of the footprint, the range of tile points whose band holds it
-  // (band starts are non-decreasing, so the range is contiguous)
-  if (tid < kDMax) {
-    int lo = 0, hi = 0;
-    if (tid < D1) {
-      lo = q1n;
-      for (int l = q1n - 1; l >= 0; --l) {
-        const int f = a.d[0].first[q1s + l] - c1lo;
-        if (f <= tid && tid < f + P) {
-          lo = l;
-          if (hi == 0) hi = l + 1;
-        }
-      }
-      if (lo == q1n) lo = hi = 0;
+  // dense band matrices of this tile: thread (row r = tid / 16, column c = tid % 16)
+  {
+    const int r = tid >> 4, c = tid & 15;
+    // direction 1, point c, DOF r (bd1) / point r, DOF c (wd1)
+    {
+      const int l = c, cd = r;
+      const int f = l < q1n ? a.d[0].first[q1s + l] - c1lo : -P;
+      const int j = cd - f;
+      const bool in = l < q1n && j >= 0 && j < P;
+      const int64_t o = (int64_t)(q1s + (l < q1n ? l : 0)) * P + (in ? j : 0);
+      bd1[0][cd][l] = in ? a.d[0].tab[o] : T(0);
+      bd1[1][cd][l] = in ? a.d[0].tab[(int64_t)a.nq[0] * P + o] : T(0);
     }
-    l1r[tid][0] = lo;
-    l1r[tid][1] = hi;
-  } else if (tid >= 64 && tid < 64 + kDMax) {
-    const int c = tid - 64;
-    int lo = 0, hi = 0;
-    if (c < D2) {
-      lo = q2n;
-      for (int m = q2n - 1; m >= 0; --m) {
-        const int f = a.d[1].first[q2s + m] - c2lo;
-        if (f <= c && c < f + P) {
-          lo = m;
-          if (hi == 0) hi = m + 1;
-        }
-      }
-      if (lo == q2n) lo = hi = 0;
+    {
+      const int l = r, cd = c;
+      const int f = l < q1n ? a.d[0].first[q1s + l] - c1lo : -P;
+      const int j = cd - f;
+      const bool in = l < q1n && j >= 0 && j < P;
+      const int64_t o = (int64_t)(q1s + (l < q1n ? l : 0)) * P + (in ? j : 0);
+      wd1[0][l][cd] = in ? a.d[0].tab[2 * (int64_t)a.nq[0] * P + o] : T(0);
+      wd1[1][l][cd] = in ? a.d[0].tab[3 * (int64_t)a.nq[0] * P + o] : T(0);
     }
-    m2r[c][0] = lo;
-    m2r[c][1] = hi;
+    // direction 2, point r, DOF c (bd2) / point c, DOF r (wd2)
+    {
+      const int m = r, cd = c;
+      const int f = m < q2n ? a.d[1].first[q2s + m] - c2lo : -P;
+      const int j = cd - f;
+      const bool in = m < q2n && j >= 0 && j < P;
+      const int64_t o = (int64_t)(q2s + (m < q2n ? m : 0)) * P + (in ? j : 0);
+      bd2[0][m][cd] = in ? a.d[1].tab[o] : T(0);
+      bd2[1][m][cd] = in ? a.d[1].tab[(int64_t)a.nq[1] * P + o] : T(0);
+    }
+    {
+      const int m = c, cd = r;
+      const int f = m < q2n ? a.d[1].first[q2s + m] - c2lo : -P;
+      const int j = cd - f;
+      const bool in = m < q2n && j >= 0 && j < P;
+      const int64_t o = (int64_t)(q2s + (m < q2n ? m : 0)) * P + (in ? j : 0);
+      wd2[0][cd][m] = in ? a.d[1].tab[2 * (int64_t)a.nq[1] * P + o] : T(0);
+      wd2[1][cd][m] = in ? a.d[1].tab[3 * (int64_t)a.nq[1] * P + o] : T(0);
+    }
+    xs[r][c] = T(0);
+    if constexpr (DUAL) xs1[r][c] = T(0);
   }
   for (int i = tid; i < (q3e - q3s) * 4 * P; i += kThreads) {
     const int lq = i / (4 * P), m = (i / P) % 4, j = i % P;
@@ -216,9 +257,8 @@ __device__ __attribute__((always_inline)) inline void fused_apply_body(const Fus
   }
   __syncthreads();
 
-  constexpr int NTR = DUAL ? 4 : 3;
-  T tr[NTR][P];  // trial partials of the window planes: (val,val), (d1,val), (val,d2) [, (val,val) of x1]
-  T te[3][P];  // test partials: (val,val), (d1,val), (val,d2) in directions (1,2)
+  T tr[NT2][P];  // trial partials of the window planes: (val,val), (d1,val), (val,d2) [, (val,val) of x1]
+  T te[3][P];    // test partials: (val,val), (d1,val), (val,d2) in directions (1,2)
 
   // x block of one DOF plane, held in registers between its (early) load and its
   // use, so that the load latency overlaps the work of the current step.
@@ -254,13 +294,9 @@ __device__ __attribute__((always_inline)) inline void fused_apply_body(const Fus
     T x[XR], w[XR], x1[XR];
   };
   using XBlock = std::conditional_t<DUAL, XBlock2, XBlock1>;
-  struct Trial1 {
-    T v00, v10, v01;
+  struct Trial {
+    T v[NT2];
   };
-  struct Trial2 {
-    T v00, v10, v01, vm;
-  };
-  using Trial = std::conditional_t<DUAL, Trial2, Trial1>;
   auto load_x = [&](int c3) __attribute__((always_inline)) {
     XBlock b;
 #pragma unroll
@@ -276,9 +312,12 @@ __device__ __attribute__((always_inline)) inline void fused_apply_body(const Fus
   // One LDS phase of the sweep.  It may (a) contract the oldest test plane (ring
   // slot 0, DOF plane c3f) in directions 2 and 1 into the tile's partial block and
   // (b) build the trial partials of a new DOF plane from an x block held in
-  // registers.  Both contractions share the phase's three barriers.
+  // registers.  Every contraction is a 16x16 product on the matrix cores:
+  //   stage 1: T1_f = X Bd1_f (per trial field f), R1_0 = Wd2_0 S_0 + Wd2_1 S_2,
+  //            R1_1 = Wd2_0 S_1
+  //   stage 2: T2 = Bd2_a T1_b (the trial plane fields), partial = R1_0 Wd1_0 + R1_1 Wd1_1
+  // with the products spread over the four waves.
   T* part = a.partial + (int64_t)k * a.part_stride + a.part_base[(int64_t)tile * a.nslab + s];
-  T* xsf = &xs[0][0];
   auto plane_phase = [&](bool do_flush, int c3f, bool do_trial, XBlock xb) __attribute__((always_inline)) {
     Trial out{};
     __syncthreads();  // LDS buffers are reused
@@ -291,88 +330,61 @@ __device__ __attribute__((always_inline)) inline void fused_apply_body(const Fus
 #pragma unroll
       for (int e = 0; e < XR; ++e) {
         if (xij[e] >= 0) {
-          const int slot = (xij[e] >> 8) * (kDMax + 1) + (xij[e] & 255);
-          xsf[slot] = nurbs ? xb.x[e] * xb.w[e] : xb.x[e];
-          if constexpr (DUAL) (&xs1[0][0])[slot] = nurbs ? xb.x1[e] * xb.w[e] : xb.x1[e];
+          const int i2 = xij[e] >> 8, i1 = xij[e] & 255;
+          xs[i2][i1] = nurbs ? xb.x[e] * xb.w[e] : xb.x[e];
+          if constexpr (DUAL) xs1[i2][i1] = nurbs ? xb.x1[e] * xb.w[e] : xb.x1[e];
         }
       }
     }
     __syncthreads();
-    // stage A: direction-2 test contraction (R1) and direction-1 trial contraction (T1)
-    const int na = do_flush ? D2 * kTQ1 : 0;
-    const int nb = do_trial ? D2 * kTQ1 : 0;
-    for (int it = tid; it < na + nb; it += kThreads) {
-      if (it < na) {
-        const int c2l = it / kTQ1, l = it % kTQ1;
-        T r0 = T(0), r1 = T(0);
-        const int mhi = m2r[c2l][1];
-        auto r1_term = [&](int m) __attribute__((always_inline)) {
-          const int j = c2l - f2s[m];
-          r0 += b2s[2][m][j] * s2s[0][m][l] + b2s[3][m][j] * s2s[2][m][l];
-          r1 += b2s[2][m][j] * s2s[1][m][l];
-        };
-        if constexpr (kUnrollBand) {
-#pragma unroll 4
-          for (int m = m2r[c2l][0]; m < mhi; ++m) r1_term(m);
-        } else {
-          for (int m = m2r[c2l][0]; m < mhi; ++m) r1_term(m);
-        }
-        r1s[0][c2l][l] = r0;
-        r1s[1][c2l][l] = r1;
-      } else {
-        const int i2 = (it - na) / kTQ1, l = (it - na) % kTQ1;
-        const int f = f1s[l];
-        T acc0 = T(0), acc1 = T(0), acc2 = T(0);
-        if (l < q1n) {
-#pragma unroll
-          for (int j = 0; j < P; ++j) {
-            const T xv = xs[i2][f + j];
-            acc0 += b1s[0][l][j] * xv;
-            acc1 += b1s[1][l][j] * xv;
-            if constexpr (DUAL) acc2 += b1s[0][l][j] * xs1[i2][f + j];
-          }
-        }
-        t1s[0][i2][l] = acc0;
-        t1s[1][i2][l] = acc1;
-        if constexpr (DUAL) t1s[DUAL ? 2 : 0][i2][l] = acc2;
+    // stage 1
+    if (do_trial) {
+      if (wave < NT1) {
+        const T* X = (DUAL && wave == 2) ? &xs1[0][0] : &xs[0][0];
+        const int f = wave == 1 ? 1 : 0;
+        V acc = mm16<T>(X, AP, &bd1[f][0][0], 16, V{0, 0, 0, 0}, lane);
+        st16<T>(&t1s[wave][0][0], 16, acc, lane);
       }
     }
-    __syncthreads();
-    // stage B: direction-1 test contraction into the partial block, and the
-    // direction-2 trial contraction of this thread's column
     if (do_flush) {
-      T* blk = part + (int64_t)(c3f - c3lo) * D2 * D1;
-#pragma unroll
-      for (int e = 0; e < XR; ++e) {
-        if (xij[e] >= 0) {
-          const int c2l = xij[e] >> 8, c1l = xij[e] & 255;
-          T acc = T(0);
-          const int lhi = l1r[c1l][1];
-          auto flush_term = [&](int l) __attribute__((always_inline)) {
-            const int j = c1l - f1s[l];
-            acc += b1s[2][l][j] * r1s[0][c2l][l] + b1s[3][l][j] * r1s[1][c2l][l];
-          };
-          if constexpr (kUnrollBand) {
-#pragma unroll 4
-            for (int l = l1r[c1l][0]; l < lhi; ++l) flush_term(l);
-          } else {
-            for (int l = l1r[c1l][0]; l < lhi; ++l) flush_term(l);
-          }
-          blk[tid + e * kThreads] = acc;
-        }
+      if (wave == 3) {
+        V acc = mm16<T>(&wd2[0][0][0], AP, &s2s[0][0][0], 16, V{0, 0, 0, 0}, lane);
+        acc = mm16<T>(&wd2[1][0][0], AP, &s2s[2][0][0], 16, acc, lane);
+        st16<T>(&r1s[0][0][0], AP, acc, lane);
+      } else if (wave == 2) {
+        V acc = mm16<T>(&wd2[0][0][0], AP, &s2s[1][0][0], 16, V{0, 0, 0, 0}, lane);
+        st16<T>(&r1s[1][0][0], AP, acc, lane);
       }
     }
-    if (do_trial && active) {
-      const int f = f2s[l2];
-#pragma unroll
-      for (int j = 0; j < P; ++j) {
-        const T b0 = b2s[0][l2][j], b1 = b2s[1][l2][j];
-        const T a0 = t1s[0][f + j][l1], a1 = t1s[1][f + j][l1];
-        out.v00 += b0 * a0;
-        out.v10 += b0 * a1;
-        out.v01 += b1 * a0;
-        if constexpr (DUAL) out.vm += b0 * t1s[DUAL ? 2 : 0][f + j][l1];
+    __syncthreads();
+    // stage 2
+    if (do_trial) {
+      // fields: (B0, T1_0), (B0, T1_1), (B1, T1_0) [, (B0, T1_2)]
+      auto field = [&](int g) __attribute__((always_inline)) {
+        const int fa = g == 2 ? 1 : 0, fb = g == 1 ? 1 : g == 3 ? 2 : 0;
+        V acc = mm16<T>(&bd2[fa][0][0], AP, &t1s[fb][0][0], 16, V{0, 0, 0, 0}, lane);
+        st16<T>(&s2s[g][0][0], 16, acc, lane);
+      };
+      if (wave < 3) field(wave);
+      if constexpr (DUAL) {
+        if (wave == 0) field(3);
       }
+    }
+    if (do_flush && wave == 3) {
+      V acc = mm16<T>(&r1s[0][0][0], AP, &wd1[0][0][0], 16, V{0, 0, 0, 0}, lane);
+      acc = mm16<T>(&r1s[1][0][0], AP, &wd1[1][0][0], 16, acc, lane);
+      T* blk = part + (int64_t)(c3f - c3lo) * D2 * D1;
+      const int c1 = lane & 15;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int c2 = MM::row(lane, r);
+        if (c1 < D1 && c2 < D2) blk[c2 * D1 + c1] = acc[r];
+      }
+    }
+    if (do_trial) {
+      __syncthreads();
+#pragma unroll
+      for (int g = 0; g < NT2; ++g) out.v[g] = active ? s2s[g][l2][l1] : T(0);
     }
     return out;
   };
@@ -388,10 +400,8 @@ __device__ __attribute__((always_inline)) inline void fused_apply_body(const Fus
 #pragma unroll
     for (int j = 0; j < P; ++j) {
       const Trial t = plane_phase(false, 0, true, xw[j]);
-      tr[0][j] = t.v00;
-      tr[1][j] = t.v10;
-      tr[2][j] = t.v01;
-      if constexpr (DUAL) tr[NTR - 1][j] = t.vm;
+#pragma unroll
+      for (int g = 0; g < NT2; ++g) tr[g][j] = t.v[g];
       te[0][j] = te[1][j] = te[2][j] = T(0);
     }
   }
@@ -408,10 +418,14 @@ __device__ __attribute__((always_inline)) inline void fused_apply_body(const Fus
   // and no register with a load in flight is ever copied.
   T cbuf[PD][NCF];
   int fbuf[PD];
+  // The coefficient stream is loaded non-temporally (it is read once per launch when
+  // there is one right-hand side): 8% faster on the p=4 n=64 sweep, 5% slower when 16
+  // right-hand sides share one stream through the L2.  A uniform run-time choice
+  // between the two load forms lost the gain (the branch serialises the loads).
   auto load_step = [&](int q3, T (&c)[NCF], int& fb) __attribute__((always_inline)) {
     const T* p = cptr + (int64_t)q3 * step_stride;
 #pragma unroll
-    for (int f = 0; f < NCF; ++f) c[f] = p[f * npts];
+    for (int f = 0; f < NCF; ++f) c[f] = __builtin_nontemporal_load(p + f * npts);
     fb = first3[q3];
   };
 #pragma unroll
@@ -426,13 +440,11 @@ __device__ __attribute__((always_inline)) inline void fused_apply_body(const Fus
 #pragma unroll
       for (int g = 0; g < 3; ++g) te[g][j] = te[g][j + 1];
 #pragma unroll
-      for (int g = 0; g < NTR; ++g) tr[g][j] = tr[g][j + 1];
+      for (int g = 0; g < NT2; ++g) tr[g][j] = tr[g][j + 1];
     }
     te[0][P - 1] = te[1][P - 1] = te[2][P - 1] = T(0);
-    tr[0][P - 1] = t.v00;
-    tr[1][P - 1] = t.v10;
-    tr[2][P - 1] = t.v01;
-    if constexpr (DUAL) tr[NTR - 1][P - 1] = t.vm;
+#pragma unroll
+    for (int g = 0; g < NT2; ++g) tr[g][P - 1] = t.v[g];
     ++cur;
     // unconditional (clamped plane) so that the load queue has the same shape on every path
     xnext = load_x(cur + P <= c3hi ? cur + P : c3hi);
@@ -453,7 +465,7 @@ __device__ __attribute__((always_inline)) inline void fused_apply_body(const Fus
       T u[4] = {T(0), T(0), T(0), T(0)};
 #pragma unroll
       for (int j = 0; j < P; ++j) {
-        u[0] += tb[0][j] * tr[DUAL ? NTR - 1 : 0][j];  // the value: from x1 for FORM_KM7X
+        u[0] += tb[0][j] * tr[DUAL ? NT2 - 1 : 0][j];  // the value: from x1 for FORM_KM7X
         u[1] += tb[0][j] * tr[1][j];
         u[2] += tb[0][j] * tr[2][j];
         u[3] += tb[1][j] * tr[0][j];
