@@ -26,7 +26,6 @@ from typing import Callable, List, Literal, Optional, Sequence, Tuple
 import numpy as np
 from scipy import sparse as sp
 from scipy.optimize import Bounds, LinearConstraint, milp
-from scipy.sparse.linalg import lsqr
 
 from .common import Constants, validate_entry
 
@@ -285,32 +284,27 @@ def compute_quadrature_layout(
     degree_target: int,
 ) -> np.ndarray:
     """Upper-triangular table ``S4[r, s]`` of the minimum number of points needed in
-    elements ``r..s`` (steps S1-S4 of ``wq_helpers.py:11-82``)."""
+    elements ``r..s`` (steps S1-S4 of ``wq_helpers.py:11-82``).
+
+    For test function i, ``mu_i[r, s]`` counts the target functions whose common
+    elements with it are exactly ``r..s``, and the number of targets confined to
+    ``r..s`` is ``sum_{r <= k <= l <= s} mu_i[k, l]``: since ``mu_i`` vanishes below the
+    diagonal this is a two-dimensional prefix sum (a reversed cumulative sum over k, a
+    cumulative sum over l), computed for all test functions at once.  ``S4`` is its
+    maximum over the test functions."""
     kv_test = np.asarray(knot_vector_test, dtype=float)
     kv_target = np.asarray(knot_vector_target, dtype=float)
     breaks = np.unique(kv_test)
     nel = len(breaks) - 1
-    sup_test = _element_supports(kv_test, degree_test, breaks)
-    sup_target = _element_supports(kv_target, degree_target, breaks)
-    S4 = np.zeros((nel, nel), dtype=np.int64)
-    for a, b in sup_test:
-        mu = np.zeros((nel, nel), dtype=np.int64)
-        for c, d in sup_target:
-            r, s = max(a, c), min(b, d)
-            if r <= s:
-                mu[r, s] += 1
-        # mu_bar(r, s) = sum_{r<=k<=l<=s} mu(k, l)
-        col = mu.cumsum(axis=0)
-        mb = np.zeros_like(mu)
-        for r in range(nel):
-            run = mu[r, r]
-            mb[r, r] = run
-            for s in range(r + 1, nel):
-                cs = col[s, s] - (col[r - 1, s] if r > 0 else 0)
-                run += cs
-                mb[r, s] = run
-        np.maximum(S4, mb, out=S4)
-    return S4
+    sup_test = np.asarray(_element_supports(kv_test, degree_test, breaks), dtype=np.int64)
+    sup_target = np.asarray(_element_supports(kv_target, degree_target, breaks), dtype=np.int64)
+    r = np.maximum(sup_test[:, None, 0], sup_target[None, :, 0])
+    s_ = np.minimum(sup_test[:, None, 1], sup_target[None, :, 1])
+    i_idx, j_idx = np.nonzero(r <= s_)
+    mu = np.zeros((len(sup_test), nel, nel), dtype=np.int64)
+    np.add.at(mu, (i_idx, r[i_idx, j_idx], s_[i_idx, j_idx]), 1)
+    mb = np.flip(np.cumsum(np.flip(mu, axis=1), axis=1), axis=1).cumsum(axis=2)
+    return mb.max(axis=0)
 
 
 def convert_to_greedy_input(S4: np.ndarray):
@@ -359,11 +353,18 @@ def _target_degree_and_knots(degree: int, knotvector: np.ndarray, method: str):
 
 def eval_nbquadpts_per_element(degree: int, knotvector: np.ndarray, quadrature_type: str) -> np.ndarray:
     """Points per element (``weighted_quad.py:207-235``), including the reference's
-    ``+1`` stability margin before the covering check."""
+    ``+1`` stability margin before the covering check.  The check ``A lb >= b`` of
+    ``greedy_min_norm1`` is evaluated from cumulative sums (the sum of ``lb`` over
+    elements ``r..s`` against ``S4[r, s]``); the constraint matrix is built only when
+    it fails and the MILP is needed."""
     deg_t, kv_t = _target_degree_and_knots(degree, knotvector, quadrature_type)
     S4 = compute_quadrature_layout(knotvector, kv_t, degree, deg_t)
-    lb, A, b = convert_to_greedy_input(S4)
-    lb = lb + 1
+    lb = np.diag(S4).copy() + 1
+    c = np.concatenate([[0], np.cumsum(lb)])
+    span = c[None, 1:] - c[:-1, None]  # span[r, s] = sum(lb[r..s]) for r <= s
+    if np.all(np.triu(span - S4, 1) >= 0):
+        return lb.astype(int)
+    _, A, b = convert_to_greedy_input(S4)
     return greedy_min_norm1(lb, A, b).astype(int)
 
 
@@ -400,12 +401,37 @@ def _knot_support(quadpts: np.ndarray, unique_kv: np.ndarray) -> np.ndarray:
 
 
 def solve_min_weighted_norm(Z: np.ndarray, A: np.ndarray, B: np.ndarray) -> np.ndarray:
-    """Row-wise ``min ||diag(z_i)^{-1} w_i||`` s.t. ``A w_i = b_i``, written as
-    ``w_i = z_i * lsqr(A diag(z_i), b_i)`` (``quadrature/utils/operations.py:107-143``)."""
-    tol = Constants.SAFEGUARD
+    """Row-wise ``min ||diag(z_i)^{-1} w_i||`` s.t. ``A w_i = b_i``, i.e.
+    ``w_i = z_i * u_i`` with ``u_i`` the minimum-norm solution of ``A diag(z_i) u = b_i``
+    (the reference runs ``lsqr`` on it, ``quadrature/utils/operations.py:107-143``).
+
+    Only the columns where ``z_i`` is nonzero (a contiguous run: the support of test
+    function i) and the rows of ``A`` that touch them (contiguous: the overlapping
+    target functions) take part; the other rows are ``0 = b`` and the other entries of
+    ``u_i`` are zero in the minimum-norm solution.  The small dense systems are solved
+    with ``numpy.linalg.pinv``, batched over rows of equal block shape."""
+    n, m = Z.shape
     out = np.zeros_like(Z)
-    for i, (z, b) in enumerate(zip(Z, B)):
-        out[i] = z * lsqr(A * z, b, atol=tol, btol=tol)[0]
+    nzA = A != 0
+    groups = {}
+    for i in range(n):
+        cols = np.flatnonzero(Z[i])
+        if cols.size == 0:
+            continue
+        c0, c1 = int(cols[0]), int(cols[-1]) + 1
+        touch = np.flatnonzero(nzA[:, c0:c1].any(axis=1))
+        if touch.size == 0:
+            continue
+        r0, r1 = int(touch[0]), int(touch[-1]) + 1
+        groups.setdefault((r1 - r0, c1 - c0), []).append((i, r0, c0))
+    for (nr, nc), items in groups.items():
+        idx = np.array([it[0] for it in items])
+        r0 = np.array([it[1] for it in items])[:, None] + np.arange(nr)
+        c0 = np.array([it[2] for it in items])[:, None] + np.arange(nc)
+        z = Z[idx[:, None], c0]
+        M = A[r0[:, :, None], c0[:, None, :]] * z[:, None, :]
+        u = np.einsum("gij,gj->gi", np.linalg.pinv(M), B[idx[:, None], r0])
+        out[idx[:, None], c0] = z * u
     return out
 
 

@@ -63,16 +63,43 @@ def test_wq_integrals_match_gauss(position_rule, quadtype):
 
 
 def test_min_norm_weights_closed_form():
-    """lsqr solution == ``z o pinv(A diag z) b`` (the minimum weighted-norm solution)."""
+    """The weights are ``z o pinv(A diag z) b`` (the minimum weighted-norm solution) on
+    the full system, although each row is solved on the block of its nonzero columns and
+    the rows of ``A`` touching them: rows with different supports, a zero inside a
+    support, rows of ``A`` touching no column of a support, and a row of ``Z`` that is
+    zero."""
     rng = np.random.default_rng(0)
-    A = rng.standard_normal((5, 12))
-    Z = np.abs(rng.standard_normal((3, 12)))
-    Z[:, :3] = 0.0
-    Bm = rng.standard_normal((3, 5))
+    A = rng.standard_normal((7, 12))
+    A[0, 6:] = 0.0  # row 0 touches only columns 0-5
+    A[6, :9] = 0.0  # row 6 touches only columns 9-11
+    Z = np.abs(rng.standard_normal((5, 12)))
+    Z[:2, :3] = 0.0
+    Z[2, 6:] = 0.0
+    Z[3, 7] = 0.0
+    Z[4] = 0.0
+    Bm = rng.standard_normal((5, 7))
     got = solve_min_weighted_norm(Z, A, Bm)
-    for i in range(3):
+    for i in range(5):
         ref = Z[i] * (np.linalg.pinv(A * Z[i]) @ Bm[i])
-        np.testing.assert_allclose(got[i], ref, rtol=1e-8, atol=1e-10)
+        np.testing.assert_allclose(got[i], ref, rtol=1e-10, atol=1e-12)
+
+
+def test_min_norm_weights_match_lsqr_on_a_wq_system():
+    """On the (well-conditioned) value-weight system of a cubic rule, the batched
+    solution equals one ``lsqr`` per row on the full system, as the reference computes
+    it (``quadrature/utils/operations.py:107-143``)."""
+    from igaops_amd.common import Constants
+    from igaops_amd.quadrature import _knot_support
+
+    kv = make_open_knotvector(3, 8)
+    wq = WeightedQuadrature(3, kv, quadtype="2")
+    gs = StandardGauss(3, kv, "legendre")
+    A = gs.eval_basis(wq.quadpts)[0].T.toarray()
+    Z = (gs.eval_basis(wq.quadpts)[0].T.toarray() * _knot_support(wq.quadpts, gs.unique_kv))
+    Bm = (gs.weights[0] @ gs.basis[0]).toarray()
+    tol = Constants.SAFEGUARD
+    ref = np.stack([z * lsqr(A * z, b, atol=tol, btol=tol)[0] for z, b in zip(Z, Bm)])
+    np.testing.assert_allclose(solve_min_weighted_norm(Z, A, Bm), ref, rtol=0, atol=1e-12)
 
 
 def test_gauss_lobatto():
