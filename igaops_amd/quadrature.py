@@ -27,7 +27,7 @@ import numpy as np
 from scipy import sparse as sp
 from scipy.optimize import Bounds, LinearConstraint, milp
 
-from .common import Constants, validate_entry
+from .common import validate_entry
 
 logger = logging.getLogger(__name__)
 
