@@ -79,6 +79,15 @@ def to_device(a: Union[np.ndarray, "object"], dtype: str = "f64"):
     return t.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(device=device(), dtype=torch_dtype(dtype))
 
 
+def upload(a: np.ndarray):
+    """Host array to the device without waiting for the copy: staged through pinned
+    memory (torch's pinned-block cache keeps the block until the copy on the current
+    stream has finished), so the host does not wait behind the device work queued
+    ahead of the copy."""
+    t = torch()
+    return t.from_numpy(np.ascontiguousarray(a)).pin_memory().to(device(), non_blocking=True)
+
+
 def empty(n, dtype: str = "f64"):
     t = torch()
     shape = (n,) if np.isscalar(n) else tuple(n)

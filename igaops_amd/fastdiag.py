@@ -34,6 +34,23 @@ def rewrite_quadrature(quadrule_list) -> List[StandardGauss]:
     return out
 
 
+def pencil_eigenpairs(q, dirichlet_lo: bool, dirichlet_hi: bool):
+    """Free indices and generalized eigenpairs ``K U = M U diag(lam)`` of one direction
+    (``space_precond.py:124-193``): ``M``, ``K`` from the Gauss rule ``q``, rows and
+    columns of Dirichlet ends removed, ``K + 1e-8 M`` when no end is removed,
+    ``scipy.linalg.eigh`` (``U^T M U = I``)."""
+    M = (q.weights[0] @ q.basis[0]).toarray()
+    K = (q.weights[-1] @ q.basis[-1]).toarray()
+    n = q.nbctrlpts
+    idx = np.arange(1 if dirichlet_lo else 0, n - (1 if dirichlet_hi else 0))
+    Mi = M[np.ix_(idx, idx)]
+    Ki = K[np.ix_(idx, idx)]
+    if len(idx) == n:
+        Ki = Ki + Constants.TINY * Mi
+    lam, U = sla.eigh(Ki, Mi)
+    return idx, np.real(lam), np.real(U)
+
+
 class SpaceFD:
     """Spatial FD for one DOF per node (``nbDoFsPerNode = 1``, the scalar problems of
     the path; the reference's vector-valued loop over DOFs per node is the same apply
@@ -57,27 +74,28 @@ class SpaceFD:
             raise NotImplementedError("one DOF per node is supported")
         qs = rewrite_quadrature(space_quadrule_list)
         nbdirs = min(table.shape[1], len(qs))
-        self.nbDoFsPerNode = table.shape[0]
-        self.nnz = [q.nbctrlpts for q in qs[:nbdirs]]
-        self.space_dirs = [ParametricDirection(i) for i in range(nbdirs)]
-        self.indices: List[np.ndarray] = []
-        self.eigenvec_by_dir_space: List[np.ndarray] = []
-        self.eigenval_by_dir_space: List[np.ndarray] = []
+        solved = {}
+        pairs = []
         for d in range(nbdirs):
             q = qs[d]
-            M = (q.weights[0] @ q.basis[0]).toarray()
-            K = (q.weights[-1] @ q.basis[-1]).toarray()
-            lo = 1 if table[0, d, 0] else 0
-            hi = self.nnz[d] - (1 if table[0, d, 1] else 0)
-            idx = np.arange(lo, hi)
-            Mi = M[np.ix_(idx, idx)]
-            Ki = K[np.ix_(idx, idx)]
-            if len(idx) == self.nnz[d]:
-                Ki = Ki + Constants.TINY * Mi
-            lam, U = sla.eigh(Ki, Mi)
-            self.indices.append(idx)
-            self.eigenvec_by_dir_space.append(np.real(U))
-            self.eigenval_by_dir_space.append(np.real(lam))
+            # directions with the same space, rule and boundary flags share one pencil
+            key = (q.degree, q.knotvector.tobytes(), q.nbquadpts, bool(table[0, d, 0]), bool(table[0, d, 1]))
+            if key not in solved:
+                solved[key] = pencil_eigenpairs(q, table[0, d, 0], table[0, d, 1])
+            pairs.append(solved[key])
+        self.set_space_eigenpairs(table, [q.nbctrlpts for q in qs[:nbdirs]], pairs)
+
+    def set_space_eigenpairs(self, space_table_dirichlet, nnz, pairs):
+        """Install per-direction ``(indices, eigenvalues, eigenvectors)`` computed by
+        :func:`pencil_eigenpairs` (possibly elsewhere, e.g. in a worker process)."""
+        table = np.asarray(space_table_dirichlet, dtype=bool)
+        self.nbDoFsPerNode = table.shape[0]
+        self.nnz = [int(n) for n in nnz]
+        nbdirs = len(self.nnz)
+        self.space_dirs = [ParametricDirection(i) for i in range(nbdirs)]
+        self.indices = [np.asarray(p[0]) for p in pairs]
+        self.eigenval_by_dir_space = [np.asarray(p[1]) for p in pairs]
+        self.eigenvec_by_dir_space = [np.asarray(p[2]) for p in pairs]
         self.space_table_dirichlet = table
         self.mass_space_corrector = [1.0]
         self.stiff_space_corrector = [np.ones(nbdirs)]
@@ -158,6 +176,8 @@ class FDPlan:
 
         self.dtype = dtype
         self.nsub = int(np.prod(ln))
+        # the free box (grid sizes, first free index, free count per direction)
+        self.box = tuple(np.array(a, dtype=np.int64) for a in (n, lo, ln))
         na, npt = _lib.i64_array(n)
         la, lpt = _lib.i64_array(lo)
         lna, lnpt = _lib.i64_array(ln)

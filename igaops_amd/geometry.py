@@ -71,6 +71,7 @@ class ExtrudedPatch:
         self.z0 = float(z0)
         self.height = float(height)
         self.name = name
+        self._plane: Optional[PlaneGeometry] = None
 
     # --- sizes -----------------------------------------------------------
     @property
@@ -114,7 +115,10 @@ class ExtrudedPatch:
     def plane_geometry(self) -> PlaneGeometry:
         """Rational map and its Jacobian on the cross-section quadrature grid
         (the ``NurbsOperations.eval_jacobien`` formula, ``operators/nurbs_ops.py:136-160``:
-        ``dX = (d(wP) - X dW) / W``)."""
+        ``dX = (d(wP) - X dW) / W``).  Evaluated once per patch: the control net and
+        the rules do not change after construction, and callers only read the arrays."""
+        if self._plane is not None:
+            return self._plane
         q1, q2 = self.quadrule_list[0], self.quadrule_list[1]
         n1, n2 = q1.nbctrlpts, q2.nbctrlpts
         B1 = q1.basis[:2]
@@ -132,7 +136,10 @@ class ExtrudedPatch:
             dX = [_apply2(B1[1], B2[0], Pw), _apply2(B1[0], B2[1], Pw)]
             for j in range(2):
                 jac[i, j] = (dX[j] - X * dW[j]) / W
-        return PlaneGeometry(x=x, jac=jac)
+        x.flags.writeable = False
+        jac.flags.writeable = False
+        self._plane = PlaneGeometry(x=x, jac=jac)
+        return self._plane
 
     def line_geometry(self) -> Tuple[np.ndarray, np.ndarray]:
         """``z(q_3)`` and ``dz/dxi_3`` on the extrusion direction."""
@@ -234,7 +241,7 @@ class ExtrudedPatch:
         parts = [det2, inv_h[(0, 0)], inv_h[(1, 1)], inv_h[(0, 1)]]
         npl = det2.size
         host = np.concatenate([np.ravel(a) for a in parts] + [np.ravel(dz)])
-        allv = t.from_numpy(host).to(dev)
+        allv = dv.upload(host)
         det2_d = allv[:npl]
         invd = {k: allv[(i + 1) * npl:(i + 2) * npl] for i, k in enumerate(((0, 0), (1, 1), (0, 1)))}
         dz_d = allv[4 * npl:]
@@ -276,14 +283,13 @@ class ExtrudedPatch:
         zh = (z - lo[2]) / span[2]
         Ez = np.exp(1j * np.pi * zh[:, None] * k[None, :])
         Zc = np.einsum("abc,zc->abz", coef, Ez).reshape(-1, zh.shape[0])  # (m*m, n3)
-        dev = dv.device()
         npl = xh.shape[0]
-        allv = t.from_numpy(np.concatenate([xh, yh, k.astype(np.float64)])).to(dev)
+        allv = dv.upload(np.concatenate([xh, yh, k.astype(np.float64)]))
         xd, yd, kd = allv[:npl], allv[npl:2 * npl], allv[2 * npl:]
         Ex = t.exp(1j * np.pi * xd[:, None] * kd[None, :])
         Ey = t.exp(1j * np.pi * yd[:, None] * kd[None, :])
         Pxy = (Ex[:, :, None] * Ey[:, None, :]).reshape(xh.shape[0], -1)
-        g = t.real(t.from_numpy(Zc).to(dev).T.contiguous() @ Pxy.T.contiguous())  # (n3, nplane)
+        g = t.real(dv.upload(Zc).T.contiguous() @ Pxy.T.contiguous())  # (n3, nplane)
         return t.exp(sigma * g)
 
 

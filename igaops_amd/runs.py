@@ -18,6 +18,12 @@ vectors only.
 of ``A_lo d = r`` in float32 vectors and operators, the residual ``r = b - A x`` and
 the solution in float64, until ``||r|| / ||b|| <= tol``.
 
+With a fused operator and an FD preconditioner (``NativeStep``), a run queues each
+Arnoldi step with one library call (``iga_arnoldi_step``: FD, operator, mask,
+orthogonalisation, and the residual estimate's copy to the host behind an event),
+which keeps the host time per step well below the device time even for small
+problems.
+
 Both expose ``advance()``, which queues whatever work is possible and returns
 whether anything happened, so that one host thread can interleave many runs on
 separate streams (the serving loop does).
@@ -51,15 +57,40 @@ def _relative_or_absolute(tol: float, rel: float, absv: float) -> str:
     return ""
 
 
+class NativeStep:
+    """The pieces of a GMRES solve that the one-call Arnoldi step (``iga_arnoldi_*``)
+    takes over from ``A``, ``P`` and ``mask``: a fused ``StiffnessMassOperator``, an
+    FD plan (``None``: no preconditioner) with its workspace, and the mask's free box
+    ``(n, lo, len)`` (``None``: no mask).  The FD writes zeros outside its own box, so
+    that box must be the mask's."""
+
+    def __init__(self, op, fd_plan=None, fd_work=None, box=None):
+        if not getattr(op, "fused", False):
+            raise ValueError("the one-call Arnoldi step needs a fused operator")
+        if fd_plan is not None:
+            if fd_work is None:
+                raise ValueError("the FD plan needs its workspace")
+            if box is None or not all(np.array_equal(a, b) for a, b in zip(fd_plan.box, box)):
+                raise ValueError("the FD's free box must be the mask's box")
+            if (op.dtype == "f64") != (fd_plan.dtype == "f64"):
+                raise ValueError("operator and FD plan take different vector types")
+        self.op, self.fd_plan, self.fd_work = op, fd_plan, fd_work
+        self.box = None if box is None else tuple(np.ascontiguousarray(a, dtype=np.int64) for a in box)
+
+
 class GmresRun:
     """One GMRES solve of ``A x = b`` right-preconditioned by ``P``.
 
     ``A(x, out)`` / ``P(x, out)`` write into ``out`` (vectors of ``dtype``); ``mask``
     zeroes the constrained entries (``linear_solver.py:119-124``).  ``maxiters`` is
-    also the Krylov dimension (the reference does not restart)."""
+    also the Krylov dimension (the reference does not restart).  With ``native`` (a
+    :class:`NativeStep` describing the same ``A``, ``P`` and ``mask``) each Arnoldi step
+    is queued by one library call instead of one call per kernel group; ``A``, ``P``
+    and ``mask`` still serve the start and the solution update."""
 
     def __init__(self, A: Callable, P: Optional[Callable], b, tol: float, maxiters: int, mask=None, x0=None,
-                 stream=None, lookahead: int = 2, dtype: Optional[str] = None, host_buf=None):
+                 stream=None, lookahead: int = 2, dtype: Optional[str] = None, host_buf=None,
+                 native: Optional[NativeStep] = None):
         t = dv.torch()
         self.A, self.P, self.mask = A, P, mask
         self.tol, self.m = float(tol), int(maxiters)
@@ -102,6 +133,8 @@ class GmresRun:
         self._y_region = host_buf[2 * (self.m + 2) + self.m * self.m + self.m:need]
         self.launched = 0      # steps queued
         self.known = -1        # last step whose residual estimate has been read
+        self.recorded = -1     # last step whose residual estimate is on its way to the host
+        self.started = False
         self.events = {}
         self.kfinal: Optional[int] = None
         self.reason = ""
@@ -111,6 +144,32 @@ class GmresRun:
         self._sol_event = None
         self._sol_host = None
         self.result = None
+        self._ctx = None
+        if native is not None:
+            if native.op.dtype != self.dtype:
+                raise ValueError("the native operator does not take the run's vector type")
+            if self.lookahead > 8:
+                raise ValueError("the one-call step keeps at most 16 residual estimates in flight")
+            op, fd = native.op, native.fd_plan
+            bx = native.box
+            nd = 0 if bx is None else len(bx[0])
+            bp = [None, None, None] if bx is None else [a.ctypes.data_as(_lib._i64p) for a in bx]
+            self._native = native
+            ctx = C.c_void_p()
+            _lib.call("iga_arnoldi_create", C.byref(ctx), dv.DTYPES[self.dtype], n, self.m, op.plan._h,
+                      dv.ptr(op.packed), dv.ptr(op.w), dv.ptr(op.workspace(1)),
+                      None if fd is None else fd._h, dv.ptr(native.fd_work), nd, *bp, dv.ptr(self.z),
+                      dv.ptr(self.w), dv.ptr(self.ls), dv.ptr(self.work), self.hist_host.data_ptr())
+            self._ctx = ctx
+
+    def __del__(self):
+        self._drop_native()
+
+    def _drop_native(self):
+        ctx = getattr(self, "_ctx", None)
+        if ctx is not None and ctx.value and _lib._lib is not None:
+            _lib.load().iga_arnoldi_destroy(ctx)
+        self._ctx = None
 
     @staticmethod
     def host_doubles(m: int) -> int:
@@ -129,6 +188,29 @@ class GmresRun:
         ev = t.cuda.Event()
         ev.record(self.stream if self.stream is not None else t.cuda.current_stream())
         self.events[step] = ev
+        self.recorded = step
+
+    def _arrived(self, j: int, block: bool) -> bool:
+        """Whether residual estimate j is on the host (waiting for it with ``block``)."""
+        if self._ctx is not None:
+            rc = _lib.load().iga_arnoldi_query(self._ctx, j)
+            if rc < 0:
+                _lib.check(rc, "iga_arnoldi_query")
+            if rc == 1:
+                return True
+            if block:
+                _lib.call("iga_arnoldi_wait", self._ctx, j)
+                return True
+            return False
+        ev = self.events[j]
+        if ev.query():
+            del self.events[j]
+            return True
+        if block:
+            ev.synchronize()
+            del self.events[j]
+            return True
+        return False
 
     # -- phases --
     def start(self):
@@ -144,10 +226,15 @@ class GmresRun:
                 if self.mask:
                     self.mask(tmp)
                 r.sub_(tmp)
-            _lib.call("iga_gmres_start", dt, self.n, self.m, dv.ptr(r), dv.ptr(self.row(0)), dv.ptr(self.ls),
-                      dv.ptr(self.work), self._sp())
-            self._record(0, self._hist_off, 0, 2)
+            if self._ctx is not None:
+                _lib.call("iga_arnoldi_start", self._ctx, dv.ptr(r), dv.ptr(self.V), self._sp())
+                self.recorded = 0
+            else:
+                _lib.call("iga_gmres_start", dt, self.n, self.m, dv.ptr(r), dv.ptr(self.row(0)), dv.ptr(self.ls),
+                          dv.ptr(self.work), self._sp())
+                self._record(0, self._hist_off, 0, 2)
         self.launched = 0
+        self.started = True
 
     def can_launch(self) -> bool:
         # step k reports dimension k - 1: one more step in flight than the lookahead
@@ -183,6 +270,15 @@ class GmresRun:
         holds P V[k-1])."""
         k = self.launched + 1
         dt = dv.DTYPES[self.dtype]
+        if self._ctx is not None:
+            with self._on_stream():
+                self._ensure_rows(k)
+            _lib.call("iga_arnoldi_step", self._ctx, k, dv.ptr(self.V), self.ldv, int(bool(preconditioned)),
+                      self._sp())
+            if k >= 2:
+                self.recorded = k - 1
+            self.launched = k
+            return
         with self._on_stream():
             self._ensure_rows(k)
             z = self.z
@@ -206,15 +302,11 @@ class GmresRun:
     def poll(self, block: bool = False) -> bool:
         """Read the residual estimates of finished steps; decide convergence."""
         progressed = False
-        while self.kfinal is None and (self.known + 1) in self.events:
+        while self.kfinal is None and self.known + 1 <= self.recorded:
             j = self.known + 1
-            ev = self.events[j]
-            if not ev.query():
-                if not block:
-                    break
-                ev.synchronize()
+            if not self._arrived(j, block):
+                break
             block = False
-            del self.events[j]
             self.known = j
             progressed = True
             g, nw = float(self.hist_host[2 * j]), float(self.hist_host[2 * j + 1])
@@ -297,6 +389,7 @@ class GmresRun:
             # arithmetic makes the least-squares solution exact; the true residual decides
             success = self._true_residual_meets_tol()
         self.result = OutputArgs(self.x, np.asarray(self.res_hist), success, int(k or 0), self.reason)
+        self._drop_native()
         return self.result
 
     def _true_residual_meets_tol(self) -> bool:
@@ -320,7 +413,7 @@ class GmresRun:
         """Queue what can be queued without waiting; with ``launch=False`` no
         Arnoldi step is queued (the caller may batch them)."""
         progressed = False
-        if self.launched == 0 and not self.events and self.known < 0:
+        if not self.started:
             self.start()
             progressed = True
         if self.result is not None:
@@ -365,9 +458,11 @@ class RefinementRun:
 
     def __init__(self, A64: Callable, A_lo: Callable, P_lo: Callable, b, tol: float = 1e-10,
                  inner_tol: float = 1e-6, inner_maxiters: int = 200, max_refinements: int = 30, mask=None,
-                 stream=None, lookahead: int = 2, lo_dtype: str = "f32", host_buf=None):
+                 stream=None, lookahead: int = 2, lo_dtype: str = "f32", host_buf=None,
+                 native_lo: Optional[NativeStep] = None):
         t = dv.torch()
         self.A64, self.A_lo, self.P_lo, self.mask = A64, A_lo, P_lo, mask
+        self.native_lo = native_lo
         self.tol, self.inner_tol = float(tol), float(inner_tol)
         self.inner_maxiters, self.max_ref = int(inner_maxiters), int(max_refinements)
         self.stream, self.lookahead, self.lo_dtype = stream, lookahead, lo_dtype
@@ -418,7 +513,7 @@ class RefinementRun:
         itol = min(1e-1, max(self.inner_tol, 0.5 * self.tol / max(rel, 1e-300)))
         self.inner = GmresRun(self.A_lo, self.P_lo, self.r_lo, itol, self.inner_maxiters, mask=self.mask,
                               stream=self.stream, lookahead=self.lookahead, dtype=self.lo_dtype,
-                              host_buf=self._inner_host)
+                              host_buf=self._inner_host, native=self.native_lo)
         self.phase = "inner"
 
     def _finish(self, success: bool):

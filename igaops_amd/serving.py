@@ -28,10 +28,14 @@ How the loop works:
   Nothing a request computes depends on which others are present, so its result
   is the same alone and in the mix (to rounding: the batched FD sums in the same
   order per column).
-* Latency is measured from arrival to completion (the true residual checked).
+* Latency is measured from arrival to completion (the true residual checked).  The
+  loop never waits for one request's stream: set-up uploads go through pinned
+  memory, residual estimates and the final true residual come back through pinned
+  memory behind events, and each Arnoldi step is one library call.
 """
 from __future__ import annotations
 
+import sys
 import time
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional, Tuple
@@ -41,6 +45,7 @@ import numpy as np
 from . import device as dv
 
 _RULES: Dict[Tuple[int, int], object] = {}
+_PENCILS: Dict[Tuple[int, int], tuple] = {}
 
 
 def wq_rule(degree: int, nel: int):
@@ -53,6 +58,60 @@ def wq_rule(degree: int, nel: int):
         q = WeightedQuadrature(degree, make_open_knotvector(degree, nel), quadtype="2")
         _RULES[key] = q
     return q
+
+
+def fd_pencil(degree: int, nel: int):
+    """Free indices and 1-D eigenpairs of the FD of ``(degree, nel)`` with both ends
+    Dirichlet (the Gauss rewrite of the rule: the same for every direction and for
+    both geometries), cached by ``(degree, nel)``."""
+    from .fastdiag import pencil_eigenpairs, rewrite_quadrature
+
+    key = (int(degree), int(nel))
+    e = _PENCILS.get(key)
+    if e is None:
+        e = pencil_eigenpairs(rewrite_quadrature([wq_rule(degree, nel)])[0], True, True)
+        _PENCILS[key] = e
+    return e
+
+
+def _host_setup(key):
+    """Worker-process body of :func:`prepare_shape_table`."""
+    return key, wq_rule(*key), fd_pencil(*key)
+
+
+def _worker_init():
+    """One BLAS thread per worker: the pool already fills the cores, and the dense
+    problems here are far too small for threaded BLAS.  The limit reaches only the
+    BLAS libraries already loaded, so scipy's LAPACK is loaded first."""
+    import scipy.linalg  # noqa: F401
+    import scipy.sparse.linalg  # noqa: F401
+    from threadpoolctl import threadpool_limits
+
+    threadpool_limits(limits=1)
+
+
+def prepare_shape_table(degrees, nels, workers: int = 8) -> dict:
+    """Build the rule and FD pencil of every ``(degree, nel)`` of a declared range
+    (the host set-up a request of that size needs) in a pool of ``workers`` processes,
+    largest first.  Entries already cached are skipped.  Returns the number built and
+    the wall time, which a server reports as its start-up cost."""
+    import multiprocessing
+    from concurrent.futures import ProcessPoolExecutor
+
+    keys = sorted({(int(p), int(n)) for p in degrees for n in nels} - (_RULES.keys() & _PENCILS.keys()),
+                  key=lambda k: -(k[0] + 1) * k[1])
+    t0 = time.perf_counter()
+    if workers > 1 and len(keys) > 1:
+        # fresh interpreters: the workers never touch the device
+        ctx = multiprocessing.get_context("spawn")
+        with ProcessPoolExecutor(max_workers=min(workers, len(keys)), mp_context=ctx,
+                                 initializer=_worker_init) as ex:
+            for key, q, e in ex.map(_host_setup, keys, chunksize=2):
+                _RULES[key], _PENCILS[key] = q, e
+    else:
+        for key in keys:
+            _host_setup(key)
+    return {"entries": len(keys), "workers": workers, "seconds": time.perf_counter() - t0}
 
 
 @dataclass
@@ -91,10 +150,8 @@ class SolveResult:
 class ShapeResources:
     """Everything a request of one (geometry, degree, nel) shares with its peers."""
 
-    def __init__(self, geometry: str, degree: int, nel: int):
-        from .fastdiag import SingleFD
+    def __init__(self, geometry: str, degree: int, nel: int, fd: "FDResources"):
         from .geometry import quarter_annulus, unit_cube
-        from .solvers import Mask
 
         q = wq_rule(degree, nel)
         build = unit_cube if geometry == "cube" else quarter_annulus
@@ -102,11 +159,8 @@ class ShapeResources:
         self.qs = self.patch.quadrule_list
         self.ndof = self.patch.nbctrlpts_total
         self.nq = self.patch.nbquadpts_total
-        fd = SingleFD("spatial")
-        fd.compute_space_eigendecomposition(self.qs, np.ones((1, 3, 2), bool))
-        fd.update_space_eigenvalues([0.0, 1.0])
-        self.space_fd = fd.space_preconditioner
-        self.mask = Mask(box=self.space_fd.box())
+        self.fd = fd
+        self.space_fd, self.mask, self.fd_plans = fd.space_fd, fd.mask, fd.plans
         self.w64 = dv.to_device(self.patch.nurbs_weights, "f64") if self.patch.is_nurbs else None
         self.nurbs_fields = None
         if self.w64 is not None:
@@ -115,7 +169,26 @@ class ShapeResources:
             self.nurbs_fields = nurbs_weight_fields(self.qs, self.w64)
         # requests read these from their own streams: finish the set-up work first
         dv.torch().cuda.current_stream().synchronize()
-        self.fd_plans = {dt: self.space_fd.plan(dt) for dt in ("f64", "f32")}
+
+
+class FDResources:
+    """The FD preconditioner of one ``(degree, nel)``, shared by both geometries: the
+    preconditioner is built on the parametric Gauss rewrite of the rules, with every
+    face Dirichlet, so it does not depend on the map."""
+
+    def __init__(self, degree: int, nel: int, dtypes=("f64", "f32")):
+        from .fastdiag import SpaceFD
+        from .solvers import Mask
+
+        q = wq_rule(degree, nel)
+        pencil = fd_pencil(degree, nel)
+        fd = SpaceFD()
+        fd.set_space_eigenpairs(np.ones((1, 3, 2), bool), [q.nbctrlpts] * 3, [pencil] * 3)
+        fd.update_space_eigenvalues([0.0, 1.0])
+        self.space_fd = fd
+        self.mask = Mask(box=fd.box())
+        self.plans = {dt: fd.plan(dt) for dt in dtypes}
+        dv.torch().cuda.current_stream().synchronize()
 
 
 def footprint_bytes(ndof: int, nq: int, nurbs: bool, mode: str, maxiters: int, inner_maxiters: int) -> int:
@@ -138,7 +211,7 @@ class _Active:
     def __init__(self, req: SolveRequest, shape: ShapeResources, mode: str, stream, tol: float, maxiters: int,
                  inner_tol: float, inner_maxiters: int, lookahead: int, admitted: float, host_buf=None):
         from .operators import StiffnessMassOperator
-        from .runs import GmresRun, RefinementRun
+        from .runs import GmresRun, NativeStep, RefinementRun
 
         t = dv.torch()
         self.req, self.shape, self.mode, self.stream = req, shape, mode, stream
@@ -178,14 +251,16 @@ class _Active:
             op64, op32 = self.A64, self.A32
             P = lambda x, o: plan.apply(x, o, 1, None, work=fd_work)
             A64 = lambda x, o: op64.apply(x, out=o)
+            # each Arnoldi step is one library call (FD, operator, mask, orthogonalisation)
+            native = NativeStep(op32 if mode == "mixed" else op64, plan, fd_work, shape.mask.box)
             if mode == "mixed":
                 A32 = lambda x, o: op32.apply(x, out=o)
                 self.run = RefinementRun(A64, A32, P, b, tol=tol, inner_tol=inner_tol,
                                          inner_maxiters=inner_maxiters, mask=shape.mask, stream=stream,
-                                         lookahead=lookahead, host_buf=host_buf)
+                                         lookahead=lookahead, host_buf=host_buf, native_lo=native)
             else:
                 self.run = GmresRun(A64, P, b, tol, maxiters, mask=shape.mask, stream=stream, lookahead=lookahead,
-                                    host_buf=host_buf)
+                                    host_buf=host_buf, native=native)
             lap("solver")
         self.t0 = time.perf_counter()
 
@@ -196,19 +271,40 @@ class _Active:
     def done(self) -> bool:
         return self.run.done
 
-    def result(self, completed: float, keep_solution: bool) -> SolveResult:
+    def queue_check(self, host2):
+        """Queue the true residual ``||b - A x||^2`` and ``||b||^2`` (float64, masked)
+        and their copy into ``host2`` (two pinned doubles) behind an event, so that
+        the loop never waits for a request's stream."""
+        from . import _lib
+
         t = dv.torch()
-        out = self.run.result
+        x = self.run.result.solution
         with t.cuda.stream(self.stream):
-            x = out.solution
             r = self.b - self.A64.apply(x)
             self.shape.mask(r)
-            rel = float(t.linalg.vector_norm(r) / t.linalg.vector_norm(self.b))
+            nrm = t.empty(2, dtype=t.float64, device=r.device)
+            work = dv.workspace(_lib.load().iga_reduce_workspace_bytes(r.numel(), 1))
+            sp = dv.stream_ptr(self.stream)
+            for i, v in enumerate((r, self.b)):
+                _lib.call("iga_dot_many", _lib.IGA_F64, v.numel(), 1, dv.ptr(v), v.numel(), dv.ptr(v),
+                          dv.ptr(nrm[i:]), dv.ptr(work), sp)
+            host2.copy_(nrm, non_blocking=True)
+            self._check = (host2, nrm, work, r)
+            self.check_event = t.cuda.Event()
+            self.check_event.record(self.stream)
+
+    check_event = None
+
+    def result(self, completed: float, keep_solution: bool) -> SolveResult:
+        out = self.run.result
+        host2 = self._check[0]
+        rel = float(host2[0]) ** 0.5 / float(host2[1]) ** 0.5
+        x = out.solution
         res = SolveResult(self.req.rid, self.req.shape, self.shape.ndof, int(out.iterations), rel,
                           bool(out.success), self.req.arrival, self.admitted, completed,
                           x.clone() if keep_solution else None)
         # release everything this request holds on the device
-        self.run = self.A64 = self.A32 = self.b = self._fd_work = None
+        self.run = self.A64 = self.A32 = self.b = self._fd_work = self._check = None
         return res
 
 
@@ -231,8 +327,40 @@ class SolveServer:
         self._batch_stream = None
         self._batch_work: Dict[tuple, object] = {}
         self.shapes: Dict[tuple, ShapeResources] = {}
+        self.fds: Dict[tuple, FDResources] = {}
         self.stats = {"max_active": 0, "max_bytes": 0, "deferred_for_memory": 0, "admit_s": 0.0, "advance_s": 0.0,
-                      "complete_s": 0.0}
+                      "complete_s": 0.0, "shape_setup_s": 0.0, "start": None}
+
+    @property
+    def dtypes(self):
+        return ("f64", "f32") if self.mode == "mixed" else ("f64",)
+
+    def prepare(self, degrees, nels, workers: int = 8) -> dict:
+        """Server start: build what depends on ``(degree, nel)`` only, for every size
+        the server accepts, so that no request waits for it: the rules and FD pencils
+        (host, in a process pool), then the device FD plans and the fused-operator
+        plans of both geometries' coefficient forms in the mode's precisions (one-off
+        uploads that would otherwise wait behind other requests' work).  Sizes outside
+        the declared ranges are still served; their set-up is then part of the first
+        such request's latency."""
+        from .operators import FORM_K6, FORM_SYM10, fused_plan
+
+        info = prepare_shape_table(degrees, nels, workers)
+        t0 = time.perf_counter()
+        for p in degrees:
+            for nel in nels:
+                key = (int(p), int(nel))
+                if key not in self.fds:
+                    self.fds[key] = FDResources(*key, dtypes=self.dtypes)
+                q = wq_rule(*key)
+                for form in (FORM_K6, FORM_SYM10):  # unit cube; quarter annulus (NURBS)
+                    for dt in self.dtypes:
+                        fused_plan([q, q, q], form, dt)
+        dv.torch().cuda.synchronize()
+        info["device_plans_seconds"] = time.perf_counter() - t0
+        info["seconds"] += info["device_plans_seconds"]
+        self.stats["start"] = info
+        return info
 
     def _batched_steps(self, states) -> bool:
         """One Arnoldi step for every group of same-shape requests ready for it, with
@@ -244,13 +372,14 @@ class SolveServer:
         for st in states:
             run = st.run.step_run() if st.run is not None else None
             if run is not None:
-                groups.setdefault((st.req.shape, run.dtype), []).append(run)
+                # the FD plan depends on (degree, nel) only: both geometries share it
+                groups.setdefault((st.req.shape[1:], run.dtype), []).append(run)
         done = False
         for (shape_key, dtype), runs in groups.items():
             if len(runs) < 2:
                 continue
             g = len(runs)
-            plan = self.shapes[shape_key].fd_plans[dtype]
+            plan = self.fds[shape_key].plans[dtype]
             steps = [r.prepare_step() for r in runs]
             for r in runs:
                 ev = t.cuda.Event()
@@ -283,8 +412,15 @@ class SolveServer:
     def shape(self, key) -> ShapeResources:
         s = self.shapes.get(key)
         if s is None:
-            s = ShapeResources(*key)
+            t0 = time.perf_counter()
+            fk = tuple(key[1:])
+            fd = self.fds.get(fk)
+            if fd is None:
+                fd = FDResources(*fk, dtypes=self.dtypes)
+                self.fds[fk] = fd
+            s = ShapeResources(*key, fd)
             self.shapes[key] = s
+            self.stats["shape_setup_s"] += time.perf_counter() - t0
         return s
 
     def footprint(self, req: SolveRequest) -> int:
@@ -293,9 +429,10 @@ class SolveServer:
         nq = q.nbquadpts ** 3
         return footprint_bytes(n, nq, req.geometry == "annulus", self.mode, self.maxiters, self.inner_maxiters)
 
-    def serve(self, requests: List[SolveRequest], realtime: bool = True) -> List[SolveResult]:
+    def serve(self, requests: List[SolveRequest], realtime: bool = True, progress_every: int = 0) -> List[SolveResult]:
         """Run the stream to completion.  With ``realtime`` the requests arrive at their
-        ``arrival`` times (wall clock from the call); otherwise all are present at once."""
+        ``arrival`` times (wall clock from the call); otherwise all are present at once.
+        ``progress_every > 0`` prints a line to stderr every that many completions."""
         t = dv.torch()
         dv.require_gpu()
         pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
@@ -307,6 +444,7 @@ class SolveServer:
         hd = (GmresRun.host_doubles(self.inner_maxiters) + 2 if self.mode == "mixed"
               else GmresRun.host_doubles(self.maxiters))
         host_bufs = [t.zeros(hd, dtype=t.float64, pin_memory=True) for _ in range(self.max_active)]
+        check_bufs = [t.zeros(2, dtype=t.float64, pin_memory=True) for _ in range(self.max_active)]
         free_streams = list(range(self.max_active))
         active: List[Tuple[_Active, int, int]] = []  # (state, stream index, bytes)
         used = 0
@@ -352,8 +490,14 @@ class SolveServer:
             self.stats["advance_s"] += tb - ta
             still = []
             for st, si, fp in active:
-                if st.done:
+                if st.done and st.check_event is None:
+                    st.queue_check(check_bufs[si])
+                    progressed = True
+                if st.done and st.check_event.query():
                     results.append(st.result(now(), self.keep_solutions))
+                    if progress_every > 0 and len(results) % progress_every == 0:
+                        print(f"[serve {self.mode}] {len(results)}/{len(pending)} done at {now():.1f} s, "
+                              f"{len(active)} active, {len(pending) - head} waiting", file=sys.stderr, flush=True)
                     self.stats["complete_s"] += time.perf_counter() - tb
                     tb = time.perf_counter()
                     free_streams.append(si)

@@ -199,6 +199,66 @@ def test_device_step_gmres_at_maxiters_and_orthogonality(dtype):
         assert rel(out.solution.cpu().numpy(), ref.solution.cpu().numpy()) <= 1e-10
 
 
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+@pytest.mark.parametrize("precond", [True, False])
+def test_one_call_arnoldi_step_matches_python_driven_steps(dtype, precond):
+    """``iga_arnoldi_step`` queues the kernels of the Python-driven step with the same
+    arguments (the FD leaves zeros outside its box, so the mask of ``z`` it skips
+    changes nothing): the same residual history, iteration count and solution bits,
+    with lookahead 1 and 3, with and without the FD."""
+    import torch
+
+    from igaops_amd.runs import GmresRun, NativeStep
+
+    patch, C, m, fd, mask, ops = _annulus_fd_problem()
+    n = patch.nbctrlpts_total
+    tdt = torch.float64 if dtype == "f64" else torch.float32
+    plan = fd.space_preconditioner.plan(dtype)
+    work = plan.workspace(1)
+    A = lambda x, o: ops[dtype].apply(x, out=o)
+    P = (lambda x, o: plan.apply(x, o, 1, None, work=work)) if precond else None
+    native = NativeStep(ops[dtype], plan if precond else None, work if precond else None, mask.box)
+    b = torch.as_tensor(np.random.default_rng(13).standard_normal(n), device="cuda", dtype=tdt)
+    mask(b)
+    tol = 1e-10 if dtype == "f64" else 1e-6
+    for la in (1, 3):
+        ref = GmresRun(A, P, b, tol, 150, mask=mask, lookahead=la).run()
+        out = GmresRun(A, P, b, tol, 150, mask=mask, lookahead=la, native=native).run()
+        assert out.iterations == ref.iterations and out.reason == ref.reason
+        assert out.iterations >= (10 if precond else 40)
+        np.testing.assert_array_equal(out.residual, ref.residual)
+        assert torch.equal(out.solution, ref.solution)
+
+
+def test_one_call_arnoldi_step_inside_refinement():
+    """Mixed-precision refinement with its inner solves stepped by ``iga_arnoldi_step``:
+    the same refinements, inner iterations and solution bits as with Python-driven
+    inner steps, and a true residual of 1e-10."""
+    import torch
+
+    from igaops_amd.runs import NativeStep, RefinementRun
+
+    patch, C, m, fd, mask, ops = _annulus_fd_problem(4, 8)
+    n = patch.nbctrlpts_total
+    plan = fd.space_preconditioner.plan("f32")
+    work = plan.workspace(1)
+    A64 = lambda x, o: ops["f64"].apply(x, out=o)
+    A32 = lambda x, o: ops["f32"].apply(x, out=o)
+    P = lambda x, o: plan.apply(x, o, 1, None, work=work)
+    b = torch.as_tensor(np.random.default_rng(14).standard_normal(n), device="cuda")
+    mask(b)
+    ref = RefinementRun(A64, A32, P, b, tol=1e-10, inner_tol=1e-5, mask=mask).run()
+    out = RefinementRun(A64, A32, P, b, tol=1e-10, inner_tol=1e-5, mask=mask,
+                        native_lo=NativeStep(ops["f32"], plan, work, mask.box)).run()
+    assert out.success and ref.success
+    assert out.history["inner_iterations"] == ref.history["inner_iterations"]
+    assert len(out.history["inner_iterations"]) >= 2
+    assert torch.equal(out.solution, ref.solution)
+    r = b - ops["f64"].apply(out.solution)
+    mask(r)
+    assert float(torch.linalg.vector_norm(r) / torch.linalg.vector_norm(b)) <= 1e-10
+
+
 def test_refinement_run_float32_vectors_reaches_1e10():
     import torch
 

@@ -43,3 +43,48 @@ def test_error_path_without_gpu_compute():
     with pytest.raises(ValueError):
         _lib.check(rc, "iga_band_create")
     assert lib.iga_version() >= 100
+
+
+def test_arnoldi_context_rejects_bad_arguments_on_the_host():
+    """The one-call Arnoldi step checks its context on the host: missing pointers,
+    a box whose grid is not the vector, and queries on no context are errors (-1)."""
+    import numpy as np
+
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    fake = ctypes.c_void_p(1 << 20)  # never dereferenced: each call fails its checks first
+    assert lib.iga_arnoldi_create(ctypes.byref(h), 0, 27, 5, None, fake, None, fake, None, None, 0, None, None, None,
+                                  fake, fake, fake, fake, fake) == -1
+    n, lo, ln = (np.array(a, dtype=np.int64) for a in ((3, 3, 3), (1, 1, 1), (1, 1, 1)))
+    ptrs = [a.ctypes.data_as(_lib._i64p) for a in (n, lo, ln)]
+    assert lib.iga_arnoldi_create(ctypes.byref(h), 0, 28, 5, fake, fake, None, fake, None, None, 3, *ptrs,
+                                  fake, fake, fake, fake, fake) == -1
+    buf = ctypes.create_string_buffer(256)
+    lib.iga_last_error(buf, 256)
+    assert b"box grid" in buf.value
+    assert lib.iga_arnoldi_query(None, 0) == -1
+    assert lib.iga_arnoldi_destroy(None) == 0
+
+
+def test_native_step_spec_checks():
+    """NativeStep refuses a non-fused operator, an FD without workspace, an FD box
+    other than the mask box and mismatched vector types."""
+    import types
+
+    import numpy as np
+
+    from igaops_amd.runs import NativeStep
+
+    box = (np.array([5, 5, 5]), np.array([1, 1, 1]), np.array([3, 3, 3]))
+    op = types.SimpleNamespace(fused=True, dtype="f64")
+    fd = types.SimpleNamespace(box=box, dtype="f64")
+    NativeStep(op, fd, object(), box)
+    NativeStep(op, None, None, box)
+    with pytest.raises(ValueError, match="fused"):
+        NativeStep(types.SimpleNamespace(fused=False, dtype="f64"))
+    with pytest.raises(ValueError, match="workspace"):
+        NativeStep(op, fd, None, box)
+    with pytest.raises(ValueError, match="box"):
+        NativeStep(op, fd, object(), (box[0], np.array([0, 1, 1]), box[2]))
+    with pytest.raises(ValueError, match="vector types"):
+        NativeStep(op, types.SimpleNamespace(box=box, dtype="f32"), object(), box)
