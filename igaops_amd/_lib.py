@@ -64,6 +64,8 @@ SIGNATURES = {
     "iga_fused_apply_sliced2": (_i32, [_vp, _vp, _i64, _vp, _vp, _i64, _i64, _vp, _i64, _i64, _i32, _vp, C.c_double,
                                        _i32, _vp, _vp]),
     "iga_fused_pack_strided": (_i32, [_vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp]),
+    "iga_fused_pack_extruded": (_i32, [_vp, _vp, _vp, _vp, _vp, _i32, _f64, _i32, _vp, _vp, _vp, _vp]),
+    "iga_lognormal_plane_modes": (_i32, [_i64, _i32, _vp, _vp, _vp, _vp]),
     "iga_gmres_state_doubles": (_i64, [_i32]),
     "iga_gmres_layout": (_i32, [_i32, _i64p]),
     "iga_sumfact_nnz": (_i64, [_i32, _i64p, C.POINTER(_i32p)]),

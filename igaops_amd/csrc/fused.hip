@@ -610,6 +610,23 @@ __global__ void __launch_bounds__(256) fused_gather_kernel(GatherArgs<T> g) {
 //   B-splines).  NURBS folding (operators/nurbs_ops.py:343-390):
 //   D_ij = C_ij / W^2, D_00 = z^T C z + m / W^2, D_0j = -(C^T z)_j / W,
 //   D_i0 = -(C z)_i / W with z = dW / W^2.
+// Coefficients of a 2-D cross-section map extruded along xi_3, generated at the
+// points instead of read from C and m (q = plane + n_plane q3, plane = q1 + nq1 q2):
+//   det = dz[q3] det2[plane],  kappa = exp(sigma Re sum_k Ez[q3][k] H[plane][k])
+//   C_ab = kappa det inv_ab[plane] (a, b < 2, symmetric),  C_22 = kappa det / dz^2,
+//   m = det (with_mass) or 0.
+// plane holds det2, inv_00, inv_11, inv_01 (4 x n_plane); H (n_plane x modes) and
+// Ez (n3 x modes) are complex (re, im interleaved); H == nullptr means kappa = 1.
+struct ExtrudedSource {
+  const double* plane = nullptr;
+  const double* dz = nullptr;
+  const double* H = nullptr;
+  const double* Ez = nullptr;
+  int modes = 0;
+  int with_mass = 0;
+  double sigma = 0.0;
+};
+
 template <typename T>
 struct PackArgs {
   int nq[3], nt1, form, ncoef;
@@ -621,6 +638,7 @@ struct PackArgs {
   const double* W;
   const double* dW;
   int64_t comp_stride;  // elements between the components of C and of dW
+  ExtrudedSource ex;    // used instead of C and m when ex.plane is set
   T* out;
 };
 
@@ -639,10 +657,32 @@ __global__ void fused_pack_kernel(PackArgs<T> p) {
     T* dst = p.out + p.coef_base[t2 * p.nt1 + t1] + q3 * p.ncoef * npts + (l1 + (int64_t)q1n * l2);
     double D[4][4] = {{0}};
     double c[3][3] = {{0}};
-    if (p.C)
-      for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) c[i][j] = p.C[(int64_t)(i * 3 + j) * p.comp_stride + q];
-    const double mm = p.m ? p.m[q] : 0.0;
+    double mm = 0.0;
+    if (p.ex.plane) {
+      const int64_t nplane = (int64_t)p.nq[0] * p.nq[1];
+      const int64_t pl = q1 + (int64_t)p.nq[0] * q2;
+      const double dz = p.ex.dz[q3];
+      const double det = dz * p.ex.plane[pl];
+      double kap = 1.0;
+      if (p.ex.H) {
+        const double* h = p.ex.H + 2 * pl * p.ex.modes;
+        const double* e = p.ex.Ez + 2 * q3 * p.ex.modes;
+        double g = 0.0;
+        for (int k = 0; k < p.ex.modes; ++k) g += h[2 * k] * e[2 * k] - h[2 * k + 1] * e[2 * k + 1];
+        kap = exp(p.ex.sigma * g);
+      }
+      const double kd = kap * det;
+      c[0][0] = kd * p.ex.plane[nplane + pl];
+      c[1][1] = kd * p.ex.plane[2 * nplane + pl];
+      c[0][1] = c[1][0] = kd * p.ex.plane[3 * nplane + pl];
+      c[2][2] = kd / (dz * dz);
+      mm = p.ex.with_mass ? det : 0.0;
+    } else {
+      if (p.C)
+        for (int i = 0; i < 3; ++i)
+          for (int j = 0; j < 3; ++j) c[i][j] = p.C[(int64_t)(i * 3 + j) * p.comp_stride + q];
+      mm = p.m ? p.m[q] : 0.0;
+    }
     if (p.W) {
       const double wi = 1.0 / p.W[q];
       double z[3];
@@ -689,6 +729,60 @@ __global__ void fused_pack_kernel(PackArgs<T> p) {
         for (int a = 0; a < 4; ++a)
           for (int b = 0; b < 4; ++b) dst[(a * 4 + b) * st] = (T)D[a][b];
     }
+  }
+}
+
+// H[plane][c] = sum_{a,b} coef[a][b][c] e^{i pi (a+1) x} e^{i pi (b+1) y} for the
+// scaled cross-section coordinates (x, y) = xy[plane], xy[n_plane + plane]: the
+// cross-section factor of the separable Fourier field of the lognormal
+// conductivity (modes <= kMaxModes wave numbers per axis, coef complex modes^3 with
+// c fastest, staged in LDS).  One thread per cross-section point.
+constexpr int kMaxModes = 16;
+
+template <int M>
+__global__ void lognormal_plane_modes_kernel(int64_t nplane, const double* __restrict__ xy,
+                                             const double* __restrict__ coef, double* __restrict__ H) {
+  __shared__ double cs[2 * M * M * M];
+  constexpr int ncoef = 2 * M * M * M;
+  for (int i = threadIdx.x; i < ncoef; i += blockDim.x) cs[i] = coef[i];
+  __syncthreads();
+  for (int64_t pl = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; pl < nplane;
+       pl += (int64_t)gridDim.x * blockDim.x) {
+    double ex[M][2], ey[M][2];
+    const double x = xy[pl], y = xy[nplane + pl];
+    for (int k = 0; k < M; ++k) {
+      sincospi((k + 1) * x, &ex[k][1], &ex[k][0]);
+      sincospi((k + 1) * y, &ey[k][1], &ey[k][0]);
+    }
+    double h[M][2];
+    for (int c = 0; c < M; ++c) h[c][0] = h[c][1] = 0.0;
+    for (int a = 0; a < M; ++a)
+      for (int b = 0; b < M; ++b) {
+        const double pr = ex[a][0] * ey[b][0] - ex[a][1] * ey[b][1];
+        const double pi = ex[a][0] * ey[b][1] + ex[a][1] * ey[b][0];
+        const double* cab = cs + 2 * (a * M + b) * M;
+        for (int c = 0; c < M; ++c) {
+          h[c][0] += cab[2 * c] * pr - cab[2 * c + 1] * pi;
+          h[c][1] += cab[2 * c] * pi + cab[2 * c + 1] * pr;
+        }
+      }
+    double* out = H + 2 * pl * M;
+    for (int c = 0; c < M; ++c) {
+      out[2 * c] = h[c][0];
+      out[2 * c + 1] = h[c][1];
+    }
+  }
+}
+
+// Dispatch of the wave-number count to its kernel instance.
+template <int M>
+void launch_plane_modes(int modes, int64_t nplane, const double* xy, const double* coef, double* H, hipStream_t s) {
+  if constexpr (M > kMaxModes) {
+    throw Error(IGA_ERR_INVALID, "modes must be 1..16");
+  } else {
+    if (modes != M) return launch_plane_modes<M + 1>(modes, nplane, xy, coef, H, s);
+    lognormal_plane_modes_kernel<M><<<grid_for(nplane, 128), 128, 0, s>>>(nplane, xy, coef, H);
+    IGA_LAUNCH_CHECK();
   }
 }
 
@@ -936,7 +1030,7 @@ static void run_apply(const iga_fused_s* f, const void* coef, const void* x, voi
 
 template <typename T>
 static void run_pack(const iga_fused_s* f, const double* C, const double* m, const double* W, const double* dW,
-                     void* out, hipStream_t s, int64_t comp_stride = 0) {
+                     void* out, hipStream_t s, int64_t comp_stride = 0, const ExtrudedSource* ex = nullptr) {
   PackArgs<T> p{};
   for (int d = 0; d < 3; ++d) p.nq[d] = f->nq[d];
   p.nt1 = f->nt1;
@@ -954,6 +1048,7 @@ static void run_pack(const iga_fused_s* f, const double* C, const double* m, con
   p.out = (T*)out;
   const int64_t nqt = (int64_t)f->nq[0] * f->nq[1] * f->nq[2];
   p.comp_stride = comp_stride > 0 ? comp_stride : nqt;
+  if (ex) p.ex = *ex;
   fused_pack_kernel<T><<<grid_for(nqt, 256), 256, 0, s>>>(p);
   IGA_LAUNCH_CHECK();
 }
@@ -1193,6 +1288,39 @@ int iga_fused_apply_stage(iga_fused_t f, int stage, const void* packed, const vo
       run_apply<double>(f, packed, x, y, nrhs, nurbs_w, alpha, accumulate, work, as_stream(stream), stage);
     else
       run_apply<float>(f, packed, x, y, nrhs, nurbs_w, alpha, accumulate, work, as_stream(stream), stage);
+  });
+}
+
+int iga_fused_pack_extruded(iga_fused_t f, const double* plane, const double* dz, const double* H, const double* Ez,
+                            int modes, double sigma, int with_mass, const double* W, const double* dW, void* packed,
+                            void* stream) {
+  return guard([&] {
+    IGA_CHECK_ARG(f && packed && plane && dz, "NULL argument");
+    IGA_CHECK_ARG((W == nullptr) == (dW == nullptr), "W and dW go together");
+    IGA_CHECK_ARG(!H || (Ez && modes >= 1 && modes <= kMaxModes), "field modes: 1..16 with H and Ez");
+    IGA_CHECK_ARG(f->form != FORM_M1, "the generated coefficients are stiffness coefficients");
+    if (f->form == FORM_KM7 || f->form == FORM_KM7X) IGA_CHECK_ARG(with_mass, "mass forms need the mass term");
+    ExtrudedSource ex;
+    ex.plane = plane;
+    ex.dz = dz;
+    ex.H = H;
+    ex.Ez = Ez;
+    ex.modes = H ? modes : 0;
+    ex.sigma = sigma;
+    ex.with_mass = with_mass;
+    if (f->dtype == IGA_F64)
+      run_pack<double>(f, nullptr, nullptr, W, dW, packed, as_stream(stream), 0, &ex);
+    else
+      run_pack<float>(f, nullptr, nullptr, W, dW, packed, as_stream(stream), 0, &ex);
+  });
+}
+
+int iga_lognormal_plane_modes(int64_t nplane, int modes, const double* xy, const double* coef, double* H,
+                              void* stream) {
+  return guard([&] {
+    IGA_CHECK_ARG(xy && coef && H && nplane >= 1, "bad argument");
+    IGA_CHECK_ARG(modes >= 1 && modes <= kMaxModes, "modes must be 1..16");
+    launch_plane_modes<1>(modes, nplane, xy, coef, H, as_stream(stream));
   });
 }
 

@@ -214,6 +214,81 @@ class ExtrudedPatch:
         return C, m
 
 
+    def extrusion_factors(self):
+        """Host factors of the extruded map that the coefficients are built from:
+        ``plane (4, n_plane)`` = ``det J_2`` and the entries 11, 22, 12 of
+        ``(J_2^T J_2)^{-1}`` on the cross-section grid; ``dz (n3)`` = ``dz/dxi_3``; and
+        the coordinates scaled to the bounding box of the control net (``xy (2,
+        n_plane)``, ``z (n3)``) at which the lognormal field is evaluated."""
+        pg = self.plane_geometry()
+        z, dz = self.line_geometry()
+        j = pg.jac
+        det2 = j[0, 0] * j[1, 1] - j[0, 1] * j[1, 0]
+        if np.any(det2 * dz[0] <= 0.0):
+            raise ValueError("Potential geometry issue: quadrature points with detJ <= 0")
+        g00 = j[0, 0] ** 2 + j[1, 0] ** 2
+        g11 = j[0, 1] ** 2 + j[1, 1] ** 2
+        g01 = j[0, 0] * j[0, 1] + j[1, 0] * j[1, 1]
+        dG = g00 * g11 - g01 ** 2
+        plane = np.stack([det2.ravel(), (g11 / dG).ravel(), (g00 / dG).ravel(), (-g01 / dG).ravel()])
+        cp = self.ctrlpts
+        lo, hi = cp.min(axis=1), cp.max(axis=1)
+        span = np.where(hi > lo, hi - lo, 1.0)
+        xy = np.stack([((pg.x[0] - lo[0]) / span[0]).ravel(), ((pg.x[1] - lo[1]) / span[1]).ravel()])
+        return plane, np.asarray(dz, dtype=float), xy, (z - lo[2]) / span[2]
+
+    @staticmethod
+    def lognormal_modes(sigma: float, seed: int, modes: int = 8):
+        """Complex mode coefficients ``coef[a, b, c]`` (wave numbers ``a+1, b+1, c+1``
+        along x, y, z) of the field of :func:`lognormal_conductivity`, from the same
+        random numbers."""
+        rng = np.random.default_rng(seed)
+        k = np.arange(1, modes + 1)
+        K1, K2, K3 = np.meshgrid(k, k, k, indexing="ij")
+        amp = rng.standard_normal(K1.shape) / (1.0 + K1 ** 2 + K2 ** 2 + K3 ** 2)
+        phase = rng.uniform(0.0, 2 * np.pi, K1.shape)
+        scale = np.sqrt(np.sum(amp ** 2) / 2.0)
+        return amp * np.exp(1j * phase) / scale
+
+    def device_extrusion(self):
+        """``extrusion_factors`` on the device (float64), computed once per patch:
+        ``plane (4 n_plane)``, ``dz``, ``xy (2 n_plane)``, and the host ``z``."""
+        cache = getattr(self, "_dev_extrusion", None)
+        if cache is None:
+            from . import device as dv
+
+            plane, dz, xy, zh = self.extrusion_factors()
+            allv = dv.upload(np.concatenate([plane.ravel(), dz, xy.ravel()]))
+            npl = plane.shape[1]
+            cache = {"plane": allv[:4 * npl], "dz": allv[4 * npl:4 * npl + dz.size],
+                     "xy": allv[4 * npl + dz.size:], "z": zh, "nplane": npl}
+            self._dev_extrusion = cache
+        return cache
+
+    def device_field_modes(self, sigma: float, seed: int, modes: int = 8, stream=None):
+        """The separable factors of ``sigma g`` of :func:`lognormal_conductivity` on the
+        device: ``H (n_plane, modes)`` (one launch of ``iga_lognormal_plane_modes``) and
+        ``Ez (n3, modes)``, complex as interleaved float64; ``g = Re sum_c Ez[q3, c]
+        H[plane, c]``.  ``None`` when ``sigma`` is 0 (``kappa = 1``)."""
+        if sigma == 0.0:
+            return None
+        from . import _lib
+        from . import device as dv
+
+        ext = self.device_extrusion()
+        coef = self.lognormal_modes(sigma, seed, modes)
+        k = np.arange(1, modes + 1)
+        Ez = np.exp(1j * np.pi * ext["z"][:, None] * k[None, :])
+        host = np.concatenate([np.ascontiguousarray(coef).view(np.float64).ravel(),
+                               np.ascontiguousarray(Ez).view(np.float64).ravel()])
+        t = dv.torch()
+        allv = dv.upload(host)
+        nc = 2 * modes ** 3
+        H = t.empty(2 * ext["nplane"] * modes, dtype=t.float64, device=allv.device)
+        _lib.call("iga_lognormal_plane_modes", ext["nplane"], modes, dv.ptr(ext["xy"]), dv.ptr(allv[:nc]),
+                  dv.ptr(H), dv.stream_ptr(stream))
+        return {"H": H, "Ez": allv[nc:], "modes": modes, "sigma": float(sigma), "_coef": allv}
+
     def device_coefficients(self, sigma: float = 0.0, seed: int = 0, modes: int = 8, mass: bool = True):
         """``conductivity_coefficients(lognormal_conductivity(self, sigma, seed, modes))``
         formed on the device: the cross-section and extrusion factors are evaluated on
@@ -225,34 +300,18 @@ class ExtrudedPatch:
 
         from . import device as dv
 
-        pg = self.plane_geometry()
-        z, dz = self.line_geometry()
-        j = pg.jac
-        det2 = j[0, 0] * j[1, 1] - j[0, 1] * j[1, 0]
-        if np.any(det2 * dz[0] <= 0.0):
-            raise ValueError("Potential geometry issue: quadrature points with detJ <= 0")
-        g00 = j[0, 0] ** 2 + j[1, 0] ** 2
-        g11 = j[0, 1] ** 2 + j[1, 1] ** 2
-        g01 = j[0, 0] * j[0, 1] + j[1, 0] * j[1, 1]
-        dG = g00 * g11 - g01 ** 2
-        dev = dv.device()
-        inv_h = {(0, 0): g11 / dG, (1, 1): g00 / dG, (0, 1): -g01 / dG}
-        # one upload of all cross-section / extrusion factors
-        parts = [det2, inv_h[(0, 0)], inv_h[(1, 1)], inv_h[(0, 1)]]
-        npl = det2.size
-        host = np.concatenate([np.ravel(a) for a in parts] + [np.ravel(dz)])
-        allv = dv.upload(host)
-        det2_d = allv[:npl]
-        invd = {k: allv[(i + 1) * npl:(i + 2) * npl] for i, k in enumerate(((0, 0), (1, 1), (0, 1)))}
-        dz_d = allv[4 * npl:]
+        ext = self.device_extrusion()
+        npl = ext["nplane"]
+        det2_d = ext["plane"][:npl]
+        invd = {k: ext["plane"][(i + 1) * npl:(i + 2) * npl] for i, k in enumerate(((0, 0), (1, 1), (0, 1)))}
+        dz_d = ext["dz"]
         det = t.outer(dz_d, det2_d)  # (n3, nplane): q3 slowest
         if sigma > 0.0:
-            kap = self._device_lognormal(sigma, seed, modes, pg, z)
-            kdet = det * kap
+            kdet = det * self._device_lognormal(sigma, seed, modes)
         else:
             kdet = det
         nq = det.numel()
-        C = t.zeros((3, 3, nq), dtype=t.float64, device=dev)
+        C = t.zeros((3, 3, nq), dtype=t.float64, device=dv.device())
         for (a, b), v in invd.items():
             val = (kdet * v[None, :]).reshape(-1)
             C[a, b] = val
@@ -262,33 +321,24 @@ class ExtrudedPatch:
         m = det.reshape(-1).clone() if mass else None
         return C, m
 
-    def _device_lognormal(self, sigma, seed, modes, pg, z):
-        """The field of ``lognormal_conductivity`` (same random numbers), on the device."""
+    def _device_lognormal(self, sigma, seed, modes):
+        """The field of ``lognormal_conductivity`` (same random numbers), on the device,
+        as one complex matrix product ``(n3, modes^2) x (modes^2, n_plane)``."""
         import torch as t
 
         from . import device as dv
 
-        rng = np.random.default_rng(seed)
+        ext = self.device_extrusion()
+        coef = self.lognormal_modes(sigma, seed, modes)
         k = np.arange(1, modes + 1)
-        K1, K2, K3 = np.meshgrid(k, k, k, indexing="ij")
-        amp = rng.standard_normal(K1.shape) / (1.0 + K1 ** 2 + K2 ** 2 + K3 ** 2)
-        phase = rng.uniform(0.0, 2 * np.pi, K1.shape)
-        scale = np.sqrt(np.sum(amp ** 2) / 2.0)
-        coef = amp * np.exp(1j * phase) / scale
-        cp = self.ctrlpts
-        lo, hi = cp.min(axis=1), cp.max(axis=1)
-        span = np.where(hi > lo, hi - lo, 1.0)
-        xh = ((pg.x[0] - lo[0]) / span[0]).ravel()
-        yh = ((pg.x[1] - lo[1]) / span[1]).ravel()
-        zh = (z - lo[2]) / span[2]
-        Ez = np.exp(1j * np.pi * zh[:, None] * k[None, :])
-        Zc = np.einsum("abc,zc->abz", coef, Ez).reshape(-1, zh.shape[0])  # (m*m, n3)
-        npl = xh.shape[0]
-        allv = dv.upload(np.concatenate([xh, yh, k.astype(np.float64)]))
-        xd, yd, kd = allv[:npl], allv[npl:2 * npl], allv[2 * npl:]
+        Ez = np.exp(1j * np.pi * ext["z"][:, None] * k[None, :])
+        Zc = np.einsum("abc,zc->abz", coef, Ez).reshape(-1, ext["z"].shape[0])  # (m*m, n3)
+        npl = ext["nplane"]
+        xd, yd = ext["xy"][:npl], ext["xy"][npl:]
+        kd = dv.upload(k.astype(np.float64))
         Ex = t.exp(1j * np.pi * xd[:, None] * kd[None, :])
         Ey = t.exp(1j * np.pi * yd[:, None] * kd[None, :])
-        Pxy = (Ex[:, :, None] * Ey[:, None, :]).reshape(xh.shape[0], -1)
+        Pxy = (Ex[:, :, None] * Ey[:, None, :]).reshape(npl, -1)
         g = t.real(dv.upload(Zc).T.contiguous() @ Pxy.T.contiguous())  # (n3, nplane)
         return t.exp(sigma * g)
 
@@ -377,13 +427,8 @@ def lognormal_conductivity(patch: ExtrudedPatch, sigma: float, seed: int, modes:
     ``s^2 = sum a_k^2 / 2`` (unit variance in expectation over the phases).  The sum is
     separable between the cross-section and the extrusion direction, so it is evaluated
     as one complex matrix product of size ``(n_q1 n_q2, modes^2) x (modes^2, n_q3)``."""
-    rng = np.random.default_rng(seed)
     k = np.arange(1, modes + 1)
-    K1, K2, K3 = np.meshgrid(k, k, k, indexing="ij")
-    amp = rng.standard_normal(K1.shape) / (1.0 + K1 ** 2 + K2 ** 2 + K3 ** 2)
-    phase = rng.uniform(0.0, 2 * np.pi, K1.shape)
-    scale = np.sqrt(np.sum(amp ** 2) / 2.0)
-    coef = amp * np.exp(1j * phase) / scale  # (k1, k2, k3)
+    coef = ExtrudedPatch.lognormal_modes(sigma, seed, modes)  # (k1, k2, k3)
     cp = patch.ctrlpts
     lo, hi = cp.min(axis=1), cp.max(axis=1)
     span = np.where(hi > lo, hi - lo, 1.0)

@@ -379,6 +379,21 @@ class FusedPlan:
         )
         return out
 
+    def pack_extruded(self, ext, field=None, with_mass=False, W=None, dW=None, stream=None):
+        """Packed device coefficients of an extruded patch generated point by point
+        (``iga_fused_pack_extruded``): ``ext`` from ``ExtrudedPatch.device_extrusion``,
+        ``field`` from ``ExtrudedPatch.device_field_modes`` (``None``: ``kappa = 1``),
+        ``W`` and ``dW`` (3 N_q, concatenated) the NURBS weight fields."""
+        out = dv.empty(self.coef_elems, self.dtype)
+        H = Ez = None
+        modes, sigma = 0, 0.0
+        if field is not None:
+            H, Ez, modes, sigma = field["H"], field["Ez"], field["modes"], field["sigma"]
+        _lib.call("iga_fused_pack_extruded", self._h, dv.ptr(ext["plane"]), dv.ptr(ext["dz"]), dv.ptr(H), dv.ptr(Ez),
+                  int(modes), float(sigma), int(bool(with_mass)), dv.ptr(W), dv.ptr(dW), dv.ptr(out),
+                  dv.stream_ptr(stream))
+        return out
+
     def apply_stage(self, stage, packed, x, y, nrhs=1, nurbs_w=None, alpha=1.0, accumulate=False, work=None, stream=None):
         """One of the two launches of ``apply`` (1: coefficient sweep, 2: gather)."""
         work = work if work is not None else self.workspace(nrhs)
@@ -517,6 +532,49 @@ class StiffnessMassOperator:
             self.terms = self._composed_terms(Cd, md)
         self.coef_count = FORM_NCOEF[self.form] if self.fused else len(self.terms)
         self._work = {}
+
+    @classmethod
+    def from_extruded(cls, patch, field=None, dtype: str = "f64", mass: bool = False, nurbs_fields=None,
+                      nurbs_w64=None, stream=None):
+        """``K (+ M)`` of an extruded patch (``geometry.ExtrudedPatch``) with the
+        conductivity ``kappa detJ J^+ J^+T``, ``kappa`` the lognormal field of
+        ``patch.device_field_modes`` (or 1 when ``field`` is None): the same operator as
+        ``StiffnessMassOperator(qs, *patch.device_coefficients(...), nurbs_weights=...)``,
+        with the coefficients generated and packed in one launch instead of going
+        through a (3, 3, N_q) array.  ``nurbs_fields`` = ``(W, dW)``, ``dW`` the three
+        gradient components concatenated, and ``nurbs_w64`` (the control-point weights
+        on the device, float64) may be passed when cached by the caller."""
+        t = dv.torch()
+        dv.require_gpu()
+        self = cls.__new__(cls)
+        self.quadrule_list = list(patch.quadrule_list)
+        self.dtype = dtype
+        self.ndof = patch.nbctrlpts_total
+        self.nq = patch.nbquadpts_total
+        self.is_nurbs = patch.is_nurbs
+        self.w64 = None
+        if self.is_nurbs:
+            self.w64 = nurbs_w64 if nurbs_w64 is not None else dv.upload(np.asarray(patch.nurbs_weights, dtype=float))
+        self.w = None if not self.is_nurbs else self.w64.to(dv.torch_dtype(dtype))
+        if not fused_applicable(self.quadrule_list):
+            raise NotImplementedError("generated coefficients need the fused operator")
+        self.fused = True
+        if self.is_nurbs:
+            self.form = FORM_SYM10
+        else:
+            self.form = FORM_KM7 if mass else FORM_K6
+        self.plan = fused_plan(self.quadrule_list, self.form, dtype, 0)
+        W = dW = None
+        if self.is_nurbs:
+            if nurbs_fields is None:
+                Wf, dWf = nurbs_weight_fields(self.quadrule_list, self.w64)
+                nurbs_fields = (Wf, t.cat(dWf))
+            W, dW = nurbs_fields
+        ext = patch.device_extrusion()
+        self.packed = self.plan.pack_extruded(ext, field, mass, W, dW, stream=stream)
+        self.coef_count = FORM_NCOEF[self.form]
+        self._work = {}
+        return self
 
     def workspace(self, nrhs: int):
         """This operator's own partial-block buffer, so that operators sharing a plan

@@ -261,8 +261,9 @@ class GmresRun:
         can fill z = P V[k-1] itself (batched with other runs) before
         ``launch(preconditioned=True)``."""
         k = self.launched + 1
-        with self._on_stream():
-            self._ensure_rows(k)
+        if k >= self.V.shape[0]:
+            with self._on_stream():
+                self._ensure_rows(k)
         return k, self.row(k - 1), self.z
 
     def launch(self, preconditioned: bool = False):
@@ -271,8 +272,9 @@ class GmresRun:
         k = self.launched + 1
         dt = dv.DTYPES[self.dtype]
         if self._ctx is not None:
-            with self._on_stream():
-                self._ensure_rows(k)
+            if k >= self.V.shape[0]:  # the basis grows (on the run's stream) only now and then
+                with self._on_stream():
+                    self._ensure_rows(k)
             _lib.call("iga_arnoldi_step", self._ctx, k, dv.ptr(self.V), self.ldv, int(bool(preconditioned)),
                       self._sp())
             if k >= 2:

@@ -161,14 +161,18 @@ class ShapeResources:
         self.nq = self.patch.nbquadpts_total
         self.fd = fd
         self.space_fd, self.mask, self.fd_plans = fd.space_fd, fd.mask, fd.plans
+        t = dv.torch()
         self.w64 = dv.to_device(self.patch.nurbs_weights, "f64") if self.patch.is_nurbs else None
         self.nurbs_fields = None
         if self.w64 is not None:
             from .operators import nurbs_weight_fields
 
-            self.nurbs_fields = nurbs_weight_fields(self.qs, self.w64)
-        # requests read these from their own streams: finish the set-up work first
-        dv.torch().cuda.current_stream().synchronize()
+            Wf, dWf = nurbs_weight_fields(self.qs, self.w64)
+            self.nurbs_fields = (Wf, t.cat(dWf))  # W and the gradient components, concatenated
+        self.patch.device_extrusion()  # cross-section and extrusion factors, on the device
+        # requests read these from their own streams, which wait for this event
+        self.ready = t.cuda.Event()
+        self.ready.record(t.cuda.current_stream())
 
 
 class FDResources:
@@ -225,18 +229,17 @@ class _Active:
             self.setup_times[name] = now - tic
             tic = now
 
+        stream.wait_event(shape.ready)
         with t.cuda.stream(stream):
-            C, _ = shape.patch.device_coefficients(req.sigma, req.seed, mass=False)
+            # the lognormal field as separable modes; the coefficients are generated
+            # at the points and packed in one launch per precision
+            field = shape.patch.device_field_modes(req.sigma, req.seed)
             lap("coefficients")
-            w = shape.w64
-            # C is symmetric by construction; the NURBS weight fields belong to the shape
-            self.A64 = StiffnessMassOperator(shape.qs, C=C, m=None, nurbs_weights=w, dtype="f64", symmetric=True,
-                                             nurbs_fields=shape.nurbs_fields)
-            self.A32 = (StiffnessMassOperator(shape.qs, C=C, m=None, nurbs_weights=w, dtype="f32", symmetric=True,
-                                              nurbs_fields=shape.nurbs_fields)
-                        if mode == "mixed" else None)
+            kw = dict(nurbs_fields=shape.nurbs_fields, nurbs_w64=shape.w64)
+            self.A64 = StiffnessMassOperator.from_extruded(shape.patch, field, "f64", **kw)
+            self.A32 = StiffnessMassOperator.from_extruded(shape.patch, field, "f32", **kw) if mode == "mixed" else None
             lap("operators")
-            del C
+            del field
             gen = t.Generator(device=dv.device())
             gen.manual_seed(int(req.seed) + 7919)
             b = t.randn(shape.ndof, generator=gen, dtype=t.float64, device=dv.device())
@@ -345,6 +348,8 @@ class SolveServer:
         such request's latency."""
         from .operators import FORM_K6, FORM_SYM10, fused_plan
 
+        # the device is opened before the worker processes start (they never use it)
+        dv.require_gpu()
         info = prepare_shape_table(degrees, nels, workers)
         t0 = time.perf_counter()
         for p in degrees:

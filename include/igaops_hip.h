@@ -281,6 +281,29 @@ int iga_fused_apply_sliced2(iga_fused_t plan, const void* packed, int64_t coef_s
 int iga_fused_pack_strided(iga_fused_t plan, const double* C, const double* m, const double* W,
                            const double* dW, int64_t comp_stride, void* packed, void* stream);
 
+/* Packs the coefficients of a cross-section map extruded along xi_3 without the
+ * (3,3,N_q) array: the same values as conductivity_coefficients of the extruded
+ * patch times a lognormal field, followed by iga_fused_pack, generated point by point
+ * (q = plane + n_plane q3, plane = q1 + nq1 q2):
+ *   det = dz[q3] det2[plane],  kappa = exp(sigma Re sum_k Ez[q3][k] H[plane][k]),
+ *   C_ab = kappa det inv_ab[plane] (a, b in {1,2}, symmetric),  C_33 = kappa det / dz^2,
+ *   m = det when with_mass, else 0.
+ * plane (device float64, 4 x n_plane): det2, inv_11, inv_22, inv_12 of the cross-section
+ * (inverse of J_2^T J_2); dz (n3): dz/dxi_3.  H (n_plane x modes) and Ez (n3 x modes)
+ * are complex (re, im interleaved); H = NULL gives kappa = 1.  W / dW as in
+ * iga_fused_pack. */
+int iga_fused_pack_extruded(iga_fused_t plan, const double* plane, const double* dz, const double* H,
+                            const double* Ez, int modes, double sigma, int with_mass, const double* W,
+                            const double* dW, void* packed, void* stream);
+
+/* Cross-section factor of the separable Fourier field of the benchmark's lognormal
+ * conductivity (no counterpart in the reference, whose callers pass coefficients):
+ *   H[p][c] = sum_{a,b} coef[a][b][c] exp(i pi (a+1) x_p) exp(i pi (b+1) y_p),
+ * xy (device, 2 x n_plane): the scaled cross-section coordinates; coef (device,
+ * complex modes^3, c fastest); modes 1..16. */
+int iga_lognormal_plane_modes(int64_t nplane, int modes, const double* xy, const double* coef, double* H,
+                              void* stream);
+
 /* ---- fast-diagonalization apply --------------------------------------------
  * Replaces SpaceFD.apply (fastdiag/utils/space_precond.py:273-320) for one DOF per
  * node: y[free] = U (Lambda^{-1} o U^T x[free]), y = 0 outside the free box, where

@@ -78,6 +78,43 @@ def test_device_coefficients_match_host():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("geometry,mass", [("cube", False), ("cube", True), ("annulus", False)])
+@pytest.mark.parametrize("sigma", [0.0, 0.6])
+def test_generated_coefficients_match_the_oracle(geometry, mass, sigma):
+    """The operator of generated coefficients (``from_extruded``: field modes, then
+    one generating pack launch) against the oracle on the host coefficients
+    (``lognormal_conductivity`` and ``conductivity_coefficients``): float64 within
+    1e-12, float32 within 5e-6; its packed coefficients equal those packed from the
+    device (3, 3, N_q) array to 1e-13."""
+    import torch
+
+    from igaops_amd.geometry import lognormal_conductivity, quarter_annulus, unit_cube
+    from igaops_amd.operators import StiffnessMassOperator
+    from oracle import operators as oo
+
+    patch = (unit_cube if geometry == "cube" else quarter_annulus)(3, 6)
+    kappa = lognormal_conductivity(patch, sigma, seed=9) if sigma > 0 else None
+    C, m = patch.conductivity_coefficients(kappa)
+    if not mass:
+        m = np.zeros_like(m)
+    w = patch.nurbs_weights if patch.is_nurbs else None
+    n = patch.nbctrlpts_total
+    x = np.random.default_rng(4).standard_normal(n)
+    ref = oo.stiffness_plus_mass(patch.quadrule_list, C, m, x, nurbs_weights=w)
+    xd = torch.as_tensor(x, device="cuda")
+    for dt, tol in (("f64", 1e-12), ("f32", 5e-6)):
+        field = patch.device_field_modes(sigma, 9)
+        op = StiffnessMassOperator.from_extruded(patch, field, dtype=dt, mass=mass)
+        y = op.apply(xd.to(op.packed.dtype)).double().cpu().numpy()
+        assert np.linalg.norm(y - ref) <= tol * np.linalg.norm(ref)
+        Cd, md = patch.device_coefficients(sigma, 9, mass=mass)
+        old = StiffnessMassOperator(patch.quadrule_list, C=Cd, m=md, nurbs_weights=w, dtype=dt, symmetric=True)
+        assert op.form == old.form
+        a, b = op.packed.double(), old.packed.double()
+        assert float((a - b).abs().max()) <= (1e-13 if dt == "f64" else 1e-6) * float(b.abs().max())
+
+
+@pytest.mark.gpu
 def test_admission_respects_memory_budget():
     from igaops_amd.serving import SolveServer
 
