@@ -24,7 +24,9 @@ How the loop works:
   work of many requests is in flight on the device at once.  Same-shape requests
   share plans; when several of them are ready for an Arnoldi step in the same
   pass, their FD preconditioner applications go out as one multi-right-hand-side
-  launch (the FD depends on the shape only) on a batch stream, ordered by events.
+  launch (the FD depends on the shape only: parametric pencils with per-direction
+  correctors from the mean geometric conductivity) on a batch stream, ordered by
+  events.
   Nothing a request computes depends on which others are present, so its result
   is the same alone and in the mix (to rounding: the batched FD sums in the same
   order per column).
@@ -46,6 +48,7 @@ from . import device as dv
 
 _RULES: Dict[Tuple[int, int], object] = {}
 _PENCILS: Dict[Tuple[int, int], tuple] = {}
+_CORRECTORS: Dict[Tuple[int, int], np.ndarray] = {}
 
 
 def wq_rule(degree: int, nel: int):
@@ -74,9 +77,29 @@ def fd_pencil(degree: int, nel: int):
     return e
 
 
+def annulus_correctors(degree: int, nel: int) -> np.ndarray:
+    """Per-direction FD stiffness correctors of the quarter annulus of ``(degree,
+    nel)``: the mean over the quadrature points of the diagonal of its geometric
+    conductivity ``detJ J^+ J^+T`` (the scalar correctors of
+    ``fastdiag/preconditioner.py:110-144``, which the reference leaves to the caller).
+    The angular direction is up to three times longer than the radial one, which a
+    parametric FD ignores.  Cached by ``(degree, nel)``."""
+    from .geometry import quarter_annulus
+
+    key = (int(degree), int(nel))
+    c = _CORRECTORS.get(key)
+    if c is None:
+        plane, dz, _, _ = quarter_annulus(degree, nel, quadrule=wq_rule(*key)).extrusion_factors()
+        det2, i00, i11 = plane[0], plane[1], plane[2]
+        c = np.array([np.mean(dz) * np.mean(det2 * i00), np.mean(dz) * np.mean(det2 * i11),
+                      np.mean(det2) * np.mean(1.0 / dz)])
+        _CORRECTORS[key] = c
+    return c
+
+
 def _host_setup(key):
     """Worker-process body of :func:`prepare_shape_table`."""
-    return key, wq_rule(*key), fd_pencil(*key)
+    return key, wq_rule(*key), fd_pencil(*key), annulus_correctors(*key)
 
 
 def _worker_init():
@@ -91,14 +114,16 @@ def _worker_init():
 
 
 def prepare_shape_table(degrees, nels, workers: int = 8) -> dict:
-    """Build the rule and FD pencil of every ``(degree, nel)`` of a declared range
-    (the host set-up a request of that size needs) in a pool of ``workers`` processes,
+    """Build the rule, FD pencil and annulus FD correctors of every ``(degree, nel)``
+    of a declared range (the host set-up a request of that size needs) in a pool of
+    ``workers`` processes,
     largest first.  Entries already cached are skipped.  Returns the number built and
     the wall time, which a server reports as its start-up cost."""
     import multiprocessing
     from concurrent.futures import ProcessPoolExecutor
 
-    keys = sorted({(int(p), int(n)) for p in degrees for n in nels} - (_RULES.keys() & _PENCILS.keys()),
+    keys = sorted({(int(p), int(n)) for p in degrees for n in nels} - (_RULES.keys() & _PENCILS.keys()
+                                                                         & _CORRECTORS.keys()),
                   key=lambda k: -(k[0] + 1) * k[1])
     t0 = time.perf_counter()
     if workers > 1 and len(keys) > 1:
@@ -106,8 +131,8 @@ def prepare_shape_table(degrees, nels, workers: int = 8) -> dict:
         ctx = multiprocessing.get_context("spawn")
         with ProcessPoolExecutor(max_workers=min(workers, len(keys)), mp_context=ctx,
                                  initializer=_worker_init) as ex:
-            for key, q, e in ex.map(_host_setup, keys, chunksize=2):
-                _RULES[key], _PENCILS[key] = q, e
+            for key, q, e, c in ex.map(_host_setup, keys, chunksize=2):
+                _RULES[key], _PENCILS[key], _CORRECTORS[key] = q, e, c
     else:
         for key in keys:
             _host_setup(key)
@@ -141,6 +166,7 @@ class SolveResult:
     admitted: float
     completed: float
     solution: object = None
+    reason: str = ""
 
     @property
     def latency(self) -> float:
@@ -176,11 +202,11 @@ class ShapeResources:
 
 
 class FDResources:
-    """The FD preconditioner of one ``(degree, nel)``, shared by both geometries: the
-    preconditioner is built on the parametric Gauss rewrite of the rules, with every
-    face Dirichlet, so it does not depend on the map."""
+    """The FD preconditioner of one ``(geometry, degree, nel)``: the parametric Gauss
+    pencils of the rules (every face Dirichlet) with per-direction stiffness
+    correctors, 1 for the unit cube and :func:`annulus_correctors` for the annulus."""
 
-    def __init__(self, degree: int, nel: int, dtypes=("f64", "f32")):
+    def __init__(self, geometry: str, degree: int, nel: int, dtypes=("f64", "f32")):
         from .fastdiag import SpaceFD
         from .solvers import Mask
 
@@ -188,6 +214,8 @@ class FDResources:
         pencil = fd_pencil(degree, nel)
         fd = SpaceFD()
         fd.set_space_eigenpairs(np.ones((1, 3, 2), bool), [q.nbctrlpts] * 3, [pencil] * 3)
+        if geometry == "annulus":
+            fd.add_scalar_space_correctors(stiffness_corrector=[annulus_correctors(degree, nel)])
         fd.update_space_eigenvalues([0.0, 1.0])
         self.space_fd = fd
         self.mask = Mask(box=fd.box())
@@ -305,7 +333,7 @@ class _Active:
         x = out.solution
         res = SolveResult(self.req.rid, self.req.shape, self.shape.ndof, int(out.iterations), rel,
                           bool(out.success), self.req.arrival, self.admitted, completed,
-                          x.clone() if keep_solution else None)
+                          x.clone() if keep_solution else None, str(getattr(out, "reason", "")))
         # release everything this request holds on the device
         self.run = self.A64 = self.A32 = self.b = self._fd_work = self._check = None
         return res
@@ -355,8 +383,9 @@ class SolveServer:
         for p in degrees:
             for nel in nels:
                 key = (int(p), int(nel))
-                if key not in self.fds:
-                    self.fds[key] = FDResources(*key, dtypes=self.dtypes)
+                for geo in ("cube", "annulus"):
+                    if (geo,) + key not in self.fds:
+                        self.fds[(geo,) + key] = FDResources(geo, *key, dtypes=self.dtypes)
                 q = wq_rule(*key)
                 for form in (FORM_K6, FORM_SYM10):  # unit cube; quarter annulus (NURBS)
                     for dt in self.dtypes:
@@ -377,8 +406,7 @@ class SolveServer:
         for st in states:
             run = st.run.step_run() if st.run is not None else None
             if run is not None:
-                # the FD plan depends on (degree, nel) only: both geometries share it
-                groups.setdefault((st.req.shape[1:], run.dtype), []).append(run)
+                groups.setdefault((st.req.shape, run.dtype), []).append(run)
         done = False
         for (shape_key, dtype), runs in groups.items():
             if len(runs) < 2:
@@ -418,7 +446,7 @@ class SolveServer:
         s = self.shapes.get(key)
         if s is None:
             t0 = time.perf_counter()
-            fk = tuple(key[1:])
+            fk = tuple(key)
             fd = self.fds.get(fk)
             if fd is None:
                 fd = FDResources(*fk, dtypes=self.dtypes)
@@ -549,8 +577,12 @@ def poisson_stream(n: int, qps: float, seed: int = 0, nel_range=(16, 96), degree
 
 def summarize(results: List[SolveResult], tol: float, wall: float) -> dict:
     lat = np.array([r.latency for r in results]) if results else np.zeros(1)
-    ok = sum(1 for r in results if r.success and r.true_rel_residual <= tol * 1.0001)
+    good = lambda r: r.success and r.true_rel_residual <= tol * 1.0001
+    ok = sum(1 for r in results if good(r))
+    failures = [{"rid": r.rid, "shape": list(r.shape), "iterations": r.iterations, "reason": r.reason,
+                 "true_rel_residual": r.true_rel_residual} for r in results if not good(r)][:10]
     return {
+        "failures": failures,
         "requests": len(results),
         "reached_tol": ok,
         "solves_per_s": ok / wall if wall > 0 else 0.0,

@@ -27,6 +27,20 @@ def test_footprint_grows_with_size_and_precision():
                                                                                          "fp64", 300, 200)
 
 
+def test_annulus_correctors_are_the_mean_geometric_conductivity():
+    """The FD correctors of an annulus shape equal the per-direction means over the
+    quadrature points of the diagonal of ``conductivity_coefficients`` (separable
+    factors on the host against the full (3, 3, N_q) array); the angular direction is
+    the smallest."""
+    from igaops_amd.geometry import quarter_annulus
+    from igaops_amd.serving import annulus_correctors, wq_rule
+
+    c = annulus_correctors(3, 7)
+    C, _ = quarter_annulus(3, 7, quadrule=wq_rule(3, 7)).conductivity_coefficients(None)
+    np.testing.assert_allclose(c, [C[d, d].mean() for d in range(3)], rtol=1e-12)
+    assert c[1] < 0.5 * min(c[0], c[2])
+
+
 def test_summarize_counts_only_converged():
     rs = [SolveResult(i, ("cube", 2, 4), 100, 10, 1e-11 if i % 2 else 1e-8, True, 0.0, 0.0, 0.1 * (i + 1))
           for i in range(10)]
