@@ -161,7 +161,8 @@ def fd_error_vs_oracle(patch, dtype, x, reps=20):
 
 def solve_section(patch, C, m, nrhs, tol=1e-10):
     """FD-preconditioned GMRES on ``(K + M) x = b`` for each right-hand side: plain
-    FP64 and FP32 operators inside FP64 iterative refinement."""
+    FP64 and FP32 operators inside FP64 iterative refinement.  The FD carries the mean
+    conductivity per direction and the mean mass coefficient as its correctors."""
     import torch
 
     from igaops_amd.fastdiag import SingleFD
@@ -174,6 +175,10 @@ def solve_section(patch, C, m, nrhs, tol=1e-10):
     n = patch.nbctrlpts_total
     fd = SingleFD("spatial")
     fd.compute_space_eigendecomposition(qs, np.ones((1, 3, 2), bool))
+    # the reference's scalar correctors (fastdiag/preconditioner.py:110-144): mean
+    # conductivity per direction and mean mass coefficient over the points
+    fd.add_scalar_space_time_correctors(mass_corrector=[float(np.mean(m))],
+                                        stiffness_corrector=[np.array([np.mean(C[d, d]) for d in range(3)])])
     fd.update_space_eigenvalues([1.0, 1.0])
     mask = Mask(box=fd.space_preconditioner.box())
     A64 = StiffnessMassOperator(qs, C=C, m=m, nurbs_weights=w, dtype="f64")
