@@ -82,12 +82,12 @@ SIGNATURES = {
     "iga_stfd_apply": (_i32, [_vp, _vp, _vp, _i32, _i32, _vp, _vp]),
     "iga_stfd_destroy": (_i32, [_vp]),
     "iga_arnoldi_create": (_i32, [C.POINTER(_vp), _i32, _i64, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _i64p, _i64p,
-                                  _i64p, _vp, _vp, _vp, _vp, _vp]),
+                                  _i64p, _vp, _vp, _vp, _vp, _vp, _i64]),
     "iga_arnoldi_start": (_i32, [_vp, _vp, _vp, _vp]),
     "iga_arnoldi_step": (_i32, [_vp, _i32, _vp, _i64, _i32, _vp]),
-    "iga_arnoldi_query": (_i32, [_vp, _i32]),
-    "iga_arnoldi_wait": (_i32, [_vp, _i32]),
     "iga_arnoldi_destroy": (_i32, [_vp]),
+    "iga_host_alloc": (_i32, [_i64, C.POINTER(_vp)]),
+    "iga_host_free": (_i32, [_vp]),
 }
 
 _ERRORS = {-1: ValueError, -2: RuntimeError, -3: MemoryError, -4: NotImplementedError, -5: RuntimeError}

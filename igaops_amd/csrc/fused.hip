@@ -28,6 +28,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "internal.hpp"
 
 namespace iga {
 
@@ -531,6 +532,8 @@ struct GatherArgs {
   int64_t y_dstride, y_rstride;  // y[dof * y_dstride + k * y_rstride]
   T alpha;
   int accumulate;
+  int box;               // zero y outside the DOF box [blo, bhi) (the Dirichlet mask)
+  int blo[3], bhi[3];
 };
 
 template <typename T>
@@ -600,6 +603,9 @@ __global__ void __launch_bounds__(256) fused_gather_kernel(GatherArgs<T> g) {
     T v = g.alpha * acc;
     if (g.w) v *= g.w[dof];
     if (g.accumulate) v += g.y[yi];
+    if (g.box && (c1 < g.blo[0] || c1 >= g.bhi[0] || c2 < g.blo[1] || c2 >= g.bhi[1] || c3 < g.blo[2] ||
+                  c3 >= g.bhi[2]))
+      v = T(0);
     g.y[yi] = v;
   }
 }
@@ -966,7 +972,8 @@ struct Layout {
 template <typename T>
 static void run_apply(const iga_fused_s* f, const void* coef, const void* x, void* y, int nrhs, const void* w,
                       double alpha, int accumulate, void* work, hipStream_t s, int stages = 3,
-                      Layout lay = Layout{0, 0, 0, 0, 0}, const void* x1 = nullptr) {
+                      Layout lay = Layout{0, 0, 0, 0, 0}, const void* x1 = nullptr,
+                      const int64_t* box_lo = nullptr, const int64_t* box_len = nullptr) {
   if (lay.xd == 0) lay = Layout::interleaved(nrhs);
   IGA_CHECK_ARG((f->form == FORM_KM7X) == (x1 != nullptr),
                 "the two-input form takes a second input (iga_fused_apply_sliced2), the others do not");
@@ -1023,6 +1030,13 @@ static void run_apply(const iga_fused_s* f, const void* coef, const void* x, voi
   g.y_rstride = lay.yr;
   g.alpha = (T)alpha;
   g.accumulate = accumulate;
+  if (box_lo) {
+    g.box = 1;
+    for (int d = 0; d < 3; ++d) {
+      g.blo[d] = (int)box_lo[d];
+      g.bhi[d] = (int)(box_lo[d] + box_len[d]);
+    }
+  }
   const int64_t total = (int64_t)f->nc[0] * f->nc[1] * f->nc[2] * nrhs;
   fused_gather_kernel<T><<<grid_for(total, 256), 256, 0, s>>>(g);
   IGA_LAUNCH_CHECK();
@@ -1374,3 +1388,20 @@ int iga_fused_apply_sliced2(iga_fused_t f, const void* packed, int64_t coef_slic
 }
 
 }  // extern "C"
+
+namespace iga {
+
+void fused_apply_masked(iga_fused_t f, const void* packed, const void* x, void* y, const void* nurbs_w, void* work,
+                        const int64_t* box_lo, const int64_t* box_len, void* stream) {
+  IGA_CHECK_ARG(f && packed && x && y && work && box_lo && box_len, "NULL argument");
+  for (int d = 0; d < 3; ++d)
+    IGA_CHECK_ARG(box_lo[d] >= 0 && box_len[d] >= 0 && box_lo[d] + box_len[d] <= f->nc[d], "box outside the grid");
+  if (f->dtype == IGA_F64)
+    run_apply<double>(f, packed, x, y, 1, nurbs_w, 1.0, 0, work, as_stream(stream), 3, Layout{0, 0, 0, 0, 0}, nullptr,
+                      box_lo, box_len);
+  else
+    run_apply<float>(f, packed, x, y, 1, nurbs_w, 1.0, 0, work, as_stream(stream), 3, Layout{0, 0, 0, 0, 0}, nullptr,
+                     box_lo, box_len);
+}
+
+}  // namespace iga

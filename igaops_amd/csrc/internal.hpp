@@ -1,0 +1,23 @@
+// Functions shared between the library's source files; not part of the C ABI.
+// They throw iga::Error (callers run them inside guard()).
+#pragma once
+
+#include <cstdint>
+
+#include "common.hpp"
+
+namespace iga {
+
+// iga_gmres_start / iga_gmres_step with the hist entries also published to
+// host-visible memory `rec` under `tag` (krylov.hip; rec == nullptr: not published).
+void gmres_start_impl(int dtype, int64_t n, int m, const void* r, void* V0, double* ls, void* work, double* rec,
+                      int64_t tag, void* stream);
+void gmres_step_impl(int dtype, int64_t n, int k, int m, void* V, int64_t ldv, void* w, double* ls, void* work,
+                     double* rec, int64_t tag, void* stream);
+
+// iga_fused_apply whose output is zeroed outside the box [lo, lo + len) of the DOF grid
+// (the Dirichlet mask, applied by the gather that writes y; fused.hip).
+void fused_apply_masked(iga_fused_t f, const void* packed, const void* x, void* y, const void* nurbs_w, void* work,
+                        const int64_t* box_lo, const int64_t* box_len, void* stream);
+
+}  // namespace iga

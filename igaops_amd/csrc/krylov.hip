@@ -4,7 +4,10 @@
 // second pass over the per-block partials.
 #include <hip/hip_runtime.h>
 
+#include <cstring>
+
 #include "common.hpp"
+#include "internal.hpp"
 
 namespace iga {
 
@@ -372,8 +375,19 @@ __global__ void gm_zero_kernel(int64_t n, double* __restrict__ p) {
     p[i] = 0.0;
 }
 
+// Hist entry d (g[d], rho_d) also goes to host-visible memory when `rec` is set:
+// rec[2 + 2d], rec[3 + 2d], then the release store of tag + d into rec[0] (as int64),
+// so that a host that reads rec[0] and finds tag + d sees the values of every entry up
+// to d.  The tag tells one solve's entries from another's in a reused buffer.
+__device__ inline void gm_publish(double* rec, int64_t tag, int d, double g, double rho) {
+  if (!rec) return;
+  rec[2 + 2 * d] = g;
+  rec[3 + 2 * d] = rho;
+  __hip_atomic_store(reinterpret_cast<int64_t*>(rec), tag + d, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // beta = ||r|| from scal[2]; g[0] = beta, scal[3] = 1/beta (0 for r = 0), hist[0] = (beta, beta)
-__global__ void gm_start_kernel(GmState st, double* ls) {
+__global__ void gm_start_kernel(GmState st, double* ls, double* rec, int64_t tag) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   double* sc = ls + st.scal();
   const double beta = sqrt(fmax(sc[2], 0.0));
@@ -381,6 +395,7 @@ __global__ void gm_start_kernel(GmState st, double* ls) {
   sc[3] = beta > 0.0 ? 1.0 / beta : 0.0;
   ls[st.hist()] = beta;
   ls[st.hist() + 1] = beta;
+  gm_publish(rec, tag, 0, beta, beta);
 }
 
 // The small dense part of step j + 1 (one block): finalise column j - 1, rotate
@@ -390,7 +405,7 @@ __global__ void gm_start_kernel(GmState st, double* ls) {
 constexpr int kGmPer = (kGmMaxDim + kGmThreads - 1) / kGmThreads;  // indices l per thread
 
 __global__ void __launch_bounds__(kGmThreads) __attribute__((amdgpu_waves_per_eu(1, 1)))
-gm_dcgs2_kernel(GmState st, int j, double* ls) {
+gm_dcgs2_kernel(GmState st, int j, double* ls, double* rec, int64_t tag) {
   extern __shared__ double sh[];  // s[j] | col[j+1] | cr[j] | sr[j]   (y[j+1] reuses col)
   __shared__ double red[2][kGmThreads / 64];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -505,6 +520,7 @@ gm_dcgs2_kernel(GmState st, int j, double* ls) {
       ls[st.g() + j - 1] = cc * gj1;
       ls[st.hist() + 2 * j] = -sv * gj1;
       ls[st.hist() + 2 * j + 1] = rho;
+      gm_publish(rec, tag, j, -sv * gj1, rho);
     }
     __syncthreads();
     double* Rc = ls + st.R() + (int64_t)(j - 1) * st.m;
@@ -717,23 +733,51 @@ int iga_gmres_layout(int m, int64_t* offsets) {
 }
 
 int iga_gmres_start(int dtype, int64_t n, int m, const void* r, void* V0, double* ls, void* work, void* stream) {
-  return guard([&] {
-    IGA_CHECK_ARG(r && V0 && ls && work && n >= 1 && m >= 1, "bad argument");
-    IGA_CHECK_ARG(m <= kGmMaxDim, "Krylov dimension above 2000");
-    hipStream_t s = as_stream(stream);
-    const GmState st{m};
-    gm_zero_kernel<<<grid_for(st.size(), 256, 1024), 256, 0, s>>>(st.size(), ls);
-    IGA_LAUNCH_CHECK();
-    IGA_CHECK(iga_dot_many(dtype, n, 1, r, n, r, ls + st.scal() + 2, work, stream));
-    gm_start_kernel<<<1, 64, 0, s>>>(st, ls);
-    IGA_LAUNCH_CHECK();
-    IGA_CHECK(iga_axpby(dtype, n, 0.0, ls + st.scal() + 3, 0, r, 0.0, V0, stream));
-  });
+  return guard([&] { gmres_start_impl(dtype, n, m, r, V0, ls, work, nullptr, 0, stream); });
 }
 
 int iga_gmres_step(int dtype, int64_t n, int k, int m, void* V, int64_t ldv, void* w, double* ls, void* work,
                    void* stream) {
+  return guard([&] { gmres_step_impl(dtype, n, k, m, V, ldv, w, ls, work, nullptr, 0, stream); });
+}
+
+int iga_host_alloc(int64_t bytes, void** out) {
   return guard([&] {
+    IGA_CHECK_ARG(out && bytes >= 1, "bad argument");
+    void* p = nullptr;
+    IGA_HIP(hipHostMalloc(&p, (size_t)bytes, hipHostMallocCoherent));
+    std::memset(p, 0, (size_t)bytes);
+    *out = p;
+  });
+}
+
+int iga_host_free(void* p) {
+  return guard([&] {
+    if (p) IGA_HIP(hipHostFree(p));
+  });
+}
+
+}  // extern "C"
+
+namespace iga {
+
+void gmres_start_impl(int dtype, int64_t n, int m, const void* r, void* V0, double* ls, void* work, double* rec,
+                      int64_t tag, void* stream) {
+  IGA_CHECK_ARG(r && V0 && ls && work && n >= 1 && m >= 1, "bad argument");
+  IGA_CHECK_ARG(m <= kGmMaxDim, "Krylov dimension above 2000");
+  hipStream_t s = as_stream(stream);
+  const GmState st{m};
+  gm_zero_kernel<<<grid_for(st.size(), 256, 1024), 256, 0, s>>>(st.size(), ls);
+  IGA_LAUNCH_CHECK();
+  IGA_CHECK(iga_dot_many(dtype, n, 1, r, n, r, ls + st.scal() + 2, work, stream));
+  gm_start_kernel<<<1, 64, 0, s>>>(st, ls, rec, tag);
+  IGA_LAUNCH_CHECK();
+  IGA_CHECK(iga_axpby(dtype, n, 0.0, ls + st.scal() + 3, 0, r, 0.0, V0, stream));
+}
+
+void gmres_step_impl(int dtype, int64_t n, int k, int m, void* V, int64_t ldv, void* w, double* ls, void* work,
+                     double* rec, int64_t tag, void* stream) {
+  {
     IGA_CHECK_ARG(V && w && ls && work && n >= 1 && ldv >= n, "bad argument");
     IGA_CHECK_ARG(k >= 1 && k <= m + 1, "step outside the Krylov dimension");
     IGA_CHECK_ARG(m <= kGmMaxDim, "Krylov dimension above 2000");
@@ -745,7 +789,7 @@ int iga_gmres_step(int dtype, int64_t n, int k, int m, void* V, int64_t ldv, voi
     const void* u = (const char*)V + (size_t)j * ldv * es;
     // first pass: V[0..j] against u = V[j] and w
     dot_many_launch(dtype, n, j + 1, V, ldv, u, w, 2, ls + st.U(), st.W() - st.U(), work, s);
-    gm_dcgs2_kernel<<<1, kGmThreads, sizeof(double) * (4 * (size_t)j + 1), s>>>(st, j, ls);
+    gm_dcgs2_kernel<<<1, kGmThreads, sizeof(double) * (4 * (size_t)j + 1), s>>>(st, j, ls, rec, tag);
     IGA_LAUNCH_CHECK();
     // second pass
     const dim3 grid(grid_for(n, 256, 8192)), block(256);
@@ -757,7 +801,7 @@ int iga_gmres_step(int dtype, int64_t n, int k, int m, void* V, int64_t ldv, voi
       gm_update_kernel<float><<<grid, block, shm, s>>>(n, j, (float*)V, ldv, (const float*)w, ls + st.cq(),
                                                        ls + st.cw(), ls + st.scal());
     IGA_LAUNCH_CHECK();
-  });
+  }
 }
 
-}  // extern "C"
+}  // namespace iga

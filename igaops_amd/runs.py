@@ -57,6 +57,33 @@ def _relative_or_absolute(tol: float, rel: float, absv: float) -> str:
     return ""
 
 
+class HostRecord:
+    """Coherent pinned host memory (``iga_host_alloc``) that the device writes the
+    residual estimates of a solve into: ``values[2 + 2d]``, ``values[3 + 2d]`` for
+    hist entry d, and ``tag + d`` into the int64 at the front once they are written."""
+
+    def __init__(self, ndoubles: int):
+        ptr = C.c_void_p()
+        _lib.call("iga_host_alloc", int(ndoubles) * 8, C.byref(ptr))
+        self.ptr = int(ptr.value)
+        self.size = int(ndoubles)
+        self.values = np.ctypeslib.as_array((C.c_double * self.size).from_address(self.ptr))
+        self.seq = self.values.view(np.int64)[:1]
+
+
+_RECORDS: dict = {}  # free host records by size; a record outlives the solves that use it
+_TAGS = iter(range(1, 1 << 30))
+
+
+def _take_record(ndoubles: int) -> HostRecord:
+    free = _RECORDS.setdefault(ndoubles, [])
+    return free.pop() if free else HostRecord(ndoubles)
+
+
+def _give_record(rec: HostRecord):
+    _RECORDS.setdefault(rec.size, []).append(rec)
+
+
 class NativeStep:
     """The pieces of a GMRES solve that the one-call Arnoldi step (``iga_arnoldi_*``)
     takes over from ``A``, ``P`` and ``mask``: a fused ``StiffnessMassOperator``, an
@@ -145,31 +172,40 @@ class GmresRun:
         self._sol_host = None
         self.result = None
         self._ctx = None
+        self._rec = None
         if native is not None:
             if native.op.dtype != self.dtype:
                 raise ValueError("the native operator does not take the run's vector type")
-            if self.lookahead > 8:
-                raise ValueError("the one-call step keeps at most 16 residual estimates in flight")
             op, fd = native.op, native.fd_plan
             bx = native.box
             nd = 0 if bx is None else len(bx[0])
             bp = [None, None, None] if bx is None else [a.ctypes.data_as(_lib._i64p) for a in bx]
             self._native = native
+            # the device publishes the residual estimates here under this solve's tag
+            self._rec = _take_record(2 * (self.m + 2))
+            self._tag = next(_TAGS)
             ctx = C.c_void_p()
             _lib.call("iga_arnoldi_create", C.byref(ctx), dv.DTYPES[self.dtype], n, self.m, op.plan._h,
                       dv.ptr(op.packed), dv.ptr(op.w), dv.ptr(op.workspace(1)),
                       None if fd is None else fd._h, dv.ptr(native.fd_work), nd, *bp, dv.ptr(self.z),
-                      dv.ptr(self.w), dv.ptr(self.ls), dv.ptr(self.work), self.hist_host.data_ptr())
+                      dv.ptr(self.w), dv.ptr(self.ls), dv.ptr(self.work), self._rec.ptr, self._tag << 32)
             self._ctx = ctx
 
     def __del__(self):
-        self._drop_native()
+        self._drop_native(give_back=False)
 
-    def _drop_native(self):
+    def _drop_native(self, give_back: bool = True):
+        """Destroy the step context; with ``give_back`` the host record returns to the
+        pool (only once every step of this solve has finished: the solution copy,
+        queued after them, has been waited for)."""
         ctx = getattr(self, "_ctx", None)
         if ctx is not None and ctx.value and _lib._lib is not None:
             _lib.load().iga_arnoldi_destroy(ctx)
         self._ctx = None
+        rec = getattr(self, "_rec", None)
+        if rec is not None and give_back:
+            _give_record(rec)
+        self._rec = None
 
     @staticmethod
     def host_doubles(m: int) -> int:
@@ -190,16 +226,19 @@ class GmresRun:
         self.events[step] = ev
         self.recorded = step
 
+    def _published(self, j: int) -> bool:
+        seq = int(self._rec.seq[0])
+        return (seq >> 32) == self._tag and (seq & 0xFFFFFFFF) >= j
+
     def _arrived(self, j: int, block: bool) -> bool:
         """Whether residual estimate j is on the host (waiting for it with ``block``)."""
         if self._ctx is not None:
-            rc = _lib.load().iga_arnoldi_query(self._ctx, j)
-            if rc < 0:
-                _lib.check(rc, "iga_arnoldi_query")
-            if rc == 1:
+            if self._published(j):
                 return True
             if block:
-                _lib.call("iga_arnoldi_wait", self._ctx, j)
+                (self.stream or dv.torch().cuda.current_stream()).synchronize()
+                if not self._published(j):
+                    raise RuntimeError(f"residual estimate {j} was not published by the device")
                 return True
             return False
         ev = self.events[j]
@@ -311,7 +350,10 @@ class GmresRun:
             block = False
             self.known = j
             progressed = True
-            g, nw = float(self.hist_host[2 * j]), float(self.hist_host[2 * j + 1])
+            if self._ctx is not None:
+                g, nw = float(self._rec.values[2 + 2 * j]), float(self._rec.values[3 + 2 * j])
+            else:
+                g, nw = float(self.hist_host[2 * j]), float(self.hist_host[2 * j + 1])
             if j == 0:
                 # relative to ||b - A x0|| (the reference shifts b by A x0 and starts
                 # from zero, linear_solver.py:228-236)
@@ -391,6 +433,10 @@ class GmresRun:
             # arithmetic makes the least-squares solution exact; the true residual decides
             success = self._true_residual_meets_tol()
         self.result = OutputArgs(self.x, np.asarray(self.res_hist), success, int(k or 0), self.reason)
+        if self._rec is not None and not k and self.launched > 0:
+            # no solution copy was queued behind the steps already launched: wait for them
+            # before the host record can serve another solve
+            (self.stream or t.cuda.current_stream()).synchronize()
         self._drop_native()
         return self.result
 

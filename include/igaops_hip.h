@@ -331,33 +331,39 @@ int iga_fd_destroy(iga_fd_t plan);
 /* ---- one right-preconditioned Arnoldi step per call ---------------------------
  * The loop body of GMRES (solvers/linear_solver.py:438-525: z = P v_{k-1},
  * w = A z, the Dirichlet mask of w, linear_solver.py:119-124, then the
- * orthogonalisation and Givens update of iga_gmres_step) queued by one call, with
- * the residual estimate of the step copied to pinned host memory behind an event.
- * A context fixes the fused operator (plan, packed coefficients, NURBS weights or
+ * orthogonalisation and Givens update of iga_gmres_step) queued by one call.  A
+ * context fixes the fused operator (plan, packed coefficients, NURBS weights or
  * NULL, its partial-block workspace), the FD preconditioner (NULL: z = v_{k-1}; the
  * FD writes zeros outside its free box, which must be the mask's box), the mask box
- * (ndim 0: none), the run's vectors z and w (n values of dtype), its least-squares
- * state ls (iga_gmres_state_doubles(m)) and reduction workspace, and hist_host,
- * pinned host memory of 2 (m + 2) doubles.
- * iga_arnoldi_start: iga_gmres_start(r -> V row 0), then hist entry 0 to the host.
- * iga_arnoldi_step (k = 1..m+1): the step k of iga_gmres_step on the basis V (rows
- * of stride ldv, at least k + 1 of them); preconditioned != 0 takes z as already
- * holding P v_{k-1} (a caller that batches the FD of several runs).  For k >= 2,
- * hist entry k-1 (g[k-1], H[k-1, k-2]) goes to hist_host[2(k-1) .. 2(k-1)+1].
- * iga_arnoldi_query(j): 1 once hist entry j is on the host, 0 before, < 0 on error
- * (also when the event was reused: at most 16 entries may be outstanding).
- * iga_arnoldi_wait(j): block until it is. */
+ * (ndim 3, or 0 for none; the operator's gather writes the zeros), the run's vectors
+ * z and w (n values of dtype), its least-squares state ls
+ * (iga_gmres_state_doubles(m)) and reduction workspace, and the host record.
+ * Host record: host_rec, host-visible memory from iga_host_alloc of at least
+ * 2 (m + 2) doubles, and tag, a multiple of 2^32 that identifies the solve.  Hist
+ * entry d (g[d] and H[d, d-1] of Krylov dimension d; for d = 0 both are beta) is
+ * written to host_rec[2 + 2d], host_rec[3 + 2d], and then tag + d to the int64 at
+ * host_rec[0] with release ordering: a host that reads tag + d there may read
+ * every entry up to d.  Entry 0 comes with iga_arnoldi_start, entry k - 1 with
+ * step k >= 2.  A buffer may serve solves one after the other (new tag each): the
+ * steps of a solve are queued on one stream, so they have finished once a later
+ * operation on that stream has.
+ * iga_arnoldi_start: iga_gmres_start(r -> V row 0).
+ * iga_arnoldi_step (k = 1..m+1): step k of iga_gmres_step on the basis V (rows of
+ * stride ldv, at least k + 1 of them); preconditioned != 0 takes z as already holding
+ * P v_{k-1} (a caller that batches the FD of several solves).
+ * iga_host_alloc: coherent pinned host memory (zeroed), device-writable;
+ * iga_host_free releases it (no device work may still write to it). */
 typedef struct iga_arnoldi_s* iga_arnoldi_t;
 
 int iga_arnoldi_create(iga_arnoldi_t* out, int dtype, int64_t n, int m, iga_fused_t op, const void* packed,
                        const void* nurbs_w, void* op_work, iga_fd_t fd, void* fd_work, int ndim, const int64_t* box_n,
                        const int64_t* box_lo, const int64_t* box_len, void* z, void* w, double* ls, void* work,
-                       double* hist_host);
+                       double* host_rec, int64_t tag);
 int iga_arnoldi_start(iga_arnoldi_t ctx, const void* r, void* V, void* stream);
 int iga_arnoldi_step(iga_arnoldi_t ctx, int k, void* V, int64_t ldv, int preconditioned, void* stream);
-int iga_arnoldi_query(iga_arnoldi_t ctx, int j);
-int iga_arnoldi_wait(iga_arnoldi_t ctx, int j);
 int iga_arnoldi_destroy(iga_arnoldi_t ctx);
+int iga_host_alloc(int64_t bytes, void** out);
+int iga_host_free(void* p);
 
 /* ---- space-time arrowhead fast-diagonalization apply ----------------------------
  * Replaces SpTimeFD.apply / apply_transpose (fastdiag/utils/time_arrow.py:111-192)

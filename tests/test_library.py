@@ -46,23 +46,30 @@ def test_error_path_without_gpu_compute():
 
 
 def test_arnoldi_context_rejects_bad_arguments_on_the_host():
-    """The one-call Arnoldi step checks its context on the host: missing pointers,
-    a box whose grid is not the vector, and queries on no context are errors (-1)."""
+    """The one-call Arnoldi step checks its context on the host: missing pointers, a
+    box whose grid is not the vector, a box that is not three-dimensional, and a tag
+    whose low 32 bits are not zero are errors (-1); destroying no context is a no-op."""
     import numpy as np
 
     lib = _lib.load()
     h = ctypes.c_void_p()
     fake = ctypes.c_void_p(1 << 20)  # never dereferenced: each call fails its checks first
+    tag = 5 << 32
     assert lib.iga_arnoldi_create(ctypes.byref(h), 0, 27, 5, None, fake, None, fake, None, None, 0, None, None, None,
-                                  fake, fake, fake, fake, fake) == -1
+                                  fake, fake, fake, fake, fake, tag) == -1
     n, lo, ln = (np.array(a, dtype=np.int64) for a in ((3, 3, 3), (1, 1, 1), (1, 1, 1)))
     ptrs = [a.ctypes.data_as(_lib._i64p) for a in (n, lo, ln)]
     assert lib.iga_arnoldi_create(ctypes.byref(h), 0, 28, 5, fake, fake, None, fake, None, None, 3, *ptrs,
-                                  fake, fake, fake, fake, fake) == -1
+                                  fake, fake, fake, fake, fake, tag) == -1
     buf = ctypes.create_string_buffer(256)
     lib.iga_last_error(buf, 256)
     assert b"box grid" in buf.value
-    assert lib.iga_arnoldi_query(None, 0) == -1
+    assert lib.iga_arnoldi_create(ctypes.byref(h), 0, 27, 5, fake, fake, None, fake, None, None, 2, *ptrs,
+                                  fake, fake, fake, fake, fake, tag) == -1
+    assert lib.iga_arnoldi_create(ctypes.byref(h), 0, 27, 5, fake, fake, None, fake, None, None, 3, *ptrs,
+                                  fake, fake, fake, fake, fake, tag + 1) == -1
+    lib.iga_last_error(buf, 256)
+    assert b"tag" in buf.value
     assert lib.iga_arnoldi_destroy(None) == 0
 
 
