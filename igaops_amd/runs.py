@@ -184,9 +184,14 @@ class GmresRun:
             # the device publishes the residual estimates here under this solve's tag
             self._rec = _take_record(2 * (self.m + 2))
             self._tag = next(_TAGS)
+            with self._on_stream():
+                # the operator's partial blocks are written on this run's stream: take
+                # them from its memory pool (a block from another stream's pool may still
+                # be read by that stream's pending work)
+                op_work = op.workspace(1)
             ctx = C.c_void_p()
             _lib.call("iga_arnoldi_create", C.byref(ctx), dv.DTYPES[self.dtype], n, self.m, op.plan._h,
-                      dv.ptr(op.packed), dv.ptr(op.w), dv.ptr(op.workspace(1)),
+                      dv.ptr(op.packed), dv.ptr(op.w), dv.ptr(op_work),
                       None if fd is None else fd._h, dv.ptr(native.fd_work), nd, *bp, dv.ptr(self.z),
                       dv.ptr(self.w), dv.ptr(self.ls), dv.ptr(self.work), self._rec.ptr, self._tag << 32)
             self._ctx = ctx

@@ -266,6 +266,11 @@ class _Active:
             kw = dict(nurbs_fields=shape.nurbs_fields, nurbs_w64=shape.w64)
             self.A64 = StiffnessMassOperator.from_extruded(shape.patch, field, "f64", **kw)
             self.A32 = StiffnessMassOperator.from_extruded(shape.patch, field, "f32", **kw) if mode == "mixed" else None
+            # the operators' partial blocks come from this request's stream pool (the
+            # inner solves of the mixed mode are built outside any stream context)
+            for op in (self.A64, self.A32):
+                if op is not None:
+                    op.workspace(1)
             lap("operators")
             del field
             gen = t.Generator(device=dv.device())
