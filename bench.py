@@ -70,24 +70,46 @@ def time_events(fn, steps, warmup):
     return [s.elapsed_time(e) for s, e in zip(start, end)]
 
 
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(patch, C, m, x, budget_s=20.0):
-    """The oracle's matvec (the reference algorithm in numpy/scipy) on the host."""
+    """The oracle's matvec (the reference algorithm in numpy/scipy) on the host: one
+    core (median of repeats), and all cores of this process's share (one matvec per
+    thread on independent right-hand sides, at most 16 threads; scipy's sparse
+    products release the GIL)."""
+    from concurrent.futures import ThreadPoolExecutor
+
     from oracle import operators as oo
 
     from threadpoolctl import threadpool_limits
 
     w = patch.nurbs_weights if patch.is_nurbs else None
-    with threadpool_limits(limits=1):  # one core, as reported
+    f = lambda v: oo.stiffness_plus_mass(patch.quadrule_list, C, m, v, nurbs_weights=w)
+    with threadpool_limits(limits=1):  # one BLAS thread per matvec
         t0 = time.perf_counter()
-        y = oo.stiffness_plus_mass(patch.quadrule_list, C, m, x, nurbs_weights=w)
+        y = f(x)
         first = time.perf_counter() - t0
         reps = max(1, min(5, int(budget_s / max(first, 1e-3)) - 1))
         times = []
         for _ in range(reps):
             t0 = time.perf_counter()
-            oo.stiffness_plus_mass(patch.quadrule_list, C, m, x, nurbs_weights=w)
+            f(x)
             times.append(time.perf_counter() - t0)
-    return y, float(np.median(times)), reps
+        cores = min(16, len(os.sched_getaffinity(0)))
+        X = [np.roll(x, k) for k in range(cores)]
+        with ThreadPoolExecutor(cores) as ex:
+            t0 = time.perf_counter()
+            list(ex.map(f, X))
+            t_all = time.perf_counter() - t0
+    return y, float(np.median(times)), reps, {"cores": cores, "seconds": t_all, "matvecs": cores}
 
 
 def fd_error_vs_oracle(patch, dtype, x, reps=20):
@@ -587,7 +609,7 @@ def main():
     cpu = None
     relerr = None
     if not args.no_cpu:
-        yref, t_cpu, reps = cpu_baseline(patch, C, m, x0)
+        yref, t_cpu, reps, allc = cpu_baseline(patch, C, m, x0)
         relerr = float(np.linalg.norm(y0 - yref) / np.linalg.norm(yref))
         cpu = {
             "value": n / t_cpu / 1e9,
@@ -597,6 +619,14 @@ def main():
             "sample": f"one right-hand side of the same workload, oracle matvec (numpy/scipy, one BLAS thread), "
                       f"median of {reps} after one warm-up call",
             "seconds_per_matvec": t_cpu,
+            "cpu_model": cpu_model(),
+            "all_cores": {
+                "value": n * allc["matvecs"] / allc["seconds"] / 1e9,
+                "unit": "GDOF/s",
+                "cores": allc["cores"],
+                "sample": f"{allc['matvecs']} matvecs of the same workload on {allc['cores']} threads at once "
+                          f"(independent right-hand sides), wall time {allc['seconds']:.2f} s",
+            },
         }
         fd_info = fd_error_vs_oracle(patch, args.dtype, x0)
         fd_err = fd_info["rel_err_vs_oracle"]

@@ -367,8 +367,34 @@ struct GmState {
   __host__ __device__ int64_t scal() const { return cw() + m + 1; }  // 1/rho, z_j/rho, beta^2, 1/beta
   __host__ __device__ int64_t H() const { return scal() + 8; }
   __host__ __device__ int64_t ldh() const { return (int64_t)m + 1; }
-  __host__ __device__ int64_t size() const { return H() + ldh() * ldh(); }
+  __host__ __device__ int64_t Y() const { return H() + ldh() * ldh(); }  // y = H s (m + 1)
+  __host__ __device__ int64_t size() const { return Y() + m + 1; }
 };
+
+// From this basis size on, y = H[0:j, 0:j] s is formed by gm_hs_kernel (one wave per
+// row, many work-groups) before the small dense kernel, whose single work-group
+// otherwise spends most of its time in the O(j^2) product.
+constexpr int kGmHsFrom = 24;
+
+// y_i = sum_l H[i][l] s_l for i < j with column j - 1 taken final (H[i][j-1] + s_i),
+// s = U[0..j); the Hessenberg rows start at column i - 1.  Row j (rho s_{j-1}) is
+// left to the small dense kernel.  One wave per row.
+__global__ void __launch_bounds__(256) gm_hs_kernel(GmState st, int j, double* ls) {
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= j) return;
+  const double* H = ls + st.H();
+  const double* s = ls + st.U();
+  const int64_t ldh = st.ldh();
+  double acc = 0.0;
+  for (int l = (i > 0 ? i - 1 : 0) + lane; l < j; l += 64) {
+    double h = H[(int64_t)i * ldh + l];
+    if (l == j - 1) h += s[i];
+    acc += h * s[l];
+  }
+  acc = wave_sum(acc);
+  if (lane == 0) ls[st.Y() + i] = acc;
+}
 
 __global__ void gm_zero_kernel(int64_t n, double* __restrict__ p) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -527,10 +553,14 @@ gm_dcgs2_kernel(GmState st, int j, double* ls, double* rec, int64_t tag) {
     for (int l = tid; l < j; l += kGmThreads) Rc[l] = col[l];
     __syncthreads();
   }
-  // y = H[0:j+1, 0:j] s (Hessenberg: row i starts at column i - 1), into col: the
+  // y = H[0:j+1, 0:j] s (Hessenberg: row i starts at column i - 1), into col: from
+  // gm_hs_kernel for rows below j at larger j (row j is rho s_{j-1}); otherwise the
   // rows of the row-major H dotted with s, 16 rows per wave at a time, lanes along
   // the row (the entries below the band are zero)
   double* y = col;
+  if (j >= kGmHsFrom) {
+    for (int l = tid; l <= j; l += kGmThreads) y[l] = l < j ? ls[st.Y() + l] : rho * s[j - 1];
+  } else
   for (int r0 = wave * 16; r0 <= j; r0 += 16 * (kGmThreads / 64)) {
     double acc[16];
 #pragma unroll
@@ -789,6 +819,10 @@ void gmres_step_impl(int dtype, int64_t n, int k, int m, void* V, int64_t ldv, v
     const void* u = (const char*)V + (size_t)j * ldv * es;
     // first pass: V[0..j] against u = V[j] and w
     dot_many_launch(dtype, n, j + 1, V, ldv, u, w, 2, ls + st.U(), st.W() - st.U(), work, s);
+    if (j >= kGmHsFrom) {
+      gm_hs_kernel<<<(unsigned)((j + 3) / 4), 256, 0, s>>>(st, j, ls);
+      IGA_LAUNCH_CHECK();
+    }
     gm_dcgs2_kernel<<<1, kGmThreads, sizeof(double) * (4 * (size_t)j + 1), s>>>(st, j, ls, rec, tag);
     IGA_LAUNCH_CHECK();
     // second pass

@@ -518,10 +518,17 @@ class SolveServer:
             self.stats["max_bytes"] = max(self.stats["max_bytes"], used)
             progressed = False
             ta = time.perf_counter()
-            if self.batch_fd:
+            if self.batch_fd and len(active) > 1:
+                # only requests whose shape another active request shares can batch their
+                # FD steps: they advance without launching, then the batches go out
+                seen, shared = set(), set()
                 for st, _, _ in active:
-                    progressed |= st.run.advance(launch=False)
-                progressed |= self._batched_steps([st for st, _, _ in active])
+                    (shared if st.req.shape in seen else seen).add(st.req.shape)
+                if shared:
+                    cand = [st for st, _, _ in active if st.req.shape in shared]
+                    for st in cand:
+                        progressed |= st.run.advance(launch=False)
+                    progressed |= self._batched_steps(cand)
             for st, _, _ in active:
                 progressed |= st.advance()
             tb = time.perf_counter()
