@@ -34,7 +34,7 @@ SIGNATURES = {
     "iga_axpy_many": (_i32, [_i32, _i64, _i32, _vp, _i64, _vp, _f64, _vp, _vp]),
     "iga_zgram_workspace_bytes": (_i64, [_i64, _i32, _i32]),
     "iga_zgram": (_i32, [_i64, _i32, _vp, _i32, _vp, _vp, _vp, _vp]),
-    "iga_zcombine": (_i32, [_i64, _i32, _vp, _i32, _vp, _f64, _vp, _vp]),
+    "iga_zcombine": (_i32, [_i64, _i32, _vp, _i32, _vp, _f64, _f64, _vp, _vp]),
     "iga_zdot_many": (_i32, [_i64, _i32, _vp, _i64, _vp, _vp, _vp, _vp]),
     "iga_zaxpy_many": (_i32, [_i64, _i32, _vp, _i64, _vp, _f64, _f64, _vp, _vp]),
     "iga_axpby": (_i32, [_i32, _i64, _f64, _vp, _i32, _vp, _f64, _vp, _vp]),

@@ -230,10 +230,10 @@ __global__ void zero_kernel(int64_t n, double* __restrict__ out) {
 constexpr int kCombMaxOut = 24;
 
 __global__ void __launch_bounds__(256) zcombine_kernel(int64_t n, int nv, int nout, const double* const* __restrict__ V,
-                                                       const double* __restrict__ coef, double beta,
+                                                       const double* __restrict__ coef, double alpha, double beta,
                                                        double* const* __restrict__ out) {
-  extern __shared__ double cs[];  // [nv][nout][2]
-  for (int e = threadIdx.x; e < nv * nout * 2; e += blockDim.x) cs[e] = coef[e];
+  extern __shared__ double cs[];  // [nv][nout][2], scaled by alpha
+  for (int e = threadIdx.x; e < nv * nout * 2; e += blockDim.x) cs[e] = alpha * coef[e];
   __syncthreads();
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     double re[kCombMaxOut], im[kCombMaxOut];
@@ -327,7 +327,7 @@ int iga_zgram(int64_t n, int na, const double* const* X, int nb, const double* c
   });
 }
 
-int iga_zcombine(int64_t n, int nv, const double* const* V, int nout, const double* coef, double beta,
+int iga_zcombine(int64_t n, int nv, const double* const* V, int nout, const double* coef, double alpha, double beta,
                  double* const* out, void* stream) {
   return guard([&] {
     IGA_CHECK_ARG(V && coef && out && n >= 0, "bad argument");
@@ -335,7 +335,8 @@ int iga_zcombine(int64_t n, int nv, const double* const* V, int nout, const doub
     const size_t shm = sizeof(double) * 2 * (size_t)nv * nout;
     IGA_CHECK_ARG(shm <= 64 * 1024, "too many coefficients");
     if (n == 0) return;
-    zcombine_kernel<<<grid_for(n, 256, 8192), 256, shm, as_stream(stream)>>>(n, nv, nout, V, coef, beta, out);
+    zcombine_kernel<<<grid_for(n, 256, 8192), 256, shm, as_stream(stream)>>>(n, nv, nout, V, coef, alpha, beta,
+                                                                              out);
     IGA_LAUNCH_CHECK();
   });
 }

@@ -11,7 +11,9 @@ is an (n, 2) float64 array, i.e. two right-hand sides (real, imaginary part).  A
 operator is therefore applied to a complex vector in one launch with ``nrhs = 2``
 (the reference applies it to the two parts separately, ``gcrodr.py:147-162``).  Tall
 products (``V^H w``, ``W^H V``, ``V y``, ``Y R^{-1}``) go through the one-pass block
-kernels ``iga_zgram`` / ``iga_zcombine``; the small dense problems (least squares,
+kernels ``iga_zgram`` / ``iga_zcombine`` (in an Arnoldi step the projection
+coefficients stay on the device between them; one host round trip per step brings
+them and the new norm for the least-squares residual); the small dense problems (least squares,
 ``eig``, QR of ``(p+1) x k`` matrices, Cholesky) run on the host.  The tall QR of the
 warm start is CholQR2 (two Cholesky-based passes) instead of Householder: the same
 factorisation up to the phases of the columns, which cancel in every later use.
@@ -87,8 +89,31 @@ class ZBlas:
                 pv = self._ptrs(Vj, 0)
                 po = self._ptrs(oc, 128)
                 b = beta if j0 == 0 else 1.0
-                _lib.call("iga_zcombine", self.n, len(Vj), dv.ptr(pv), len(oc), dv.ptr(cdev), float(b), dv.ptr(po),
-                          dv.stream_ptr())
+                _lib.call("iga_zcombine", self.n, len(Vj), dv.ptr(pv), len(oc), dv.ptr(cdev), 1.0, float(b),
+                          dv.ptr(po), dv.stream_ptr())
+
+    # -- device-resident coefficients: rows given by a device array of their addresses
+    #    (uploaded once), results and coefficients left on the device --
+    def row_ptrs(self, rows):
+        """Device int64 array of the rows' addresses (one upload)."""
+        t = dv.torch()
+        return t.tensor([r.data_ptr() for r in rows], dtype=t.int64).to(dv.device())
+
+    def gram_into(self, px, na: int, py, out):
+        """``out[a] = X_a^H y`` for a < na (one right-hand row y, ``py`` its address
+        array) into the complex device tensor ``out``."""
+        for a0 in range(0, na, 32):
+            nc = min(32, na - a0)
+            _lib.call("iga_zgram", self.n, nc, dv.ptr(px) + 8 * a0, 1, dv.ptr(py), dv.ptr(out) + 16 * a0,
+                      dv.ptr(self.work), dv.stream_ptr())
+
+    def combine_into(self, pv, nv: int, coef, pout, alpha: float = 1.0, beta: float = 0.0):
+        """``out = beta out + alpha sum_j coef[j] V_j`` for one output row (``pout``),
+        ``coef`` a complex device tensor of nv values."""
+        for j0 in range(0, nv, 64):
+            nc = min(64, nv - j0)
+            _lib.call("iga_zcombine", self.n, nc, dv.ptr(pv) + 8 * j0, 1, dv.ptr(coef) + 16 * j0, float(alpha),
+                      float(beta if j0 == 0 else 1.0), dv.ptr(pout), dv.stream_ptr())
 
     def norm(self, v) -> float:
         return float(np.sqrt(max(self.gram([v], [v])[0, 0].real, 0.0)))
@@ -141,8 +166,6 @@ class GCRODR:
             raise ValueError("Max no. cycles should greater than 0")
         if krylov_size < recycled_size:
             raise ValueError("Not consistent with the method")
-        if recycled_size > 24 or krylov_size > 63:
-            raise ValueError("this implementation supports krylov_size <= 63 and recycled_size <= 24")
         self.tol, self.maxiters = tolerance, maxiters
         self.m, self.k, self.max_cycles = krylov_size, recycled_size, max_cycles
         self.verbose = verbose
@@ -179,18 +202,32 @@ class GCRODR:
         if beta <= Constants.SAFEGUARD:
             return V, H, B, k, resvec[:k], y, res, beta
         zb.scale(r, 1.0 / beta, V[0])
+        # the projections stay on the device: each gram writes its coefficients where
+        # the combine that subtracts them reads them, and one copy per step brings the
+        # step's coefficients and the new norm to the host ([C part | pass 1 | pass 2 | norm])
+        pV, pw = zb.row_ptrs(V), zb.row_ptrs([w])
+        pC = zb.row_ptrs(C_rows) if kc else None
+        coef = t.empty(kc + 2 * m + 1, dtype=t.complex128, device=r.device)
+        host = t.empty(kc + 2 * m + 1, dtype=t.complex128, pin_memory=True)
         for k in range(1, m + 1):
             A(V[k - 1], tmp)
             M(tmp, w)
             if kc:
-                bcol = zb.gram(C_rows, [w])[:, 0]
-                B[:, k - 1] = bcol
-                zb.combine(C_rows, -bcol[:, None], [w], beta=1.0)
-            for _pass in range(2):
-                h = zb.gram(V[:k], [w])[:, 0]
-                H[:k, k - 1] += h
-                zb.combine(V[:k], -h[:, None], [w], beta=1.0)
-            nw = zb.norm(w)
+                zb.gram_into(pC, kc, pw, coef[:kc])
+                zb.combine_into(pC, kc, coef[:kc], pw, alpha=-1.0, beta=1.0)
+            for p in range(2):
+                seg = coef[kc + p * m: kc + p * m + k]
+                zb.gram_into(pV, k, pw, seg)
+                zb.combine_into(pV, k, seg, pw, alpha=-1.0, beta=1.0)
+            zb.gram_into(pw, 1, pw, coef[kc + 2 * m:])
+            host.copy_(coef, non_blocking=True)
+            t.cuda.current_stream().synchronize()
+            hv = host.numpy()
+            if kc:
+                B[:, k - 1] = hv[:kc]
+            H[:k, k - 1] += hv[kc:kc + k]
+            H[:k, k - 1] += hv[kc + m:kc + m + k]
+            nw = float(np.sqrt(max(hv[kc + 2 * m].real, 0.0)))
             H[k, k - 1] = nw
             if nw > 0.0:
                 zb.scale(w, 1.0 / nw, V[k])

@@ -111,7 +111,7 @@ int iga_zaxpy_many(int64_t n, int k, const double* V, int64_t ldv, const double*
  * row addresses (the tall products of gcrodr.py:266-276,288-292,327-363 and
  * helpers.py:240-253):
  *   iga_zgram:    out[a*nb + b] = sum_i conj(X_a[i]) Y_b[i]   (complex, 1 <= na, nb <= 64)
- *   iga_zcombine: Out_c = beta Out_c + sum_j coef[j*nout + c] V_j   (1 <= nout <= 24)
+ *   iga_zcombine: Out_c = beta Out_c + alpha sum_j coef[j*nout + c] V_j   (1 <= nout <= 24, alpha real)
  * Rows are complex128 (interleaved re, im), each starting on a 16-byte boundary.
  * iga_zgram runs on the FP64 matrix cores (X^H Y as a real product over the 2n
  * interleaved doubles), or for at most 2 x 2 rows one complex element per thread,
@@ -119,7 +119,7 @@ int iga_zaxpy_many(int64_t n, int k, const double* V, int64_t ldv, const double*
 int64_t iga_zgram_workspace_bytes(int64_t n, int na, int nb);
 int iga_zgram(int64_t n, int na, const double* const* X, int nb, const double* const* Y, double* out,
               void* work, void* stream);
-int iga_zcombine(int64_t n, int nv, const double* const* V, int nout, const double* coef, double beta,
+int iga_zcombine(int64_t n, int nv, const double* const* V, int nout, const double* coef, double alpha, double beta,
                  double* const* out, void* stream);
 
 /* y[i] = a * x[i] + b * y[i]; a and b are read from device memory when

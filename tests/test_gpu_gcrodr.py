@@ -76,3 +76,38 @@ def test_gcrodr_matches_oracle_iterations():
         assert abs(o1.iterations - o2.iterations) <= 2
         x1, x2 = o1.solution.cpu().numpy(), o2.solution
         assert np.linalg.norm(x1 - x2) <= 1e-6 * np.linalg.norm(x2)
+
+
+@pytest.mark.parametrize("m,k", [(80, 30), (100, 70)])
+def test_gcrodr_beyond_64_basis_vectors_matches_oracle(m, k):
+    """Krylov and recycled sizes past one block call (the tall products are split over
+    32-row gram and 64-row combine calls): the oracle's matvec counts (+-2) and
+    solutions over a sequence in which the recycled space is carried across solves."""
+    import torch
+
+    from igaops_amd.gcrodr import GCRODR
+
+    n = 3000
+    Ad = np.diag(np.full(n, 2.01)) - np.diag(np.ones(n - 1), 1) - np.diag(np.ones(n - 1), -1)
+
+    def A(x, out):
+        out.copy_(2.01 * x)
+        out[:-1] -= x[1:]
+        out[1:] -= x[:-1]
+        return out
+
+    rng = np.random.default_rng(3)
+    b = rng.standard_normal(n)
+    dg = GCRODR(1e-8, 5000, krylov_size=m, recycled_size=k)
+    og = osv.GCRODR(1e-8, 5000, krylov_size=m, recycled_size=k)
+    for s in (1.0, 1.02, 0.97):
+        bd = torch.as_tensor(s * b, device="cuda")
+        o1 = dg.solve_device(A, bd)
+        o2 = og.solve(lambda v: Ad @ v, s * b)
+        assert o1.success and o2.success
+        if s == 1.0:
+            assert o1.iterations > m  # more than one cycle: the recycled space is built and used
+        assert abs(o1.iterations - o2.iterations) <= 2
+        x1, x2 = o1.solution.cpu().numpy(), o2.solution
+        assert np.linalg.norm(x1 - x2) <= 1e-6 * np.linalg.norm(x2)
+        assert np.linalg.norm(s * b - Ad @ x1) <= 10 * 1e-8 * np.linalg.norm(s * b)
