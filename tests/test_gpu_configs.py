@@ -11,7 +11,8 @@
 * Space-time heat: cube degree 3 x 32 elements by time degree 2 x 64 elements
   (2.83 M DOF): the arrowhead FD inverts the Gauss-rule operator, which the WQ
   operator equals for unit diffusivity, so ``A P b = b`` on the free nodes; GCRODR
-  (krylov 40, recycled 20) over 10 perturbed right-hand sides to a true 1e-10.
+  (krylov 40, recycled 20) over 10 perturbed right-hand sides to a true 1e-10; and
+  with a lognormal diffusivity, where GCRODR runs several cycles and recycles.
 
 Each step prints what it runs (``pytest -s`` shows it).
 """
@@ -185,3 +186,54 @@ def test_spacetime_heat_full_size():
         xi = torch.as_tensor(rng.standard_normal(n), device="cuda")
         A._mask(xi, 1)
         b = b + 0.05 * torch.linalg.vector_norm(b) / n ** 0.5 * xi
+
+
+def test_spacetime_heat_full_size_variable_diffusivity():
+    """The space-time size of the heat case with a lognormal diffusivity (sigma 0.5,
+    constant in time), where the FD is no longer the exact inverse: GCRODR (krylov 40,
+    recycled 20) over the 10 perturbed right-hand sides.  GCRODR is left-preconditioned
+    (``gcrodr.py:147-162``): each solve reaches ``||P (b - A x)|| <= 1e-10 ||P b||``,
+    recomputed here from the returned solution, and a true relative residual within
+    1e-8; the recycled space is carried from solve to solve (later solves take fewer
+    iterations than the first on average)."""
+    import torch
+
+    from igaops_amd.fastdiag import SingleFD
+    from igaops_amd.gcrodr import GCRODR
+    from igaops_amd.geometry import unit_cube
+    from igaops_amd.quadrature import WeightedQuadrature, make_open_knotvector
+    from igaops_amd.spacetime import BoxMasked, SpaceTimeOperator
+
+    say("space-time heat with lognormal diffusivity, space p=3 nel=32, time p=2 nel=64: set-up")
+    qs = [WeightedQuadrature(3, make_open_knotvector(3, 32), quadtype="2") for _ in range(3)]
+    qt = WeightedQuadrature(2, make_open_knotvector(2, 64), quadtype="2")
+    Cs, _ = unit_cube(3, 32, quadrule=qs[0]).device_coefficients(0.5, 11, mass=False)  # (3, 3, N_q space)
+    C = Cs.repeat(1, 1, qt.nbquadpts)  # constant in time; time is the slowest point index
+    del Cs
+    fd = SingleFD("spacetime")
+    fd.compute_space_eigendecomposition(qs, np.ones((1, 3, 2), bool))
+    fd.compute_time_schurdecomposition(qt)
+    fd.update_space_eigenvalues([0.0, 1.0])
+    op = SpaceTimeOperator(qs + [qt], 1.0, C)
+    del C
+    A = BoxMasked(op, fd.sptm_preconditioner.box())
+    rng = np.random.default_rng(6)
+    b = torch.as_tensor(rng.standard_normal(op.ndof), device="cuda")
+    A._mask(b, 1)
+    g = GCRODR(1e-10, 3000, krylov_size=40, recycled_size=20)
+    its = []
+    P = fd.sptm_preconditioner
+    for i in range(10):
+        out = g.solve_device(A, b, Pfun=P)
+        r = b - A.apply_device(out.solution)
+        tr = float(torch.linalg.vector_norm(r) / torch.linalg.vector_norm(b))
+        pr = float(torch.linalg.vector_norm(P.apply_device(r)) / torch.linalg.vector_norm(P.apply_device(b)))
+        say(f"  rhs {i}: {out.iterations} iterations, preconditioned relative residual {pr:.2e}, "
+            f"true {tr:.2e}")
+        assert out.success and pr <= 1e-10 * 1.01 and tr <= 1e-8
+        its.append(out.iterations)
+        xi = torch.as_tensor(rng.standard_normal(op.ndof), device="cuda")
+        A._mask(xi, 1)
+        b = b + 0.05 * torch.linalg.vector_norm(b) / op.ndof ** 0.5 * xi
+    assert its[0] > 40  # more than one cycle
+    assert np.mean(its[1:]) < its[0]
