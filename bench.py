@@ -294,6 +294,42 @@ def serve_section(nreq=2000, qps=200.0, seed=0, max_active=16, budget_gb=200.0, 
     return out
 
 
+def fd_section(nels=(64, 128), reps=50):
+    """The FD apply alone (degree 4, every face Dirichlet, ``Lambda = M + K``): free boxes
+    66^3 and 130^3, FP64 / FP32 / BF16-input plans, event-timed on the launch stream;
+    flops ``12 n^4`` per apply (two products of ``n x n`` factors per direction and
+    way) against the dense matrix-core peak of the precision."""
+    import torch
+
+    from igaops_amd.fastdiag import SingleFD
+    from igaops_amd.quadrature import WeightedQuadrature, make_open_knotvector
+
+    out = {}
+    for nel in nels:
+        q = WeightedQuadrature(4, make_open_knotvector(4, nel), quadtype="2")
+        fd = SingleFD("spatial")
+        fd.compute_space_eigendecomposition([q, q, q], np.ones((1, 3, 2), bool))
+        fd.update_space_eigenvalues([1.0, 1.0])
+        sp = fd.space_preconditioner
+        n = q.nbctrlpts ** 3
+        box = q.nbctrlpts - 2
+        flops = 12.0 * box ** 4
+        row = {}
+        for dt in ("f64", "f32", "bf16"):
+            sp.float32_plan = "bf16" if dt == "bf16" else "f32"
+            plan = sp.plan(dt)
+            work = plan.workspace(1)
+            x = torch.randn(n, device="cuda", dtype=torch.float64 if dt == "f64" else torch.float32)
+            y = torch.empty_like(x)
+            us = float(np.mean(time_events(lambda: plan.apply(x, y, 1, None, work=work), reps, 3))) * 1e3
+            row[dt] = {"us_per_apply": us, "tflops": flops / us * 1e-6,
+                       "mfma_frac": flops / us * 1e-6 / MFMA_PEAK_TFLOPS[dt]}
+        sp.float32_plan = "f32"
+        out[f"box{box}"] = row
+        print(f"fd box {box}: {json.dumps(row)}", file=sys.stderr, flush=True)
+    return out
+
+
 def krylov_section(n=68 ** 3, m=200, sizes=(16, 64, 128), reps=20):
     """One device GMRES Arnoldi orthogonalisation step (``iga_gmres_step``: two-vector
     many-dot pass, small dense kernel, update pass) at the annulus p=4 n=64 vector
@@ -518,6 +554,7 @@ def main():
     ap.add_argument("--no-traffic", action="store_true")
     ap.add_argument("--spacetime", action="store_true", help="only the space-time GCRODR section")
     ap.add_argument("--krylov", action="store_true", help="only the GMRES orthogonalisation step timings")
+    ap.add_argument("--fd", action="store_true", help="only the FD apply timings (66^3 and 130^3 boxes)")
     ap.add_argument("--assembly", action="store_true", help="only the sparse assembly timing")
     ap.add_argument("--serve", type=int, default=0, help="only the serving section, with this many requests")
     ap.add_argument("--qps", type=float, default=200.0)
@@ -535,6 +572,9 @@ def main():
         return
     if args.krylov:
         print(json.dumps({"krylov": krylov_section()}))
+        return
+    if args.fd:
+        print(json.dumps({"fd": fd_section()}))
         return
     if args.assembly:
         print(json.dumps({"assembly": assembly_section()}))
