@@ -384,28 +384,70 @@ __device__ inline void fd_mma_loop(typename FdMma<MODE>::V (&acc)[TPW], int K, c
 // over the four waves, columns over the 64 lanes (CC = ceil(cols / 64) <= 3 passes), RG
 // rows per wave per round with every load of the round issued before the stores.
 // Clamped indices keep every load in bounds; out-of-range elements are not stored.
-template <int MODE, int CC, class RowPtr>
-__device__ inline void fd_stage(typename FdMma<MODE>::S* dst, int p, int rows, int cols, const RowPtr& rowptr,
-                                int64_t cstride, int wave, int lane) {
-  using M = FdMma<MODE>;
-  using T = typename M::T;
-  constexpr int RG = CC == 1 ? 8 : CC == 2 ? 6 : 4;
-  for (int r0 = 0; r0 < rows; r0 += 4 * RG) {
-    T v[RG][CC];
+// Rows of a staged operand per wave and round (CC lane-columns of 64 per row).
+template <int CC>
+struct FdStageShape {
+  static constexpr int RG = CC == 1 ? 8 : CC == 2 ? 6 : 4;
+};
+
+// Loads of one round: rows r0 + wave + 4 i (clamped, so always in bounds), columns
+// lane + 64 cc (clamped).
+template <int MODE, int CC, typename RowPtr>
+__device__ inline void fd_stage_load(typename FdMma<MODE>::T (&v)[FdStageShape<CC>::RG][CC], int r0, int rows,
+                                     int cols, const RowPtr& rowptr, int64_t cstride, int wave, int lane) {
+  constexpr int RG = FdStageShape<CC>::RG;
 #pragma unroll
-    for (int i = 0; i < RG; ++i) {
-      const T* src = rowptr(min(r0 + wave + 4 * i, rows - 1));
+  for (int i = 0; i < RG; ++i) {
+    const auto* src = rowptr(min(r0 + wave + 4 * i, rows - 1));
 #pragma unroll
-      for (int cc = 0; cc < CC; ++cc) v[i][cc] = src[(int64_t)min(lane + 64 * cc, cols - 1) * cstride];
+    for (int cc = 0; cc < CC; ++cc) v[i][cc] = src[(int64_t)min(lane + 64 * cc, cols - 1) * cstride];
+  }
+}
+
+template <int MODE, int CC>
+__device__ inline void fd_stage_store(typename FdMma<MODE>::S* dst, int p, const typename FdMma<MODE>::T (
+                                          &v)[FdStageShape<CC>::RG][CC], int r0, int rows, int cols, int wave,
+                                      int lane) {
+  constexpr int RG = FdStageShape<CC>::RG;
+#pragma unroll
+  for (int i = 0; i < RG; ++i) {
+    const int r = r0 + wave + 4 * i;
+#pragma unroll
+    for (int cc = 0; cc < CC; ++cc) {
+      const int c = lane + 64 * cc;
+      if (r < rows && c < cols) dst[r * p + c] = FdMma<MODE>::to_s(v[i][cc]);
     }
+  }
+}
+
+// Stage rows [0, rows) x columns [0, cols) of a global operand into LDS (row pitch p,
+// converted to the operand type): rows over the waves, columns over the lanes, two
+// rounds in flight (the loads of the next round are issued before the current round
+// is written), so a stage costs about one memory round trip instead of one per round.
+// `pre`, when given, holds the loads of the first round, issued earlier by the caller.
+template <int MODE, int CC, typename RowPtr>
+__device__ inline void fd_stage(typename FdMma<MODE>::S* dst, int p, int rows, int cols, const RowPtr& rowptr,
+                                int64_t cstride, int wave, int lane,
+                                const typename FdMma<MODE>::T (*pre)[FdStageShape<CC>::RG][CC] = nullptr) {
+  using T = typename FdMma<MODE>::T;
+  constexpr int RG = FdStageShape<CC>::RG;
+  constexpr int step = 4 * RG;
+  T v0[RG][CC], v1[RG][CC];
+  if (pre) {
 #pragma unroll
-    for (int i = 0; i < RG; ++i) {
-      const int r = r0 + wave + 4 * i;
+    for (int i = 0; i < RG; ++i)
 #pragma unroll
-      for (int cc = 0; cc < CC; ++cc) {
-        const int c = lane + 64 * cc;
-        if (r < rows && c < cols) dst[r * p + c] = M::to_s(v[i][cc]);
-      }
+      for (int cc = 0; cc < CC; ++cc) v0[i][cc] = (*pre)[i][cc];
+  } else {
+    fd_stage_load<MODE, CC>(v0, 0, rows, cols, rowptr, cstride, wave, lane);
+  }
+  for (int r0 = 0; r0 < rows; r0 += 2 * step) {
+    const bool next = r0 + step < rows;
+    if (next) fd_stage_load<MODE, CC>(v1, r0 + step, rows, cols, rowptr, cstride, wave, lane);
+    fd_stage_store<MODE, CC>(dst, p, v0, r0, rows, cols, wave, lane);
+    if (next) {
+      if (r0 + 2 * step < rows) fd_stage_load<MODE, CC>(v0, r0 + 2 * step, rows, cols, rowptr, cstride, wave, lane);
+      fd_stage_store<MODE, CC>(dst, p, v1, r0 + step, rows, cols, wave, lane);
     }
   }
 }
@@ -471,6 +513,10 @@ __global__ void __launch_bounds__(256) fd_plane_kernel(FdPlaneArgs<typename FdMm
   for (int q = tid; q < (16 - qrows) * p; q += 256) Q[qrows * p + q] = M::to_s(T(0));
   fd_zero_cols<MODE>(P, p, n2, n1, K1, wave, lane);
   fd_zero_cols<MODE>(Q, p, qrows, n1, K1, wave, lane);
+  // the first round of A2 is loaded now and stays in flight during the first product
+  auto a2row = [&](int r) { return a.A2 + (int64_t)r * n2; };
+  T a2pre[FdStageShape<TPW>::RG][TPW];
+  fd_stage_load<MODE, TPW>(a2pre, 0, n2, n2, a2row, 1, wave, lane);
   __syncthreads();
   const int lr = lane & 15;
   const S* brow[TPW];  // this lane's B row (its output column) in each tile, clamped
@@ -493,7 +539,7 @@ __global__ void __launch_bounds__(256) fd_plane_kernel(FdPlaneArgs<typename FdMm
       if (col < K2) Q[M::row(lane, r) * p + col] = M::to_s(col < n2 ? acc[u][r] : T(0));
     }
   }
-  fd_stage<MODE, TPW>(P, p, n2, n2, [&](int r) { return a.A2 + (int64_t)r * n2; }, 1, wave, lane);
+  fd_stage<MODE, TPW>(P, p, n2, n2, a2row, 1, wave, lane, &a2pre);
   fd_zero_cols<MODE>(P, p, n2, n2, K2, wave, lane);
   __syncthreads();
   // step 2: Y(i1, i2) = sum_k2 Y1(i1, k2) A2[i2][k2]; B(k2, i2) = A2[i2][k2]
