@@ -452,6 +452,48 @@ __device__ inline void fd_stage(typename FdMma<MODE>::S* dst, int p, int rows, i
   }
 }
 
+// Flat staging of a rows x cols operand through registers, for operands of at most
+// 256 E elements: thread t holds elements t + 256 i (i < E), element e at row e / cols,
+// column e % cols, so no lane idles on a short row.  load() issues every load (clamped
+// past the end, so always in bounds) and store() writes them to LDS later: the caller
+// places other work, or other loads, between the two, and a stage costs one memory
+// round trip whatever its row count.
+template <int MODE, int E>
+struct FdFlat {
+  using T = typename FdMma<MODE>::T;
+  T v[E];
+  template <typename RowPtr>
+  __device__ void load(int rows, int cols, const RowPtr& rowptr, int64_t cstride, int tid) {
+    const int q = cols <= 256 ? 256 / cols : 0, rr = cols <= 256 ? 256 % cols : 256;
+    int r = tid / cols, c = tid % cols;
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      const bool ok = r < rows;
+      v[i] = rowptr(ok ? r : rows - 1)[(int64_t)(ok ? c : cols - 1) * cstride];
+      r += q;
+      c += rr;
+      if (c >= cols) {
+        c -= cols;
+        ++r;
+      }
+    }
+  }
+  __device__ void store(typename FdMma<MODE>::S* dst, int p, int rows, int cols, int tid) const {
+    const int q = cols <= 256 ? 256 / cols : 0, rr = cols <= 256 ? 256 % cols : 256;
+    int r = tid / cols, c = tid % cols;
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      if (r < rows) dst[r * p + c] = FdMma<MODE>::to_s(v[i]);
+      r += q;
+      c += rr;
+      if (c >= cols) {
+        c -= cols;
+        ++r;
+      }
+    }
+  }
+};
+
 // Zero columns [c0, c1) of rows [0, rows) (c1 - c0 < 64): the K padding of an operand.
 template <int MODE>
 __device__ inline void fd_zero_cols(typename FdMma<MODE>::S* dst, int p, int rows, int c0, int c1, int wave,
@@ -476,7 +518,9 @@ struct FdPlaneArgs {
 // rows of S, then Y1[i1][k2]); then the output tile staging [i2][17] in T over P.  Rows
 // are zero-padded in k up to a multiple of KS.  Tiles past n2 read clamped (finite) rows:
 // their garbage stays in output columns that are never stored, and Y1's columns past n2
-// are stored as zeros.  TPW = ceil(ceil(n2 / 16) / 4) tiles per wave.
+// are stored as zeros.  TPW = ceil(ceil(n2 / 16) / 4) tiles per wave.  (Staging the
+// operands flat, all loads at once, measured no faster for FP64 and slower for FP32 /
+// BF16 than these row rounds.)
 template <int MODE, bool BWD, int TPW>
 __global__ void __launch_bounds__(256) fd_plane_kernel(FdPlaneArgs<typename FdMma<MODE>::T> a) {
   using M = FdMma<MODE>;
@@ -507,14 +551,16 @@ __global__ void __launch_bounds__(256) fd_plane_kernel(FdPlaneArgs<typename FdMm
   const int64_t row0 = BWD ? (int64_t)n1 * n2 * i3 * nr + k
                            : ((a.lo1) + a.nf1 * (a.lo2 + a.nf2 * (a.lo3 + i3))) * nr + k;
   const int64_t rstep = BWD ? (int64_t)n1 * nr : a.nf1 * nr;
-  fd_stage<MODE, TPW>(P, p, n2, n1, [&](int r) { return a.in + row0 + rstep * r; }, nr, wave, lane);
+  auto xrow = [&](int r) { return a.in + row0 + rstep * r; };
+  auto a1row = [&](int r) { return a.A1 + (int64_t)(i1s + r) * n1; };
+  auto a2row = [&](int r) { return a.A2 + (int64_t)r * n2; };
   const int qrows = min(16, n1 - i1s);  // rows of S inside the box; the rest are zero
-  fd_stage<MODE, TPW>(Q, p, qrows, n1, [&](int r) { return a.A1 + (int64_t)(i1s + r) * n1; }, 1, wave, lane);
+  fd_stage<MODE, TPW>(P, p, n2, n1, xrow, nr, wave, lane);
+  fd_stage<MODE, TPW>(Q, p, qrows, n1, a1row, 1, wave, lane);
   for (int q = tid; q < (16 - qrows) * p; q += 256) Q[qrows * p + q] = M::to_s(T(0));
   fd_zero_cols<MODE>(P, p, n2, n1, K1, wave, lane);
   fd_zero_cols<MODE>(Q, p, qrows, n1, K1, wave, lane);
   // the first round of A2 is loaded now and stays in flight during the first product
-  auto a2row = [&](int r) { return a.A2 + (int64_t)r * n2; };
   T a2pre[FdStageShape<TPW>::RG][TPW];
   fd_stage_load<MODE, TPW>(a2pre, 0, n2, n2, a2row, 1, wave, lane);
   __syncthreads();
@@ -600,7 +646,10 @@ struct FdLineArgs {
 // LDS: U (K3 rows x p: U3^T[k3][i3], rows past n3 zero; the second product reads it
 // transposed) and X (16 rows x xp: X^T[j][i3], then Z^T[j][k3]).  Tile rows past n3 are clamped: their garbage
 // stays in output columns that are never stored, and Z^T's columns past n3 are zeros.
-template <int MODE, int TPW>
+// E > 0 (n3 <= 128, U3^T of at most 256 E elements): every global load of the kernel
+// (U3^T flat, the 16 columns of W, the Lambda^-1 entries of its outputs) is issued
+// before the first LDS store, one round trip; E = 0: row rounds.
+template <int MODE, int TPW, int E>
 __global__ void __launch_bounds__(256) fd_line_kernel(FdLineArgs<typename FdMma<MODE>::T> a) {
   using M = FdMma<MODE>;
   using T = typename M::T;
@@ -615,8 +664,44 @@ __global__ void __launch_bounds__(256) fd_line_kernel(FdLineArgs<typename FdMma<
   S* X = Ub + (kTwoCopies ? (int64_t)n3 * p : 0);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int64_t j0 = (int64_t)blockIdx.x * 16;
-  // U3^T rows (and for BF16 its transpose), the 16 columns of W transposed into X
-  fd_stage<MODE, TPW>(U, p, n3, n3, [&](int r) { return a.Ut + (int64_t)r * n3; }, 1, wave, lane);
+  const int lr = lane & 15;
+  int mrow[TPW];  // this lane's row of U3 / U3^T in each of its tiles, clamped
+#pragma unroll
+  for (int u = 0; u < TPW; ++u) mrow[u] = min((wave + 4 * u) * 16 + lr, n3 - 1);
+  // Lambda^-1 of this lane's outputs of the first product (eigen-index k3 = tile row,
+  // column j), loaded ahead
+  T linv[TPW][4];
+  auto load_inv = [&]() __attribute__((always_inline)) {
+#pragma unroll
+    for (int u = 0; u < TPW; ++u)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t j = j0 + M::row(lane, r);
+        const int64_t c = (j < a.ncol ? j : a.ncol - 1) / a.nrhs;
+        linv[u][r] = a.inv[c + a.plane * mrow[u]];
+      }
+  };
+  auto utrow = [&](int r) { return a.Ut + (int64_t)r * n3; };
+  const int jc = tid & 15;
+  const int64_t jg = min(j0 + jc, a.ncol - 1);
+  if constexpr (E > 0) {
+    // U3^T, the 16 columns of W (X[j][i3] = W(i3, j0 + j): 16 threads per W row, rows
+    // i3 = tid / 16 + 16 e) and Lambda^-1: all loads first
+    FdFlat<MODE, E> uflat;
+    T xv[8];
+    uflat.load(n3, n3, utrow, 1, tid);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) xv[e] = a.w[(int64_t)min((tid >> 4) + 16 * e, n3 - 1) * a.ncol + jg];
+    load_inv();
+    uflat.store(U, p, n3, n3, tid);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int r = (tid >> 4) + 16 * e;
+      if (r < n3) X[jc * xp + r] = M::to_s(j0 + jc < a.ncol ? xv[e] : T(0));
+    }
+  } else {
+    fd_stage<MODE, TPW>(U, p, n3, n3, utrow, 1, wave, lane);
+  }
   if constexpr (kTwoCopies) {
     // Ub[i3][k3] = U3^T[k3][i3]: row i3 of Ub is column i3 of U3^T
     fd_stage<MODE, TPW>(Ub, p, n3, n3, [&](int r) { return a.Ut + r; }, n3, wave, lane);
@@ -624,10 +709,8 @@ __global__ void __launch_bounds__(256) fd_line_kernel(FdLineArgs<typename FdMma<
   }
   for (int q = tid; q < (K3 - n3) * p; q += 256) U[n3 * p + q] = M::to_s(T(0));  // rows past n3
   fd_zero_cols<MODE>(U, p, n3, n3, K3, wave, lane);
-  {
+  if constexpr (E == 0) {
     // X[j][i3] = W(i3, j0 + j): 16 threads per W row, rows i3 = tid / 16 + 16 e
-    const int jc = tid & 15;
-    const int64_t jg = min(j0 + jc, a.ncol - 1);
     for (int r0 = 0; r0 < n3; r0 += 16 * 8) {
       T v[8];
 #pragma unroll
@@ -638,13 +721,10 @@ __global__ void __launch_bounds__(256) fd_line_kernel(FdLineArgs<typename FdMma<
         if (r < n3) X[jc * xp + r] = M::to_s(j0 + jc < a.ncol ? v[e] : T(0));
       }
     }
-    for (int q = tid; q < (K3 - n3) * 16; q += 256) X[(q & 15) * xp + n3 + (q >> 4)] = M::to_s(T(0));
+    load_inv();
   }
+  for (int q = tid; q < (K3 - n3) * 16; q += 256) X[(q & 15) * xp + n3 + (q >> 4)] = M::to_s(T(0));
   __syncthreads();
-  const int lr = lane & 15;
-  int mrow[TPW];  // this lane's row of U3 / U3^T in each of its tiles, clamped
-#pragma unroll
-  for (int u = 0; u < TPW; ++u) mrow[u] = min((wave + 4 * u) * 16 + lr, n3 - 1);
   const S* arow = X + lr * xp;
   V acc[TPW];
   // step 1: Z^T(j, k3) = sum_i3 X^T(j, i3) U3^T[k3][i3]
@@ -662,9 +742,7 @@ __global__ void __launch_bounds__(256) fd_line_kernel(FdLineArgs<typename FdMma<
     for (int r = 0; r < 4; ++r) {
       const int jr = M::row(lane, r);
       const int64_t j = j0 + jr;
-      const int64_t c = (j < a.ncol ? j : a.ncol - 1) / a.nrhs;
-      if (m < K3)
-        X[jr * xp + m] = M::to_s((m < n3 && j < a.ncol) ? acc[u][r] * a.inv[c + a.plane * min(m, n3 - 1)] : T(0));
+      if (m < K3) X[jr * xp + m] = M::to_s((m < n3 && j < a.ncol) ? acc[u][r] * linv[u][r] : T(0));
     }
   }
   __syncthreads();
@@ -734,7 +812,7 @@ static void fd3_allow_lds(Kern kern, size_t bytes, std::atomic<size_t>& granted)
   granted.store(bytes, std::memory_order_relaxed);
 }
 
-template <int MODE, int TPW>
+template <int MODE, int TPW, int E>
 static void fd3_launch(const iga_fd_s* f, FdPlaneArgs<typename FdMma<MODE>::T> pa,
                        FdLineArgs<typename FdMma<MODE>::T> la, const Fd3Shape& sh, int nrhs, int which,
                        hipStream_t s) {
@@ -744,8 +822,8 @@ static void fd3_launch(const iga_fd_s* f, FdPlaneArgs<typename FdMma<MODE>::T> p
     fd3_allow_lds(fd_plane_kernel<MODE, false, TPW>, sh.lds_plane, granted[0]);
     fd_plane_kernel<MODE, false, TPW><<<dim3(nsb, (unsigned)pa.n3, (unsigned)nrhs), 256, sh.lds_plane, s>>>(pa);
   } else if (which == 1) {
-    fd3_allow_lds(fd_line_kernel<MODE, TPW>, sh.lds_line, granted[1]);
-    fd_line_kernel<MODE, TPW><<<(unsigned)((la.ncol + 15) / 16), 256, sh.lds_line, s>>>(la);
+    fd3_allow_lds(fd_line_kernel<MODE, TPW, E>, sh.lds_line, granted[1]);
+    fd_line_kernel<MODE, TPW, E><<<(unsigned)((la.ncol + 15) / 16), 256, sh.lds_line, s>>>(la);
   } else {
     const unsigned extra = (unsigned)(f->n[2] - pa.n3);
     fd3_allow_lds(fd_plane_kernel<MODE, true, TPW>, sh.lds_plane, granted[2]);
@@ -755,17 +833,23 @@ static void fd3_launch(const iga_fd_s* f, FdPlaneArgs<typename FdMma<MODE>::T> p
   IGA_LAUNCH_CHECK();
 }
 
+// Tiles per wave and, for the line kernel, the flat staging size from the longest
+// operand side n of a launch: an n x n operand takes ceil(n^2 / 256) registers per
+// thread when staged flat (up to n = 128); longer sides stage in row rounds.
 template <int MODE>
-static void fd3_launch_tpw(const iga_fd_s* f, FdPlaneArgs<typename FdMma<MODE>::T> pa,
+static void fd3_launch_len(const iga_fd_s* f, FdPlaneArgs<typename FdMma<MODE>::T> pa,
                            FdLineArgs<typename FdMma<MODE>::T> la, const Fd3Shape& sh, int nrhs, int which,
-                           int tiles, hipStream_t s) {
-  const int tpw = (tiles + 3) / 4;
-  if (tpw <= 1)
-    fd3_launch<MODE, 1>(f, pa, la, sh, nrhs, which, s);
-  else if (tpw == 2)
-    fd3_launch<MODE, 2>(f, pa, la, sh, nrhs, which, s);
+                           int n, hipStream_t s) {
+  if (n <= 64)
+    fd3_launch<MODE, 1, 16>(f, pa, la, sh, nrhs, which, s);
+  else if (n <= 78)
+    fd3_launch<MODE, 2, 24>(f, pa, la, sh, nrhs, which, s);
+  else if (n <= 101)
+    fd3_launch<MODE, 2, 40>(f, pa, la, sh, nrhs, which, s);
+  else if (n <= 128)
+    fd3_launch<MODE, 2, 64>(f, pa, la, sh, nrhs, which, s);
   else
-    fd3_launch<MODE, 3>(f, pa, la, sh, nrhs, which, s);
+    fd3_launch<MODE, 3, 0>(f, pa, la, sh, nrhs, which, s);
 }
 
 template <int MODE>
@@ -803,16 +887,16 @@ static void fd3_apply(const iga_fd_s* f, const typename FdMma<MODE>::T* x, typen
   pa.A1 = (const T*)f->d_Ut[0];
   pa.A2 = (const T*)f->d_Ut[1];
   // the plane kernels' tiles per wave also set their column passes: cover both lengths
-  const int plane_tiles = (std::max(n1, n2) + 15) / 16;
-  fd3_launch_tpw<MODE>(f, pa, la, sh, nrhs, 0, plane_tiles, s);
+  const int plane_len = std::max(n1, n2);
+  fd3_launch_len<MODE>(f, pa, la, sh, nrhs, 0, plane_len, s);
   // direction 3 with the eigenvalue scaling, in place on W
-  fd3_launch_tpw<MODE>(f, pa, la, sh, nrhs, 1, (n3 + 15) / 16, s);
+  fd3_launch_len<MODE>(f, pa, la, sh, nrhs, 1, n3, s);
   // backward: W -> y (directions 1, 2), zeros outside the box
   pa.in = w;
   pa.out = y;
   pa.A1 = (const T*)f->d_U[0];
   pa.A2 = (const T*)f->d_U[1];
-  fd3_launch_tpw<MODE>(f, pa, la, sh, nrhs, 2, plane_tiles, s);
+  fd3_launch_len<MODE>(f, pa, la, sh, nrhs, 2, plane_len, s);
 }
 
 template <typename T>

@@ -44,6 +44,43 @@ def test_fd_apply_matches_oracle(table, p, nel):
     assert rel(y32, ofd_.apply(X[:, 0])) <= 1e-4  # float32 GEMMs on a conditioned inverse
 
 
+@pytest.mark.parametrize("dt,nels", [("f64", (42, 72, 97)), ("f64", (122, 112, 66)), ("f32", (102, 62, 122)),
+                                     ("f32", (132, 52, 142)), ("bf16", (72, 97, 122))])
+def test_fd_apply_every_staging_size(dt, nels):
+    """Three-launch FD applies on boxes of unequal sides (degree 2, every face Dirichlet,
+    box side nel), so that the plane and line kernels run every operand-staging size:
+    flat with 16 / 24 / 40 / 64 registers per thread (sides up to 64 / 78 / 101 / 128)
+    and row rounds past 128; two right-hand sides, against the oracle's apply."""
+    import torch
+
+    from igaops_amd.fastdiag import SingleFD
+    from igaops_amd.quadrature import WeightedQuadrature
+
+    qs = [WeightedQuadrature(2, make_open_knotvector(2, n), "2") for n in nels]
+    tab = np.ones((1, 3, 2), bool)
+    fd = SingleFD("spatial")
+    fd.compute_space_eigendecomposition(qs, tab)
+    fd.update_space_eigenvalues([0.5, 1.0])
+    sp = fd.space_preconditioner
+    o = ofd.SpaceFD([StandardGauss(q.degree, q.knotvector, "legendre") for q in qs], tab)
+    o.scalar_coefs = (0.5, 1.0)
+    n = int(np.prod([q.nbctrlpts for q in qs]))
+    X = np.random.default_rng(21).standard_normal((n, 2))
+    ref = o.apply(X)
+    Xd = torch.as_tensor(X, device="cuda")
+    if dt != "f64":
+        sp.float32_plan = dt
+        Xd = Xd.float()
+    y = sp.apply_device(Xd.reshape(-1).contiguous(), nrhs=2).reshape(n, 2).double().cpu().numpy()
+    sp.float32_plan = "f32"
+    err = rel(y, ref)
+    print(f"FD {dt} box {nels}: relative error {err:.2e}", flush=True)
+    assert err <= {"f64": 1e-12, "f32": 1e-5, "bf16": 1e-2}[dt]
+    free = np.zeros(n, bool)
+    free[o.free_nodes()] = True
+    assert np.all(y[~free] == 0)
+
+
 def build_problem(geom, p, nel, sigma=0.5):
     patch = unit_cube(p, nel) if geom == "cube" else quarter_annulus(p, nel)
     kappa = lognormal_conductivity(patch, sigma, seed=1) if sigma > 0 else None
