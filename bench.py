@@ -295,6 +295,40 @@ def serve_section(nreq=2000, qps=200.0, seed=0, max_active=16, budget_gb=200.0, 
         out[mode] = s
         print(f"serve {mode}: {json.dumps(s)}", file=sys.stderr, flush=True)
     out["speedup_mixed_vs_fp64"] = out["mixed"]["solves_per_s"] / max(out["fp64"]["solves_per_s"], 1e-30)
+    out["same_shape"] = same_shape_section()
+    return out
+
+
+def same_shape_section(nreq=96, max_active=16):
+    """A stream whose requests share three shapes (annulus p=3 nel=24, cube p=4
+    nel=32, annulus p=2 nel=40; their own fields, sigma in [0.3, 0.8), and right-hand
+    sides), all present at the start: served with same-shape batching (one
+    multi-right-hand-side FD launch and one grouped operator launch per group step)
+    and without, per mode; solves/s over the whole stream and the batched steps."""
+    from igaops_amd.serving import SolveRequest, SolveServer, summarize
+
+    shapes = [("annulus", 3, 24), ("cube", 4, 32), ("annulus", 2, 40)]
+    reqs = lambda: [SolveRequest(i, *shapes[i % 3], 0.3 + 0.5 * ((i * 37) % 100) / 100.0, 1000 + i)
+                    for i in range(nreq)]
+    out = {"requests": nreq, "shapes": [list(s) for s in shapes], "max_active": max_active}
+    for mode in ("fp64", "mixed"):
+        row = {}
+        for batch in (False, True):
+            srv = SolveServer(mode=mode, max_active=max_active, batch_fd=batch, batch_operators=batch)
+            srv.prepare((2, 3, 4), (24, 32, 40), workers=1)
+            srv.serve(reqs()[:max_active], realtime=False)  # shape set-up and allocator warm-up
+            t1 = time.perf_counter()
+            res = srv.serve(reqs(), realtime=False)
+            wall = time.perf_counter() - t1
+            s = summarize(res, 1e-10, wall)
+            row["batched" if batch else "unbatched"] = {
+                "solves_per_s": s["solves_per_s"], "reached_tol": s["reached_tol"],
+                "mean_iterations": s["mean_iterations"], "wall_s": wall,
+                "batched_fd_steps": srv.stats.get("batched_fd_steps", 0),
+                "batched_operator_steps": srv.stats.get("batched_operator_steps", 0)}
+        row["speedup_batched"] = row["batched"]["solves_per_s"] / max(row["unbatched"]["solves_per_s"], 1e-30)
+        out[mode] = row
+        print(f"same-shape {mode}: {json.dumps(row)}", file=sys.stderr, flush=True)
     return out
 
 
@@ -656,6 +690,7 @@ def main():
     ap.add_argument("--c3", action="store_true", help="only the mixed-precision case at 128 elements, p 2, 4, 6")
     ap.add_argument("--assembly", action="store_true", help="only the sparse assembly timing")
     ap.add_argument("--serve", type=int, default=0, help="only the serving section, with this many requests")
+    ap.add_argument("--same-shape", action="store_true", help="only the same-shape batching stream")
     ap.add_argument("--qps", type=float, default=200.0)
     ap.add_argument("--no-serve", action="store_true", help="skip the short serving stream of the default run")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
@@ -680,6 +715,9 @@ def main():
         return
     if args.assembly:
         print(json.dumps({"assembly": assembly_section()}))
+        return
+    if args.same_shape:
+        print(json.dumps({"same_shape": same_shape_section()}))
         return
     if args.serve:
         print(json.dumps({"serve": serve_section(args.serve, args.qps)}))

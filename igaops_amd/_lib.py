@@ -53,6 +53,7 @@ SIGNATURES = {
     "iga_fused_pack": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "iga_fused_apply": (_i32, [_vp, _vp, _vp, _vp, _i32, _vp, _f64, _i32, _vp, _vp]),
     "iga_fused_apply_noalloc": (_i32, [_vp, _vp, _vp, _vp, _i32, _vp, _f64, _i32, _vp]),
+    "iga_fused_apply_group": (_i32, [_vp, _i32, C.POINTER(_vp), _vp, _vp, _vp, _vp, _i32, _i64p, _i64p, _vp]),
     "iga_fused_apply_stage": (_i32, [_vp, _i32, _vp, _vp, _vp, _i32, _vp, _f64, _i32, _vp, _vp]),
     "iga_fd_create": (_i32, [C.POINTER(_vp), _i32, _i64p, _i64p, _i64p, C.POINTER(_f64p), _i32]),
     "iga_fd_set_eigenvalues": (_i32, [_vp, _f64p]),

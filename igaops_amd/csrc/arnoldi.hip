@@ -9,6 +9,8 @@
 //
 //   z = P V[k-1]          iga_fd_apply (zero outside the FD's free box), or a copy
 //   w = A z, masked       the fused apply, its gather writing zeros outside the box
+//                         (a caller that batches these two over several solves hands
+//                         in z, or w, instead)
 //   Arnoldi / Givens      the step of iga_gmres_step, whose small dense kernel also
 //                         publishes the residual estimate to host-visible memory
 //
@@ -101,19 +103,22 @@ int iga_arnoldi_step(iga_arnoldi_t c, int k, void* V, int64_t ldv, int precondit
   return guard([&] {
     IGA_CHECK_ARG(c && V && ldv >= c->n, "bad argument");
     IGA_CHECK_ARG(k >= 1 && k <= c->m + 1, "step outside the Krylov dimension");
+    IGA_CHECK_ARG(preconditioned >= 0 && preconditioned <= 2, "preconditioned is 0, 1 or 2");
     hipStream_t s = as_stream(stream);
     const size_t es = c->dtype == IGA_F64 ? 8 : 4;
     const void* v = (const char*)V + (size_t)(k - 1) * ldv * es;
-    if (!preconditioned) {
+    if (preconditioned == 0) {
       if (c->fd)
         IGA_CHECK(iga_fd_apply(c->fd, v, c->z, 1, c->fd_work, stream));
       else
         IGA_HIP(hipMemcpyAsync(c->z, v, (size_t)c->n * es, hipMemcpyDeviceToDevice, s));
     }
-    if (c->masked)
-      fused_apply_masked(c->op, c->packed, c->z, c->w, c->nurbs_w, c->op_work, c->box_lo, c->box_len, stream);
-    else
-      IGA_CHECK(iga_fused_apply(c->op, c->packed, c->z, c->w, 1, c->nurbs_w, 1.0, 0, c->op_work, stream));
+    if (preconditioned <= 1) {
+      if (c->masked)
+        fused_apply_masked(c->op, c->packed, c->z, c->w, c->nurbs_w, c->op_work, c->box_lo, c->box_len, stream);
+      else
+        IGA_CHECK(iga_fused_apply(c->op, c->packed, c->z, c->w, 1, c->nurbs_w, 1.0, 0, c->op_work, stream));
+    }
     gmres_step_impl(c->dtype, c->n, k, c->m, V, ldv, c->w, c->ls, c->work, c->rec, c->tag, stream);
   });
 }

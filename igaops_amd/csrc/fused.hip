@@ -35,10 +35,20 @@ namespace iga {
 // FORM_KM7X: the KM7 fields with the trial value taken from a second input x1 (the
 // gradient from x): the space-time operator's stiffness on the time-interpolated
 // vector and mass on the time-differentiated one in one sweep.
-enum FusedForm { FORM_M1 = 0, FORM_K6 = 1, FORM_KM7 = 2, FORM_SYM10 = 3, FORM_GEN16 = 4, FORM_KM7X = 5 };
+// FORM_GV4 / FORM_UG4: the first-order terms, D_i0 (test derivative i, trial value) /
+// D_0j (test value, trial derivative j) plus D_00 (the NURBS correction, zero for
+// B-splines): four fields per point.
+enum FusedForm {
+  FORM_M1 = 0, FORM_K6 = 1, FORM_KM7 = 2, FORM_SYM10 = 3, FORM_GEN16 = 4, FORM_KM7X = 5, FORM_GV4 = 6, FORM_UG4 = 7
+};
 
 __host__ __device__ constexpr int form_ncoef(int f) {
-  return f == FORM_M1 ? 1 : f == FORM_K6 ? 6 : (f == FORM_KM7 || f == FORM_KM7X) ? 7 : f == FORM_SYM10 ? 10 : 16;
+  return f == FORM_M1 ? 1
+         : f == FORM_K6 ? 6
+         : (f == FORM_KM7 || f == FORM_KM7X) ? 7
+         : f == FORM_SYM10 ? 10
+         : (f == FORM_GV4 || f == FORM_UG4) ? 4
+                                            : 16;
 }
 
 constexpr int kTQ1 = 16, kTQ2 = 16, kThreads = kTQ1 * kTQ2;
@@ -48,6 +58,7 @@ constexpr int kTQ1 = 16, kTQ2 = 16, kThreads = kTQ1 * kTQ2;
 constexpr int kDMax = 16;
 constexpr int kKMax = 4;   // max tiles/slabs whose footprint contains one DOF index
 constexpr int kSlabMax = 128;  // max q3 planes of one slab (direction-3 table staged in LDS)
+constexpr int kGroupMax = 16;  // members of a grouped apply (coefficient sets of one launch)
 
 template <typename T>
 struct DirTab {
@@ -69,6 +80,9 @@ struct FusedArgs {
   // at coef + k * coef_rstride (0: one coefficient set shared by all of them)
   int64_t x_dstride, x_rstride, coef_rstride;
   const T* coef;
+  // grouped apply (ngroup > 0): right-hand side k reads its own coefficient set gcoef[k]
+  int ngroup;
+  const T* gcoef[kGroupMax];
   const T* x;
   const T* x1;  // FORM_KM7X: the input of the value (mass) term, strides of x
   const T* w;   // NURBS weights or nullptr
@@ -103,6 +117,16 @@ __device__ inline void point_contract(const T* __restrict__ c, int64_t stride, c
     f[1] = c11 * u[1] + c12 * u[2] + c13 * u[3];
     f[2] = c12 * u[1] + c22 * u[2] + c23 * u[3];
     f[3] = c13 * u[1] + c23 * u[2] + c33 * u[3];
+  } else if constexpr (FORM == FORM_GV4) {
+    const T d00 = c[0], d10 = c[stride], d20 = c[2 * stride], d30 = c[3 * stride];
+    f[0] = d00 * u[0];
+    f[1] = d10 * u[0];
+    f[2] = d20 * u[0];
+    f[3] = d30 * u[0];
+  } else if constexpr (FORM == FORM_UG4) {
+    const T d00 = c[0], d01 = c[stride], d02 = c[2 * stride], d03 = c[3 * stride];
+    f[0] = d00 * u[0] + d01 * u[1] + d02 * u[2] + d03 * u[3];
+    f[1] = f[2] = f[3] = T(0);
   } else if constexpr (FORM == FORM_SYM10) {
     const T d00 = c[0], d11 = c[stride], d22 = c[2 * stride], d33 = c[3 * stride];
     const T d01 = c[4 * stride], d02 = c[5 * stride], d03 = c[6 * stride];
@@ -411,7 +435,8 @@ __device__ __attribute__((always_inline)) inline void fused_apply_body(const Fus
 
   const int npts = q1n * q2n;
   // inactive lanes read the tile's first point (a valid address) and discard it
-  const T* cptr = a.coef + (int64_t)k * a.coef_rstride + a.coef_base[tile] + (active ? (l1 + q1n * l2) : 0);
+  const T* coefk = a.ngroup > 0 ? a.gcoef[k] : a.coef + (int64_t)k * a.coef_rstride;
+  const T* cptr = coefk + a.coef_base[tile] + (active ? (l1 + q1n * l2) : 0);
   constexpr int NCF = form_ncoef(FORM);
   const int64_t step_stride = (int64_t)NCF * npts;
   // Coefficients of plane q3 are loaded PD planes ahead into a ring of buffers;
@@ -683,6 +708,24 @@ __global__ void fused_pack_kernel(PackArgs<T> p) {
       c[0][1] = c[1][0] = kd * p.ex.plane[3 * nplane + pl];
       c[2][2] = kd / (dz * dz);
       mm = p.ex.with_mass ? det : 0.0;
+    } else if (p.form == FORM_GV4 || p.form == FORM_UG4) {
+      // C holds the first-order coefficients c (3, nq)
+      double cv[3], cz = 0.0;
+      for (int i = 0; i < 3; ++i) cv[i] = p.C[(int64_t)i * p.comp_stride + q];
+      double wi2 = 1.0;
+      if (p.W) {
+        // rational basis (operators/nurbs_ops.py:392-430): c W~^2 on the B-spline
+        // first-order term, -(c . z) W~ on the mass term, z = grad W W~^2
+        const double wi = 1.0 / p.W[q];
+        wi2 = wi * wi;
+        for (int i = 0; i < 3; ++i) cz += cv[i] * p.dW[(int64_t)i * p.comp_stride + q] * wi2;
+        cz *= -wi;
+      }
+      // both forms hold (D_00, c_1, c_2, c_3): the test derivative (GV4) or the trial
+      // derivative (UG4) of c_i is the kernel's choice
+      dst[0] = (T)cz;
+      for (int i = 0; i < 3; ++i) dst[(i + 1) * npts] = (T)(cv[i] * wi2);
+      continue;
     } else {
       if (p.C)
         for (int i = 0; i < 3; ++i)
@@ -920,6 +963,8 @@ static int resident_blocks_p(int form) {
     case FORM_KM7: return resident_blocks_pf<T, P, FORM_KM7>();
     case FORM_SYM10: return resident_blocks_pf<T, P, FORM_SYM10>();
     case FORM_KM7X: return resident_blocks_pf<T, P, FORM_KM7X>();
+    case FORM_GV4: return resident_blocks_pf<T, P, FORM_GV4>();
+    case FORM_UG4: return resident_blocks_pf<T, P, FORM_UG4>();
     default: return resident_blocks_pf<T, P, FORM_GEN16>();
   }
 }
@@ -944,6 +989,8 @@ static void launch_apply_p(const iga_fused_s* f, const FusedArgs<T>& a, int nblk
     case FORM_KM7: launch_apply_pf<T, P, FORM_KM7>(a, nblk, s); break;
     case FORM_SYM10: launch_apply_pf<T, P, FORM_SYM10>(a, nblk, s); break;
     case FORM_KM7X: launch_apply_pf<T, P, FORM_KM7X>(a, nblk, s); break;
+    case FORM_GV4: launch_apply_pf<T, P, FORM_GV4>(a, nblk, s); break;
+    case FORM_UG4: launch_apply_pf<T, P, FORM_UG4>(a, nblk, s); break;
     default: launch_apply_pf<T, P, FORM_GEN16>(a, nblk, s); break;
   }
 }
@@ -973,7 +1020,8 @@ template <typename T>
 static void run_apply(const iga_fused_s* f, const void* coef, const void* x, void* y, int nrhs, const void* w,
                       double alpha, int accumulate, void* work, hipStream_t s, int stages = 3,
                       Layout lay = Layout{0, 0, 0, 0, 0}, const void* x1 = nullptr,
-                      const int64_t* box_lo = nullptr, const int64_t* box_len = nullptr) {
+                      const int64_t* box_lo = nullptr, const int64_t* box_len = nullptr,
+                      const void* const* gcoef = nullptr) {
   if (lay.xd == 0) lay = Layout::interleaved(nrhs);
   IGA_CHECK_ARG((f->form == FORM_KM7X) == (x1 != nullptr),
                 "the two-input form takes a second input (iga_fused_apply_sliced2), the others do not");
@@ -999,6 +1047,11 @@ static void run_apply(const iga_fused_s* f, const void* coef, const void* x, voi
   a.x_rstride = lay.xr;
   a.coef_rstride = lay.cr;
   a.coef = (const T*)coef;
+  if (gcoef) {
+    IGA_CHECK_ARG(nrhs <= kGroupMax, "too many members for one grouped apply");
+    a.ngroup = nrhs;
+    for (int k = 0; k < nrhs; ++k) a.gcoef[k] = (const T*)gcoef[k];
+  }
   a.x = (const T*)x;
   a.x1 = (const T*)x1;
   a.w = (const T*)w;
@@ -1102,7 +1155,7 @@ int iga_fused_create(iga_fused_t* out, const int32_t* nc, const int32_t* nq, int
   return guard([&] {
     IGA_CHECK_ARG(out && nc && nq && first && tabs, "NULL argument");
     IGA_CHECK_ARG(P >= 3 && P <= 8, "band width p+1 must be in 3..8");
-    IGA_CHECK_ARG(form >= FORM_M1 && form <= FORM_KM7X, "unknown coefficient form");
+    IGA_CHECK_ARG(form >= FORM_M1 && form <= FORM_UG4, "unknown coefficient form");
     IGA_CHECK_ARG(dtype == IGA_F64 || dtype == IGA_F32, "unknown dtype");
     for (int d = 0; d < 3; ++d) {
       IGA_CHECK_ARG(nc[d] >= P && nq[d] >= 1, "bad sizes");
@@ -1272,6 +1325,29 @@ int iga_fused_apply(iga_fused_t f, const void* packed, const void* x, void* y, i
   });
 }
 
+int iga_fused_apply_group(iga_fused_t f, int nmember, const void* const* packed, const void* x, void* y,
+                          const void* nurbs_w, void* work, int ndim, const int64_t* box_lo, const int64_t* box_len,
+                          void* stream) {
+  return guard([&] {
+    IGA_CHECK_ARG(f && packed && x && y && work, "NULL argument");
+    IGA_CHECK_ARG(nmember >= 1 && nmember <= kGroupMax, "a grouped apply takes 1 to 16 members");
+    for (int k = 0; k < nmember; ++k) IGA_CHECK_ARG(packed[k], "NULL coefficient set");
+    IGA_CHECK_ARG(f->form != FORM_KM7X, "the two-input form is not grouped");
+    IGA_CHECK_ARG(ndim == 0 || ndim == 3, "the mask box of the fused operator has three directions");
+    IGA_CHECK_ARG(ndim == 0 || (box_lo && box_len), "box arrays missing");
+    for (int d = 0; d < ndim; ++d)
+      IGA_CHECK_ARG(box_lo[d] >= 0 && box_len[d] >= 0 && box_lo[d] + box_len[d] <= f->nc[d], "box outside the grid");
+    const int64_t* lo = ndim == 3 ? box_lo : nullptr;
+    const int64_t* len = ndim == 3 ? box_len : nullptr;
+    if (f->dtype == IGA_F64)
+      run_apply<double>(f, packed[0], x, y, nmember, nurbs_w, 1.0, 0, work, as_stream(stream), 3,
+                        Layout{0, 0, 0, 0, 0}, nullptr, lo, len, packed);
+    else
+      run_apply<float>(f, packed[0], x, y, nmember, nurbs_w, 1.0, 0, work, as_stream(stream), 3,
+                       Layout{0, 0, 0, 0, 0}, nullptr, lo, len, packed);
+  });
+}
+
 int iga_fused_apply_noalloc(iga_fused_t f, const void* packed, const void* x, void* y, int nrhs,
                             const void* nurbs_w, double alpha, int accumulate, void* stream) {
   return guard([&] {
@@ -1312,7 +1388,8 @@ int iga_fused_pack_extruded(iga_fused_t f, const double* plane, const double* dz
     IGA_CHECK_ARG(f && packed && plane && dz, "NULL argument");
     IGA_CHECK_ARG((W == nullptr) == (dW == nullptr), "W and dW go together");
     IGA_CHECK_ARG(!H || (Ez && modes >= 1 && modes <= kMaxModes), "field modes: 1..16 with H and Ez");
-    IGA_CHECK_ARG(f->form != FORM_M1, "the generated coefficients are stiffness coefficients");
+    IGA_CHECK_ARG(f->form != FORM_M1 && f->form != FORM_GV4 && f->form != FORM_UG4,
+                  "the generated coefficients are stiffness coefficients");
     if (f->form == FORM_KM7 || f->form == FORM_KM7X) IGA_CHECK_ARG(with_mass, "mass forms need the mass term");
     ExtrudedSource ex;
     ex.plane = plane;

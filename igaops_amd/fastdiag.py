@@ -202,8 +202,11 @@ class FDPlan:
         _lib.call("iga_fd_set_eigenvalues", self._h, lp)
         self.eig_version = version
 
+    def workspace_bytes(self, nrhs: int = 1) -> int:
+        return int(_lib.load().iga_fd_workspace_bytes(self._h, int(nrhs)))
+
     def workspace(self, nrhs: int = 1):
-        return dv.workspace(_lib.load().iga_fd_workspace_bytes(self._h, int(nrhs)))
+        return dv.workspace(self.workspace_bytes(nrhs))
 
     def apply(self, x, out, nrhs: int = 1, stream=None, work=None):
         """``out = P x``; ``work`` (from ``workspace``) lets concurrent callers of one

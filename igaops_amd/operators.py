@@ -308,7 +308,8 @@ def nurbs_weight_fields(quadrule_list, w_dev):
 
 FORM_M1, FORM_K6, FORM_KM7, FORM_SYM10, FORM_GEN16 = 0, 1, 2, 3, 4
 FORM_KM7X = 5  # the KM7 fields, value (mass) term on a second input
-FORM_NCOEF = {0: 1, 1: 6, 2: 7, 3: 10, 4: 16, 5: 7}
+FORM_GV4, FORM_UG4 = 6, 7  # first-order terms: gradient on the test / on the trial function
+FORM_NCOEF = {0: 1, 1: 6, 2: 7, 3: 10, 4: 16, 5: 7, 6: 4, 7: 4}
 
 
 class FusedPlan:
@@ -650,6 +651,91 @@ class StiffnessMassOperator:
         return b * (2 * self.ndof * nrhs + c * self.nq)
 
 
+class FirstOrderOperator:
+    """Device-resident first-order term in one fused sweep: ``kind`` "gradu_v",
+    ``y_A = sum_i int d_i N_A c_i u`` (``compute_mf_scalar_gradu_v``,
+    ``operators/bsplines_ops.py:357-390``), or "u_gradv", ``y_A = sum_j int N_A c_j
+    d_j u`` (``bsplines_ops.py:392-429``); ``c`` is (3, N_q).  For a NURBS patch the
+    rational correction of ``operators/nurbs_ops.py:392-430`` (``c W~^2`` on the
+    first-order term, ``-(c . z) W~`` on the value term) is folded into the four
+    packed fields per point, so the sweep reads ``4 N_q`` coefficients and no
+    ``N_q``-sized intermediate reaches HBM."""
+
+    def __init__(self, quadrule_list, c, kind: str, nurbs_weights=None, dtype: str = "f64"):
+        dv.require_gpu()
+        t = dv.torch()
+        if kind not in ("gradu_v", "u_gradv"):
+            raise ValueError("kind must be 'gradu_v' or 'u_gradv'")
+        self.quadrule_list = list(quadrule_list)
+        if not fused_applicable(self.quadrule_list):
+            raise NotImplementedError("the fused first-order sweep takes three directions of WQ type 2 rules")
+        self.dtype = dtype
+        self.kind = kind
+        self.ndof = int(np.prod([q.nbctrlpts for q in quadrule_list]))
+        self.nq = int(np.prod([q.nbquadpts for q in quadrule_list]))
+        self.is_nurbs = nurbs_weights is not None and int(
+            nurbs_weights.numel() if isinstance(nurbs_weights, t.Tensor) else np.size(nurbs_weights)) > 0
+        cd = dv.to_device(c, "f64").reshape(3, -1).contiguous()
+        self.form = FORM_GV4 if kind == "gradu_v" else FORM_UG4
+        self.plan = fused_plan(self.quadrule_list, self.form, dtype)
+        W = dW = None
+        self.w = None
+        if self.is_nurbs:
+            w64 = dv.to_device(nurbs_weights, "f64")
+            self.w = w64.to(dv.torch_dtype(dtype))
+            Wf, dWf = nurbs_weight_fields(self.quadrule_list, w64)
+            W, dW = Wf, t.cat(dWf)
+        self.packed = self.plan.pack(cd, None, W, dW)
+        self.fused = True
+        self._work = {}
+
+    def workspace(self, nrhs: int):
+        w = self._work.get(nrhs)
+        if w is None:
+            w = dv.workspace(self.plan.workspace_bytes(nrhs))
+            self._work[nrhs] = w
+        return w
+
+    def apply(self, x, out=None, nrhs: int = 1, alpha: float = 1.0, accumulate: bool = False, stream=None):
+        t = dv.torch()
+        if out is None:
+            out = t.empty_like(x)
+        self.plan.apply(self.packed, x, out, nrhs, self.w, alpha, accumulate, work=self.workspace(nrhs), stream=stream)
+        return out
+
+    __call__ = apply
+
+    def algorithmic_bytes(self, nrhs: int = 1) -> int:
+        """``b (2 N_dof r + c N_q)``, c = 3 (B-spline) / 4 (NURBS: the value-value term)."""
+        b = 8 if self.dtype == "f64" else 4
+        return b * (2 * self.ndof * nrhs + (4 if self.is_nurbs else 3) * self.nq)
+
+
+def apply_group(ops, X, Y, box=None, work=None, stream=None):
+    """The fused operators of several solves that share one plan (same rules, form
+    and dtype: requests of one shape, each with its own coefficients) in one launch
+    (``iga_fused_apply_group``): member k applies ``ops[k]`` to column k of ``X``
+    (``(N, g)`` C order) into column k of ``Y``, bit for bit its own ``apply``.
+    ``box`` = ``(lo, len)`` (int64 arrays) zeroes ``Y`` outside the Dirichlet box;
+    ``work`` holds ``plan.workspace_bytes(g)`` bytes (the plan's own buffer if None)."""
+    g = len(ops)
+    if g < 1:
+        raise ValueError("no operators")
+    plan = ops[0].plan if ops[0].fused else None
+    if plan is None or any((not op.fused) or op.plan is not plan for op in ops):
+        raise ValueError("a grouped apply takes fused operators of one plan")
+    ptrs = (C.c_void_p * g)(*[dv.ptr(op.packed) for op in ops])
+    work = work if work is not None else plan.workspace(g)
+    nd, lop, lenp = 0, None, None
+    if box is not None:
+        lo = np.ascontiguousarray(box[0], dtype=np.int64)
+        ln = np.ascontiguousarray(box[1], dtype=np.int64)
+        nd, lop, lenp = 3, lo.ctypes.data_as(_lib._i64p), ln.ctypes.data_as(_lib._i64p)
+    _lib.call("iga_fused_apply_group", plan._h, g, ptrs, dv.ptr(X), dv.ptr(Y), dv.ptr(ops[0].w), dv.ptr(work), nd,
+              lop, lenp, dv.stream_ptr(stream))
+    return Y
+
+
 # ---------------------------------------------------------------------------
 # Reference-signature operator classes
 # ---------------------------------------------------------------------------
@@ -676,6 +762,10 @@ def _operator_call(kind, quadrule_list, coefficients, array_in, time_ders, nurbs
             op = StiffnessMassOperator(qs, C=None, m=coef.reshape(-1), nurbs_weights=nurbs_weights, dtype=dtype)
         else:
             op = StiffnessMassOperator(qs, C=coef.reshape(3, 3, -1), m=None, nurbs_weights=nurbs_weights, dtype=dtype)
+        y = op.apply(x, nrhs=r)
+        return _from_dev(y, was_np, shape)
+    if kind in ("gradu_v", "u_gradv") and fused_applicable(qs, time_ders) and tuple(coefficients.shape[:1]) == (3,):
+        op = FirstOrderOperator(qs, coef.reshape(3, -1), kind, nurbs_weights=nurbs_weights, dtype=dtype)
         y = op.apply(x, nrhs=r)
         return _from_dev(y, was_np, shape)
     tdt = dv.torch_dtype(dtype)

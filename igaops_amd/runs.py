@@ -310,17 +310,17 @@ class GmresRun:
                 self._ensure_rows(k)
         return k, self.row(k - 1), self.z
 
-    def launch(self, preconditioned: bool = False):
+    def launch(self, preconditioned: bool = False, applied: bool = False):
         """Queue Arnoldi step k = launched + 1 (with ``preconditioned``, ``z`` already
-        holds P V[k-1])."""
+        holds P V[k-1]; with ``applied``, ``w`` already holds the masked A P V[k-1])."""
         k = self.launched + 1
         dt = dv.DTYPES[self.dtype]
+        stage = 2 if applied else int(bool(preconditioned))
         if self._ctx is not None:
             if k >= self.V.shape[0]:  # the basis grows (on the run's stream) only now and then
                 with self._on_stream():
                     self._ensure_rows(k)
-            _lib.call("iga_arnoldi_step", self._ctx, k, dv.ptr(self.V), self.ldv, int(bool(preconditioned)),
-                      self._sp())
+            _lib.call("iga_arnoldi_step", self._ctx, k, dv.ptr(self.V), self.ldv, stage, self._sp())
             if k >= 2:
                 self.recorded = k - 1
             self.launched = k
@@ -328,17 +328,17 @@ class GmresRun:
         with self._on_stream():
             self._ensure_rows(k)
             z = self.z
-            if preconditioned:
-                pass
-            elif self.P is not None:
-                self.P(self.row(k - 1), z)
-            else:
-                z.copy_(self.row(k - 1))
-            if self.mask:
-                self.mask(z)
-            self.A(z, self.w)
-            if self.mask:
-                self.mask(self.w)
+            if stage == 0:
+                if self.P is not None:
+                    self.P(self.row(k - 1), z)
+                else:
+                    z.copy_(self.row(k - 1))
+            if stage <= 1:
+                if self.mask:
+                    self.mask(z)
+                self.A(z, self.w)
+                if self.mask:
+                    self.mask(self.w)
             _lib.call("iga_gmres_step", dt, self.n, k, self.m, dv.ptr(self.V), self.ldv, dv.ptr(self.w),
                       dv.ptr(self.ls), dv.ptr(self.work), self._sp())
             if k >= 2:

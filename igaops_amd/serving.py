@@ -25,11 +25,13 @@ How the loop works:
   share plans; when several of them are ready for an Arnoldi step in the same
   pass, their FD preconditioner applications go out as one multi-right-hand-side
   launch (the FD depends on the shape only: parametric pencils with per-direction
-  correctors from the mean geometric conductivity) on a batch stream, ordered by
-  events.
+  correctors from the mean geometric conductivity) and their operators as one
+  grouped launch of the shared plan with each request's coefficients, on a batch
+  stream, ordered by events.
   Nothing a request computes depends on which others are present, so its result
   is the same alone and in the mix (to rounding: the batched FD sums in the same
-  order per column).
+  order per column, and each column of the grouped operator is that request's own
+  apply bit for bit).
 * Latency is measured from arrival to completion (the true residual checked).  The
   loop never waits for one request's stream: set-up uploads go through pinned
   memory, residual estimates and the final true residual come back through pinned
@@ -349,7 +351,7 @@ class SolveServer:
 
     def __init__(self, mode: str = "mixed", hbm_budget_gb: float = 200.0, max_active: int = 16, tol: float = 1e-10,
                  maxiters: int = 800, inner_tol: float = 1e-6, inner_maxiters: int = 300, lookahead: int = 2,
-                 keep_solutions: bool = False, batch_fd: bool = True):
+                 keep_solutions: bool = False, batch_fd: bool = True, batch_operators: bool = True):
         if mode not in ("mixed", "fp64"):
             raise ValueError("mode must be 'mixed' or 'fp64'")
         self.mode = mode
@@ -360,6 +362,7 @@ class SolveServer:
         self.lookahead = int(lookahead)
         self.keep_solutions = keep_solutions
         self.batch_fd = bool(batch_fd)
+        self.batch_operators = bool(batch_operators)
         self._batch_stream = None
         self._batch_work: Dict[tuple, object] = {}
         self.shapes: Dict[tuple, ShapeResources] = {}
@@ -403,9 +406,13 @@ class SolveServer:
 
     def _batched_steps(self, states) -> bool:
         """One Arnoldi step for every group of same-shape requests ready for it, with
-        their preconditioner applications in one multi-right-hand-side FD launch on a
-        shared stream (the FD plan depends on the shape only).  Events order the
-        batch after each request's previous work and each request after the batch."""
+        their preconditioner applications in one multi-right-hand-side FD launch and
+        their operators in one grouped launch (``iga_fused_apply_group``: one plan, the
+        coefficients of each request) on a shared stream; each request then queues
+        only its orthogonalisation.  Events order the batch after each request's
+        previous work and each request after the batch."""
+        from .operators import apply_group
+
         t = dv.torch()
         groups: Dict[tuple, list] = {}
         for st in states:
@@ -423,29 +430,48 @@ class SolveServer:
                 ev = t.cuda.Event()
                 ev.record(r.stream)
                 self._batch_stream.wait_event(ev)
-            wkey = (shape_key, dtype, g)
-            work = self._batch_work.get(wkey)
-            if work is None:
-                with t.cuda.stream(self._batch_stream):
-                    work = plan.workspace(g)
-                self._batch_work[wkey] = work
+            natives = [getattr(r, "_native", None) if r._ctx is not None else None for r in runs]
+            grouped = self.batch_operators and all(nv is not None for nv in natives) and g <= 16
+            fd_work = self._batch_workspace((shape_key, dtype, g, "fd"), lambda: plan.workspace_bytes(g))
+            op_work = self._batch_workspace((shape_key, dtype, g, "op"),
+                                            lambda: natives[0].op.plan.workspace_bytes(g)) if grouped else None
             with t.cuda.stream(self._batch_stream):
                 n = runs[0].n
                 X = t.empty((n, g), dtype=runs[0].V.dtype, device=runs[0].V.device)
                 Y = t.empty_like(X)
                 for j, (_, v, _) in enumerate(steps):
                     X[:, j].copy_(v)
-                plan.apply(X.view(-1), Y.view(-1), g, None, work=work)
-                for j, (_, _, z) in enumerate(steps):
-                    z.copy_(Y[:, j])
+                plan.apply(X.view(-1), Y.view(-1), g, None, work=fd_work)
+                if grouped:
+                    # w_j = masked A_j z_j for every member, z_j = column j of Y
+                    box = natives[0].box
+                    apply_group([nv.op for nv in natives], Y, X, box=(box[1], box[2]), work=op_work,
+                                stream=self._batch_stream)
+                    for j, r in enumerate(runs):
+                        r.w.copy_(X[:, j])
+                else:
+                    for j, (_, _, z) in enumerate(steps):
+                        z.copy_(Y[:, j])
                 done_ev = t.cuda.Event()
                 done_ev.record(self._batch_stream)
             for r in runs:
                 r.stream.wait_event(done_ev)
-                r.launch(preconditioned=True)
+                r.launch(preconditioned=True, applied=grouped)
             self.stats["batched_fd_steps"] = self.stats.get("batched_fd_steps", 0) + g
+            if grouped:
+                self.stats["batched_operator_steps"] = self.stats.get("batched_operator_steps", 0) + g
             done = True
         return done
+
+    def _batch_workspace(self, key, nbytes):
+        """Scratch of the batch stream, from that stream's memory pool, kept per key."""
+        w = self._batch_work.get(key)
+        if w is None:
+            t = dv.torch()
+            with t.cuda.stream(self._batch_stream):
+                w = dv.workspace(nbytes())
+            self._batch_work[key] = w
+        return w
 
     def shape(self, key) -> ShapeResources:
         s = self.shapes.get(key)

@@ -202,7 +202,12 @@ int iga_box_scatter(int dtype, int ndim, const int64_t* n, const int64_t* lo, co
  * form: 0 mass (1 coefficient/point), 1 stiffness with symmetric C (6),
  *       2 stiffness + mass (7), 3 symmetric 4x4 (10: NURBS stiffness [+ mass]),
  *       4 general 4x4 (16), 5 the fields of 2 with the mass term on a second input
- *       (iga_fused_apply_sliced2).
+ *       (iga_fused_apply_sliced2), 6 first order with the gradient on the test
+ *       function (4: compute_mf_scalar_gradu_v, operators/bsplines_ops.py:357-390),
+ *       7 first order with the gradient on the trial function (4:
+ *       compute_mf_scalar_u_gradv, bsplines_ops.py:392-429); for forms 6 and 7 the C of
+ *       iga_fused_pack is the (3, nq) coefficient array c and m is unused (the NURBS
+ *       form adds its value-value correction, operators/nurbs_ops.py:392-430).
  * nslab: number of q3 slabs (0 = automatic). */
 typedef struct iga_fused_s* iga_fused_t;
 
@@ -244,6 +249,21 @@ int iga_fused_apply(iga_fused_t plan, const void* packed, const void* x, void* y
  * iga_fused_apply with the same arguments (same kernels, same summation order). */
 int iga_fused_apply_noalloc(iga_fused_t plan, const void* packed, const void* x, void* y, int nrhs,
                             const void* nurbs_w, double alpha, int accumulate, void* stream);
+
+/* The operators of nmember solves that share one plan (same rules and form, hence the
+ * same shape; coefficients of their own) in one launch: member k's coefficients at
+ * packed[k] (a host array of nmember device pointers, 1 <= nmember <= 16), its vector
+ * in column k of x and y ((N, nmember) C order, as iga_fused_apply with nrhs =
+ * nmember), nurbs_w shared (the shape's).  With ndim = 3, y is zero outside the DOF box
+ * [box_lo, box_lo + box_len) (the Dirichlet mask of linear_solver.py:119-124); ndim = 0,
+ * no mask.  Each column is the member's iga_fused_apply result bit for bit (same
+ * kernels and summation order).  The reference batches only right-hand sides of one
+ * operator (operators/utils/matrix_free.py:36,117-119); this entry batches the
+ * same-shape operator work of different requests of a solve server.  work:
+ * iga_fused_workspace_bytes(plan, nmember) bytes. */
+int iga_fused_apply_group(iga_fused_t plan, int nmember, const void* const* packed, const void* x, void* y,
+                          const void* nurbs_w, void* work, int ndim, const int64_t* box_lo,
+                          const int64_t* box_len, void* stream);
 
 /* The two launches of iga_fused_apply, separately (for timing): stage 1 runs the
  * coefficient sweep into the partial blocks of `work`, stage 2 adds them into y. */
@@ -349,8 +369,10 @@ int iga_fd_destroy(iga_fd_t plan);
  * operation on that stream has.
  * iga_arnoldi_start: iga_gmres_start(r -> V row 0).
  * iga_arnoldi_step (k = 1..m+1): step k of iga_gmres_step on the basis V (rows of
- * stride ldv, at least k + 1 of them); preconditioned != 0 takes z as already holding
- * P v_{k-1} (a caller that batches the FD of several solves).
+ * stride ldv, at least k + 1 of them); preconditioned = 1 takes z as already holding
+ * P v_{k-1} (a caller that batches the FD of several solves), preconditioned = 2 takes
+ * w as already holding the masked A P v_{k-1} (a caller that also batches the operator,
+ * iga_fused_apply_group) and queues only the orthogonalisation.
  * iga_host_alloc: coherent pinned host memory (zeroed), device-writable;
  * iga_host_free releases it (no device work may still write to it). */
 typedef struct iga_arnoldi_s* iga_arnoldi_t;

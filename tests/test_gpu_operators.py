@@ -215,3 +215,71 @@ def test_unit_cube_p3_n32_full_size():
                     tol=1e-10, maxiters=50, cleandod=dod)
     assert out.success and ref.success and out.iterations <= 2 and abs(out.iterations - ref.iterations) <= 1
     assert rel(out.solution, ref.solution) <= 1e-9
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+@pytest.mark.parametrize("geom", ["cube", "annulus"])
+def test_grouped_apply_equals_each_members_apply(dtype, geom):
+    """Three operators of one shape with their own lognormal fields in one grouped
+    launch (iga_fused_apply_group): every column is that operator's own apply bit for
+    bit, with and without the Dirichlet box, and the float64 columns match the oracle."""
+    import torch
+
+    from igaops_amd.operators import StiffnessMassOperator, apply_group
+
+    print("case", geom, dtype, flush=True)
+    patch = unit_cube(3, 6) if geom == "cube" else quarter_annulus(3, 6)
+    qs = patch.quadrule_list
+    w = patch.nurbs_weights if patch.is_nurbs else None
+    coefs = [patch.conductivity_coefficients(lognormal_conductivity(patch, 0.3 + 0.2 * k, seed=k)) for k in range(3)]
+    ops = [StiffnessMassOperator(qs, C=C, m=m, nurbs_weights=w, dtype=dtype) for C, m in coefs]
+    assert all(op.fused and op.plan is ops[0].plan for op in ops)
+    n = patch.nbctrlpts_total
+    tdt = torch.float64 if dtype == "f64" else torch.float32
+    X = torch.randn(n, 3, dtype=torch.float64, device="cuda").to(tdt)
+    Y = torch.empty_like(X)
+    apply_group(ops, X, Y)
+    alone = [op.apply(X[:, k].contiguous()) for k, op in enumerate(ops)]
+    for k in range(3):
+        assert torch.equal(Y[:, k], alone[k])
+    nc = [q.nbctrlpts for q in qs]
+    lo, ln = np.array([1, 1, 1]), np.array([c - 2 for c in nc])
+    apply_group(ops, X, Y, box=(lo, ln))
+    inside = torch.zeros(nc[2], nc[1], nc[0], dtype=torch.bool, device="cuda")
+    inside[1:-1, 1:-1, 1:-1] = True
+    inside = inside.reshape(-1)
+    for k in range(3):
+        assert torch.equal(Y[:, k], torch.where(inside, alone[k], torch.zeros_like(alone[k])))
+    Xh = X.double().cpu().numpy()
+    for k in (0, 2):
+        C, m = coefs[k]
+        ref = oo.stiffness_plus_mass(qs, C, m, Xh[:, k], nurbs_weights=w)
+        assert rel(alone[k].double().cpu().numpy(), ref) <= (F64_TOL if dtype == "f64" else F32_TOL)
+
+
+@pytest.mark.parametrize("kind", ["gradu_v", "u_gradv"])
+@pytest.mark.parametrize("geom", ["cube", "annulus"])
+def test_first_order_terms_in_one_sweep(kind, geom):
+    """The first-order terms through the fused sweep (four packed fields per point,
+    the NURBS correction folded in): float64 and float32, two right-hand sides, against
+    the oracle's composed restatement; the reference-signature call takes this path."""
+    import torch
+
+    from igaops_amd.operators import FirstOrderOperator
+
+    print("case", kind, geom, flush=True)
+    patch = unit_cube(3, 7) if geom == "cube" else quarter_annulus(4, 6)
+    qs = patch.quadrule_list
+    w = patch.nurbs_weights if patch.is_nurbs else None
+    rng = np.random.default_rng(31)
+    c = rng.standard_normal((3, patch.nbquadpts_total))
+    n = patch.nbctrlpts_total
+    X = rng.standard_normal((n, 2))
+    fn = getattr(oo, ("bspline_" if geom == "cube" else "nurbs_") + kind)
+    kw = {} if w is None else {"nurbs_weights": w}
+    ref = np.stack([fn(qs, c, X[:, k], **kw) for k in range(2)], axis=1)
+    for dtype, tol in (("f64", F64_TOL), ("f32", F32_TOL)):
+        op = FirstOrderOperator(qs, c, kind, nurbs_weights=w, dtype=dtype)
+        xd = torch.as_tensor(X, device="cuda").to(torch.float64 if dtype == "f64" else torch.float32)
+        y = op.apply(xd.reshape(-1).contiguous(), nrhs=2).reshape(n, 2).double().cpu().numpy()
+        assert rel(y, ref) <= tol, (dtype, rel(y, ref))

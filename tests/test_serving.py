@@ -79,6 +79,33 @@ def test_results_same_alone_and_in_mix(mode):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_same_shape_requests_batch_fd_and_operator(mode):
+    """Four requests of one shape (their own fields and right-hand sides) admitted
+    together: their FD applications and their operators go out as batched launches
+    (one multi-right-hand-side FD, one grouped operator launch per step of the group),
+    and every solution equals the one served alone."""
+    from igaops_amd.serving import SolveServer
+
+    print("case", mode, flush=True)
+    reqs = lambda: [SolveRequest(i, "annulus", 3, 8, 0.3 + 0.2 * i, 500 + i) for i in range(4)]
+    srv = SolveServer(mode=mode, max_active=4, keep_solutions=True, maxiters=200)
+    mix = {r.rid: r for r in srv.serve(reqs(), realtime=False)}
+    print("batched FD / operator steps", srv.stats.get("batched_fd_steps", 0),
+          srv.stats.get("batched_operator_steps", 0), flush=True)
+    assert srv.stats.get("batched_operator_steps", 0) > 0
+    for req in reqs():
+        alone = SolveServer(mode=mode, max_active=1, keep_solutions=True, maxiters=200).serve([req],
+                                                                                            realtime=False)[0]
+        got = mix[req.rid]
+        assert got.success and alone.success
+        assert got.true_rel_residual <= 1e-10 and alone.true_rel_residual <= 1e-10
+        assert got.iterations == alone.iterations
+        a, b = alone.solution.cpu().numpy(), got.solution.cpu().numpy()
+        assert np.linalg.norm(a - b) <= 1e-12 * np.linalg.norm(a)
+
+
+@pytest.mark.gpu
 def test_device_coefficients_match_host():
     import torch
 
