@@ -249,6 +249,40 @@ def solve_section(patch, C, m, nrhs, tol=1e-10):
             "reached_1e-10": ok,
         }
     res["speedup_mixed_vs_fp64"] = res["mixed"]["solves_per_s"] / res["fp64"]["solves_per_s"]
+    # all right-hand sides at once: their Arnoldi steps batched (one multi-right-hand-side
+    # FD launch and one operator launch reading the coefficient stream once per group step)
+    from igaops_amd.runs import run_batched
+
+    def make(mode, b):
+        if mode == "fp64":
+            return GmresRun(a64, P["f64"], b, tol, 400, mask=mask,
+                            native=NativeStep(A64, plans["f64"], works["f64"], mask.box))
+        return RefinementRun(a64, a32, P["f32"], b, tol=tol, inner_tol=1e-6, inner_maxiters=300, mask=mask,
+                             native_lo=NativeStep(A32, plans["f32"], works["f32"], mask.box))
+
+    for mode in ("fp64", "mixed"):
+        run_batched([make(mode, b) for b in B[:2]])  # untimed: the batch workspaces
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        outs, done_at = run_batched([make(mode, b) for b in B])
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        lat = [d - t0 for d in done_at]
+        true_res = []
+        for b, out in zip(B, outs):
+            r = b - A64.apply(out.solution)
+            mask(r)
+            true_res.append(float(torch.linalg.vector_norm(r) / torch.linalg.vector_norm(b)))
+        res[mode + "_batched"] = {
+            "solves_per_s": len(B) / wall,
+            "p50_ms": float(np.percentile(lat, 50) * 1e3),
+            "p99_ms": float(np.percentile(lat, 99) * 1e3),
+            "iterations_mean": float(np.mean([o.iterations for o in outs])),
+            "max_true_rel_residual": float(max(true_res)),
+            "reached_1e-10": int(sum(tr <= tol * 1.0001 for tr in true_res)),
+        }
+    res["speedup_mixed_vs_fp64_batched"] = (res["mixed_batched"]["solves_per_s"]
+                                            / res["fp64_batched"]["solves_per_s"])
     return res
 
 
@@ -288,7 +322,7 @@ def serve_section(nreq=2000, qps=200.0, seed=0, max_active=16, budget_gb=200.0, 
                   "deferred_for_memory": srv.stats["deferred_for_memory"], "shapes": len(srv.shapes),
                   "host_admit_s": srv.stats["admit_s"], "host_advance_s": srv.stats["advance_s"],
                   "host_complete_s": srv.stats["complete_s"], "host_shape_setup_s": srv.stats["shape_setup_s"],
-                  "batched_fd_steps": srv.stats.get("batched_fd_steps", 0),
+                  "batched_steps": srv.stats.get("batched_steps", 0),
                   "host_setup_s": {k[6:-2]: v for k, v in srv.stats.items() if k.startswith("setup_")},
                   "allocator": {k: srv.stats[k] for k in ("alloc_retries", "device_frees", "device_allocs",
                                                            "peak_reserved_gb")}})
@@ -304,7 +338,8 @@ def same_shape_section(nreq=96, max_active=16):
     nel=32, annulus p=2 nel=40; their own fields, sigma in [0.3, 0.8), and right-hand
     sides), all present at the start: served with same-shape batching (one
     multi-right-hand-side FD launch and one grouped operator launch per group step)
-    and without, per mode; solves/s over the whole stream and the batched steps."""
+    and without, per mode; solves/s over the whole stream and the number of
+    request steps that went out in groups."""
     from igaops_amd.serving import SolveRequest, SolveServer, summarize
 
     shapes = [("annulus", 3, 24), ("cube", 4, 32), ("annulus", 2, 40)]
@@ -314,7 +349,7 @@ def same_shape_section(nreq=96, max_active=16):
     for mode in ("fp64", "mixed"):
         row = {}
         for batch in (False, True):
-            srv = SolveServer(mode=mode, max_active=max_active, batch_fd=batch, batch_operators=batch)
+            srv = SolveServer(mode=mode, max_active=max_active, batch_steps=batch)
             srv.prepare((2, 3, 4), (24, 32, 40), workers=1)
             srv.serve(reqs()[:max_active], realtime=False)  # shape set-up and allocator warm-up
             t1 = time.perf_counter()
@@ -324,8 +359,7 @@ def same_shape_section(nreq=96, max_active=16):
             row["batched" if batch else "unbatched"] = {
                 "solves_per_s": s["solves_per_s"], "reached_tol": s["reached_tol"],
                 "mean_iterations": s["mean_iterations"], "wall_s": wall,
-                "batched_fd_steps": srv.stats.get("batched_fd_steps", 0),
-                "batched_operator_steps": srv.stats.get("batched_operator_steps", 0)}
+                "batched_steps": srv.stats.get("batched_steps", 0)}
         row["speedup_batched"] = row["batched"]["solves_per_s"] / max(row["unbatched"]["solves_per_s"], 1e-30)
         out[mode] = row
         print(f"same-shape {mode}: {json.dumps(row)}", file=sys.stderr, flush=True)
@@ -691,6 +725,7 @@ def main():
     ap.add_argument("--assembly", action="store_true", help="only the sparse assembly timing")
     ap.add_argument("--serve", type=int, default=0, help="only the serving section, with this many requests")
     ap.add_argument("--same-shape", action="store_true", help="only the same-shape batching stream")
+    ap.add_argument("--solve-only", action="store_true", help="only the solves of the workload's right-hand sides")
     ap.add_argument("--qps", type=float, default=200.0)
     ap.add_argument("--no-serve", action="store_true", help="skip the short serving stream of the default run")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
@@ -718,6 +753,11 @@ def main():
         return
     if args.same_shape:
         print(json.dumps({"same_shape": same_shape_section()}))
+        return
+    if args.solve_only:
+        cfg = dict(WORKLOADS[args.workload])
+        patch, C, m = build_patch(cfg)
+        print(json.dumps({"solve": solve_section(patch, C, m, cfg["nrhs"])}))
         return
     if args.serve:
         print(json.dumps({"serve": serve_section(args.serve, args.qps)}))

@@ -22,6 +22,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+#include <vector>
+
 #include "common.hpp"
 #include "internal.hpp"
 
@@ -125,6 +128,124 @@ int iga_arnoldi_step(iga_arnoldi_t c, int k, void* V, int64_t ldv, int precondit
 
 int iga_arnoldi_destroy(iga_arnoldi_t c) {
   return guard([&] { delete c; });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// One Arnoldi step of several solves of one shape (same FD plan, operator plan, mask
+// box, vector type and length) as one host call: their vectors V_i[k_i - 1] stacked as
+// the columns of X, one multi-right-hand-side FD launch X -> Y, one grouped operator
+// launch Y -> w_i (each solve's coefficients, outputs masked), then each solve's
+// orthogonalisation on its own stream.  Events order the batch after each member's
+// previous work and each member's orthogonalisation after the batch.
+// ---------------------------------------------------------------------------
+
+namespace {
+
+constexpr int kStepGroupMax = 16;
+
+template <typename T>
+struct StackArgs {
+  const T* src[kStepGroupMax];
+};
+
+// X[row * g + i] = src_i[row]: the members' vectors as the columns of an (n, g) array
+template <typename T>
+__global__ void __launch_bounds__(256) stack_columns_kernel(int g, int64_t n, StackArgs<T> a, T* __restrict__ X) {
+  const int64_t total = n * g;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(idx % g);
+    X[idx] = a.src[i][idx / g];
+  }
+}
+
+// events for ordering the streams of a group step, per device; an event may be
+// recorded again once the waits on its previous record have been enqueued
+std::mutex g_ev_mu;
+std::vector<std::vector<hipEvent_t>> g_events;
+
+hipEvent_t* group_events(int count) {
+  int dev = 0;
+  IGA_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_ev_mu);
+  if ((int)g_events.size() <= dev) g_events.resize(dev + 1);
+  auto& pool = g_events[dev];
+  while ((int)pool.size() < count) {
+    hipEvent_t e;
+    IGA_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    pool.push_back(e);
+  }
+  return pool.data();
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t iga_arnoldi_group_workspace_bytes(iga_arnoldi_t c, int g) {
+  if (!c || g < 1) return 0;
+  const int64_t es = c->dtype == IGA_F64 ? 8 : 4;
+  return 2 * c->n * g * es + 256;
+}
+
+int iga_arnoldi_step_group(int g, const iga_arnoldi_t* ctxs, const int* ks, void* const* Vs, const int64_t* ldvs,
+                           void* const* streams, void* xy, void* fd_work, void* op_work, void* batch_stream) {
+  return guard([&] {
+    IGA_CHECK_ARG(g >= 1 && g <= kStepGroupMax, "a group step takes 1 to 16 solves");
+    IGA_CHECK_ARG(ctxs && ks && Vs && ldvs && streams && xy && op_work, "NULL argument");
+    const iga_arnoldi_s* c0 = ctxs[0];
+    IGA_CHECK_ARG(c0 && c0->fd && fd_work, "a group step needs the FD preconditioner and its workspace");
+    for (int i = 0; i < g; ++i) {
+      const iga_arnoldi_s* c = ctxs[i];
+      IGA_CHECK_ARG(c && Vs[i] && ldvs[i] >= c0->n, "bad member");
+      IGA_CHECK_ARG(ks[i] >= 1 && ks[i] <= c->m + 1, "step outside the Krylov dimension");
+      IGA_CHECK_ARG(c->dtype == c0->dtype && c->n == c0->n && c->fd == c0->fd && c->op == c0->op &&
+                        c->masked == c0->masked && c->nurbs_w == c0->nurbs_w,
+                    "the members of a group step share the FD, the operator plan, the mask and the weights");
+      for (int d = 0; d < 3; ++d)
+        IGA_CHECK_ARG(c->box_lo[d] == c0->box_lo[d] && c->box_len[d] == c0->box_len[d], "members share the box");
+    }
+    hipStream_t bs = as_stream(batch_stream);
+    hipEvent_t* ev = group_events(g + 1);
+    for (int i = 0; i < g; ++i) {
+      hipStream_t si = as_stream(streams[i]);
+      if (si == bs) continue;
+      IGA_HIP(hipEventRecord(ev[i], si));
+      IGA_HIP(hipStreamWaitEvent(bs, ev[i], 0));
+    }
+    const int64_t n = c0->n;
+    const size_t es = c0->dtype == IGA_F64 ? 8 : 4;
+    void* X = xy;
+    void* Y = (char*)xy + ((n * g * (int64_t)es + 255) & ~int64_t(255));
+    const void* packed[kStepGroupMax];
+    void* outs[kStepGroupMax];
+    if (c0->dtype == IGA_F64) {
+      StackArgs<double> a{};
+      for (int i = 0; i < g; ++i) a.src[i] = (const double*)Vs[i] + (int64_t)(ks[i] - 1) * ldvs[i];
+      stack_columns_kernel<double><<<grid_for(n * g, 256), 256, 0, bs>>>(g, n, a, (double*)X);
+    } else {
+      StackArgs<float> a{};
+      for (int i = 0; i < g; ++i) a.src[i] = (const float*)Vs[i] + (int64_t)(ks[i] - 1) * ldvs[i];
+      stack_columns_kernel<float><<<grid_for(n * g, 256), 256, 0, bs>>>(g, n, a, (float*)X);
+    }
+    IGA_LAUNCH_CHECK();
+    IGA_CHECK(iga_fd_apply(c0->fd, X, Y, g, fd_work, batch_stream));
+    for (int i = 0; i < g; ++i) {
+      packed[i] = ctxs[i]->packed;
+      outs[i] = ctxs[i]->w;
+    }
+    IGA_CHECK(iga_fused_apply_group(c0->op, g, packed, Y, outs, c0->nurbs_w, op_work, c0->masked ? 3 : 0,
+                                    c0->box_lo, c0->box_len, batch_stream));
+    IGA_HIP(hipEventRecord(ev[g], bs));
+    for (int i = 0; i < g; ++i) {
+      const iga_arnoldi_s* c = ctxs[i];
+      hipStream_t si = as_stream(streams[i]);
+      if (si != bs) IGA_HIP(hipStreamWaitEvent(si, ev[g], 0));
+      gmres_step_impl(c->dtype, c->n, ks[i], c->m, Vs[i], ldvs[i], c->w, c->ls, c->work, c->rec, c->tag, streams[i]);
+    }
+  });
 }
 
 }  // extern "C"

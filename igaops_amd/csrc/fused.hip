@@ -559,19 +559,24 @@ struct GatherArgs {
   int accumulate;
   int box;               // zero y outside the DOF box [blo, bhi) (the Dirichlet mask)
   int blo[3], bhi[3];
+  // grouped apply (ngroup > 0): right-hand side k = blockIdx.y is written to gy[k][dof]
+  int ngroup;
+  T* gy[kGroupMax];
 };
 
 template <typename T>
 __global__ void __launch_bounds__(256) fused_gather_kernel(GatherArgs<T> g) {
   const int64_t ndof = (int64_t)g.nc[0] * g.nc[1] * g.nc[2];
-  const int64_t total = ndof * g.nrhs;
+  // grouped: one grid row per right-hand side, DOF fastest
+  const int64_t total = g.ngroup > 0 ? ndof : ndof * g.nrhs;
   for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
        idx += (int64_t)gridDim.x * blockDim.x) {
     // right-hand side fastest for interleaved y, DOF fastest for sliced y (coalesced stores)
     const bool interleaved = g.y_rstride == 1;
-    const int k = interleaved ? (int)(idx % g.nrhs) : (int)(idx / ndof);
-    const int64_t dof = interleaved ? idx / g.nrhs : idx % ndof;
-    const int64_t yi = dof * g.y_dstride + (int64_t)k * g.y_rstride;
+    const int k = g.ngroup > 0 ? (int)blockIdx.y : interleaved ? (int)(idx % g.nrhs) : (int)(idx / ndof);
+    const int64_t dof = g.ngroup > 0 ? idx : interleaved ? idx / g.nrhs : idx % ndof;
+    T* yk = g.ngroup > 0 ? g.gy[k] : g.y;
+    const int64_t yi = g.ngroup > 0 ? dof : dof * g.y_dstride + (int64_t)k * g.y_rstride;
     const int c1 = (int)(dof % g.nc[0]);
     const int c2 = (int)((dof / g.nc[0]) % g.nc[1]);
     const int c3 = (int)(dof / ((int64_t)g.nc[0] * g.nc[1]));
@@ -627,11 +632,11 @@ __global__ void __launch_bounds__(256) fused_gather_kernel(GatherArgs<T> g) {
     }
     T v = g.alpha * acc;
     if (g.w) v *= g.w[dof];
-    if (g.accumulate) v += g.y[yi];
+    if (g.accumulate) v += yk[yi];
     if (g.box && (c1 < g.blo[0] || c1 >= g.bhi[0] || c2 < g.blo[1] || c2 >= g.bhi[1] || c3 < g.blo[2] ||
                   c3 >= g.bhi[2]))
       v = T(0);
-    g.y[yi] = v;
+    yk[yi] = v;
   }
 }
 
@@ -1021,7 +1026,7 @@ static void run_apply(const iga_fused_s* f, const void* coef, const void* x, voi
                       double alpha, int accumulate, void* work, hipStream_t s, int stages = 3,
                       Layout lay = Layout{0, 0, 0, 0, 0}, const void* x1 = nullptr,
                       const int64_t* box_lo = nullptr, const int64_t* box_len = nullptr,
-                      const void* const* gcoef = nullptr) {
+                      const void* const* gcoef = nullptr, void* const* gy = nullptr) {
   if (lay.xd == 0) lay = Layout::interleaved(nrhs);
   IGA_CHECK_ARG((f->form == FORM_KM7X) == (x1 != nullptr),
                 "the two-input form takes a second input (iga_fused_apply_sliced2), the others do not");
@@ -1090,8 +1095,15 @@ static void run_apply(const iga_fused_s* f, const void* coef, const void* x, voi
       g.bhi[d] = (int)(box_lo[d] + box_len[d]);
     }
   }
-  const int64_t total = (int64_t)f->nc[0] * f->nc[1] * f->nc[2] * nrhs;
-  fused_gather_kernel<T><<<grid_for(total, 256), 256, 0, s>>>(g);
+  const int64_t ndof = (int64_t)f->nc[0] * f->nc[1] * f->nc[2];
+  if (gy) {
+    IGA_CHECK_ARG(nrhs <= kGroupMax, "too many members for one grouped apply");
+    g.ngroup = nrhs;
+    for (int k = 0; k < nrhs; ++k) g.gy[k] = (T*)gy[k];
+    fused_gather_kernel<T><<<dim3(grid_for(ndof, 256), (unsigned)nrhs), 256, 0, s>>>(g);
+  } else {
+    fused_gather_kernel<T><<<grid_for(ndof * nrhs, 256), 256, 0, s>>>(g);
+  }
   IGA_LAUNCH_CHECK();
 }
 
@@ -1325,13 +1337,13 @@ int iga_fused_apply(iga_fused_t f, const void* packed, const void* x, void* y, i
   });
 }
 
-int iga_fused_apply_group(iga_fused_t f, int nmember, const void* const* packed, const void* x, void* y,
+int iga_fused_apply_group(iga_fused_t f, int nmember, const void* const* packed, const void* x, void* const* y,
                           const void* nurbs_w, void* work, int ndim, const int64_t* box_lo, const int64_t* box_len,
                           void* stream) {
   return guard([&] {
     IGA_CHECK_ARG(f && packed && x && y && work, "NULL argument");
     IGA_CHECK_ARG(nmember >= 1 && nmember <= kGroupMax, "a grouped apply takes 1 to 16 members");
-    for (int k = 0; k < nmember; ++k) IGA_CHECK_ARG(packed[k], "NULL coefficient set");
+    for (int k = 0; k < nmember; ++k) IGA_CHECK_ARG(packed[k] && y[k], "NULL coefficient set or output");
     IGA_CHECK_ARG(f->form != FORM_KM7X, "the two-input form is not grouped");
     IGA_CHECK_ARG(ndim == 0 || ndim == 3, "the mask box of the fused operator has three directions");
     IGA_CHECK_ARG(ndim == 0 || (box_lo && box_len), "box arrays missing");
@@ -1340,11 +1352,11 @@ int iga_fused_apply_group(iga_fused_t f, int nmember, const void* const* packed,
     const int64_t* lo = ndim == 3 ? box_lo : nullptr;
     const int64_t* len = ndim == 3 ? box_len : nullptr;
     if (f->dtype == IGA_F64)
-      run_apply<double>(f, packed[0], x, y, nmember, nurbs_w, 1.0, 0, work, as_stream(stream), 3,
-                        Layout{0, 0, 0, 0, 0}, nullptr, lo, len, packed);
+      run_apply<double>(f, packed[0], x, y[0], nmember, nurbs_w, 1.0, 0, work, as_stream(stream), 3,
+                        Layout{0, 0, 0, 0, 0}, nullptr, lo, len, packed, y);
     else
-      run_apply<float>(f, packed[0], x, y, nmember, nurbs_w, 1.0, 0, work, as_stream(stream), 3,
-                       Layout{0, 0, 0, 0, 0}, nullptr, lo, len, packed);
+      run_apply<float>(f, packed[0], x, y[0], nmember, nurbs_w, 1.0, 0, work, as_stream(stream), 3,
+                       Layout{0, 0, 0, 0, 0}, nullptr, lo, len, packed, y);
   });
 }
 

@@ -536,15 +536,16 @@ class StiffnessMassOperator:
 
     @classmethod
     def from_extruded(cls, patch, field=None, dtype: str = "f64", mass: bool = False, nurbs_fields=None,
-                      nurbs_w64=None, stream=None):
+                      nurbs_w64=None, nurbs_w=None, stream=None):
         """``K (+ M)`` of an extruded patch (``geometry.ExtrudedPatch``) with the
         conductivity ``kappa detJ J^+ J^+T``, ``kappa`` the lognormal field of
         ``patch.device_field_modes`` (or 1 when ``field`` is None): the same operator as
         ``StiffnessMassOperator(qs, *patch.device_coefficients(...), nurbs_weights=...)``,
         with the coefficients generated and packed in one launch instead of going
         through a (3, 3, N_q) array.  ``nurbs_fields`` = ``(W, dW)``, ``dW`` the three
-        gradient components concatenated, and ``nurbs_w64`` (the control-point weights
-        on the device, float64) may be passed when cached by the caller."""
+        gradient components concatenated, ``nurbs_w64`` (the control-point weights on
+        the device, float64) and ``nurbs_w`` (the same in ``dtype``, shared by the
+        operators of one shape) may be passed when cached by the caller."""
         t = dv.torch()
         dv.require_gpu()
         self = cls.__new__(cls)
@@ -556,7 +557,9 @@ class StiffnessMassOperator:
         self.w64 = None
         if self.is_nurbs:
             self.w64 = nurbs_w64 if nurbs_w64 is not None else dv.upload(np.asarray(patch.nurbs_weights, dtype=float))
-        self.w = None if not self.is_nurbs else self.w64.to(dv.torch_dtype(dtype))
+        self.w = None
+        if self.is_nurbs:
+            self.w = nurbs_w if nurbs_w is not None else self.w64.to(dv.torch_dtype(dtype))
         if not fused_applicable(self.quadrule_list):
             raise NotImplementedError("generated coefficients need the fused operator")
         self.fused = True
@@ -711,29 +714,31 @@ class FirstOrderOperator:
         return b * (2 * self.ndof * nrhs + (4 if self.is_nurbs else 3) * self.nq)
 
 
-def apply_group(ops, X, Y, box=None, work=None, stream=None):
+def apply_group(ops, X, outs, box=None, work=None, stream=None):
     """The fused operators of several solves that share one plan (same rules, form
-    and dtype: requests of one shape, each with its own coefficients) in one launch
-    (``iga_fused_apply_group``): member k applies ``ops[k]`` to column k of ``X``
-    (``(N, g)`` C order) into column k of ``Y``, bit for bit its own ``apply``.
-    ``box`` = ``(lo, len)`` (int64 arrays) zeroes ``Y`` outside the Dirichlet box;
-    ``work`` holds ``plan.workspace_bytes(g)`` bytes (the plan's own buffer if None)."""
+    and dtype: requests of one shape with their own coefficients, or one operator
+    repeated for several right-hand sides) in one launch (``iga_fused_apply_group``):
+    member k applies ``ops[k]`` to column k of ``X`` (``(N, g)`` C order) into the
+    vector ``outs[k]``, bit for bit its own ``apply``.  ``box`` = ``(lo, len)`` (int64
+    arrays) zeroes the outputs outside the Dirichlet box; ``work`` holds
+    ``plan.workspace_bytes(g)`` bytes (the plan's own buffer if None)."""
     g = len(ops)
-    if g < 1:
-        raise ValueError("no operators")
+    if g < 1 or len(outs) != g:
+        raise ValueError("one output per operator")
     plan = ops[0].plan if ops[0].fused else None
     if plan is None or any((not op.fused) or op.plan is not plan for op in ops):
         raise ValueError("a grouped apply takes fused operators of one plan")
     ptrs = (C.c_void_p * g)(*[dv.ptr(op.packed) for op in ops])
+    yptrs = (C.c_void_p * g)(*[dv.ptr(y) for y in outs])
     work = work if work is not None else plan.workspace(g)
     nd, lop, lenp = 0, None, None
     if box is not None:
         lo = np.ascontiguousarray(box[0], dtype=np.int64)
         ln = np.ascontiguousarray(box[1], dtype=np.int64)
         nd, lop, lenp = 3, lo.ctypes.data_as(_lib._i64p), ln.ctypes.data_as(_lib._i64p)
-    _lib.call("iga_fused_apply_group", plan._h, g, ptrs, dv.ptr(X), dv.ptr(Y), dv.ptr(ops[0].w), dv.ptr(work), nd,
+    _lib.call("iga_fused_apply_group", plan._h, g, ptrs, dv.ptr(X), yptrs, dv.ptr(ops[0].w), dv.ptr(work), nd,
               lop, lenp, dv.stream_ptr(stream))
-    return Y
+    return outs
 
 
 # ---------------------------------------------------------------------------

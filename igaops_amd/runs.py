@@ -345,6 +345,12 @@ class GmresRun:
                 self._record(k - 1, self._hist_off + 2 * (k - 1), 2 * (k - 1), 2)
         self.launched = k
 
+    def _launched_native(self, k: int):
+        """Bookkeeping of one-call step k queued by someone else (a group step)."""
+        if k >= 2:
+            self.recorded = k - 1
+        self.launched = k
+
     def poll(self, block: bool = False) -> bool:
         """Read the residual estimates of finished steps; decide convergence."""
         progressed = False
@@ -648,3 +654,87 @@ class RefinementRun:
                 elif self._ev is not None:
                     self._ev.synchronize()
         return self.result
+
+
+def launch_group(group, work: dict, batch_stream=None):
+    """One Arnoldi step of every run of ``group`` (``GmresRun`` objects with one-call
+    steps sharing the FD plan, the operator plan, the mask box and the vector type)
+    queued by one library call (``iga_arnoldi_step_group``): their basis vectors as the
+    columns of one multi-right-hand-side FD launch, one grouped operator launch into
+    each run's ``w``, then each run's orthogonalisation on its own stream.  The batch
+    goes on ``batch_stream`` (the first run's stream if None); ``work`` caches the
+    batch scratch per group size (allocated on the batch stream)."""
+    t = dv.torch()
+    g = len(group)
+    if g < 1 or g > 16:
+        raise ValueError("a group step takes 1 to 16 runs")
+    nv = group[0]._native
+    bs = batch_stream if batch_stream is not None else group[0].stream
+    for r in group:
+        if r._ctx is None:
+            raise ValueError("group steps need runs with one-call steps")
+        r.prepare_step()
+    key = (id(nv.fd_plan), id(nv.op.plan), g)
+    if key not in work:
+        lib = _lib.load()
+        with (t.cuda.stream(bs) if bs is not None else contextlib.nullcontext()):
+            work[key] = (dv.workspace(lib.iga_arnoldi_group_workspace_bytes(group[0]._ctx, g)),
+                         nv.fd_plan.workspace(g), dv.workspace(nv.op.plan.workspace_bytes(g)))
+    xy, fd_work, op_work = work[key]
+    ks = [r.launched + 1 for r in group]
+    ctxs = (C.c_void_p * g)(*[r._ctx.value for r in group])
+    karr = (C.c_int * g)(*ks)
+    Vs = (C.c_void_p * g)(*[dv.ptr(r.V) for r in group])
+    ldvs = (C.c_int64 * g)(*[r.ldv for r in group])
+    streams = (C.c_void_p * g)(*[r._sp() for r in group])
+    _lib.call("iga_arnoldi_step_group", g, ctxs, karr, Vs, ldvs, streams, dv.ptr(xy), dv.ptr(fd_work),
+              dv.ptr(op_work), dv.stream_ptr(bs))
+    for r, k in zip(group, ks):
+        r._launched_native(k)
+
+
+def run_batched(runs, max_group: int = 16):
+    """Drive several solves of one shape to completion with their Arnoldi steps
+    batched: ``runs`` are ``GmresRun`` or ``RefinementRun`` objects on one stream whose
+    one-call steps (``native`` / ``native_lo``) share the FD plan and the mask box and
+    whose operators share one fused plan (the same operator for several right-hand
+    sides, or one per solve).  Every pass, the runs that can take a step take it
+    together (up to ``max_group`` per launch); everything else (starts, refinement
+    residuals, solution updates) stays per run.  Returns the results in order and the
+    host clock (``time.perf_counter``) at which each completed."""
+    import time
+
+    ctxs = []
+    for r in runs:
+        inner_native = r.native_lo if isinstance(r, RefinementRun) else getattr(r, "_native", None)
+        if inner_native is None or inner_native.fd_plan is None or inner_native.box is None:
+            raise ValueError("batched steps need one-call steps with an FD plan and a mask box")
+        ctxs.append(inner_native)
+    nv0 = ctxs[0]
+    for nv in ctxs:
+        if nv.fd_plan is not nv0.fd_plan or nv.op.plan is not nv0.op.plan or \
+                not all(np.array_equal(a, b) for a, b in zip(nv.box, nv0.box)):
+            raise ValueError("batched runs share the FD plan, the operator plan and the mask box")
+    if len({id(r.stream) for r in runs}) != 1:
+        raise ValueError("batched runs share one stream")
+    t = dv.torch()
+    stream = runs[0].stream
+    work: dict = {}
+    done_at = [None] * len(runs)
+    with (t.cuda.stream(stream) if stream is not None else contextlib.nullcontext()):
+        while True:
+            pending = [i for i, r in enumerate(runs) if not r.done]
+            if not pending:
+                break
+            progressed = False
+            for i in pending:
+                progressed |= runs[i].advance(launch=False)
+                if runs[i].done:
+                    done_at[i] = time.perf_counter()
+            ready = [s for s in (runs[i].step_run() for i in pending if not runs[i].done) if s is not None]
+            for j in range(0, len(ready), max_group):
+                launch_group(ready[j:j + max_group], work, stream)
+                progressed = True
+            if not progressed:
+                time.sleep(20e-6)  # nothing to queue: the device is busy with the queued steps
+    return [r.result for r in runs], done_at

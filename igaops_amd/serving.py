@@ -191,6 +191,9 @@ class ShapeResources:
         self.space_fd, self.mask, self.fd_plans = fd.space_fd, fd.mask, fd.plans
         t = dv.torch()
         self.w64 = dv.to_device(self.patch.nurbs_weights, "f64") if self.patch.is_nurbs else None
+        # the weights in each operator precision, shared by the requests of this shape
+        # (a grouped operator launch takes one weight vector for all its members)
+        self.w_by_dtype = {"f64": self.w64, "f32": None if self.w64 is None else self.w64.float()}
         self.nurbs_fields = None
         if self.w64 is not None:
             from .operators import nurbs_weight_fields
@@ -266,8 +269,10 @@ class _Active:
             field = shape.patch.device_field_modes(req.sigma, req.seed)
             lap("coefficients")
             kw = dict(nurbs_fields=shape.nurbs_fields, nurbs_w64=shape.w64)
-            self.A64 = StiffnessMassOperator.from_extruded(shape.patch, field, "f64", **kw)
-            self.A32 = StiffnessMassOperator.from_extruded(shape.patch, field, "f32", **kw) if mode == "mixed" else None
+            self.A64 = StiffnessMassOperator.from_extruded(shape.patch, field, "f64", nurbs_w=shape.w_by_dtype["f64"],
+                                                           **kw)
+            self.A32 = StiffnessMassOperator.from_extruded(shape.patch, field, "f32", nurbs_w=shape.w_by_dtype["f32"],
+                                                           **kw) if mode == "mixed" else None
             # the operators' partial blocks come from this request's stream pool (the
             # inner solves of the mixed mode are built outside any stream context)
             for op in (self.A64, self.A32):
@@ -351,7 +356,7 @@ class SolveServer:
 
     def __init__(self, mode: str = "mixed", hbm_budget_gb: float = 200.0, max_active: int = 16, tol: float = 1e-10,
                  maxiters: int = 800, inner_tol: float = 1e-6, inner_maxiters: int = 300, lookahead: int = 2,
-                 keep_solutions: bool = False, batch_fd: bool = True, batch_operators: bool = True):
+                 keep_solutions: bool = False, batch_steps: bool = True):
         if mode not in ("mixed", "fp64"):
             raise ValueError("mode must be 'mixed' or 'fp64'")
         self.mode = mode
@@ -361,8 +366,7 @@ class SolveServer:
         self.inner_tol, self.inner_maxiters = float(inner_tol), int(inner_maxiters)
         self.lookahead = int(lookahead)
         self.keep_solutions = keep_solutions
-        self.batch_fd = bool(batch_fd)
-        self.batch_operators = bool(batch_operators)
+        self.batch_steps = bool(batch_steps)
         self._batch_stream = None
         self._batch_work: Dict[tuple, object] = {}
         self.shapes: Dict[tuple, ShapeResources] = {}
@@ -405,73 +409,29 @@ class SolveServer:
         return info
 
     def _batched_steps(self, states) -> bool:
-        """One Arnoldi step for every group of same-shape requests ready for it, with
-        their preconditioner applications in one multi-right-hand-side FD launch and
-        their operators in one grouped launch (``iga_fused_apply_group``: one plan, the
-        coefficients of each request) on a shared stream; each request then queues
-        only its orthogonalisation.  Events order the batch after each request's
-        previous work and each request after the batch."""
-        from .operators import apply_group
+        """One Arnoldi step for every group of same-shape requests ready for it, as one
+        library call per group (``iga_arnoldi_step_group``): their preconditioner
+        applications in one multi-right-hand-side FD launch and their operators in one
+        grouped launch (one plan, each request's coefficients) on the batch stream,
+        then each request's orthogonalisation on its own stream; events order the batch
+        after each request's previous work and each request after the batch."""
+        from .runs import launch_group
 
-        t = dv.torch()
         groups: Dict[tuple, list] = {}
         for st in states:
             run = st.run.step_run() if st.run is not None else None
-            if run is not None:
+            if run is not None and run._ctx is not None:
                 groups.setdefault((st.req.shape, run.dtype), []).append(run)
         done = False
-        for (shape_key, dtype), runs in groups.items():
-            if len(runs) < 2:
-                continue
-            g = len(runs)
-            plan = self.fds[shape_key].plans[dtype]
-            steps = [r.prepare_step() for r in runs]
-            for r in runs:
-                ev = t.cuda.Event()
-                ev.record(r.stream)
-                self._batch_stream.wait_event(ev)
-            natives = [getattr(r, "_native", None) if r._ctx is not None else None for r in runs]
-            grouped = self.batch_operators and all(nv is not None for nv in natives) and g <= 16
-            fd_work = self._batch_workspace((shape_key, dtype, g, "fd"), lambda: plan.workspace_bytes(g))
-            op_work = self._batch_workspace((shape_key, dtype, g, "op"),
-                                            lambda: natives[0].op.plan.workspace_bytes(g)) if grouped else None
-            with t.cuda.stream(self._batch_stream):
-                n = runs[0].n
-                X = t.empty((n, g), dtype=runs[0].V.dtype, device=runs[0].V.device)
-                Y = t.empty_like(X)
-                for j, (_, v, _) in enumerate(steps):
-                    X[:, j].copy_(v)
-                plan.apply(X.view(-1), Y.view(-1), g, None, work=fd_work)
-                if grouped:
-                    # w_j = masked A_j z_j for every member, z_j = column j of Y
-                    box = natives[0].box
-                    apply_group([nv.op for nv in natives], Y, X, box=(box[1], box[2]), work=op_work,
-                                stream=self._batch_stream)
-                    for j, r in enumerate(runs):
-                        r.w.copy_(X[:, j])
-                else:
-                    for j, (_, _, z) in enumerate(steps):
-                        z.copy_(Y[:, j])
-                done_ev = t.cuda.Event()
-                done_ev.record(self._batch_stream)
-            for r in runs:
-                r.stream.wait_event(done_ev)
-                r.launch(preconditioned=True, applied=grouped)
-            self.stats["batched_fd_steps"] = self.stats.get("batched_fd_steps", 0) + g
-            if grouped:
-                self.stats["batched_operator_steps"] = self.stats.get("batched_operator_steps", 0) + g
-            done = True
+        for runs in groups.values():
+            for j in range(0, len(runs), 16):
+                grp = runs[j:j + 16]
+                if len(grp) < 2:
+                    continue
+                launch_group(grp, self._batch_work, self._batch_stream)
+                self.stats["batched_steps"] = self.stats.get("batched_steps", 0) + len(grp)
+                done = True
         return done
-
-    def _batch_workspace(self, key, nbytes):
-        """Scratch of the batch stream, from that stream's memory pool, kept per key."""
-        w = self._batch_work.get(key)
-        if w is None:
-            t = dv.torch()
-            with t.cuda.stream(self._batch_stream):
-                w = dv.workspace(nbytes())
-            self._batch_work[key] = w
-        return w
 
     def shape(self, key) -> ShapeResources:
         s = self.shapes.get(key)
@@ -502,6 +462,7 @@ class SolveServer:
         pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
         streams = [t.cuda.Stream() for _ in range(self.max_active)]
         self._batch_stream = t.cuda.Stream()
+        self._batch_work = {}  # group-step scratch, allocated on this batch stream
         # one pinned host buffer per stream slot, lent to the solver runs of that slot
         from .runs import GmresRun
 
@@ -544,7 +505,7 @@ class SolveServer:
             self.stats["max_bytes"] = max(self.stats["max_bytes"], used)
             progressed = False
             ta = time.perf_counter()
-            if self.batch_fd and len(active) > 1:
+            if self.batch_steps and len(active) > 1:
                 # only requests whose shape another active request shares can batch their
                 # FD steps: they advance without launching, then the batches go out
                 seen, shared = set(), set()

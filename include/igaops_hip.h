@@ -251,17 +251,19 @@ int iga_fused_apply_noalloc(iga_fused_t plan, const void* packed, const void* x,
                             const void* nurbs_w, double alpha, int accumulate, void* stream);
 
 /* The operators of nmember solves that share one plan (same rules and form, hence the
- * same shape; coefficients of their own) in one launch: member k's coefficients at
- * packed[k] (a host array of nmember device pointers, 1 <= nmember <= 16), its vector
- * in column k of x and y ((N, nmember) C order, as iga_fused_apply with nrhs =
- * nmember), nurbs_w shared (the shape's).  With ndim = 3, y is zero outside the DOF box
+ * same shape; coefficients of their own, or the same ones for several right-hand
+ * sides of one operator) in one launch: member k's coefficients at packed[k] (a host
+ * array of nmember device pointers, 1 <= nmember <= 16), its input in column k of x
+ * ((N, nmember) C order, as iga_fused_apply with nrhs = nmember), its output at y[k]
+ * (a host array of nmember device pointers, N values each), nurbs_w shared (the
+ * shape's).  With ndim = 3, y is zero outside the DOF box
  * [box_lo, box_lo + box_len) (the Dirichlet mask of linear_solver.py:119-124); ndim = 0,
  * no mask.  Each column is the member's iga_fused_apply result bit for bit (same
  * kernels and summation order).  The reference batches only right-hand sides of one
  * operator (operators/utils/matrix_free.py:36,117-119); this entry batches the
  * same-shape operator work of different requests of a solve server.  work:
  * iga_fused_workspace_bytes(plan, nmember) bytes. */
-int iga_fused_apply_group(iga_fused_t plan, int nmember, const void* const* packed, const void* x, void* y,
+int iga_fused_apply_group(iga_fused_t plan, int nmember, const void* const* packed, const void* x, void* const* y,
                           const void* nurbs_w, void* work, int ndim, const int64_t* box_lo,
                           const int64_t* box_len, void* stream);
 
@@ -384,6 +386,24 @@ int iga_arnoldi_create(iga_arnoldi_t* out, int dtype, int64_t n, int m, iga_fuse
 int iga_arnoldi_start(iga_arnoldi_t ctx, const void* r, void* V, void* stream);
 int iga_arnoldi_step(iga_arnoldi_t ctx, int k, void* V, int64_t ldv, int preconditioned, void* stream);
 int iga_arnoldi_destroy(iga_arnoldi_t ctx);
+
+/* One Arnoldi step of g solves of one shape (1 <= g <= 16; contexts sharing the FD
+ * plan, the fused-operator plan, the mask box, the NURBS weights, the vector type and
+ * length; each with its own coefficients, basis and state) as one call: step ks[i] of
+ * solve i on basis Vs[i] (row stride ldvs[i]).  The vectors V_i[k_i - 1] become the
+ * columns of one multi-right-hand-side FD apply and the results go through one
+ * iga_fused_apply_group launch into each context's w (masked); then each solve's
+ * orthogonalisation (iga_gmres_step, residual estimate published as by
+ * iga_arnoldi_step) is queued on streams[i].  The batch runs on batch_stream, after
+ * every member stream's earlier work and before the orthogonalisations (events; a
+ * member stream equal to batch_stream needs none).  xy: scratch of
+ * iga_arnoldi_group_workspace_bytes(ctx, g) bytes; fd_work: iga_fd_workspace_bytes(fd,
+ * g); op_work: iga_fused_workspace_bytes(op, g).  The reference has no counterpart: it
+ * runs one solve at a time (solvers/linear_solver.py:438-525); this entry point
+ * batches the same-shape steps of a solve server or of several right-hand sides. */
+int64_t iga_arnoldi_group_workspace_bytes(iga_arnoldi_t ctx, int g);
+int iga_arnoldi_step_group(int g, const iga_arnoldi_t* ctxs, const int* ks, void* const* Vs, const int64_t* ldvs,
+                           void* const* streams, void* xy, void* fd_work, void* op_work, void* batch_stream);
 int iga_host_alloc(int64_t bytes, void** out);
 int iga_host_free(void* p);
 

@@ -237,11 +237,17 @@ def test_grouped_apply_equals_each_members_apply(dtype, geom):
     n = patch.nbctrlpts_total
     tdt = torch.float64 if dtype == "f64" else torch.float32
     X = torch.randn(n, 3, dtype=torch.float64, device="cuda").to(tdt)
-    Y = torch.empty_like(X)
+    Y = [torch.empty(n, dtype=tdt, device="cuda") for _ in range(3)]
     apply_group(ops, X, Y)
     alone = [op.apply(X[:, k].contiguous()) for k, op in enumerate(ops)]
     for k in range(3):
-        assert torch.equal(Y[:, k], alone[k])
+        assert torch.equal(Y[k], alone[k])
+    # one operator for several right-hand sides: the multi-right-hand-side apply
+    Z = [torch.empty(n, dtype=tdt, device="cuda") for _ in range(3)]
+    apply_group([ops[1]] * 3, X, Z)
+    multi = ops[1].apply(X.reshape(-1).contiguous(), nrhs=3).reshape(n, 3)
+    for k in range(3):
+        assert torch.equal(Z[k], multi[:, k])
     nc = [q.nbctrlpts for q in qs]
     lo, ln = np.array([1, 1, 1]), np.array([c - 2 for c in nc])
     apply_group(ops, X, Y, box=(lo, ln))
@@ -249,7 +255,7 @@ def test_grouped_apply_equals_each_members_apply(dtype, geom):
     inside[1:-1, 1:-1, 1:-1] = True
     inside = inside.reshape(-1)
     for k in range(3):
-        assert torch.equal(Y[:, k], torch.where(inside, alone[k], torch.zeros_like(alone[k])))
+        assert torch.equal(Y[k], torch.where(inside, alone[k], torch.zeros_like(alone[k])))
     Xh = X.double().cpu().numpy()
     for k in (0, 2):
         C, m = coefs[k]

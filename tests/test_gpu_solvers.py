@@ -48,9 +48,10 @@ def test_fd_apply_matches_oracle(table, p, nel):
                                      ("f32", (132, 52, 142)), ("bf16", (72, 97, 122))])
 def test_fd_apply_every_staging_size(dt, nels):
     """Three-launch FD applies on boxes of unequal sides (degree 2, every face Dirichlet,
-    box side nel), so that the plane and line kernels run every operand-staging size:
-    flat with 16 / 24 / 40 / 64 registers per thread (sides up to 64 / 78 / 101 / 128)
-    and row rounds past 128; two right-hand sides, against the oracle's apply."""
+    box side nel), so that the line kernel runs every operand-staging size (flat with
+    16 / 24 / 40 / 64 registers per thread for sides up to 64 / 78 / 101 / 128, row
+    rounds past 128) and the plane kernels one to three tiles per wave; two right-hand
+    sides, against the oracle's apply."""
     import torch
 
     from igaops_amd.fastdiag import SingleFD
@@ -294,6 +295,48 @@ def test_one_call_arnoldi_step_inside_refinement():
     r = b - ops["f64"].apply(out.solution)
     mask(r)
     assert float(torch.linalg.vector_norm(r) / torch.linalg.vector_norm(b)) <= 1e-10
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_batched_steps_of_several_right_hand_sides_match_single_runs(mode):
+    """Five right-hand sides of one operator driven together (``run_batched``: the FD
+    of the ready runs in one multi-right-hand-side launch, their operators in one
+    grouped launch, then each run's orthogonalisation): the same iteration counts and
+    solution bits as each solve driven alone, and a true residual of 1e-10."""
+    import torch
+
+    from igaops_amd.runs import GmresRun, NativeStep, RefinementRun, run_batched
+
+    print("case", mode, flush=True)
+    patch, C, m, fd, mask, ops = _annulus_fd_problem(4, 8)
+    n = patch.nbctrlpts_total
+    dt = "f64" if mode == "fp64" else "f32"
+    plan = fd.space_preconditioner.plan(dt)
+    work = plan.workspace(1)
+    A64 = lambda x, o: ops["f64"].apply(x, out=o)
+    A32 = lambda x, o: ops["f32"].apply(x, out=o)
+    P = lambda x, o: plan.apply(x, o, 1, None, work=work)
+    rng = np.random.default_rng(15)
+    B = [torch.as_tensor(rng.standard_normal(n), device="cuda") for _ in range(5)]
+    for b in B:
+        mask(b)
+
+    def make(b):
+        if mode == "fp64":
+            return GmresRun(A64, P, b, 1e-10, 200, mask=mask, native=NativeStep(ops["f64"], plan, work, mask.box))
+        return RefinementRun(A64, A32, P, b, tol=1e-10, inner_tol=1e-5, mask=mask,
+                             native_lo=NativeStep(ops["f32"], plan, work, mask.box))
+
+    alone = [make(b).run() for b in B]
+    together, done_at = run_batched([make(b) for b in B])
+    assert all(d is not None for d in done_at)
+    for b, a, t in zip(B, alone, together):
+        assert a.success and t.success
+        assert t.iterations == a.iterations
+        assert torch.equal(t.solution, a.solution)
+        r = b - ops["f64"].apply(t.solution)
+        mask(r)
+        assert float(torch.linalg.vector_norm(r) / torch.linalg.vector_norm(b)) <= 1e-10
 
 
 def test_refinement_run_float32_vectors_reaches_1e10():
