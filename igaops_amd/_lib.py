@@ -53,13 +53,14 @@ SIGNATURES = {
     "iga_fused_pack": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "iga_fused_apply": (_i32, [_vp, _vp, _vp, _vp, _i32, _vp, _f64, _i32, _vp, _vp]),
     "iga_fused_apply_noalloc": (_i32, [_vp, _vp, _vp, _vp, _i32, _vp, _f64, _i32, _vp]),
-    "iga_fused_apply_group": (_i32, [_vp, _i32, C.POINTER(_vp), _vp, C.POINTER(_vp), _vp, _vp, _i32, _i64p, _i64p,
-                                     _vp]),
+    "iga_fused_apply_group": (_i32, [_vp, _i32, C.POINTER(_vp), _vp, _i64, _i64, C.POINTER(_vp), _vp, _vp, _i32,
+                                     _i64p, _i64p, _vp]),
     "iga_fused_apply_stage": (_i32, [_vp, _i32, _vp, _vp, _vp, _i32, _vp, _f64, _i32, _vp, _vp]),
     "iga_fd_create": (_i32, [C.POINTER(_vp), _i32, _i64p, _i64p, _i64p, C.POINTER(_f64p), _i32]),
     "iga_fd_set_eigenvalues": (_i32, [_vp, _f64p]),
     "iga_fd_workspace_bytes": (_i64, [_vp, _i32]),
     "iga_fd_apply": (_i32, [_vp, _vp, _vp, _i32, _vp, _vp]),
+    "iga_fd_apply_rhs_major": (_i32, [_vp, _vp, _vp, _i32, _vp, _vp]),
     "iga_fd_destroy": (_i32, [_vp]),
     "iga_fused_apply_sliced": (_i32, [_vp, _vp, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _i32, _vp, C.c_double, _i32,
                                       _vp, _vp]),
@@ -88,6 +89,8 @@ SIGNATURES = {
     "iga_arnoldi_start": (_i32, [_vp, _vp, _vp, _vp]),
     "iga_arnoldi_step": (_i32, [_vp, _i32, _vp, _i64, _i32, _vp]),
     "iga_arnoldi_group_workspace_bytes": (_i64, [_vp, _i32]),
+    "iga_arnoldi_increment_doubles": (_i64, [_i32]),
+    "iga_arnoldi_set_increment": (_i32, [_vp, _vp, _i64, _vp, _vp]),
     "iga_arnoldi_step_group": (_i32, [_i32, C.POINTER(_vp), C.POINTER(C.c_int), C.POINTER(_vp), _i64p,
                                       C.POINTER(_vp), _vp, _vp, _vp, _vp]),
     "iga_arnoldi_destroy": (_i32, [_vp]),

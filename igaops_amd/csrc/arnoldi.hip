@@ -46,6 +46,12 @@ struct iga_arnoldi_s {
   void* work = nullptr;
   double* rec = nullptr;
   int64_t tag = 0;
+  // increment criterion (iga_arnoldi_set_increment): the basis of preconditioned
+  // vectors, the Gram row of the step (device) and the solve's increment record
+  void* Z = nullptr;
+  int64_t ldz = 0;
+  double* gram = nullptr;
+  double* inc_rec = nullptr;
 };
 
 using namespace iga;
@@ -110,17 +116,30 @@ int iga_arnoldi_step(iga_arnoldi_t c, int k, void* V, int64_t ldv, int precondit
     hipStream_t s = as_stream(stream);
     const size_t es = c->dtype == IGA_F64 ? 8 : 4;
     const void* v = (const char*)V + (size_t)(k - 1) * ldv * es;
+    // with the increment criterion z_{k-1} = P v_{k-1} is kept as row k - 1 of Z
+    void* z = c->Z ? (char*)c->Z + (size_t)(k - 1) * c->ldz * es : c->z;
+    IGA_CHECK_ARG(!c->Z || preconditioned == 0, "the increment criterion needs the step's own FD and operator");
     if (preconditioned == 0) {
       if (c->fd)
-        IGA_CHECK(iga_fd_apply(c->fd, v, c->z, 1, c->fd_work, stream));
+        IGA_CHECK(iga_fd_apply(c->fd, v, z, 1, c->fd_work, stream));
       else
-        IGA_HIP(hipMemcpyAsync(c->z, v, (size_t)c->n * es, hipMemcpyDeviceToDevice, s));
+        IGA_HIP(hipMemcpyAsync(z, v, (size_t)c->n * es, hipMemcpyDeviceToDevice, s));
     }
     if (preconditioned <= 1) {
       if (c->masked)
-        fused_apply_masked(c->op, c->packed, c->z, c->w, c->nurbs_w, c->op_work, c->box_lo, c->box_len, stream);
+        fused_apply_masked(c->op, c->packed, z, c->w, c->nurbs_w, c->op_work, c->box_lo, c->box_len, stream);
       else
-        IGA_CHECK(iga_fused_apply(c->op, c->packed, c->z, c->w, 1, c->nurbs_w, 1.0, 0, c->op_work, stream));
+        IGA_CHECK(iga_fused_apply(c->op, c->packed, z, c->w, 1, c->nurbs_w, 1.0, 0, c->op_work, stream));
+    }
+    if (c->Z) {
+      // Gram row z_{k-1} . z_0..z_{k-1}, published with the step's increment record entry
+      dot_many_impl(c->dtype, c->n, k, c->Z, c->ldz, z, c->gram, c->work, stream);
+      GmIncrement inc;
+      inc.gram = c->gram;
+      inc.rec = c->inc_rec;
+      inc.tag = c->tag;
+      gmres_step_impl(c->dtype, c->n, k, c->m, V, ldv, c->w, c->ls, c->work, c->rec, c->tag, stream, &inc);
+      return;
     }
     gmres_step_impl(c->dtype, c->n, k, c->m, V, ldv, c->w, c->ls, c->work, c->rec, c->tag, stream);
   });
@@ -130,12 +149,26 @@ int iga_arnoldi_destroy(iga_arnoldi_t c) {
   return guard([&] { delete c; });
 }
 
+int64_t iga_arnoldi_increment_doubles(int m) { return m >= 1 ? gmres_increment_doubles(m) : 0; }
+
+int iga_arnoldi_set_increment(iga_arnoldi_t c, void* Z, int64_t ldz, double* gram, double* inc_rec) {
+  return guard([&] {
+    IGA_CHECK_ARG(c, "bad argument");
+    IGA_CHECK_ARG(!Z || (gram && inc_rec && ldz >= c->n), "the increment criterion needs Z, the Gram row and the "
+                                                         "record");
+    c->Z = Z;
+    c->ldz = ldz;
+    c->gram = gram;
+    c->inc_rec = inc_rec;
+  });
+}
+
 }  // extern "C"
 
 // ---------------------------------------------------------------------------
 // One Arnoldi step of several solves of one shape (same FD plan, operator plan, mask
-// box, vector type and length) as one host call: their vectors V_i[k_i - 1] stacked as
-// the columns of X, one multi-right-hand-side FD launch X -> Y, one grouped operator
+// box, vector type and length) as one host call: their vectors V_i[k_i - 1] stacked one
+// after the other in X, one multi-right-hand-side FD launch X -> Y, one grouped operator
 // launch Y -> w_i (each solve's coefficients, outputs masked), then each solve's
 // orthogonalisation on its own stream.  Events order the batch after each member's
 // previous work and each member's orthogonalisation after the batch.
@@ -150,15 +183,13 @@ struct StackArgs {
   const T* src[kStepGroupMax];
 };
 
-// X[row * g + i] = src_i[row]: the members' vectors as the columns of an (n, g) array
+// X[i * n + row] = src_i[row]: the members' vectors one after the other (grid row i)
 template <typename T>
-__global__ void __launch_bounds__(256) stack_columns_kernel(int g, int64_t n, StackArgs<T> a, T* __restrict__ X) {
-  const int64_t total = n * g;
-  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
-       idx += (int64_t)gridDim.x * blockDim.x) {
-    const int i = (int)(idx % g);
-    X[idx] = a.src[i][idx / g];
-  }
+__global__ void __launch_bounds__(256) stack_rows_kernel(int64_t n, StackArgs<T> a, T* __restrict__ X) {
+  const T* src = a.src[blockIdx.y];
+  T* dst = X + (int64_t)blockIdx.y * n;
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x)
+    dst[r] = src[r];
 }
 
 // events for ordering the streams of a group step, per device; an event may be
@@ -201,6 +232,7 @@ int iga_arnoldi_step_group(int g, const iga_arnoldi_t* ctxs, const int* ks, void
       const iga_arnoldi_s* c = ctxs[i];
       IGA_CHECK_ARG(c && Vs[i] && ldvs[i] >= c0->n, "bad member");
       IGA_CHECK_ARG(ks[i] >= 1 && ks[i] <= c->m + 1, "step outside the Krylov dimension");
+      IGA_CHECK_ARG(!c->Z, "the increment criterion is not taken by group steps");
       IGA_CHECK_ARG(c->dtype == c0->dtype && c->n == c0->n && c->fd == c0->fd && c->op == c0->op &&
                         c->masked == c0->masked && c->nurbs_w == c0->nurbs_w,
                     "the members of a group step share the FD, the operator plan, the mask and the weights");
@@ -221,22 +253,24 @@ int iga_arnoldi_step_group(int g, const iga_arnoldi_t* ctxs, const int* ks, void
     void* Y = (char*)xy + ((n * g * (int64_t)es + 255) & ~int64_t(255));
     const void* packed[kStepGroupMax];
     void* outs[kStepGroupMax];
+    const dim3 sgrid(grid_for(n, 256, 4096), (unsigned)g);
     if (c0->dtype == IGA_F64) {
       StackArgs<double> a{};
       for (int i = 0; i < g; ++i) a.src[i] = (const double*)Vs[i] + (int64_t)(ks[i] - 1) * ldvs[i];
-      stack_columns_kernel<double><<<grid_for(n * g, 256), 256, 0, bs>>>(g, n, a, (double*)X);
+      stack_rows_kernel<double><<<sgrid, 256, 0, bs>>>(n, a, (double*)X);
     } else {
       StackArgs<float> a{};
       for (int i = 0; i < g; ++i) a.src[i] = (const float*)Vs[i] + (int64_t)(ks[i] - 1) * ldvs[i];
-      stack_columns_kernel<float><<<grid_for(n * g, 256), 256, 0, bs>>>(g, n, a, (float*)X);
+      stack_rows_kernel<float><<<sgrid, 256, 0, bs>>>(n, a, (float*)X);
     }
     IGA_LAUNCH_CHECK();
-    IGA_CHECK(iga_fd_apply(c0->fd, X, Y, g, fd_work, batch_stream));
+    // member-major throughout: every kernel reads and writes each member's vector contiguously
+    IGA_CHECK(iga_fd_apply_rhs_major(c0->fd, X, Y, g, fd_work, batch_stream));
     for (int i = 0; i < g; ++i) {
       packed[i] = ctxs[i]->packed;
       outs[i] = ctxs[i]->w;
     }
-    IGA_CHECK(iga_fused_apply_group(c0->op, g, packed, Y, outs, c0->nurbs_w, op_work, c0->masked ? 3 : 0,
+    IGA_CHECK(iga_fused_apply_group(c0->op, g, packed, Y, 1, n, outs, c0->nurbs_w, op_work, c0->masked ? 3 : 0,
                                     c0->box_lo, c0->box_len, batch_stream));
     IGA_HIP(hipEventRecord(ev[g], bs));
     for (int i = 0; i < g; ++i) {

@@ -505,6 +505,9 @@ __device__ inline void fd_zero_cols(typename FdMma<MODE>::S* dst, int p, int row
 template <typename T>
 struct FdPlaneArgs {
   int n1, n2, n3, nrhs;
+  // element e of right-hand side k of the input / output at e * es + k * ks: interleaved
+  // (N, nrhs) arrays have es = nrhs, ks = 1; right-hand-side-major ones es = 1, ks = N
+  int64_t es_in, ks_in, es_out, ks_out;
   int64_t nf1, nf2, nf3;  // full lengths of the three directions
   int64_t lo1, lo2, lo3;  // box start
   const T* in;            // forward: full x; backward: box-dense W
@@ -533,14 +536,13 @@ __global__ void __launch_bounds__(256) fd_plane_kernel(FdPlaneArgs<typename FdMm
   S* Q = P + (int64_t)n2 * p;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int k = blockIdx.z;
-  const int nr = a.nrhs;
   if (BWD && (int)blockIdx.y >= a.n3) {
     // a plane of y outside the box in direction 3: zeros, split over blockIdx.x
     const int t = blockIdx.y - a.n3;
     const int64_t i3f = t < a.lo3 ? t : a.lo3 + a.n3 + (t - a.lo3);
     const int64_t np = a.nf1 * a.nf2;
     for (int64_t q = (int64_t)blockIdx.x * 256 + tid; q < np; q += (int64_t)gridDim.x * 256)
-      a.out[(q + np * i3f) * nr + k] = T(0);
+      a.out[(q + np * i3f) * a.es_out + k * a.ks_out] = T(0);
     return;
   }
   const int i3 = blockIdx.y;
@@ -548,14 +550,14 @@ __global__ void __launch_bounds__(256) fd_plane_kernel(FdPlaneArgs<typename FdMm
   const int K1 = (n1 + M::KS - 1) / M::KS * M::KS;  // zero-padded inner lengths
   const int K2 = (n2 + M::KS - 1) / M::KS * M::KS;
   // the plane X[k2][k1] (rows k2, k1 contiguous in both layouts) and the A1 rows of S
-  const int64_t row0 = BWD ? (int64_t)n1 * n2 * i3 * nr + k
-                           : ((a.lo1) + a.nf1 * (a.lo2 + a.nf2 * (a.lo3 + i3))) * nr + k;
-  const int64_t rstep = BWD ? (int64_t)n1 * nr : a.nf1 * nr;
+  const int64_t row0 = BWD ? (int64_t)n1 * n2 * i3 * a.es_in + k * a.ks_in
+                           : ((a.lo1) + a.nf1 * (a.lo2 + a.nf2 * (a.lo3 + i3))) * a.es_in + k * a.ks_in;
+  const int64_t rstep = BWD ? (int64_t)n1 * a.es_in : a.nf1 * a.es_in;
   auto xrow = [&](int r) { return a.in + row0 + rstep * r; };
   auto a1row = [&](int r) { return a.A1 + (int64_t)(i1s + r) * n1; };
   auto a2row = [&](int r) { return a.A2 + (int64_t)r * n2; };
   const int qrows = min(16, n1 - i1s);  // rows of S inside the box; the rest are zero
-  fd_stage<MODE, TPW>(P, p, n2, n1, xrow, nr, wave, lane);
+  fd_stage<MODE, TPW>(P, p, n2, n1, xrow, a.es_in, wave, lane);
   fd_stage<MODE, TPW>(Q, p, qrows, n1, a1row, 1, wave, lane);
   for (int q = tid; q < (16 - qrows) * p; q += 256) Q[qrows * p + q] = M::to_s(T(0));
   fd_zero_cols<MODE>(P, p, n2, n1, K1, wave, lane);
@@ -611,20 +613,20 @@ __global__ void __launch_bounds__(256) fd_plane_kernel(FdPlaneArgs<typename FdMm
     if (i1 < n1) {
       const T v = R[i2 * RP + il];
       if constexpr (BWD)
-        a.out[((a.lo1 + i1) + a.nf1 * ((a.lo2 + i2) + a.nf2 * (a.lo3 + i3))) * nr + k] = v;
+        a.out[((a.lo1 + i1) + a.nf1 * ((a.lo2 + i2) + a.nf2 * (a.lo3 + i3))) * a.es_out + k * a.ks_out] = v;
       else
-        a.out[((int64_t)i1 + (int64_t)n1 * ((int64_t)i2 + (int64_t)n2 * i3)) * nr + k] = v;
+        a.out[((int64_t)i1 + (int64_t)n1 * ((int64_t)i2 + (int64_t)n2 * i3)) * a.es_out + k * a.ks_out] = v;
     }
   }
   if (BWD && blockIdx.x == 0) {
     // entries of this plane of y outside the box in directions 1 and 2: rows over the
     // waves, columns over the lanes
     const int nf1 = (int)a.nf1, nf2 = (int)a.nf2;
-    T* pl = a.out + (a.nf1 * a.nf2 * (a.lo3 + i3)) * nr + k;
+    T* pl = a.out + (a.nf1 * a.nf2 * (a.lo3 + i3)) * a.es_out + k * a.ks_out;
     for (int i2f = wave; i2f < nf2; i2f += 4) {
       const bool row_in = i2f >= a.lo2 && i2f < a.lo2 + n2;
       for (int i1f = lane; i1f < nf1; i1f += 64)
-        if (!row_in || i1f < a.lo1 || i1f >= a.lo1 + n1) pl[((int64_t)i2f * nf1 + i1f) * nr] = T(0);
+        if (!row_in || i1f < a.lo1 || i1f >= a.lo1 + n1) pl[((int64_t)i2f * nf1 + i1f) * a.es_out] = T(0);
     }
   }
 }
@@ -632,9 +634,10 @@ __global__ void __launch_bounds__(256) fd_plane_kernel(FdPlaneArgs<typename FdMm
 template <typename T>
 struct FdLineArgs {
   int n3, p, xp;
-  int nrhs;
+  int nrhs;        // right-hand sides interleaved in the columns of W (1: one per grid row)
   int64_t ncol;    // n1 n2 nrhs columns of W
   int64_t plane;   // n1 n2: stride of the eigen-index k3 in Lambda
+  int64_t kstride; // right-hand-side-major W: elements per right-hand side (grid row y)
   T* w;            // box-dense (row i3 of ncol values), overwritten
   const T* Ut;     // U3^T row-major n3 x n3: Ut[k3][i3]
   const T* inv;    // Lambda^-1 over the box (direction 1 fastest)
@@ -664,6 +667,7 @@ __global__ void __launch_bounds__(256) fd_line_kernel(FdLineArgs<typename FdMma<
   S* X = Ub + (kTwoCopies ? (int64_t)n3 * p : 0);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int64_t j0 = (int64_t)blockIdx.x * 16;
+  T* w = a.w + (int64_t)blockIdx.y * a.kstride;  // this grid row's right-hand side
   const int lr = lane & 15;
   int mrow[TPW];  // this lane's row of U3 / U3^T in each of its tiles, clamped
 #pragma unroll
@@ -691,7 +695,7 @@ __global__ void __launch_bounds__(256) fd_line_kernel(FdLineArgs<typename FdMma<
     T xv[8];
     uflat.load(n3, n3, utrow, 1, tid);
 #pragma unroll
-    for (int e = 0; e < 8; ++e) xv[e] = a.w[(int64_t)min((tid >> 4) + 16 * e, n3 - 1) * a.ncol + jg];
+    for (int e = 0; e < 8; ++e) xv[e] = w[(int64_t)min((tid >> 4) + 16 * e, n3 - 1) * a.ncol + jg];
     load_inv();
     uflat.store(U, p, n3, n3, tid);
 #pragma unroll
@@ -714,7 +718,7 @@ __global__ void __launch_bounds__(256) fd_line_kernel(FdLineArgs<typename FdMma<
     for (int r0 = 0; r0 < n3; r0 += 16 * 8) {
       T v[8];
 #pragma unroll
-      for (int e = 0; e < 8; ++e) v[e] = a.w[(int64_t)min(r0 + (tid >> 4) + 16 * e, n3 - 1) * a.ncol + jg];
+      for (int e = 0; e < 8; ++e) v[e] = w[(int64_t)min(r0 + (tid >> 4) + 16 * e, n3 - 1) * a.ncol + jg];
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
         const int r = r0 + (tid >> 4) + 16 * e;
@@ -764,7 +768,7 @@ __global__ void __launch_bounds__(256) fd_line_kernel(FdLineArgs<typename FdMma<
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int64_t j = j0 + M::row(lane, r);
-      if (m < n3 && j < a.ncol) a.w[(int64_t)m * a.ncol + j] = acc[u][r];
+      if (m < n3 && j < a.ncol) w[(int64_t)m * a.ncol + j] = acc[u][r];
     }
   }
 }
@@ -823,7 +827,8 @@ static void fd3_launch(const iga_fd_s* f, FdPlaneArgs<typename FdMma<MODE>::T> p
     fd_plane_kernel<MODE, false, TPW><<<dim3(nsb, (unsigned)pa.n3, (unsigned)nrhs), 256, sh.lds_plane, s>>>(pa);
   } else if (which == 1) {
     fd3_allow_lds(fd_line_kernel<MODE, TPW, E>, sh.lds_line, granted[1]);
-    fd_line_kernel<MODE, TPW, E><<<(unsigned)((la.ncol + 15) / 16), 256, sh.lds_line, s>>>(la);
+    const unsigned ky = la.kstride > 0 ? (unsigned)nrhs : 1u;
+    fd_line_kernel<MODE, TPW, E><<<dim3((unsigned)((la.ncol + 15) / 16), ky), 256, sh.lds_line, s>>>(la);
   } else {
     const unsigned extra = (unsigned)(f->n[2] - pa.n3);
     fd3_allow_lds(fd_plane_kernel<MODE, true, TPW>, sh.lds_plane, granted[2]);
@@ -852,9 +857,11 @@ static void fd3_launch_len(const iga_fd_s* f, FdPlaneArgs<typename FdMma<MODE>::
     fd3_launch<MODE, 3, 0>(f, pa, la, sh, nrhs, which, s);
 }
 
+// rhs_major: x, y and the box-dense intermediate hold right-hand side k at k * N
+// (N, the length of one); otherwise (N, nrhs) interleaved.
 template <int MODE>
 static void fd3_apply(const iga_fd_s* f, const typename FdMma<MODE>::T* x, typename FdMma<MODE>::T* y, int nrhs,
-                      typename FdMma<MODE>::T* w, hipStream_t s) {
+                      typename FdMma<MODE>::T* w, hipStream_t s, bool rhs_major = false) {
   using T = typename FdMma<MODE>::T;
   IGA_CHECK_ARG(nrhs <= 65535, "too many right-hand sides for one FD launch");
   const Fd3Shape sh = fd3_shape(f->len, MODE);
@@ -871,19 +878,25 @@ static void fd3_apply(const iga_fd_s* f, const typename FdMma<MODE>::T* x, typen
   pa.lo2 = f->lo[1];
   pa.lo3 = f->lo[2];
   pa.p = sh.p_plane;
+  const int64_t nfull = f->n[0] * f->n[1] * f->n[2], nbox = (int64_t)n1 * n2 * n3;
   FdLineArgs<T> la{};
   la.n3 = n3;
   la.p = sh.p_line;
   la.xp = sh.xp_line;
-  la.nrhs = nrhs;
   la.plane = (int64_t)n1 * n2;
-  la.ncol = la.plane * nrhs;
+  la.ncol = rhs_major ? la.plane : la.plane * nrhs;
+  la.nrhs = rhs_major ? 1 : nrhs;
+  la.kstride = rhs_major ? nbox : 0;
   la.w = w;
   la.Ut = (const T*)f->d_Ut[2];
   la.inv = (const T*)f->d_inv;
   // forward: x -> W (directions 1, 2)
   pa.in = x;
   pa.out = w;
+  pa.es_in = rhs_major ? 1 : nrhs;
+  pa.ks_in = rhs_major ? nfull : 1;
+  pa.es_out = rhs_major ? 1 : nrhs;
+  pa.ks_out = rhs_major ? nbox : 1;
   pa.A1 = (const T*)f->d_Ut[0];
   pa.A2 = (const T*)f->d_Ut[1];
   // the plane kernels' tiles per wave also set their column passes: cover both lengths
@@ -894,6 +907,8 @@ static void fd3_apply(const iga_fd_s* f, const typename FdMma<MODE>::T* x, typen
   // backward: W -> y (directions 1, 2), zeros outside the box
   pa.in = w;
   pa.out = y;
+  pa.ks_in = rhs_major ? nbox : 1;
+  pa.ks_out = rhs_major ? nfull : 1;
   pa.A1 = (const T*)f->d_U[0];
   pa.A2 = (const T*)f->d_U[1];
   fd3_launch_len<MODE>(f, pa, la, sh, nrhs, 2, plane_len, s);
@@ -1041,6 +1056,30 @@ int iga_fd_apply(iga_fd_t f, const void* x, void* y, int nrhs, void* work, void*
       float* w0 = (float*)work;
       fd_apply_t<float>(f, (const float*)x, (float*)y, nrhs, w0, w0 + half, as_stream(stream));
     }
+  });
+}
+
+int iga_fd_apply_rhs_major(iga_fd_t f, const void* x, void* y, int nrhs, void* work, void* stream) {
+  return guard([&] {
+    IGA_CHECK_ARG(f && x && y && work && nrhs >= 1, "NULL argument");
+    IGA_CHECK_ARG(f->d_inv, "eigenvalues are not set");
+    const int mode = f->dtype == IGA_F64 ? FD_F64 : f->dtype == IGA_F32 ? FD_F32 : FD_BF16;
+    if (fd3_applicable(f, mode)) {
+      if (mode == FD_F64)
+        fd3_apply<FD_F64>(f, (const double*)x, (double*)y, nrhs, (double*)work, as_stream(stream), true);
+      else if (mode == FD_F32)
+        fd3_apply<FD_F32>(f, (const float*)x, (float*)y, nrhs, (float*)work, as_stream(stream), true);
+      else
+        fd3_apply<FD_BF16>(f, (const float*)x, (float*)y, nrhs, (float*)work, as_stream(stream), true);
+      return;
+    }
+    // other shapes: one apply per right-hand side
+    int64_t nfull = 1;
+    for (int d = 0; d < f->ndim; ++d) nfull *= f->n[d];
+    const size_t es = f->dtype == IGA_F64 ? 8 : 4;
+    for (int k = 0; k < nrhs; ++k)
+      IGA_CHECK(iga_fd_apply(f, (const char*)x + (size_t)k * nfull * es, (char*)y + (size_t)k * nfull * es, 1, work,
+                             stream));
   });
 }
 

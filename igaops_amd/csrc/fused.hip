@@ -1337,11 +1337,12 @@ int iga_fused_apply(iga_fused_t f, const void* packed, const void* x, void* y, i
   });
 }
 
-int iga_fused_apply_group(iga_fused_t f, int nmember, const void* const* packed, const void* x, void* const* y,
-                          const void* nurbs_w, void* work, int ndim, const int64_t* box_lo, const int64_t* box_len,
-                          void* stream) {
+int iga_fused_apply_group(iga_fused_t f, int nmember, const void* const* packed, const void* x, int64_t x_dstride,
+                          int64_t x_mstride, void* const* y, const void* nurbs_w, void* work, int ndim,
+                          const int64_t* box_lo, const int64_t* box_len, void* stream) {
   return guard([&] {
     IGA_CHECK_ARG(f && packed && x && y && work, "NULL argument");
+    IGA_CHECK_ARG(x_dstride >= 1 && x_mstride >= 0, "bad input strides");
     IGA_CHECK_ARG(nmember >= 1 && nmember <= kGroupMax, "a grouped apply takes 1 to 16 members");
     for (int k = 0; k < nmember; ++k) IGA_CHECK_ARG(packed[k] && y[k], "NULL coefficient set or output");
     IGA_CHECK_ARG(f->form != FORM_KM7X, "the two-input form is not grouped");
@@ -1351,12 +1352,14 @@ int iga_fused_apply_group(iga_fused_t f, int nmember, const void* const* packed,
       IGA_CHECK_ARG(box_lo[d] >= 0 && box_len[d] >= 0 && box_lo[d] + box_len[d] <= f->nc[d], "box outside the grid");
     const int64_t* lo = ndim == 3 ? box_lo : nullptr;
     const int64_t* len = ndim == 3 ? box_len : nullptr;
+    // the output strides are unused: the gather writes member k to y[k][dof]
+    const Layout lay{x_dstride, x_mstride, 1, 0, 0};
     if (f->dtype == IGA_F64)
-      run_apply<double>(f, packed[0], x, y[0], nmember, nurbs_w, 1.0, 0, work, as_stream(stream), 3,
-                        Layout{0, 0, 0, 0, 0}, nullptr, lo, len, packed, y);
+      run_apply<double>(f, packed[0], x, y[0], nmember, nurbs_w, 1.0, 0, work, as_stream(stream), 3, lay, nullptr, lo,
+                        len, packed, y);
     else
-      run_apply<float>(f, packed[0], x, y[0], nmember, nurbs_w, 1.0, 0, work, as_stream(stream), 3,
-                       Layout{0, 0, 0, 0, 0}, nullptr, lo, len, packed, y);
+      run_apply<float>(f, packed[0], x, y[0], nmember, nurbs_w, 1.0, 0, work, as_stream(stream), 3, lay, nullptr, lo,
+                       len, packed, y);
   });
 }
 

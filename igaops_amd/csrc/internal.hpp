@@ -8,12 +8,28 @@
 
 namespace iga {
 
+// The increment criterion's record of a GMRES step: gram holds z_j . z_0..z_j (device),
+// rec the host-visible record of the solve (rec == nullptr: none), tag its tag.
+struct GmIncrement {
+  const double* gram = nullptr;
+  double* rec = nullptr;
+  int64_t tag = 0;
+};
+
+// doubles of the increment record of a Krylov dimension m (krylov.hip)
+int64_t gmres_increment_doubles(int m);
+
 // iga_gmres_start / iga_gmres_step with the hist entries also published to
-// host-visible memory `rec` under `tag` (krylov.hip; rec == nullptr: not published).
+// host-visible memory `rec` under `tag` (krylov.hip; rec == nullptr: not published),
+// and with `inc` the increment record entry of the step.
 void gmres_start_impl(int dtype, int64_t n, int m, const void* r, void* V0, double* ls, void* work, double* rec,
                       int64_t tag, void* stream);
 void gmres_step_impl(int dtype, int64_t n, int k, int m, void* V, int64_t ldv, void* w, double* ls, void* work,
-                     double* rec, int64_t tag, void* stream);
+                     double* rec, int64_t tag, void* stream, const GmIncrement* inc = nullptr);
+
+// iga_dot_many (krylov.hip): out[i] = V[i] . x for i < k
+void dot_many_impl(int dtype, int64_t n, int k, const void* V, int64_t ldv, const void* x, double* out, void* work,
+                   void* stream);
 
 // iga_fused_apply whose output is zeroed outside the box [lo, lo + len) of the DOF grid
 // (the Dirichlet mask, applied by the gather that writes y; fused.hip).

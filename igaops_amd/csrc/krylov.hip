@@ -413,6 +413,12 @@ __global__ void gm_zero_kernel(int64_t n, double* __restrict__ p) {
 // rec[2 + 2d], rec[3 + 2d], then the release store of tag + d into rec[0] (as int64),
 // so that a host that reads rec[0] and finds tag + d sees the values of every entry up
 // to d.  The tag tells one solve's entries from another's in a reused buffer.
+// Increment-criterion record of a solve (host-visible, written by the small dense
+// kernel): entry j, at 1 + j * gm_inc_stride(m), holds R[0..j-1][j-1] (m slots), g[j - 1]
+// (slot m) and the Gram row z_j . z_0..z_j of the preconditioned vectors (slots m + 1
+// on); the int64 at 0 carries tag + j once entry j is complete.
+__host__ __device__ constexpr int64_t gm_inc_stride(int m) { return 2 * (int64_t)m + 3; }
+
 __device__ inline void gm_publish(double* rec, int64_t tag, int d, double g, double rho) {
   if (!rec) return;
   rec[2 + 2 * d] = g;
@@ -439,9 +445,10 @@ __global__ void gm_start_kernel(GmState st, double* ls, double* rec, int64_t tag
 constexpr int kGmPer = (kGmMaxDim + kGmThreads - 1) / kGmThreads;  // indices l per thread
 
 __global__ void __launch_bounds__(kGmThreads) __attribute__((amdgpu_waves_per_eu(1, 1)))
-gm_dcgs2_kernel(GmState st, int j, double* ls, double* rec, int64_t tag) {
+gm_dcgs2_kernel(GmState st, int j, double* ls, double* rec, int64_t tag, GmIncrement inc) {
   extern __shared__ double sh[];  // s[j] | col[j+1] | cr[j] | sr[j]   (y[j+1] reuses col)
   __shared__ double red[2][kGmThreads / 64];
+  __shared__ double pub[3];  // residual estimate, rho, final g[j - 1]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int64_t ldh = st.ldh();
   double* H = ls + st.H();
@@ -554,13 +561,33 @@ gm_dcgs2_kernel(GmState st, int j, double* ls, double* rec, int64_t tag) {
       ls[st.g() + j - 1] = cc * gj1;
       ls[st.hist() + 2 * j] = -sv * gj1;
       ls[st.hist() + 2 * j + 1] = rho;
-      gm_publish(rec, tag, j, -sv * gj1, rho);
+      pub[0] = -sv * gj1;
+      pub[1] = rho;
+      pub[2] = cc * gj1;
     }
     __syncthreads();
     double* Rc = ls + st.R() + (int64_t)(j - 1) * st.m;
     for (int l = tid; l < j; l += kGmThreads) Rc[l] = col[l];
+    if (inc.rec) {
+      double* e = inc.rec + 1 + (int64_t)j * gm_inc_stride(st.m);
+      for (int l = tid; l < j; l += kGmThreads) e[l] = col[l];
+      if (tid == 0) e[st.m] = pub[2];
+    }
     __syncthreads();
   }
+  if (inc.rec) {
+    // increment record entry j: R column j - 1 and g[j - 1] (above), the Gram row of the
+    // preconditioned vector z_j against z_0..z_j; every thread's writes reach the
+    // system before thread 0 releases the entry's tag
+    double* e = inc.rec + 1 + (int64_t)j * gm_inc_stride(st.m);
+    for (int l = tid; l <= j; l += kGmThreads) e[st.m + 1 + l] = inc.gram[l];
+    __threadfence_system();
+    __syncthreads();
+    if (tid == 0)
+      __hip_atomic_store(reinterpret_cast<int64_t*>(inc.rec), inc.tag + j, __ATOMIC_RELEASE,
+                         __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+  if (j > 0 && tid == 0) gm_publish(rec, tag, j, pub[0], pub[1]);
   // y = H[0:j+1, 0:j] s (Hessenberg: row i starts at column i - 1), into col: from
   // gm_hs_kernel for rows below j at larger j (row j is rho s_{j-1}); otherwise the
   // rows of the row-major H dotted with s, 16 rows per wave at a time, lanes along
@@ -814,7 +841,7 @@ void gmres_start_impl(int dtype, int64_t n, int m, const void* r, void* V0, doub
 }
 
 void gmres_step_impl(int dtype, int64_t n, int k, int m, void* V, int64_t ldv, void* w, double* ls, void* work,
-                     double* rec, int64_t tag, void* stream) {
+                     double* rec, int64_t tag, void* stream, const GmIncrement* inc) {
   {
     IGA_CHECK_ARG(V && w && ls && work && n >= 1 && ldv >= n, "bad argument");
     IGA_CHECK_ARG(k >= 1 && k <= m + 1, "step outside the Krylov dimension");
@@ -831,7 +858,8 @@ void gmres_step_impl(int dtype, int64_t n, int k, int m, void* V, int64_t ldv, v
       gm_hs_kernel<<<(unsigned)((j + 3) / 4), 256, 0, s>>>(st, j, ls);
       IGA_LAUNCH_CHECK();
     }
-    gm_dcgs2_kernel<<<1, kGmThreads, sizeof(double) * (4 * (size_t)j + 1), s>>>(st, j, ls, rec, tag);
+    gm_dcgs2_kernel<<<1, kGmThreads, sizeof(double) * (4 * (size_t)j + 1), s>>>(st, j, ls, rec, tag,
+                                                                                 inc ? *inc : GmIncrement{});
     IGA_LAUNCH_CHECK();
     // second pass
     const dim3 grid(grid_for(n, 256, 8192)), block(256);
@@ -844,6 +872,16 @@ void gmres_step_impl(int dtype, int64_t n, int k, int m, void* V, int64_t ldv, v
                                                        ls + st.cw(), ls + st.scal());
     IGA_LAUNCH_CHECK();
   }
+}
+
+int64_t gmres_increment_doubles(int m) { return 1 + ((int64_t)m + 1) * gm_inc_stride(m); }
+
+void dot_many_impl(int dtype, int64_t n, int k, const void* V, int64_t ldv, const void* x, double* out, void* work,
+                   void* stream) {
+  IGA_CHECK_ARG(V && x && out && work && n >= 0 && k >= 0 && ldv >= n, "bad argument");
+  IGA_CHECK_ARG(k <= kDotMaxVectors, "too many vectors");
+  if (k == 0) return;
+  dot_many_launch(dtype, n, k, V, ldv, x, nullptr, 1, out, k, work, as_stream(stream));
 }
 
 }  // namespace iga

@@ -208,16 +208,19 @@ class FDPlan:
     def workspace(self, nrhs: int = 1):
         return dv.workspace(self.workspace_bytes(nrhs))
 
-    def apply(self, x, out, nrhs: int = 1, stream=None, work=None):
+    def apply(self, x, out, nrhs: int = 1, stream=None, work=None, rhs_major: bool = False):
         """``out = P x``; ``work`` (from ``workspace``) lets concurrent callers of one
-        plan keep their scratch apart (the plan's own buffer is used otherwise)."""
+        plan keep their scratch apart (the plan's own buffer is used otherwise).
+        Several right-hand sides are ``(N, nrhs)`` C order, or ``(nrhs, N)`` with
+        ``rhs_major`` (``iga_fd_apply_rhs_major``)."""
         w = work
         if w is None:
             w = self._work.get(nrhs)
         if w is None:
             w = self.workspace(nrhs)
             self._work[nrhs] = w
-        _lib.call("iga_fd_apply", self._h, dv.ptr(x), dv.ptr(out), int(nrhs), dv.ptr(w), dv.stream_ptr(stream))
+        _lib.call("iga_fd_apply_rhs_major" if rhs_major else "iga_fd_apply", self._h, dv.ptr(x), dv.ptr(out),
+                  int(nrhs), dv.ptr(w), dv.stream_ptr(stream))
         return out
 
 

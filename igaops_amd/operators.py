@@ -714,12 +714,13 @@ class FirstOrderOperator:
         return b * (2 * self.ndof * nrhs + (4 if self.is_nurbs else 3) * self.nq)
 
 
-def apply_group(ops, X, outs, box=None, work=None, stream=None):
+def apply_group(ops, X, outs, box=None, work=None, stream=None, member_major: bool = False):
     """The fused operators of several solves that share one plan (same rules, form
     and dtype: requests of one shape with their own coefficients, or one operator
     repeated for several right-hand sides) in one launch (``iga_fused_apply_group``):
-    member k applies ``ops[k]`` to column k of ``X`` (``(N, g)`` C order) into the
-    vector ``outs[k]``, bit for bit its own ``apply``.  ``box`` = ``(lo, len)`` (int64
+    member k applies ``ops[k]`` to column k of ``X`` (``(N, g)`` C order; row k of a
+    ``(g, N)`` array with ``member_major``) into the vector ``outs[k]``, bit for bit its
+    own ``apply``.  ``box`` = ``(lo, len)`` (int64
     arrays) zeroes the outputs outside the Dirichlet box; ``work`` holds
     ``plan.workspace_bytes(g)`` bytes (the plan's own buffer if None)."""
     g = len(ops)
@@ -736,8 +737,10 @@ def apply_group(ops, X, outs, box=None, work=None, stream=None):
         lo = np.ascontiguousarray(box[0], dtype=np.int64)
         ln = np.ascontiguousarray(box[1], dtype=np.int64)
         nd, lop, lenp = 3, lo.ctypes.data_as(_lib._i64p), ln.ctypes.data_as(_lib._i64p)
-    _lib.call("iga_fused_apply_group", plan._h, g, ptrs, dv.ptr(X), yptrs, dv.ptr(ops[0].w), dv.ptr(work), nd,
-              lop, lenp, dv.stream_ptr(stream))
+    n = ops[0].ndof
+    xd, xm = (1, n) if member_major else (g, 1)
+    _lib.call("iga_fused_apply_group", plan._h, g, ptrs, dv.ptr(X), xd, xm, yptrs, dv.ptr(ops[0].w), dv.ptr(work),
+              nd, lop, lenp, dv.stream_ptr(stream))
     return outs
 
 

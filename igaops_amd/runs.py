@@ -117,7 +117,7 @@ class GmresRun:
 
     def __init__(self, A: Callable, P: Optional[Callable], b, tol: float, maxiters: int, mask=None, x0=None,
                  stream=None, lookahead: int = 2, dtype: Optional[str] = None, host_buf=None,
-                 native: Optional[NativeStep] = None):
+                 native: Optional[NativeStep] = None, increment: bool = False):
         t = dv.torch()
         self.A, self.P, self.mask = A, P, mask
         self.tol, self.m = float(tol), int(maxiters)
@@ -173,6 +173,9 @@ class GmresRun:
         self.result = None
         self._ctx = None
         self._rec = None
+        self._inc = None
+        if increment and native is None:
+            raise ValueError("the increment criterion of the device steps needs the one-call step (native)")
         if native is not None:
             if native.op.dtype != self.dtype:
                 raise ValueError("the native operator does not take the run's vector type")
@@ -195,9 +198,66 @@ class GmresRun:
                       None if fd is None else fd._h, dv.ptr(native.fd_work), nd, *bp, dv.ptr(self.z),
                       dv.ptr(self.w), dv.ptr(self.ls), dv.ptr(self.work), self._rec.ptr, self._tag << 32)
             self._ctx = ctx
+            if increment:
+                self._setup_increment()
 
     def __del__(self):
         self._drop_native(give_back=False)
+
+    # -- increment criterion (linear_solver.py:107-117,503-515) --
+    def _setup_increment(self):
+        """Keep the preconditioned vectors Z (rows like V) and read, per step, the
+        increment record the device writes (``iga_arnoldi_set_increment``)."""
+        t = dv.torch()
+        m = self.m
+        with self._on_stream():
+            self.Z = t.empty(self.V.shape, dtype=self.V.dtype, device=self.V.device)
+            self._gram = t.empty(m + 1, dtype=t.float64, device=self.V.device)
+        nd = int(_lib.load().iga_arnoldi_increment_doubles(m))
+        self._inc = _take_record(nd)
+        self._inc_stride = 2 * m + 3
+        self._incR = np.zeros((m + 1, m + 1))
+        self._incg = np.zeros(m + 1)
+        self._incG = np.zeros((m + 2, m + 2))
+        self._inc_read = -1
+        self._inc_y = np.zeros(0)
+        self.increments = []
+        self._bind_increment()
+
+    def _bind_increment(self):
+        _lib.call("iga_arnoldi_set_increment", self._ctx, dv.ptr(self.Z), self.ldv, dv.ptr(self._gram),
+                  self._inc.ptr)
+
+    def _increment(self, j: int) -> float:
+        """||x_j - x_{j-1}|| / (||x_j|| + 1e-14) of Krylov dimension j >= 1, with
+        x_j = Z_j y_j, y_j = R_j^-1 g_j, the norms from the Gram matrix of Z."""
+        from scipy.linalg import solve_triangular
+
+        vals = self._inc.values
+        stride, m = self._inc_stride, self.m
+        seq = int(self._inc.seq[0])
+        if (seq >> 32) != self._tag or (seq & 0xFFFFFFFF) < j:
+            raise RuntimeError(f"increment record {j} was not published by the device")
+        while self._inc_read < j:
+            e = self._inc_read + 1
+            base = 1 + e * stride
+            if e >= 1:
+                self._incR[:e, e - 1] = vals[base:base + e]
+                self._incg[e - 1] = vals[base + m]
+            row = vals[base + m + 1:base + m + 2 + e]
+            self._incG[e, :e + 1] = row
+            self._incG[:e + 1, e] = row
+            self._inc_read = e
+        R = self._incR[:j, :j]
+        y = solve_triangular(R, self._incg[:j]) if np.all(np.diag(R) != 0) else np.linalg.lstsq(R, self._incg[:j],
+                                                                                                  rcond=None)[0]
+        d = y.copy()
+        d[:len(self._inc_y)] -= self._inc_y
+        G = self._incG[:j, :j]
+        num = max(float(d @ G @ d), 0.0) ** 0.5
+        den = max(float(y @ G @ y), 0.0) ** 0.5
+        self._inc_y = y
+        return num / (den + Constants.SAFEGUARD)
 
     def _drop_native(self, give_back: bool = True):
         """Destroy the step context; with ``give_back`` the host record returns to the
@@ -211,6 +271,10 @@ class GmresRun:
         if rec is not None and give_back:
             _give_record(rec)
         self._rec = None
+        inc = getattr(self, "_inc", None)
+        if inc is not None and give_back:
+            _give_record(inc)
+        self._inc = None
 
     @staticmethod
     def host_doubles(m: int) -> int:
@@ -299,6 +363,11 @@ class GmresRun:
         V = t.empty((rows, self.ldv), dtype=self.V.dtype, device=self.V.device)
         V[:self.V.shape[0]].copy_(self.V)
         self.V = V
+        if self._inc is not None:
+            Z = t.empty((rows, self.ldv), dtype=self.Z.dtype, device=self.Z.device)
+            Z[:self.Z.shape[0]].copy_(self.Z)
+            self.Z = Z
+            self._bind_increment()
 
     def prepare_step(self):
         """Capacity for step k = launched + 1; returns (k, V[k-1], z) so that a caller
@@ -377,6 +446,11 @@ class GmresRun:
             res = abs(g)
             self.res_hist.append(res / self.ref)
             reason = _relative_or_absolute(self.tol, res / self.ref, res)
+            if self._inc is not None and nw != 0.0:
+                incr = self._increment(j)
+                self.increments.append(incr)
+                if not reason and incr < self.tol:
+                    reason = "increment"
             if reason or j >= self.m or nw == 0.0:
                 self.kfinal = j
                 self.reason = reason or ("breakdown" if nw == 0.0 else "maxiters")
@@ -438,7 +512,7 @@ class GmresRun:
                 if self.mask:
                     self.mask(z)
                 self.x.add_(z)
-        success = self.reason in ("relative", "absolute")
+        success = self.reason in ("relative", "absolute", "increment")
         if self.reason == "breakdown":
             # a vanishing basis vector means the Krylov space is invariant, which in exact
             # arithmetic makes the least-squares solution exact; the true residual decides

@@ -253,8 +253,9 @@ int iga_fused_apply_noalloc(iga_fused_t plan, const void* packed, const void* x,
 /* The operators of nmember solves that share one plan (same rules and form, hence the
  * same shape; coefficients of their own, or the same ones for several right-hand
  * sides of one operator) in one launch: member k's coefficients at packed[k] (a host
- * array of nmember device pointers, 1 <= nmember <= 16), its input in column k of x
- * ((N, nmember) C order, as iga_fused_apply with nrhs = nmember), its output at y[k]
+ * array of nmember device pointers, 1 <= nmember <= 16), entry a of its input at
+ * x[a * x_dstride + k * x_mstride] ((N, nmember) C order: nmember, 1; member-major:
+ * 1, N), its output at y[k]
  * (a host array of nmember device pointers, N values each), nurbs_w shared (the
  * shape's).  With ndim = 3, y is zero outside the DOF box
  * [box_lo, box_lo + box_len) (the Dirichlet mask of linear_solver.py:119-124); ndim = 0,
@@ -263,9 +264,9 @@ int iga_fused_apply_noalloc(iga_fused_t plan, const void* packed, const void* x,
  * operator (operators/utils/matrix_free.py:36,117-119); this entry batches the
  * same-shape operator work of different requests of a solve server.  work:
  * iga_fused_workspace_bytes(plan, nmember) bytes. */
-int iga_fused_apply_group(iga_fused_t plan, int nmember, const void* const* packed, const void* x, void* const* y,
-                          const void* nurbs_w, void* work, int ndim, const int64_t* box_lo,
-                          const int64_t* box_len, void* stream);
+int iga_fused_apply_group(iga_fused_t plan, int nmember, const void* const* packed, const void* x,
+                          int64_t x_dstride, int64_t x_mstride, void* const* y, const void* nurbs_w, void* work,
+                          int ndim, const int64_t* box_lo, const int64_t* box_len, void* stream);
 
 /* The two launches of iga_fused_apply, separately (for timing): stage 1 runs the
  * coefficient sweep into the partial blocks of `work`, stage 2 adds them into y. */
@@ -348,6 +349,10 @@ int iga_fd_create(iga_fd_t* out, int ndim, const int64_t* n, const int64_t* lo, 
 int iga_fd_set_eigenvalues(iga_fd_t plan, const double* eigenvalues);
 int64_t iga_fd_workspace_bytes(iga_fd_t plan, int nrhs);
 int iga_fd_apply(iga_fd_t plan, const void* x, void* y, int nrhs, void* work, void* stream);
+/* iga_fd_apply on right-hand-side-major arrays: right-hand side k of x and y at k * N
+ * (N values each, N the full grid) instead of interleaved; every kernel then reads
+ * and writes each right-hand side contiguously.  Same results per right-hand side. */
+int iga_fd_apply_rhs_major(iga_fd_t plan, const void* x, void* y, int nrhs, void* work, void* stream);
 int iga_fd_destroy(iga_fd_t plan);
 
 /* ---- one right-preconditioned Arnoldi step per call ---------------------------
@@ -387,11 +392,28 @@ int iga_arnoldi_start(iga_arnoldi_t ctx, const void* r, void* V, void* stream);
 int iga_arnoldi_step(iga_arnoldi_t ctx, int k, void* V, int64_t ldv, int preconditioned, void* stream);
 int iga_arnoldi_destroy(iga_arnoldi_t ctx);
 
+/* The reference's third stopping criterion, the relative increment
+ * ||x_d - x_{d-1}|| / (||x_d|| + 1e-14) (solvers/linear_solver.py:107-117,503-515,
+ * solvers/utils/convergence_manager.py:281-327), for one-call steps: each step keeps
+ * z_{k-1} = P v_{k-1} as row k - 1 of Z (row stride ldz, at least m + 1 rows; Z may be
+ * set again, with its rows copied, when the caller grows it), forms the Gram row
+ * z_{k-1} . z_0..z_{k-1} into gram (device, m + 1 doubles) and the small dense kernel
+ * writes the increment record entry j = k - 1 to inc_rec (host-visible, from
+ * iga_host_alloc, iga_arnoldi_increment_doubles(m) doubles): at 1 + j (2m + 3), R[0..j-1][j-1]
+ * of the rotated Hessenberg (m slots), g[j - 1] (slot m) and the Gram row of z_j (slots
+ * m + 1 on); the int64 at inc_rec[0] then holds tag + j (release ordering), before the
+ * residual estimate of entry j is published.  The host forms y_d = R_d^-1 g_d and the
+ * increment from the Gram matrix, so x is never formed per step.  Z == NULL switches
+ * the criterion off.  Only the step's own FD and operator (preconditioned = 0). */
+int64_t iga_arnoldi_increment_doubles(int m);
+int iga_arnoldi_set_increment(iga_arnoldi_t ctx, void* Z, int64_t ldz, double* gram, double* inc_rec);
+
 /* One Arnoldi step of g solves of one shape (1 <= g <= 16; contexts sharing the FD
  * plan, the fused-operator plan, the mask box, the NURBS weights, the vector type and
  * length; each with its own coefficients, basis and state) as one call: step ks[i] of
- * solve i on basis Vs[i] (row stride ldvs[i]).  The vectors V_i[k_i - 1] become the
- * columns of one multi-right-hand-side FD apply and the results go through one
+ * solve i on basis Vs[i] (row stride ldvs[i]).  The vectors V_i[k_i - 1] go, one
+ * after the other, through one multi-right-hand-side FD apply (iga_fd_apply_rhs_major)
+ * and the results through one
  * iga_fused_apply_group launch into each context's w (masked); then each solve's
  * orthogonalisation (iga_gmres_step, residual estimate published as by
  * iga_arnoldi_step) is queued on streams[i].  The batch runs on batch_stream, after

@@ -242,6 +242,11 @@ def test_grouped_apply_equals_each_members_apply(dtype, geom):
     alone = [op.apply(X[:, k].contiguous()) for k, op in enumerate(ops)]
     for k in range(3):
         assert torch.equal(Y[k], alone[k])
+    # member-major input
+    W = [torch.empty(n, dtype=tdt, device="cuda") for _ in range(3)]
+    apply_group(ops, X.t().contiguous(), W, member_major=True)
+    for k in range(3):
+        assert torch.equal(W[k], alone[k])
     # one operator for several right-hand sides: the multi-right-hand-side apply
     Z = [torch.empty(n, dtype=tdt, device="cuda") for _ in range(3)]
     apply_group([ops[1]] * 3, X, Z)

@@ -72,7 +72,13 @@ def test_fd_apply_every_staging_size(dt, nels):
     if dt != "f64":
         sp.float32_plan = dt
         Xd = Xd.float()
-    y = sp.apply_device(Xd.reshape(-1).contiguous(), nrhs=2).reshape(n, 2).double().cpu().numpy()
+    yd = sp.apply_device(Xd.reshape(-1).contiguous(), nrhs=2)
+    # the right-hand-side-major entry gives the same bits per right-hand side
+    plan = sp.plan(dt)
+    ym = torch.empty(2, n, dtype=Xd.dtype, device="cuda")
+    plan.apply(Xd.t().contiguous(), ym, 2, rhs_major=True)
+    assert torch.equal(ym.t(), yd.reshape(n, 2))
+    y = yd.reshape(n, 2).double().cpu().numpy()
     sp.float32_plan = "f32"
     err = rel(y, ref)
     print(f"FD {dt} box {nels}: relative error {err:.2e}", flush=True)
@@ -337,6 +343,41 @@ def test_batched_steps_of_several_right_hand_sides_match_single_runs(mode):
         r = b - ops["f64"].apply(t.solution)
         mask(r)
         assert float(torch.linalg.vector_norm(r) / torch.linalg.vector_norm(b)) <= 1e-10
+
+
+@pytest.mark.parametrize("tol", [1e-6, 1e-10])
+def test_device_steps_with_the_increment_criterion_match_oracle(tol):
+    """One-call device steps with the reference's third criterion, the relative
+    increment of the iterate (linear_solver.py:107-117, 503-515), which is what stops
+    the reference's GMRES first on this FD-preconditioned annulus problem: the same
+    stop reason and iteration count as the oracle's GMRES, the increment history, and
+    the solution within 1e-8."""
+    import torch
+
+    from igaops_amd.runs import GmresRun, NativeStep
+
+    patch, C, m, fd, mask, ops = _annulus_fd_problem(3, 6)
+    qs, w, n = patch.quadrule_list, patch.nurbs_weights, patch.nbctrlpts_total
+    plan = fd.space_preconditioner.plan("f64")
+    work = plan.workspace(1)
+    A = lambda x, o: ops["f64"].apply(x, out=o)
+    P = lambda x, o: plan.apply(x, o, 1, None, work=work)
+    b = np.random.default_rng(3).standard_normal(n)
+    o = ofd.SpaceFD([StandardGauss(q.degree, q.knotvector, "legendre") for q in qs], np.ones((1, 3, 2), bool))
+    o.scalar_coefs = (1.0, 1.0)
+    dod = np.setdiff1d(np.arange(n), o.free_nodes()).tolist()
+    ref = osv.gmres(lambda v: oo.stiffness_plus_mass(qs, C, m, v, nurbs_weights=w), b, o.apply, tol=tol,
+                    maxiters=200, cleandod=dod)
+    bd = torch.as_tensor(b, device="cuda")
+    mask(bd)
+    run = GmresRun(A, P, bd, tol, 200, mask=mask, native=NativeStep(ops["f64"], plan, work, mask.box),
+                   increment=True)
+    out = run.run()
+    print("oracle", ref.iterations, ref.reason, "device", out.iterations, out.reason, flush=True)
+    assert ref.reason == "increment" and out.reason == "increment"
+    assert out.success and out.iterations == ref.iterations
+    assert rel(out.solution.cpu().numpy(), ref.solution) <= 1e-8
+    assert min(run.increments) < tol <= min(run.increments[:-1])
 
 
 def test_refinement_run_float32_vectors_reaches_1e10():
