@@ -48,6 +48,7 @@ SIGNATURES = {
         [C.POINTER(_vp), _i32p, _i32p, _i32, C.POINTER(_i32p), C.POINTER(_f64p), _i32, _i32, _i32],
     ),
     "iga_fused_destroy": (_i32, [_vp]),
+    "iga_fused_set_rhs_block": (_i32, [_vp, _i32]),
     "iga_fused_info": (_i32, [_vp, _i64p, _i64p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "iga_fused_workspace_bytes": (_i64, [_vp, _i32]),
     "iga_fused_pack": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
@@ -84,6 +85,9 @@ SIGNATURES = {
     "iga_stfd_workspace_bytes": (_i64, [_vp, _i32]),
     "iga_stfd_apply": (_i32, [_vp, _vp, _vp, _i32, _i32, _vp, _vp]),
     "iga_stfd_destroy": (_i32, [_vp]),
+    "iga_stfd_create_qz": (_i32, [C.POINTER(_vp), _i64p, _i64p, _i64p, C.POINTER(_f64p), _f64p, _f64p, _f64p,
+                                  _f64p, _i32]),
+    "iga_stfd_set_qz": (_i32, [_vp, _f64p, _f64p, _f64p, _f64p, _f64p, _f64]),
     "iga_arnoldi_create": (_i32, [C.POINTER(_vp), _i32, _i64, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _i64p, _i64p,
                                   _i64p, _vp, _vp, _vp, _vp, _vp, _i64]),
     "iga_arnoldi_start": (_i32, [_vp, _vp, _vp, _vp]),

@@ -1178,11 +1178,49 @@ struct iga_stfd_s {
   void* d_lam = nullptr;              // spatial eigenvalues over the space box
   void* d_D[2] = {nullptr, nullptr};  // adv * D  (re, im), length L
   void* d_b[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [apply/transpose][re/im], length L-1
+  // legacy QZ form (time_legacy.py): adv S and T of the QZ pencil, row-major L x L (re, im)
+  int qz = 0;
+  void* d_S[2] = {nullptr, nullptr};
+  void* d_T[2] = {nullptr, nullptr};
   bool blocks_set = false;
   std::vector<void*> owned;
 };
 
 namespace iga {
+
+// Legacy QZ middle solve (time_legacy.py:88-101, 146-156): per column j (spatial
+// eigen-index j / nrhs) the triangular system (adv S + lambda_j T) y = r, upper and by
+// back substitution, or transposed (lower, forward substitution), complex, in place on
+// z = [2][nt][ncol].  The matrix entries are the same for every thread (uniform loads).
+template <typename T>
+__global__ void __launch_bounds__(256) qz_solve_kernel(int nt, int64_t ncol, int nrhs, int transpose,
+                                                        const T* __restrict__ lam, const T* __restrict__ sre,
+                                                        const T* __restrict__ sim, const T* __restrict__ tre,
+                                                        const T* __restrict__ tim, T* __restrict__ z) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ncol) return;
+  const T lj = lam[j / nrhs];
+  T* zr = z + j;
+  T* zi = z + (int64_t)nt * ncol + j;
+  for (int s = 0; s < nt; ++s) {
+    const int i = transpose ? s : nt - 1 - s;
+    T ar = zr[(int64_t)i * ncol], ai = zi[(int64_t)i * ncol];
+    // known unknowns: k > i (upper) or k < i (transposed, lower)
+    const int k0 = transpose ? 0 : i + 1, k1 = transpose ? i : nt;
+    for (int k = k0; k < k1; ++k) {
+      const int64_t e = transpose ? (int64_t)k * nt + i : (int64_t)i * nt + k;  // m_ik or (m^T)_ik = m_ki
+      const T mr = sre[e] + lj * tre[e], mi = sim[e] + lj * tim[e];
+      const T yr = zr[(int64_t)k * ncol], yi = zi[(int64_t)k * ncol];
+      ar -= mr * yr - mi * yi;
+      ai -= mr * yi + mi * yr;
+    }
+    const int64_t d = (int64_t)i * nt + i;
+    const T dr = sre[d] + lj * tre[d], di = sim[d] + lj * tim[d];
+    const T den = T(1) / (dr * dr + di * di);
+    zr[(int64_t)i * ncol] = (ar * dr + ai * di) * den;
+    zi[(int64_t)i * ncol] = (ai * dr - ar * di) * den;
+  }
+}
 
 static void* st_upload(iga_stfd_s* f, const std::vector<double>& h) {
   void* d = nullptr;
@@ -1228,9 +1266,14 @@ static void stfd_apply_t(const iga_stfd_s* f, const T* x, T* y, int nrhs, int tr
   for (int d = 0; d < 3; ++d) space_pass(d, true);
   mode_gemm_rect<T>((const T*)f->d_F[tr], cur, nxt, 1, 2 * L, L, cols, nullptr, s);
   std::swap(cur, nxt);
-  arrowhead_kernel<T><<<grid_for(cols, 256), 256, 0, s>>>((int)L, cols, nrhs, (const T*)f->d_lam,
-                                                          (const T*)f->d_D[0], (const T*)f->d_D[1],
-                                                          (const T*)f->d_b[tr][0], (const T*)f->d_b[tr][1], cur);
+  if (f->qz)
+    qz_solve_kernel<T><<<grid_for(cols, 256), 256, 0, s>>>((int)L, cols, nrhs, tr, (const T*)f->d_lam,
+                                                           (const T*)f->d_S[0], (const T*)f->d_S[1],
+                                                           (const T*)f->d_T[0], (const T*)f->d_T[1], cur);
+  else
+    arrowhead_kernel<T><<<grid_for(cols, 256), 256, 0, s>>>((int)L, cols, nrhs, (const T*)f->d_lam,
+                                                            (const T*)f->d_D[0], (const T*)f->d_D[1],
+                                                            (const T*)f->d_b[tr][0], (const T*)f->d_b[tr][1], cur);
   IGA_LAUNCH_CHECK();
   mode_gemm_rect<T>((const T*)f->d_G[tr], cur, nxt, 1, L, 2 * L, cols, nullptr, s);
   std::swap(cur, nxt);
@@ -1314,10 +1357,75 @@ int iga_stfd_create(iga_stfd_t* out, const int64_t* n, const int64_t* lo, const 
   });
 }
 
+int iga_stfd_create_qz(iga_stfd_t* out, const int64_t* n, const int64_t* lo, const int64_t* len, const double* const* U,
+                       const double* q_re, const double* q_im, const double* z_re, const double* z_im, int dtype) {
+  return guard([&] {
+    IGA_CHECK_ARG(out && q_re && q_im && z_re && z_im, "NULL argument");
+    // the space factors and the buffers of the arrowhead plan; Q stands in for U_t
+    IGA_CHECK(iga_stfd_create(out, n, lo, len, U, q_re, q_im, dtype));
+    iga_stfd_s* f = *out;
+    try {
+      const int64_t L = len[3];
+      for (int tr = 0; tr < 2; ++tr) {
+        // apply: Q^H forward, Z backward (time_legacy.py:81-105); transpose: Z^T
+        // forward, conj(Q) backward (time_legacy.py:140-160).  F = [Re A; Im A],
+        // G = [Re B, -Im B] for the forward A and backward B time matrices.
+        std::vector<double> F((size_t)2 * L * L), G((size_t)L * 2 * L);
+        for (int64_t i = 0; i < L; ++i)
+          for (int64_t k = 0; k < L; ++k) {
+            if (tr == 0) {
+              F[i * L + k] = q_re[k * L + i];
+              F[(L + i) * L + k] = -q_im[k * L + i];
+              G[i * 2 * L + k] = z_re[i * L + k];
+              G[i * 2 * L + L + k] = -z_im[i * L + k];
+            } else {
+              F[i * L + k] = z_re[k * L + i];
+              F[(L + i) * L + k] = z_im[k * L + i];
+              G[i * 2 * L + k] = q_re[i * L + k];
+              G[i * 2 * L + L + k] = q_im[i * L + k];
+            }
+          }
+        st_write(f, f->d_F[tr], F);
+        st_write(f, f->d_G[tr], G);
+      }
+      for (int c = 0; c < 2; ++c) {
+        f->d_S[c] = st_upload(f, std::vector<double>((size_t)L * L, 0.0));
+        f->d_T[c] = st_upload(f, std::vector<double>((size_t)L * L, 0.0));
+      }
+      f->qz = 1;
+    } catch (...) {
+      free_stfd(f);
+      delete f;
+      *out = nullptr;
+      throw;
+    }
+  });
+}
+
+int iga_stfd_set_qz(iga_stfd_t f, const double* lam_space, const double* s_re, const double* s_im,
+                    const double* t_re, const double* t_im, double adv) {
+  return guard([&] {
+    IGA_CHECK_ARG(f && f->qz && lam_space && s_re && s_im && t_re && t_im, "NULL argument or not a QZ plan");
+    const int64_t L = f->len[3];
+    st_write(f, f->d_lam, std::vector<double>(lam_space, lam_space + f->nspace));
+    std::vector<double> sr((size_t)L * L), si((size_t)L * L);
+    for (int64_t e = 0; e < L * L; ++e) {
+      sr[e] = adv * s_re[e];
+      si[e] = adv * s_im[e];
+    }
+    st_write(f, f->d_S[0], sr);
+    st_write(f, f->d_S[1], si);
+    st_write(f, f->d_T[0], std::vector<double>(t_re, t_re + L * L));
+    st_write(f, f->d_T[1], std::vector<double>(t_im, t_im + L * L));
+    f->blocks_set = true;
+  });
+}
+
 int iga_stfd_set_blocks(iga_stfd_t f, const double* lam_space, const double* d_re, const double* d_im,
                         const double* g_re, const double* g_im, double adv) {
   return guard([&] {
     IGA_CHECK_ARG(f && lam_space && d_re && d_im && g_re && g_im, "NULL argument");
+    IGA_CHECK_ARG(!f->qz, "a QZ plan takes iga_stfd_set_qz");
     const int64_t L = f->len[3];
     st_write(f, f->d_lam, std::vector<double>(lam_space, lam_space + f->nspace));
     std::vector<double> dr(L), di(L), b[2][2];

@@ -182,18 +182,23 @@ __device__ inline void st16(T* C, int ldc, typename PlaneMma<T>::V acc, int lane
   for (int r = 0; r < 4; ++r) C[PlaneMma<T>::row(lane, r) * ldc + (lane & 15)] = acc[r];
 }
 
-template <typename T, int P, int FORM>
+// NR right-hand sides per work-group (1 or 2; 2 only for forms with one input and
+// shared coefficients): the coefficient loads, the band tables and the barriers of the
+// plane phases serve NR right-hand sides, each with its own register rings and LDS
+// planes.
+template <typename T, int P, int FORM, int NR>
 __device__ __attribute__((always_inline)) inline void fused_apply_body(const FusedArgs<T>& a) {
   using MM = PlaneMma<T>;
   using V = typename MM::V;
   // prefetch depth (q3 planes) of the coefficient stream
   constexpr int PD = sizeof(T) == 8 ? 2 : 3;
   constexpr bool DUAL = FORM == FORM_KM7X;
+  static_assert(NR == 1 || (NR == 2 && !DUAL), "two right-hand sides per work-group for one-input forms only");
   constexpr int NT1 = DUAL ? 3 : 2;  // direction-1 trial partials: B0 x, B1 x (, B0 x1)
   constexpr int NT2 = DUAL ? 4 : 3;  // trial plane fields
   constexpr int AP = 17;             // pitch of arrays read as the A operand (rows over lanes)
   // The footprint block of a DOF plane in LDS, zero outside D1 x D2 (set once)
-  __shared__ T xs[kTQ2][AP];
+  __shared__ T xs[NR][kTQ2][AP];
   __shared__ T xs1[DUAL ? kTQ2 : 1][AP];
   // dense band matrices of the tile (zero outside the band and past the tile):
   //   bd1[f][c1][l1] = B_f of direction 1 at point l1, DOF c1   (B operand, K = c1)
@@ -204,16 +209,17 @@ __device__ __attribute__((always_inline)) inline void fused_apply_body(const Fus
   __shared__ T bd2[2][16][AP];
   __shared__ T wd2[2][16][AP];
   __shared__ T wd1[2][16][16];
-  __shared__ T s2s[NT2 > 3 ? NT2 : 3][16][16];  // test values [m][l], then the trial plane [l2][l1]
-  __shared__ T t1s[NT1][16][16];                // direction-1 trial partials [c2][l1]
-  __shared__ T r1s[2][16][AP];                  // direction-2 test partials [c2][l]
+  __shared__ T s2s[NR][NT2 > 3 ? NT2 : 3][16][16];  // test values [m][l], then the trial plane [l2][l1]
+  __shared__ T t1s[NR][NT1][16][16];                // direction-1 trial partials [c2][l1]
+  __shared__ T r1s[NR][2][16][AP];                  // direction-2 test partials [c2][l]
   __shared__ T b3s[kSlabMax][4][P];             // direction-3 band values of the slab's planes
 
   const int L = xcd_logical(blockIdx.x, gridDim.x);
   if (L >= a.nlogical) return;  // padding blocks of the XCD remap (whole block exits)
   int rem = L;
-  const int k = rem % a.nrhs;
-  rem /= a.nrhs;
+  const int nrb = a.nrhs / NR;  // work-groups per tile-slab (the host ensures NR | nrhs)
+  const int k = (rem % nrb) * NR;  // first right-hand side of this work-group
+  rem /= nrb;
   const int s = rem % a.nslab;
   rem /= a.nslab;
   const int t1 = rem % a.nt1;
@@ -273,7 +279,8 @@ __device__ __attribute__((always_inline)) inline void fused_apply_body(const Fus
       wd2[0][cd][m] = in ? a.d[1].tab[2 * (int64_t)a.nq[1] * P + o] : T(0);
       wd2[1][cd][m] = in ? a.d[1].tab[3 * (int64_t)a.nq[1] * P + o] : T(0);
     }
-    xs[r][c] = T(0);
+#pragma unroll
+    for (int q = 0; q < NR; ++q) xs[q][r][c] = T(0);
     if constexpr (DUAL) xs1[r][c] = T(0);
   }
   for (int i = tid; i < (q3e - q3s) * 4 * P; i += kThreads) {
@@ -282,8 +289,8 @@ __device__ __attribute__((always_inline)) inline void fused_apply_body(const Fus
   }
   __syncthreads();
 
-  T tr[NT2][P];  // trial partials of the window planes: (val,val), (d1,val), (val,d2) [, (val,val) of x1]
-  T te[3][P];    // test partials: (val,val), (d1,val), (val,d2) in directions (1,2)
+  T tr[NR][NT2][P];  // trial partials of the window planes: (val,val), (d1,val), (val,d2) [, (val,val) of x1]
+  T te[NR][3][P];    // test partials: (val,val), (d1,val), (val,d2) in directions (1,2)
 
   // x block of one DOF plane, held in registers between its (early) load and its
   // use, so that the load latency overlaps the work of the current step.
@@ -291,8 +298,10 @@ __device__ __attribute__((always_inline)) inline void fused_apply_body(const Fus
   // compiler does not wait on them at the point of issue; NURBS weights are kept
   // apart and applied when the block is used.
   constexpr int XR = (kDMax * kDMax + kThreads - 1) / kThreads;
-  const T* xk = a.x + (int64_t)k * a.x_rstride;
-  const T* xk1 = DUAL ? a.x1 + (int64_t)k * a.x_rstride : xk;
+  const T* xk[NR];
+#pragma unroll
+  for (int q = 0; q < NR; ++q) xk[q] = a.x + (int64_t)(k + q) * a.x_rstride;
+  const T* xk1 = DUAL ? a.x1 + (int64_t)k * a.x_rstride : xk[0];
   const int64_t xds = a.x_dstride;
   const bool nurbs = a.w != nullptr;
   // Per-thread offsets of its x-block elements within a DOF plane and their LDS
@@ -313,21 +322,22 @@ __device__ __attribute__((always_inline)) inline void fused_apply_body(const Fus
   const int64_t plane = (int64_t)n1 * n2;
   // the second input's registers exist only in the two-input form
   struct XBlock1 {
-    T x[XR], w[XR];
+    T x[NR][XR], w[XR];
   };
   struct XBlock2 {
-    T x[XR], w[XR], x1[XR];
+    T x[NR][XR], w[XR], x1[XR];
   };
   using XBlock = std::conditional_t<DUAL, XBlock2, XBlock1>;
   struct Trial {
-    T v[NT2];
+    T v[NR][NT2];
   };
   auto load_x = [&](int c3) __attribute__((always_inline)) {
     XBlock b;
 #pragma unroll
     for (int e = 0; e < XR; ++e) {
       const int64_t dof = xoff[e] + plane * c3;
-      b.x[e] = xk[dof * xds];
+#pragma unroll
+      for (int q = 0; q < NR; ++q) b.x[q][e] = xk[q][dof * xds];
       if constexpr (DUAL) b.x1[e] = xk1[dof * xds];
       b.w[e] = nurbs ? a.w[dof] : T(1);
     }
@@ -342,74 +352,120 @@ __device__ __attribute__((always_inline)) inline void fused_apply_body(const Fus
   //            R1_1 = Wd2_0 S_1
   //   stage 2: T2 = Bd2_a T1_b (the trial plane fields), partial = R1_0 Wd1_0 + R1_1 Wd1_1
   // with the products spread over the four waves.
-  T* part = a.partial + (int64_t)k * a.part_stride + a.part_base[(int64_t)tile * a.nslab + s];
+  T* part[NR];
+#pragma unroll
+  for (int q = 0; q < NR; ++q)
+    part[q] = a.partial + (int64_t)(k + q) * a.part_stride + a.part_base[(int64_t)tile * a.nslab + s];
+  // the flush of right-hand side q's oldest test plane: R1 Wd1 into its partial block
+  auto flush_block = [&](int q, int c3f) __attribute__((always_inline)) {
+    V acc = mm16<T>(&r1s[q][0][0][0], AP, &wd1[0][0][0], 16, V{0, 0, 0, 0}, lane);
+    acc = mm16<T>(&r1s[q][1][0][0], AP, &wd1[1][0][0], 16, acc, lane);
+    T* blk = part[q] + (int64_t)(c3f - c3lo) * D2 * D1;
+    const int c1 = lane & 15;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int c2 = MM::row(lane, r);
+      if (c1 < D1 && c2 < D2) blk[c2 * D1 + c1] = acc[r];
+    }
+  };
+  // trial plane field g of right-hand side q: fields (B0, T1_0), (B0, T1_1), (B1, T1_0)
+  // [, (B0, T1_2)]
+  auto field = [&](int q, int g) __attribute__((always_inline)) {
+    const int fa = g == 2 ? 1 : 0, fb = g == 1 ? 1 : g == 3 ? 2 : 0;
+    V acc = mm16<T>(&bd2[fa][0][0], AP, &t1s[q][fb][0][0], 16, V{0, 0, 0, 0}, lane);
+    st16<T>(&s2s[q][g][0][0], 16, acc, lane);
+  };
   auto plane_phase = [&](bool do_flush, int c3f, bool do_trial, XBlock xb) __attribute__((always_inline)) {
     Trial out{};
     __syncthreads();  // LDS buffers are reused
     if (do_flush) {
-      s2s[0][l2][l1] = active ? te[0][0] : T(0);
-      s2s[1][l2][l1] = active ? te[1][0] : T(0);
-      s2s[2][l2][l1] = active ? te[2][0] : T(0);
+#pragma unroll
+      for (int q = 0; q < NR; ++q) {
+        s2s[q][0][l2][l1] = active ? te[q][0][0] : T(0);
+        s2s[q][1][l2][l1] = active ? te[q][1][0] : T(0);
+        s2s[q][2][l2][l1] = active ? te[q][2][0] : T(0);
+      }
     }
     if (do_trial) {
 #pragma unroll
       for (int e = 0; e < XR; ++e) {
         if (xij[e] >= 0) {
           const int i2 = xij[e] >> 8, i1 = xij[e] & 255;
-          xs[i2][i1] = nurbs ? xb.x[e] * xb.w[e] : xb.x[e];
+#pragma unroll
+          for (int q = 0; q < NR; ++q) xs[q][i2][i1] = nurbs ? xb.x[q][e] * xb.w[e] : xb.x[q][e];
           if constexpr (DUAL) xs1[i2][i1] = nurbs ? xb.x1[e] * xb.w[e] : xb.x1[e];
         }
       }
     }
     __syncthreads();
-    // stage 1
-    if (do_trial) {
-      if (wave < NT1) {
-        const T* X = (DUAL && wave == 2) ? &xs1[0][0] : &xs[0][0];
-        const int f = wave == 1 ? 1 : 0;
-        V acc = mm16<T>(X, AP, &bd1[f][0][0], 16, V{0, 0, 0, 0}, lane);
-        st16<T>(&t1s[wave][0][0], 16, acc, lane);
+    // stage 1: trial T1_f = X Bd1_f; test R1_0 = Wd2_0 S_0 + Wd2_1 S_2, R1_1 = Wd2_0 S_1
+    if constexpr (NR == 1) {
+      if (do_trial) {
+        if (wave < NT1) {
+          const T* X = (DUAL && wave == 2) ? &xs1[0][0] : &xs[0][0][0];
+          const int f = wave == 1 ? 1 : 0;
+          V acc = mm16<T>(X, AP, &bd1[f][0][0], 16, V{0, 0, 0, 0}, lane);
+          st16<T>(&t1s[0][wave][0][0], 16, acc, lane);
+        }
       }
-    }
-    if (do_flush) {
-      if (wave == 3) {
-        V acc = mm16<T>(&wd2[0][0][0], AP, &s2s[0][0][0], 16, V{0, 0, 0, 0}, lane);
-        acc = mm16<T>(&wd2[1][0][0], AP, &s2s[2][0][0], 16, acc, lane);
-        st16<T>(&r1s[0][0][0], AP, acc, lane);
-      } else if (wave == 2) {
-        V acc = mm16<T>(&wd2[0][0][0], AP, &s2s[1][0][0], 16, V{0, 0, 0, 0}, lane);
-        st16<T>(&r1s[1][0][0], AP, acc, lane);
+      if (do_flush) {
+        if (wave == 3) {
+          V acc = mm16<T>(&wd2[0][0][0], AP, &s2s[0][0][0][0], 16, V{0, 0, 0, 0}, lane);
+          acc = mm16<T>(&wd2[1][0][0], AP, &s2s[0][2][0][0], 16, acc, lane);
+          st16<T>(&r1s[0][0][0][0], AP, acc, lane);
+        } else if (wave == 2) {
+          V acc = mm16<T>(&wd2[0][0][0], AP, &s2s[0][1][0][0], 16, V{0, 0, 0, 0}, lane);
+          st16<T>(&r1s[0][1][0][0], AP, acc, lane);
+        }
+      }
+    } else {
+      // wave w: trial (q, f) = (w / 2, w % 2); test R1_0 of q = w (waves 0, 1) or R1_1 of
+      // q = w - 2 (waves 2, 3)
+      if (do_trial) {
+        const int q = wave >> 1, f = wave & 1;
+        V acc = mm16<T>(&xs[q][0][0], AP, &bd1[f][0][0], 16, V{0, 0, 0, 0}, lane);
+        st16<T>(&t1s[q][f][0][0], 16, acc, lane);
+      }
+      if (do_flush) {
+        if (wave < 2) {
+          V acc = mm16<T>(&wd2[0][0][0], AP, &s2s[wave][0][0][0], 16, V{0, 0, 0, 0}, lane);
+          acc = mm16<T>(&wd2[1][0][0], AP, &s2s[wave][2][0][0], 16, acc, lane);
+          st16<T>(&r1s[wave][0][0][0], AP, acc, lane);
+        } else {
+          V acc = mm16<T>(&wd2[0][0][0], AP, &s2s[wave - 2][1][0][0], 16, V{0, 0, 0, 0}, lane);
+          st16<T>(&r1s[wave - 2][1][0][0], AP, acc, lane);
+        }
       }
     }
     __syncthreads();
-    // stage 2
-    if (do_trial) {
-      // fields: (B0, T1_0), (B0, T1_1), (B1, T1_0) [, (B0, T1_2)]
-      auto field = [&](int g) __attribute__((always_inline)) {
-        const int fa = g == 2 ? 1 : 0, fb = g == 1 ? 1 : g == 3 ? 2 : 0;
-        V acc = mm16<T>(&bd2[fa][0][0], AP, &t1s[fb][0][0], 16, V{0, 0, 0, 0}, lane);
-        st16<T>(&s2s[g][0][0], 16, acc, lane);
-      };
-      if (wave < 3) field(wave);
-      if constexpr (DUAL) {
-        if (wave == 0) field(3);
+    // stage 2: the trial plane fields T2 = Bd2 T1, the flush R1 Wd1 into the partial block
+    if constexpr (NR == 1) {
+      if (do_trial) {
+        if (wave < 3) field(0, wave);
+        if constexpr (DUAL) {
+          if (wave == 0) field(0, 3);
+        }
       }
-    }
-    if (do_flush && wave == 3) {
-      V acc = mm16<T>(&r1s[0][0][0], AP, &wd1[0][0][0], 16, V{0, 0, 0, 0}, lane);
-      acc = mm16<T>(&r1s[1][0][0], AP, &wd1[1][0][0], 16, acc, lane);
-      T* blk = part + (int64_t)(c3f - c3lo) * D2 * D1;
-      const int c1 = lane & 15;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int c2 = MM::row(lane, r);
-        if (c1 < D1 && c2 < D2) blk[c2 * D1 + c1] = acc[r];
+      if (do_flush && wave == 3) flush_block(0, c3f);
+    } else {
+      // waves 0, 1: field 0 / 1 of both right-hand sides; waves 2, 3: field 2 of one and
+      // the flush of the other
+      if (do_trial) {
+        if (wave < 2) {
+          field(0, wave);
+          field(1, wave);
+        } else {
+          field(wave - 2, 2);
+        }
       }
+      if (do_flush && wave >= 2) flush_block(3 - wave, c3f);
     }
     if (do_trial) {
       __syncthreads();
 #pragma unroll
-      for (int g = 0; g < NT2; ++g) out.v[g] = active ? s2s[g][l2][l1] : T(0);
+      for (int q = 0; q < NR; ++q)
+#pragma unroll
+        for (int g = 0; g < NT2; ++g) out.v[q][g] = active ? s2s[q][g][l2][l1] : T(0);
     }
     return out;
   };
@@ -426,8 +482,11 @@ __device__ __attribute__((always_inline)) inline void fused_apply_body(const Fus
     for (int j = 0; j < P; ++j) {
       const Trial t = plane_phase(false, 0, true, xw[j]);
 #pragma unroll
-      for (int g = 0; g < NT2; ++g) tr[g][j] = t.v[g];
-      te[0][j] = te[1][j] = te[2][j] = T(0);
+      for (int q = 0; q < NR; ++q) {
+#pragma unroll
+        for (int g = 0; g < NT2; ++g) tr[q][g][j] = t.v[q][g];
+        te[q][0][j] = te[q][1][j] = te[q][2][j] = T(0);
+      }
     }
   }
   // x block of the next plane to enter the trial window
@@ -462,15 +521,18 @@ __device__ __attribute__((always_inline)) inline void fused_apply_body(const Fus
   auto advance = [&]() __attribute__((always_inline)) {
     const Trial t = plane_phase(true, cur, true, xnext);
 #pragma unroll
-    for (int j = 0; j < P - 1; ++j) {
+    for (int q = 0; q < NR; ++q) {
 #pragma unroll
-      for (int g = 0; g < 3; ++g) te[g][j] = te[g][j + 1];
+      for (int j = 0; j < P - 1; ++j) {
 #pragma unroll
-      for (int g = 0; g < NT2; ++g) tr[g][j] = tr[g][j + 1];
+        for (int g = 0; g < 3; ++g) te[q][g][j] = te[q][g][j + 1];
+#pragma unroll
+        for (int g = 0; g < NT2; ++g) tr[q][g][j] = tr[q][g][j + 1];
+      }
+      te[q][0][P - 1] = te[q][1][P - 1] = te[q][2][P - 1] = T(0);
+#pragma unroll
+      for (int g = 0; g < NT2; ++g) tr[q][g][P - 1] = t.v[q][g];
     }
-    te[0][P - 1] = te[1][P - 1] = te[2][P - 1] = T(0);
-#pragma unroll
-    for (int g = 0; g < NT2; ++g) tr[g][P - 1] = t.v[g];
     ++cur;
     // unconditional (clamped plane) so that the load queue has the same shape on every path
     xnext = load_x(cur + P <= c3hi ? cur + P : c3hi);
@@ -488,21 +550,24 @@ __device__ __attribute__((always_inline)) inline void fused_apply_body(const Fus
     }
     if (active) {
       const T (&tb)[4][P] = b3s[q3 - q3s];
-      T u[4] = {T(0), T(0), T(0), T(0)};
 #pragma unroll
-      for (int j = 0; j < P; ++j) {
-        u[0] += tb[0][j] * tr[DUAL ? NT2 - 1 : 0][j];  // the value: from x1 for FORM_KM7X
-        u[1] += tb[0][j] * tr[1][j];
-        u[2] += tb[0][j] * tr[2][j];
-        u[3] += tb[1][j] * tr[0][j];
-      }
-      T fv[4];
-      point_contract<T, P, FORM>(c, 1, u, fv);
+      for (int q = 0; q < NR; ++q) {
+        T u[4] = {T(0), T(0), T(0), T(0)};
 #pragma unroll
-      for (int j = 0; j < P; ++j) {
-        te[0][j] += tb[2][j] * fv[0] + tb[3][j] * fv[3];
-        te[1][j] += tb[2][j] * fv[1];
-        te[2][j] += tb[2][j] * fv[2];
+        for (int j = 0; j < P; ++j) {
+          u[0] += tb[0][j] * tr[q][DUAL ? NT2 - 1 : 0][j];  // the value: from x1 for FORM_KM7X
+          u[1] += tb[0][j] * tr[q][1][j];
+          u[2] += tb[0][j] * tr[q][2][j];
+          u[3] += tb[1][j] * tr[q][0][j];
+        }
+        T fv[4];
+        point_contract<T, P, FORM>(c, 1, u, fv);
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+          te[q][0][j] += tb[2][j] * fv[0] + tb[3][j] * fv[3];
+          te[q][1][j] += tb[2][j] * fv[1];
+          te[q][2][j] += tb[2][j] * fv[2];
+        }
       }
     }
     // refill this buffer with the plane PD steps ahead (past the slab end: a valid plane,
@@ -518,10 +583,12 @@ __device__ __attribute__((always_inline)) inline void fused_apply_body(const Fus
   for (int j = 0; j < P; ++j) {
     plane_phase(true, cur, false, xnext);
 #pragma unroll
-    for (int jj = 0; jj < P - 1; ++jj) {
+    for (int q = 0; q < NR; ++q)
 #pragma unroll
-      for (int g = 0; g < 3; ++g) te[g][jj] = te[g][jj + 1];
-    }
+      for (int jj = 0; jj < P - 1; ++jj) {
+#pragma unroll
+        for (int g = 0; g < 3; ++g) te[q][g][jj] = te[q][g][jj + 1];
+      }
     ++cur;
   }
 }
@@ -529,16 +596,19 @@ __device__ __attribute__((always_inline)) inline void fused_apply_body(const Fus
 // Two launch variants of the same sweep: with an occupancy target of three waves
 // per SIMD (used when the band is narrow enough to fit without spilling) and
 // without (wider FP64 bands, two waves per SIMD).
-template <typename T, int P, int FORM>
+template <typename T, int P, int FORM, int NR>
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8)))
 fused_apply_kernel(FusedArgs<T> a) {
-  fused_apply_body<T, P, FORM>(a);
+  fused_apply_body<T, P, FORM, NR>(a);
 }
 
-template <typename T, int P, int FORM>
+template <typename T, int P, int FORM, int NR>
 __global__ void __launch_bounds__(kThreads) fused_apply_kernel_wide(FusedArgs<T> a) {
-  fused_apply_body<T, P, FORM>(a);
+  fused_apply_body<T, P, FORM, NR>(a);
 }
+
+// forms with a two-right-hand-side variant (the solver and benchmark forms)
+__host__ __device__ constexpr bool form_has_nr2(int f) { return f == FORM_K6 || f == FORM_KM7 || f == FORM_SYM10; }
 
 // y[a*r + k] = alpha * wscale[a] * sum of the partial blocks covering DOF a (+ y)
 template <typename T>
@@ -849,6 +919,7 @@ struct iga_fused_s {
   int nc[3] = {0, 0, 0}, nq[3] = {0, 0, 0};
   int nt1 = 0, nt2 = 0, nslab = 0;
   int64_t coef_elems = 0, part_stride = 0, part_blk = 0;
+  int rhs_block = 1;  // right-hand sides per work-group when several share the coefficients
   // device tables
   int32_t* d_first[3] = {nullptr, nullptr, nullptr};
   void* d_tab[3] = {nullptr, nullptr, nullptr};  // T[4][nq][P]
@@ -927,13 +998,14 @@ static DirTiling tile_direction(const int32_t* first, int nq, int ncdof, int P, 
 // The three-wave variant is used unless the register budget it imposes makes the
 // compiler spill (private scratch), which costs more than the occupancy gains;
 // decided once per instance from the code object's attributes.
-template <typename T, int P, int FORM>
+template <typename T, int P, int FORM, int NR>
 static bool use_wide() {
   static std::atomic<int> wide{-1};
   int w = wide.load(std::memory_order_relaxed);
   if (w < 0) {
     hipFuncAttributes fa{};
-    w = (hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&fused_apply_kernel<T, P, FORM>)) == hipSuccess &&
+    w = (hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&fused_apply_kernel<T, P, FORM, NR>)) ==
+             hipSuccess &&
          fa.localSizeBytes > 0)
             ? 1
             : 0;
@@ -942,20 +1014,34 @@ static bool use_wide() {
   return w != 0;
 }
 
-template <typename T, int P, int FORM>
-static void launch_apply_pf(const FusedArgs<T>& a, int nblk, hipStream_t s) {
-  if (use_wide<T, P, FORM>())
-    fused_apply_kernel_wide<T, P, FORM><<<nblk, kThreads, 0, s>>>(a);
+template <typename T, int P, int FORM, int NR>
+static void launch_apply_pfn(const FusedArgs<T>& a, int nblk, hipStream_t s) {
+  if (use_wide<T, P, FORM, NR>())
+    fused_apply_kernel_wide<T, P, FORM, NR><<<nblk, kThreads, 0, s>>>(a);
   else
-    fused_apply_kernel<T, P, FORM><<<nblk, kThreads, 0, s>>>(a);
+    fused_apply_kernel<T, P, FORM, NR><<<nblk, kThreads, 0, s>>>(a);
 }
 
-// resident work-groups per CU of the variant launch_apply_pf picks (0 if unknown)
+// nr right-hand sides per work-group (2 only where the form has that variant)
+template <typename T, int P, int FORM>
+static void launch_apply_pf(const FusedArgs<T>& a, int nblk, int nr, hipStream_t s) {
+  if constexpr (form_has_nr2(FORM)) {
+    if (nr == 2) {
+      launch_apply_pfn<T, P, FORM, 2>(a, nblk, s);
+      return;
+    }
+  }
+  launch_apply_pfn<T, P, FORM, 1>(a, nblk, s);
+}
+
+// resident work-groups per CU of the variant launch_apply_pf picks for one right-hand
+// side per work-group (0 if unknown)
 template <typename T, int P, int FORM>
 static int resident_blocks_pf() {
   int nb = 0;
-  const void* fn = use_wide<T, P, FORM>() ? reinterpret_cast<const void*>(&fused_apply_kernel_wide<T, P, FORM>)
-                                          : reinterpret_cast<const void*>(&fused_apply_kernel<T, P, FORM>);
+  const void* fn = use_wide<T, P, FORM, 1>()
+                       ? reinterpret_cast<const void*>(&fused_apply_kernel_wide<T, P, FORM, 1>)
+                       : reinterpret_cast<const void*>(&fused_apply_kernel<T, P, FORM, 1>);
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, kThreads, 0) != hipSuccess) return 0;
   return nb;
 }
@@ -987,28 +1073,28 @@ static int resident_blocks(int P, int form) {
 }
 
 template <typename T, int P>
-static void launch_apply_p(const iga_fused_s* f, const FusedArgs<T>& a, int nblk, hipStream_t s) {
+static void launch_apply_p(const iga_fused_s* f, const FusedArgs<T>& a, int nblk, int nr, hipStream_t s) {
   switch (f->form) {
-    case FORM_M1: launch_apply_pf<T, P, FORM_M1>(a, nblk, s); break;
-    case FORM_K6: launch_apply_pf<T, P, FORM_K6>(a, nblk, s); break;
-    case FORM_KM7: launch_apply_pf<T, P, FORM_KM7>(a, nblk, s); break;
-    case FORM_SYM10: launch_apply_pf<T, P, FORM_SYM10>(a, nblk, s); break;
-    case FORM_KM7X: launch_apply_pf<T, P, FORM_KM7X>(a, nblk, s); break;
-    case FORM_GV4: launch_apply_pf<T, P, FORM_GV4>(a, nblk, s); break;
-    case FORM_UG4: launch_apply_pf<T, P, FORM_UG4>(a, nblk, s); break;
-    default: launch_apply_pf<T, P, FORM_GEN16>(a, nblk, s); break;
+    case FORM_M1: launch_apply_pf<T, P, FORM_M1>(a, nblk, nr, s); break;
+    case FORM_K6: launch_apply_pf<T, P, FORM_K6>(a, nblk, nr, s); break;
+    case FORM_KM7: launch_apply_pf<T, P, FORM_KM7>(a, nblk, nr, s); break;
+    case FORM_SYM10: launch_apply_pf<T, P, FORM_SYM10>(a, nblk, nr, s); break;
+    case FORM_KM7X: launch_apply_pf<T, P, FORM_KM7X>(a, nblk, nr, s); break;
+    case FORM_GV4: launch_apply_pf<T, P, FORM_GV4>(a, nblk, nr, s); break;
+    case FORM_UG4: launch_apply_pf<T, P, FORM_UG4>(a, nblk, nr, s); break;
+    default: launch_apply_pf<T, P, FORM_GEN16>(a, nblk, nr, s); break;
   }
 }
 
 template <typename T>
-static void launch_apply(const iga_fused_s* f, const FusedArgs<T>& a, int nblk, hipStream_t s) {
+static void launch_apply(const iga_fused_s* f, const FusedArgs<T>& a, int nblk, int nr, hipStream_t s) {
   switch (f->P) {
-    case 3: launch_apply_p<T, 3>(f, a, nblk, s); break;
-    case 4: launch_apply_p<T, 4>(f, a, nblk, s); break;
-    case 5: launch_apply_p<T, 5>(f, a, nblk, s); break;
-    case 6: launch_apply_p<T, 6>(f, a, nblk, s); break;
-    case 7: launch_apply_p<T, 7>(f, a, nblk, s); break;
-    case 8: launch_apply_p<T, 8>(f, a, nblk, s); break;
+    case 3: launch_apply_p<T, 3>(f, a, nblk, nr, s); break;
+    case 4: launch_apply_p<T, 4>(f, a, nblk, nr, s); break;
+    case 5: launch_apply_p<T, 5>(f, a, nblk, nr, s); break;
+    case 6: launch_apply_p<T, 6>(f, a, nblk, nr, s); break;
+    case 7: launch_apply_p<T, 7>(f, a, nblk, nr, s); break;
+    case 8: launch_apply_p<T, 8>(f, a, nblk, nr, s); break;
     default: throw Error(IGA_ERR_UNSUPPORTED, "band width p+1 must be in 3..8");
   }
 }
@@ -1061,11 +1147,14 @@ static void run_apply(const iga_fused_s* f, const void* coef, const void* x, voi
   a.x1 = (const T*)x1;
   a.w = (const T*)w;
   a.partial = (T*)work;
-  const int64_t nlog = (int64_t)f->nt1 * f->nt2 * f->nslab * nrhs;
+  // two right-hand sides per work-group when they share one coefficient set (every
+  // coefficient load, band table and plane-phase barrier then serves both)
+  const int nr = (f->rhs_block == 2 && form_has_nr2(f->form) && nrhs % 2 == 0 && !gcoef && lay.cr == 0) ? 2 : 1;
+  const int64_t nlog = (int64_t)f->nt1 * f->nt2 * f->nslab * (nrhs / nr);
   IGA_CHECK_ARG(nlog < (1LL << 30), "too many work-groups");
   a.nlogical = (int)nlog;
   const int nblk = (int)((nlog + 7) / 8 * 8);
-  launch_apply<T>(f, a, nblk, s);
+  launch_apply<T>(f, a, nblk, nr, s);
   IGA_LAUNCH_CHECK();
   }
   if (!(stages & 2)) return;
@@ -1286,6 +1375,13 @@ int iga_fused_create(iga_fused_t* out, const int32_t* nc, const int32_t* nq, int
   });
 }
 
+int iga_fused_set_rhs_block(iga_fused_t f, int nr) {
+  return guard([&] {
+    IGA_CHECK_ARG(f && (nr == 1 || nr == 2), "right-hand sides per work-group: 1 or 2");
+    f->rhs_block = nr;
+  });
+}
+
 int iga_fused_destroy(iga_fused_t f) {
   return guard([&] {
     if (!f) return;
@@ -1354,12 +1450,17 @@ int iga_fused_apply_group(iga_fused_t f, int nmember, const void* const* packed,
     const int64_t* len = ndim == 3 ? box_len : nullptr;
     // the output strides are unused: the gather writes member k to y[k][dof]
     const Layout lay{x_dstride, x_mstride, 1, 0, 0};
+    // one coefficient set for every member (right-hand sides of one operator): the
+    // sweep's shared-coefficient path
+    bool shared = true;
+    for (int k = 1; k < nmember; ++k) shared = shared && packed[k] == packed[0];
+    const void* const* gco = shared ? nullptr : packed;
     if (f->dtype == IGA_F64)
       run_apply<double>(f, packed[0], x, y[0], nmember, nurbs_w, 1.0, 0, work, as_stream(stream), 3, lay, nullptr, lo,
-                        len, packed, y);
+                        len, gco, y);
     else
       run_apply<float>(f, packed[0], x, y[0], nmember, nurbs_w, 1.0, 0, work, as_stream(stream), 3, lay, nullptr, lo,
-                       len, packed, y);
+                       len, gco, y);
   });
 }
 

@@ -257,11 +257,10 @@ class SingleFastDiagonalization:
     def compute_time_schurdecomposition(self, time_quadrule, use_arrowhead: bool = True):
         if self._preconditioner_type == "spatial":
             raise RuntimeError("Space-time preconditioner not defined.")
-        if not use_arrowhead:
-            raise NotImplementedError("only the arrowhead space-time FD is provided")
-        from .spacetime import SpaceTimeFD
+        from .spacetime import SpaceTimeFD, SpaceTimeQZFD
 
-        st = SpaceTimeFD(self.space_preconditioner)
+        # the arrowhead form, or the legacy QZ form (fastdiag/preconditioner.py:65-88)
+        st = (SpaceTimeFD if use_arrowhead else SpaceTimeQZFD)(self.space_preconditioner)
         st.compute_time_schurdecomposition(time_quadrule)
         self._sptime_fd = st
 

@@ -355,6 +355,12 @@ class FusedPlan:
             _lib.load().iga_fused_destroy(h)
             self._h = None
 
+    def set_rhs_block(self, nr: int):
+        """Right-hand sides per work-group when several share the coefficients (1, the
+        default, or 2: each coefficient load and plane-phase barrier then serves two;
+        the results agree to rounding)."""
+        _lib.call("iga_fused_set_rhs_block", self._h, int(nr))
+
     def workspace_bytes(self, nrhs: int) -> int:
         return int(_lib.load().iga_fused_workspace_bytes(self._h, int(nrhs)))
 
@@ -617,14 +623,17 @@ class StiffnessMassOperator:
             terms += terms_u_v(qs, (md * wi * wi).to(tdt).contiguous())
         return terms
 
-    def apply(self, x, out=None, nrhs: int = 1, alpha: float = 1.0, accumulate: bool = False, stream=None):
-        """``out = alpha A x (+ out)`` on device arrays of the operator dtype."""
+    def apply(self, x, out=None, nrhs: int = 1, alpha: float = 1.0, accumulate: bool = False, stream=None,
+              work=None):
+        """``out = alpha A x (+ out)`` on device arrays of the operator dtype; ``work``
+        (``plan.workspace_bytes(nrhs)`` bytes) keeps the partial blocks of callers on
+        different streams apart (the operator's own buffer otherwise)."""
         t = dv.torch()
         if out is None:
             out = t.empty_like(x)
         if self.fused:
-            self.plan.apply(self.packed, x, out, nrhs, self.w, alpha, accumulate, work=self.workspace(nrhs),
-                            stream=stream)
+            self.plan.apply(self.packed, x, out, nrhs, self.w, alpha, accumulate,
+                            work=work if work is not None else self.workspace(nrhs), stream=stream)
             return out
         xin = x
         if self.is_nurbs:

@@ -767,15 +767,18 @@ def launch_group(group, work: dict, batch_stream=None):
         r._launched_native(k)
 
 
-def run_batched(runs, max_group: int = 16):
+def run_batched(runs, max_group: int = 16, batch_stream=None):
     """Drive several solves of one shape to completion with their Arnoldi steps
-    batched: ``runs`` are ``GmresRun`` or ``RefinementRun`` objects on one stream whose
-    one-call steps (``native`` / ``native_lo``) share the FD plan and the mask box and
-    whose operators share one fused plan (the same operator for several right-hand
-    sides, or one per solve).  Every pass, the runs that can take a step take it
-    together (up to ``max_group`` per launch); everything else (starts, refinement
-    residuals, solution updates) stays per run.  Returns the results in order and the
-    host clock (``time.perf_counter``) at which each completed."""
+    batched: ``runs`` are ``GmresRun`` or ``RefinementRun`` objects whose one-call
+    steps (``native`` / ``native_lo``) share the FD plan and the mask box and whose
+    operators share one fused plan (the same operator for several right-hand sides, or
+    one per solve).  Every pass, the runs that can take a step take it together (up to
+    ``max_group`` per launch): the FD and operator launches go on ``batch_stream`` (the
+    runs' stream when they all share one), each run's orthogonalisation on its own
+    stream, so that the small kernels of different runs overlap when the runs have
+    streams of their own.  Everything else (starts, refinement residuals, solution
+    updates) stays per run.  Returns the results in order and the host clock
+    (``time.perf_counter``) at which each completed."""
     import time
 
     ctxs = []
@@ -789,10 +792,12 @@ def run_batched(runs, max_group: int = 16):
         if nv.fd_plan is not nv0.fd_plan or nv.op.plan is not nv0.op.plan or \
                 not all(np.array_equal(a, b) for a, b in zip(nv.box, nv0.box)):
             raise ValueError("batched runs share the FD plan, the operator plan and the mask box")
-    if len({id(r.stream) for r in runs}) != 1:
-        raise ValueError("batched runs share one stream")
     t = dv.torch()
-    stream = runs[0].stream
+    if batch_stream is None:
+        if len({id(r.stream) for r in runs}) != 1:
+            raise ValueError("runs on different streams need a batch stream")
+        batch_stream = runs[0].stream
+    stream = batch_stream
     work: dict = {}
     done_at = [None] * len(runs)
     with (t.cuda.stream(stream) if stream is not None else contextlib.nullcontext()):

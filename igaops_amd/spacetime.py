@@ -184,6 +184,73 @@ class SpaceTimeFD:
         return self.apply_device(x, out, nrhs, stream, transpose=True)
 
 
+class STFDQZPlan(STFDPlan):
+    """Device plan of the legacy QZ space-time FD (``iga_stfd_create_qz``)."""
+
+    def __init__(self, n, lo, ln, space_eigvecs, Q, Z, dtype: str):
+        self.dtype = dtype
+        self.n, self.lo, self.ln = (np.asarray(v, dtype=np.int64) for v in (n, lo, ln))
+        na, npt = _lib.i64_array(self.n)
+        la, lpt = _lib.i64_array(self.lo)
+        lna, lnpt = _lib.i64_array(self.ln)
+        keep = [np.ascontiguousarray(U, dtype=np.float64) for U in space_eigvecs]
+        arr = (_lib._f64p * 3)(*[k.ctypes.data_as(_lib._f64p) for k in keep])
+        qr, qrp = _lib.f64_array(np.real(Q))
+        qi, qip = _lib.f64_array(np.imag(Q))
+        zr, zrp = _lib.f64_array(np.real(Z))
+        zi, zip_ = _lib.f64_array(np.imag(Z))
+        h = C.c_void_p()
+        _lib.call("iga_stfd_create_qz", C.byref(h), npt, lpt, lnpt, arr, qrp, qip, zrp, zip_, dv.DTYPES[dtype])
+        self._h = h
+        self.version = None
+        self._work = {}
+
+    def set_qz(self, lam_space, S, T, adv: float, version):
+        lam, lp = _lib.f64_array(lam_space)
+        sr, srp = _lib.f64_array(np.real(S))
+        si, sip = _lib.f64_array(np.imag(S))
+        tr, trp = _lib.f64_array(np.real(T))
+        ti, tip = _lib.f64_array(np.imag(T))
+        _lib.call("iga_stfd_set_qz", self._h, lp, srp, sip, trp, tip, float(adv))
+        self.version = version
+
+
+class SpaceTimeQZFD(SpaceTimeFD):
+    """The legacy QZ space-time FD (``SpTimeFD`` of ``fastdiag/utils/time_legacy.py:16-171``,
+    ``SingleFD.compute_time_schurdecomposition(..., use_arrowhead=False)``) for one DOF per
+    node: the complex QZ decomposition ``W_t = Q S Z^H``, ``M_t = Q T Z^H`` of the time
+    pencil (host), applied on the device as the arrowhead form with ``Q^H`` / ``Z`` time
+    transforms and one upper-triangular solve with ``adv S + lambda T`` per spatial
+    eigenvalue in place of the arrowhead solve."""
+
+    def compute_time_schurdecomposition(self, time_quadrule):
+        Wt, Mt, nt = time_pencil(time_quadrule)
+        S, T, Q, Z = sla.qz(Wt, Mt, output="complex")  # time_legacy.py:42-48
+        self.schur_adv, self.schur_mass, self.schur_VSL, self.schur_VSR = S, T, Q, Z
+        self._eigenvec_time = Q
+        self._eigenblocks_time = []
+        self._nt = nt
+        self._advection_time_corrector = [1.0] * max(1, self.nbDoFsPerNode)
+        self._plans = {}
+
+    def plan(self, dtype: str) -> STFDPlan:
+        if self._nt == 0:
+            raise RuntimeError("Space-time preconditioner not defined.")
+        p = self._plans.get(dtype)
+        if p is None:
+            n, lo, ln = self.box()
+            if len(n) != 4:
+                raise NotImplementedError("the space-time FD apply is provided for three space directions")
+            p = STFDQZPlan(n, lo, ln, self.space_fd.eigenvec_by_dir_space, self.schur_VSL, self.schur_VSR, dtype)
+            self._plans[dtype] = p
+        lam = self.space_fd.space_eigenvalues[0]
+        adv = float(self._advection_time_corrector[0])
+        version = (self.space_fd.eig_version, adv)
+        if p.version != version:
+            p.set_qz(lam, self.schur_adv, self.schur_mass, adv, version)
+        return p
+
+
 class SpaceTimeOperator:
     """The space-time heat operator ``A x = u_v(c; time_ders=(0, 1)) x +
     gradu_gradv(C; time_ders=(0, 0)) x`` (``examples/performance_matvec_sptm.py:44-57``;

@@ -215,6 +215,12 @@ int iga_fused_create(iga_fused_t* out, const int32_t* nc, const int32_t* nq, int
                      const int32_t* const* first, const double* const* tabs, int form, int dtype,
                      int nslab);
 int iga_fused_destroy(iga_fused_t plan);
+/* Right-hand sides per work-group of the sweep when several share one coefficient set
+ * (an even nrhs; forms 1, 2 and 3): 2 reads each coefficient and runs each plane-phase
+ * barrier once for two of them (twice the registers per thread), 1 (default) processes
+ * them one by one.  The results agree to rounding (the compiler may contract the
+ * per-point sums differently in the two variants). */
+int iga_fused_set_rhs_block(iga_fused_t plan, int nr);
 
 /* Sizes: packed coefficient length (elements), partial-block workspace per
  * right-hand side (elements), tile and slab counts. */
@@ -453,6 +459,19 @@ int iga_stfd_set_blocks(iga_stfd_t plan, const double* lam_space, const double* 
 int64_t iga_stfd_workspace_bytes(iga_stfd_t plan, int nrhs);
 int iga_stfd_apply(iga_stfd_t plan, const void* x, void* y, int nrhs, int transpose, void* work, void* stream);
 int iga_stfd_destroy(iga_stfd_t plan);
+
+/* The legacy QZ form of the same apply (SpTimeFD of fastdiag/utils/time_legacy.py:16-171,
+ * chosen by SingleFD.compute_time_schurdecomposition(..., use_arrowhead=False),
+ * fastdiag/preconditioner.py:65-88): the complex QZ decomposition W_t = Q S Z^H,
+ * M_t = Q T Z^H of the time pencil; apply = (U_s^T (x) Q^H) transform, per spatial
+ * eigenvalue the upper-triangular solve with adv S + lambda T, (U_s (x) Z)
+ * back-transform, real part; transpose = Z^T, the transposed (lower) solve, conj(Q).
+ * q_*, z_* (and s_*, t_*) row-major len[3] x len[3]. */
+int iga_stfd_create_qz(iga_stfd_t* out, const int64_t* n, const int64_t* lo, const int64_t* len,
+                       const double* const* U, const double* q_re, const double* q_im, const double* z_re,
+                       const double* z_im, int dtype);
+int iga_stfd_set_qz(iga_stfd_t plan, const double* lam_space, const double* s_re, const double* s_im,
+                    const double* t_re, const double* t_im, double adv);
 
 #ifdef __cplusplus
 }

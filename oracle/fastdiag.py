@@ -13,12 +13,19 @@ complex ``eig`` of its leading block plus an M-normalised last column gives ``U_
 ``U_t^H W_t U_t = D_t`` arrowhead; apply = complex Kronecker transform, arrowhead solve
 per spatial eigenvalue through the Schur complement, back-transform, real part.
 
+Legacy QZ space-time FD (``fastdiag/utils/time_legacy.py:16-171``): the complex QZ
+decomposition ``W_t = Q S Z^H``, ``M_t = Q T Z^H`` of the same time pencil; apply =
+``(U_s^T (x) Q^H)`` transform, per spatial eigenvalue the upper-triangular solve with
+``adv S + lambda T``, ``(U_s (x) Z)`` back-transform, real part (transpose: ``Z^T``,
+the transposed triangular solve, ``conj(Q)``).
+
 Pinned by (tests/test_oracle_fastdiag.py): dense ``kron(U)`` reconstruction
 (``tests/test_tensor_operations.py:130-156``), eigenvalues against an independent
 ``eigh`` + Kronecker sum (``tests/test_fast_diagonalization.py:20-119``), the exact
 inverse of the separable stiffness on the Gauss cube (direct sparse solve), and for
 the arrowhead: ``U_t^H M_t U_t = I``, ``U_t^H W_t U_t = D_t`` and a dense solve of the
-assembled space-time system.
+assembled space-time system; for the QZ form the factorisation identities and the same
+exact inverse of the separable space-time operator (apply and transpose).
 """
 from __future__ import annotations
 
@@ -214,4 +221,52 @@ class SpaceTimeFD:
         t = mf.apply(self.space.eigvecs + [self.Ut], xin.astype(complex), is_transpose=True)
         t = solve_arrowhead(H, b, t, S)
         out[free] = np.real(mf.apply(self.space.eigvecs + [self.Ut.conj()], t))
+        return out
+
+
+class SpaceTimeQZFD(SpaceTimeFD):
+    """Legacy QZ space-time FD (``time_legacy.py:16-171``) for one DOF per node."""
+
+    def __init__(self, space: SpaceFD, gauss_time):
+        super().__init__(space, gauss_time)
+        Wt, Mt = time_pencil(gauss_time)
+        # time_legacy.py:42-48
+        self.S, self.T, self.Q, self.Z = sla.qz(Wt, Mt, output="complex")
+
+    def _solve(self, t: np.ndarray, transpose: bool) -> np.ndarray:
+        """Per spatial eigenvalue ``(adv S + lambda T) y = t`` (transposed: lower solve
+        with the transpose, ``time_legacy.py:97-101, 152-156``); ``t`` is
+        ``(n_t - 1, n_space)`` with time slowest."""
+        lam = self.space.eigenvalues()
+        out = np.empty_like(t)
+        for j, lj in enumerate(lam):
+            mat = self.adv_corr * self.S + lj * self.T
+            if transpose:
+                out[:, j] = sla.solve_triangular(mat.T, t[:, j], lower=True)
+            else:
+                out[:, j] = sla.solve_triangular(mat, t[:, j], lower=False)
+        return out
+
+    def apply(self, x: np.ndarray) -> np.ndarray:
+        """``time_legacy.py:62-108``."""
+        free = self.free_nodes()
+        out = np.zeros_like(x, dtype=float)
+        xin = x[free]
+        if not np.any(xin != 0.0):
+            return out
+        t = mf.apply(self.space.eigvecs + [self.Q.conj()], xin.astype(complex), is_transpose=True)
+        t = self._solve(t.reshape(self.nt - 1, -1), False).ravel()
+        out[free] = np.real(mf.apply(self.space.eigvecs + [self.Z], t))
+        return out
+
+    def apply_transpose(self, x: np.ndarray) -> np.ndarray:
+        """``time_legacy.py:110-171``."""
+        free = self.free_nodes()
+        out = np.zeros_like(x, dtype=float)
+        xin = x[free]
+        if not np.any(xin != 0.0):
+            return out
+        t = mf.apply(self.space.eigvecs + [self.Z], xin.astype(complex), is_transpose=True)
+        t = self._solve(t.reshape(self.nt - 1, -1), True).ravel()
+        out[free] = np.real(mf.apply(self.space.eigvecs + [self.Q.conj()], t))
         return out

@@ -294,3 +294,39 @@ def test_first_order_terms_in_one_sweep(kind, geom):
         xd = torch.as_tensor(X, device="cuda").to(torch.float64 if dtype == "f64" else torch.float32)
         y = op.apply(xd.reshape(-1).contiguous(), nrhs=2).reshape(n, 2).double().cpu().numpy()
         assert rel(y, ref) <= tol, (dtype, rel(y, ref))
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+@pytest.mark.parametrize("geom,mass", [("annulus", True), ("cube", True), ("cube", False)])
+def test_two_right_hand_sides_per_workgroup_same_bits(dtype, geom, mass):
+    """Right-hand sides sharing one coefficient set swept two per work-group (the
+    coefficient loads and plane phases shared) agree with one per work-group to
+    rounding (the compiler may contract the per-point sums differently), for the NURBS
+    (10 fields), B-spline K+M (7) and K (6) forms, and match the oracle."""
+    import torch
+
+    from igaops_amd.operators import StiffnessMassOperator
+
+    print("case", geom, mass, dtype, flush=True)
+    patch = unit_cube(3, 7) if geom == "cube" else quarter_annulus(4, 6)
+    qs = patch.quadrule_list
+    w = patch.nurbs_weights if patch.is_nurbs else None
+    C, m = patch.conductivity_coefficients(lognormal_conductivity(patch, 0.5, seed=4))
+    op = StiffnessMassOperator(qs, C=C, m=m if mass else None, nurbs_weights=w, dtype=dtype)
+    n = patch.nbctrlpts_total
+    tdt = torch.float64 if dtype == "f64" else torch.float32
+    X = torch.randn(n, 6, dtype=torch.float64, device="cuda").to(tdt)
+    outs = {}
+    for nr in (1, 2):
+        op.plan.set_rhs_block(nr)
+        outs[nr] = op.apply(X.reshape(-1).contiguous(), nrhs=6).reshape(n, 6)
+    op.plan.set_rhs_block(2)
+    d = float((outs[1] - outs[2]).abs().max() / outs[1].abs().max())
+    print("two-per-work-group difference", d, flush=True)
+    assert d <= (1e-14 if dtype == "f64" else 1e-6)
+    Xh = X.double().cpu().numpy()
+    for k in (0, 5):
+        ref = (oo.stiffness_plus_mass(qs, C, m, Xh[:, k], nurbs_weights=w) if mass
+               else (oo.nurbs_gradu_gradv(qs, C, Xh[:, k], nurbs_weights=w) if w is not None
+                     else oo.bspline_gradu_gradv(qs, C, Xh[:, k])))
+        assert rel(outs[2][:, k].double().cpu().numpy(), ref) <= (F64_TOL if dtype == "f64" else F32_TOL)

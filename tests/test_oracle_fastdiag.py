@@ -130,3 +130,44 @@ def test_spacetime_fd_inverts_separable_spacetime_operator():
     refT = np.zeros_like(v)
     refT[free] = np.linalg.solve(A[np.ix_(free, free)].T, v[free])
     np.testing.assert_allclose(st.apply_transpose(v), refT, rtol=1e-7, atol=1e-9 * np.abs(refT).max())
+
+
+def test_qz_spacetime_fd_inverts_separable_spacetime_operator():
+    """The legacy QZ form: ``Q S Z^H = W_t``, ``Q T Z^H = M_t`` with S, T upper
+    triangular and Q, Z unitary, and, with Gauss rules, the same exact inverse of the
+    separable space-time operator as the arrowhead form (apply and transpose).  A
+    planted error (the transposed solve without the transpose) fails."""
+    qs = gauss(2, 3)
+    qt = StandardGauss(2, make_open_knotvector(2, 4), "legendre")
+    sfd = ofd.SpaceFD([qs] * 3, np.ones((1, 3, 2), dtype=bool))
+    st = ofd.SpaceTimeQZFD(sfd, qt)
+    Wt, Mt = ofd.time_pencil(qt)
+    np.testing.assert_allclose(st.Q @ st.S @ st.Z.conj().T, Wt, atol=1e-12)
+    np.testing.assert_allclose(st.Q @ st.T @ st.Z.conj().T, Mt, atol=1e-12)
+    assert np.allclose(np.tril(st.S, -1), 0) and np.allclose(np.tril(st.T, -1), 0)
+    np.testing.assert_allclose(st.Q.conj().T @ st.Q, np.eye(len(Wt)), atol=1e-12)
+    M = (qs.weights[0] @ qs.basis[0]).toarray()
+    K = (qs.weights[-1] @ qs.basis[-1]).toarray()
+    Mtf = (qt.weights[0] @ qt.basis[0]).toarray()
+    Wtf = (qt.weights[1] @ qt.basis[1]).toarray()
+    kron3 = lambda a, b, c: np.kron(c, np.kron(b, a))
+    Ms = kron3(M, M, M)
+    Ks = kron3(K, M, M) + kron3(M, K, M) + kron3(M, M, K)
+    A = np.kron(Wtf, Ms) + np.kron(Mtf, Ks)
+    free = st.free_nodes()
+    v = np.random.default_rng(3).standard_normal(A.shape[0])
+    ref = np.zeros_like(v)
+    ref[free] = np.linalg.solve(A[np.ix_(free, free)], v[free])
+    np.testing.assert_allclose(st.apply(v), ref, rtol=1e-7, atol=1e-9 * np.abs(ref).max())
+    refT = np.zeros_like(v)
+    refT[free] = np.linalg.solve(A[np.ix_(free, free)].T, v[free])
+    np.testing.assert_allclose(st.apply_transpose(v), refT, rtol=1e-7, atol=1e-9 * np.abs(refT).max())
+    arrow = ofd.SpaceTimeFD(sfd, qt)
+    np.testing.assert_allclose(st.apply(v), arrow.apply(v), atol=1e-9 * np.abs(ref).max())
+
+    class Planted(ofd.SpaceTimeQZFD):
+        def _solve(self, t, transpose):
+            return super()._solve(t, False)
+
+    bad = Planted(sfd, qt)
+    assert not np.allclose(bad.apply_transpose(v), refT, rtol=1e-7, atol=1e-9 * np.abs(refT).max())

@@ -55,18 +55,18 @@ def test_free_nodes_match_oracle():
     assert list(lo) == [1, 1, 1, 1] and int(np.prod(ln)) == ost.free_nodes().size
 
 
-def _pair(tab, adv=1.0, coefs=(0.0, 1.0)):
+def _pair(tab, adv=1.0, coefs=(0.0, 1.0), arrowhead=True):
     from igaops_amd.fastdiag import SingleFD
 
     qs, qt = rules()
     fd = SingleFD("spacetime")
     fd.compute_space_eigendecomposition(qs, tab)
-    fd.compute_time_schurdecomposition(qt)
+    fd.compute_time_schurdecomposition(qt, use_arrowhead=arrowhead)
     fd.update_space_eigenvalues(list(coefs))
     fd.add_scalar_space_time_correctors(advection_corrector=[adv])
     sfd = ofd.SpaceFD([gauss_of(q) for q in qs], tab)
     sfd.scalar_coefs = coefs
-    ost = ofd.SpaceTimeFD(sfd, gauss_of(qt))
+    ost = (ofd.SpaceTimeFD if arrowhead else ofd.SpaceTimeQZFD)(sfd, gauss_of(qt))
     ost.adv_corr = adv
     n = int(np.prod([q.nbctrlpts for q in qs])) * qt.nbctrlpts
     return fd, ost, n
@@ -85,6 +85,34 @@ def test_spacetime_apply_matches_oracle(table, adv):
     x = np.random.default_rng(3).standard_normal(n)
     assert rel(fd.apply(x), ost.apply(x)) <= 1e-12
     assert rel(fd.apply_transpose(x), ost.apply_transpose(x)) <= 1e-12
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("table", ["all", "mixed"])
+@pytest.mark.parametrize("adv", [1.0, 0.7])
+def test_legacy_qz_spacetime_apply_matches_oracle(table, adv):
+    """``compute_time_schurdecomposition(..., use_arrowhead=False)``: the legacy QZ form
+    (complex QZ of the time pencil, a triangular solve per spatial eigenvalue) on the
+    device, apply and transpose, one and two right-hand sides, against the oracle's
+    restatement of it; and it agrees with the arrowhead form on the separable case."""
+    import torch
+
+    tab = np.ones((1, 3, 2), bool)
+    if table == "mixed":
+        tab[0, 1, 1] = False
+        tab[0, 2, 0] = False
+    fd, ost, n = _pair(tab, adv, (0.2, 1.0), arrowhead=False)
+    print("case", table, adv, n, flush=True)
+    X = np.random.default_rng(5).standard_normal((n, 2))
+    assert rel(fd.apply(X[:, 0]), ost.apply(X[:, 0])) <= 1e-12
+    assert rel(fd.apply_transpose(X[:, 0]), ost.apply_transpose(X[:, 0])) <= 1e-12
+    ref = np.stack([ost.apply(X[:, k]) for k in range(2)], axis=1)
+    assert rel(fd.apply(X), ref) <= 1e-12
+    y32 = fd.apply(torch.as_tensor(X[:, 1], device="cuda").float()).double().cpu().numpy()
+    assert rel(y32, ref[:, 1]) <= 1e-4
+    fa, oa, _ = _pair(tab, adv, (0.2, 1.0), arrowhead=True)
+    # both forms invert the same separable time pencil and spatial Kronecker sum
+    assert rel(fd.apply(X[:, 0]), fa.apply(X[:, 0])) <= 1e-9
 
 
 @pytest.mark.gpu
