@@ -594,10 +594,10 @@ __device__ __attribute__((always_inline)) inline void fused_apply_body(const Fus
 }
 
 // Two launch variants of the same sweep: with an occupancy target of three waves
-// per SIMD (used when the band is narrow enough to fit without spilling) and
-// without (wider FP64 bands, two waves per SIMD).
+// per SIMD for FP64 and four for FP32 (used when the band is narrow enough to fit
+// without spilling) and without (wider bands, one wave per SIMD fewer).
 template <typename T, int P, int FORM, int NR>
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8)))
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 3, 8)))
 fused_apply_kernel(FusedArgs<T> a) {
   fused_apply_body<T, P, FORM, NR>(a);
 }
@@ -995,9 +995,11 @@ static DirTiling tile_direction(const int32_t* first, int nq, int ncdof, int P, 
   return d;
 }
 
-// The three-wave variant is used unless the register budget it imposes makes the
-// compiler spill (private scratch), which costs more than the occupancy gains;
-// decided once per instance from the code object's attributes.
+// The occupancy-target variant (three waves per SIMD for FP64, four for FP32) is used
+// unless the register budget it imposes makes the compiler spill more than
+// kSpillTolerance bytes per lane (private scratch), which costs more than the
+// occupancy gains; decided once per instance from the code object's attributes.
+constexpr size_t kSpillTolerance = 16;
 template <typename T, int P, int FORM, int NR>
 static bool use_wide() {
   static std::atomic<int> wide{-1};
@@ -1006,7 +1008,7 @@ static bool use_wide() {
     hipFuncAttributes fa{};
     w = (hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&fused_apply_kernel<T, P, FORM, NR>)) ==
              hipSuccess &&
-         fa.localSizeBytes > 0)
+         fa.localSizeBytes > kSpillTolerance)
             ? 1
             : 0;
     wide.store(w, std::memory_order_relaxed);
