@@ -186,12 +186,11 @@ __device__ inline void st16(T* C, int ldc, typename PlaneMma<T>::V acc, int lane
 // shared coefficients): the coefficient loads, the band tables and the barriers of the
 // plane phases serve NR right-hand sides, each with its own register rings and LDS
 // planes.
-template <typename T, int P, int FORM, int NR>
+template <typename T, int P, int FORM, int NR, int PD>
 __device__ __attribute__((always_inline)) inline void fused_apply_body(const FusedArgs<T>& a) {
   using MM = PlaneMma<T>;
   using V = typename MM::V;
-  // prefetch depth (q3 planes) of the coefficient stream
-  constexpr int PD = sizeof(T) == 8 ? 2 : 3;
+  // PD: prefetch depth (q3 planes) of the coefficient stream
   constexpr bool DUAL = FORM == FORM_KM7X;
   static_assert(NR == 1 || (NR == 2 && !DUAL), "two right-hand sides per work-group for one-input forms only");
   constexpr int NT1 = DUAL ? 3 : 2;  // direction-1 trial partials: B0 x, B1 x (, B0 x1)
@@ -593,18 +592,25 @@ __device__ __attribute__((always_inline)) inline void fused_apply_body(const Fus
   }
 }
 
+// Coefficient prefetch depth (q3 planes) of the FP64 variant without an occupancy target,
+// which has registers to spare at two waves per SIMD.
+#ifndef IGA_FUSED_WIDE_PD_F64
+#define IGA_FUSED_WIDE_PD_F64 2
+#endif
+constexpr int kWidePrefetchF64 = IGA_FUSED_WIDE_PD_F64;
+
 // Two launch variants of the same sweep: with an occupancy target of three waves
 // per SIMD for FP64 and four for FP32 (used when the band is narrow enough to fit
 // without spilling) and without (wider bands, one wave per SIMD fewer).
 template <typename T, int P, int FORM, int NR>
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 3, 8)))
 fused_apply_kernel(FusedArgs<T> a) {
-  fused_apply_body<T, P, FORM, NR>(a);
+  fused_apply_body<T, P, FORM, NR, sizeof(T) == 8 ? 2 : 3>(a);
 }
 
 template <typename T, int P, int FORM, int NR>
 __global__ void __launch_bounds__(kThreads) fused_apply_kernel_wide(FusedArgs<T> a) {
-  fused_apply_body<T, P, FORM, NR>(a);
+  fused_apply_body<T, P, FORM, NR, sizeof(T) == 8 ? kWidePrefetchF64 : 3>(a);
 }
 
 // forms with a two-right-hand-side variant (the solver and benchmark forms)
