@@ -327,6 +327,7 @@ def serve_section(nreq=2000, qps=200.0, seed=0, max_active=16, budget_gb=200.0, 
 
     for mode in ("fp64", "mixed"):
         torch.cuda.empty_cache()  # each mode starts from an empty allocator cache
+        torch.cuda.reset_peak_memory_stats()
         srv = SolveServer(mode=mode, max_active=max_active, hbm_budget_gb=budget_gb)
         start = srv.prepare(range(2, 6), range(16, 97), workers=workers)
         t1 = time.perf_counter()
@@ -335,7 +336,8 @@ def serve_section(nreq=2000, qps=200.0, seed=0, max_active=16, budget_gb=200.0, 
         s = summarize(res, 1e-10, wall)
         s.update({"server_start": dict(start, note="all 324 (p, nel) of p 2..5, nel 16..96: rules and FD pencils "
                                                    "in a process pool (built once per process), device FD and "
-                                                   "operator plans of this mode"),
+                                                   "operator plans of this mode, per-shape device "
+                                                   "resources of both geometries"),
                   "max_concurrent": srv.stats["max_active"], "peak_admitted_gb": srv.stats["max_bytes"] / 1e9,
                   "deferred_for_memory": srv.stats["deferred_for_memory"], "shapes": len(srv.shapes),
                   "host_admit_s": srv.stats["admit_s"], "host_advance_s": srv.stats["advance_s"],

@@ -7,6 +7,7 @@ device every factor is a band: row ``i`` keeps ``width`` entries starting at col
 """
 from __future__ import annotations
 
+import weakref
 from typing import Sequence, Tuple
 
 import numpy as np
@@ -92,8 +93,24 @@ def same_matrix(a, b) -> bool:
     return d.nnz == 0 or float(np.max(np.abs(d.data))) == 0.0
 
 
+_ALPHA_ONLY = weakref.WeakKeyDictionary()
+
+
 def test_weights_depend_on_alpha_only(quadrule) -> bool:
     """True for rules with ``weights = [W0, W0, W1, W1]`` (Gauss and WQ type "2"),
-    the case handled by the fused kernel."""
+    the case handled by the fused kernel.  The answer is kept per rule object for as
+    long as the rule holds the same weight matrices (rules are not modified in place
+    after construction)."""
     w = quadrule.weights
-    return len(w) >= 4 and same_matrix(w[0], w[1]) and same_matrix(w[2], w[3])
+    try:
+        hit = _ALPHA_ONLY.get(quadrule)
+    except TypeError:  # not weakly referenceable
+        hit = None
+    if hit is not None and len(hit[0]) == len(w) and all(a is b for a, b in zip(hit[0], w)):
+        return hit[1]
+    ans = len(w) >= 4 and same_matrix(w[0], w[1]) and same_matrix(w[2], w[3])
+    try:
+        _ALPHA_ONLY[quadrule] = (tuple(w), ans)
+    except TypeError:
+        pass
+    return ans

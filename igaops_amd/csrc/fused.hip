@@ -729,6 +729,8 @@ __global__ void __launch_bounds__(256) fused_gather_kernel(GatherArgs<T> g) {
 //   m = det (with_mass) or 0.
 // plane holds det2, inv_00, inv_11, inv_01 (4 x n_plane); H (n_plane x modes) and
 // Ez (n3 x modes) are complex (re, im interleaved); H == nullptr means kappa = 1.
+// With this source, W (n_plane) and dW (2 x n_plane: d/dxi_1, d/dxi_2) are fields of the
+// cross-section.
 struct ExtrudedSource {
   const double* plane = nullptr;
   const double* dz = nullptr;
@@ -770,9 +772,18 @@ __global__ void fused_pack_kernel(PackArgs<T> p) {
     double D[4][4] = {{0}};
     double c[3][3] = {{0}};
     double mm = 0.0;
+    // NURBS weight function and its gradient at this point (rational: p.W set)
+    double wq = 1.0, dwq[3] = {0.0, 0.0, 0.0};
     if (p.ex.plane) {
       const int64_t nplane = (int64_t)p.nq[0] * p.nq[1];
       const int64_t pl = q1 + (int64_t)p.nq[0] * q2;
+      if (p.W) {
+        // cross-section fields: the control weights repeat along xi_3, so W does not
+        // depend on xi_3 and its xi_3 derivative is zero
+        wq = p.W[pl];
+        dwq[0] = p.dW[pl];
+        dwq[1] = p.dW[nplane + pl];
+      }
       const double dz = p.ex.dz[q3];
       const double det = dz * p.ex.plane[pl];
       double kap = 1.0;
@@ -812,11 +823,15 @@ __global__ void fused_pack_kernel(PackArgs<T> p) {
         for (int i = 0; i < 3; ++i)
           for (int j = 0; j < 3; ++j) c[i][j] = p.C[(int64_t)(i * 3 + j) * p.comp_stride + q];
       mm = p.m ? p.m[q] : 0.0;
+      if (p.W) {
+        wq = p.W[q];
+        for (int i = 0; i < 3; ++i) dwq[i] = p.dW[(int64_t)i * p.comp_stride + q];
+      }
     }
     if (p.W) {
-      const double wi = 1.0 / p.W[q];
+      const double wi = 1.0 / wq;
       double z[3];
-      for (int i = 0; i < 3; ++i) z[i] = p.dW[(int64_t)i * p.comp_stride + q] * wi * wi;
+      for (int i = 0; i < 3; ++i) z[i] = dwq[i] * wi * wi;
       double cz[3] = {0, 0, 0}, ctz[3] = {0, 0, 0}, zcz = 0.0;
       for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) {

@@ -49,6 +49,8 @@ class PlaneGeometry:
 
     x: np.ndarray  # (2, nq2, nq1) physical coordinates
     jac: np.ndarray  # (2, 2, nq2, nq1): jac[i, j] = d x_i / d xi_j
+    # (3, nq2, nq1): the NURBS weight function W and dW/dxi_1, dW/dxi_2 (None for B-splines)
+    w: Optional[np.ndarray] = None
 
 
 class ExtrudedPatch:
@@ -138,7 +140,11 @@ class ExtrudedPatch:
                 jac[i, j] = (dX[j] - X * dW[j]) / W
         x.flags.writeable = False
         jac.flags.writeable = False
-        self._plane = PlaneGeometry(x=x, jac=jac)
+        wf = None
+        if self.weights2d is not None:
+            wf = np.stack([W, dW[0], dW[1]])
+            wf.flags.writeable = False
+        self._plane = PlaneGeometry(x=x, jac=jac, w=wf)
         return self._plane
 
     def line_geometry(self) -> Tuple[np.ndarray, np.ndarray]:
@@ -252,16 +258,22 @@ class ExtrudedPatch:
 
     def device_extrusion(self):
         """``extrusion_factors`` on the device (float64), computed once per patch:
-        ``plane (4 n_plane)``, ``dz``, ``xy (2 n_plane)``, and the host ``z``."""
+        ``plane (4 n_plane)``, ``dz``, ``xy (2 n_plane)``, the host ``z``, and for a
+        NURBS cross-section ``nurbs (3 n_plane)`` = ``W``, ``dW/dxi_1``, ``dW/dxi_2`` on
+        the cross-section grid (the weights repeat along ``xi_3``, so these are the
+        weight function of the whole patch; ``None`` for B-splines)."""
         cache = getattr(self, "_dev_extrusion", None)
         if cache is None:
             from . import device as dv
 
             plane, dz, xy, zh = self.extrusion_factors()
-            allv = dv.upload(np.concatenate([plane.ravel(), dz, xy.ravel()]))
+            wf = self.plane_geometry().w
+            parts = [plane.ravel(), dz, xy.ravel()] + ([wf.ravel()] if wf is not None else [])
+            allv = dv.upload(np.concatenate(parts))
             npl = plane.shape[1]
-            cache = {"plane": allv[:4 * npl], "dz": allv[4 * npl:4 * npl + dz.size],
-                     "xy": allv[4 * npl + dz.size:], "z": zh, "nplane": npl}
+            o = 4 * npl + dz.size
+            cache = {"plane": allv[:4 * npl], "dz": allv[4 * npl:o], "xy": allv[o:o + 2 * npl], "z": zh,
+                     "nplane": npl, "nurbs": allv[o + 2 * npl:] if wf is not None else None}
             self._dev_extrusion = cache
         return cache
 

@@ -390,7 +390,8 @@ class FusedPlan:
         """Packed device coefficients of an extruded patch generated point by point
         (``iga_fused_pack_extruded``): ``ext`` from ``ExtrudedPatch.device_extrusion``,
         ``field`` from ``ExtrudedPatch.device_field_modes`` (``None``: ``kappa = 1``),
-        ``W`` and ``dW`` (3 N_q, concatenated) the NURBS weight fields."""
+        ``W`` (n_plane) and ``dW`` (2 n_plane, concatenated) the NURBS weight function
+        and its in-plane gradient on the cross-section grid."""
         out = dv.empty(self.coef_elems, self.dtype)
         H = Ez = None
         modes, sigma = 0, 0.0
@@ -541,18 +542,18 @@ class StiffnessMassOperator:
         self._work = {}
 
     @classmethod
-    def from_extruded(cls, patch, field=None, dtype: str = "f64", mass: bool = False, nurbs_fields=None,
-                      nurbs_w64=None, nurbs_w=None, stream=None):
+    def from_extruded(cls, patch, field=None, dtype: str = "f64", mass: bool = False, nurbs_w64=None, nurbs_w=None,
+                      stream=None):
         """``K (+ M)`` of an extruded patch (``geometry.ExtrudedPatch``) with the
         conductivity ``kappa detJ J^+ J^+T``, ``kappa`` the lognormal field of
         ``patch.device_field_modes`` (or 1 when ``field`` is None): the same operator as
         ``StiffnessMassOperator(qs, *patch.device_coefficients(...), nurbs_weights=...)``,
         with the coefficients generated and packed in one launch instead of going
-        through a (3, 3, N_q) array.  ``nurbs_fields`` = ``(W, dW)``, ``dW`` the three
-        gradient components concatenated, ``nurbs_w64`` (the control-point weights on
-        the device, float64) and ``nurbs_w`` (the same in ``dtype``, shared by the
-        operators of one shape) may be passed when cached by the caller."""
-        t = dv.torch()
+        through a (3, 3, N_q) array.  For a NURBS cross-section the weight function and
+        its gradient come from the cross-section grid (``patch.device_extrusion``), not
+        from N_q-sized fields.  ``nurbs_w64`` (the control-point weights on the device,
+        float64) and ``nurbs_w`` (the same in ``dtype``, shared by the operators of one
+        shape) may be passed when cached by the caller."""
         dv.require_gpu()
         self = cls.__new__(cls)
         self.quadrule_list = list(patch.quadrule_list)
@@ -574,13 +575,11 @@ class StiffnessMassOperator:
         else:
             self.form = FORM_KM7 if mass else FORM_K6
         self.plan = fused_plan(self.quadrule_list, self.form, dtype, 0)
+        ext = patch.device_extrusion()
         W = dW = None
         if self.is_nurbs:
-            if nurbs_fields is None:
-                Wf, dWf = nurbs_weight_fields(self.quadrule_list, self.w64)
-                nurbs_fields = (Wf, t.cat(dWf))
-            W, dW = nurbs_fields
-        ext = patch.device_extrusion()
+            npl = ext["nplane"]
+            W, dW = ext["nurbs"][:npl], ext["nurbs"][npl:]
         self.packed = self.plan.pack_extruded(ext, field, mass, W, dW, stream=stream)
         self.coef_count = FORM_NCOEF[self.form]
         self._work = {}

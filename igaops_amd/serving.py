@@ -194,13 +194,9 @@ class ShapeResources:
         # the weights in each operator precision, shared by the requests of this shape
         # (a grouped operator launch takes one weight vector for all its members)
         self.w_by_dtype = {"f64": self.w64, "f32": None if self.w64 is None else self.w64.float()}
-        self.nurbs_fields = None
-        if self.w64 is not None:
-            from .operators import nurbs_weight_fields
-
-            Wf, dWf = nurbs_weight_fields(self.qs, self.w64)
-            self.nurbs_fields = (Wf, t.cat(dWf))  # W and the gradient components, concatenated
-        self.patch.device_extrusion()  # cross-section and extrusion factors, on the device
+        # cross-section and extrusion factors (and the NURBS weight function on the
+        # cross-section grid), on the device
+        self.patch.device_extrusion()
         # requests read these from their own streams, which wait for this event
         self.ready = t.cuda.Event()
         self.ready.record(t.cuda.current_stream())
@@ -268,7 +264,7 @@ class _Active:
             # at the points and packed in one launch per precision
             field = shape.patch.device_field_modes(req.sigma, req.seed)
             lap("coefficients")
-            kw = dict(nurbs_fields=shape.nurbs_fields, nurbs_w64=shape.w64)
+            kw = dict(nurbs_w64=shape.w64)
             self.A64 = StiffnessMassOperator.from_extruded(shape.patch, field, "f64", nurbs_w=shape.w_by_dtype["f64"],
                                                            **kw)
             self.A32 = StiffnessMassOperator.from_extruded(shape.patch, field, "f32", nurbs_w=shape.w_by_dtype["f32"],
@@ -378,14 +374,16 @@ class SolveServer:
     def dtypes(self):
         return ("f64", "f32") if self.mode == "mixed" else ("f64",)
 
-    def prepare(self, degrees, nels, workers: int = 8) -> dict:
+    def prepare(self, degrees, nels, workers: int = 8, shapes: bool = True) -> dict:
         """Server start: build what depends on ``(degree, nel)`` only, for every size
         the server accepts, so that no request waits for it: the rules and FD pencils
         (host, in a process pool), then the device FD plans and the fused-operator
         plans of both geometries' coefficient forms in the mode's precisions (one-off
         uploads that would otherwise wait behind other requests' work).  Sizes outside
         the declared ranges are still served; their set-up is then part of the first
-        such request's latency."""
+        such request's latency.  With ``shapes`` the per-shape device resources of both
+        geometries are built too (patch, device extrusion factors and NURBS weights:
+        a few MB per shape); their time is reported as ``shape_seconds``."""
         from .operators import FORM_K6, FORM_SYM10, fused_plan
 
         # the device is opened before the worker processes start (they never use it)
@@ -405,6 +403,16 @@ class SolveServer:
         dv.torch().cuda.synchronize()
         info["device_plans_seconds"] = time.perf_counter() - t0
         info["seconds"] += info["device_plans_seconds"]
+        if shapes:
+            t0 = time.perf_counter()
+            for p in degrees:
+                for nel in nels:
+                    for geo in ("cube", "annulus"):
+                        self.shape((geo, int(p), int(nel)))
+            dv.torch().cuda.synchronize()
+            info["shape_seconds"] = time.perf_counter() - t0
+            info["seconds"] += info["shape_seconds"]
+            self.stats["shape_setup_s"] = 0.0  # counts set-up during serving only
         self.stats["start"] = info
         return info
 

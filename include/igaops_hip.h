@@ -319,7 +319,10 @@ int iga_fused_pack_strided(iga_fused_t plan, const double* C, const double* m, c
  *   m = det when with_mass, else 0.
  * plane (device float64, 4 x n_plane): det2, inv_11, inv_22, inv_12 of the cross-section
  * (inverse of J_2^T J_2); dz (n3): dz/dxi_3.  H (n_plane x modes) and Ez (n3 x modes)
- * are complex (re, im interleaved); H = NULL gives kappa = 1.  W / dW as in
+ * are complex (re, im interleaved); H = NULL gives kappa = 1.  NURBS (W, dW not NULL):
+ * the control weights are those of the cross-section repeated along xi_3, so the
+ * weight function is a cross-section field: W (n_plane) and dW (2 x n_plane, its
+ * xi_1 and xi_2 derivatives; the xi_3 derivative is zero), folded as in
  * iga_fused_pack. */
 int iga_fused_pack_extruded(iga_fused_t plan, const double* plane, const double* dz, const double* H,
                             const double* Ez, int modes, double sigma, int with_mass, const double* W,
