@@ -148,3 +148,37 @@ def test_lognormal_field_statistics():
     np.testing.assert_array_equal(k1, k2)
     g = np.log(k1) / 0.5
     assert np.all(k1 > 0) and 0.3 < g.std() < 2.0
+
+
+def test_cross_section_nurbs_weight_function_closed_form():
+    """The quarter annulus's NURBS weight function on the cross-section grid (what the
+    generated coefficients fold in): W(xi_2) of the quadratic rational arc, whatever the
+    degree elevation and refinement, constant along the radial xi_1; the xi_3 weights are
+    the cross-section's repeated, so these are the weight function of the whole patch."""
+    patch = quarter_annulus(4, 6, quadclass="gauss", quadtype="legendre")
+    t = patch.quadrule_list[1].quadpts
+    wm = np.cos(np.pi / 4)
+    W = (1 - t) ** 2 + 2 * t * (1 - t) * wm + t ** 2
+    dW = -2 * (1 - t) + 2 * (1 - 2 * t) * wm + 2 * t
+    w = patch.plane_geometry().w
+    assert w.shape == (3,) + patch.plane_geometry().x.shape[1:]
+    np.testing.assert_allclose(w[0], np.broadcast_to(W[:, None], w[0].shape), rtol=1e-13)
+    np.testing.assert_allclose(w[1], 0.0, atol=1e-12)
+    np.testing.assert_allclose(w[2], np.broadcast_to(dW[:, None], w[2].shape), rtol=1e-12, atol=1e-13)
+    n1, n2, n3 = patch.nbctrlpts
+    np.testing.assert_array_equal(patch.nurbs_weights.reshape(n3, n1 * n2), np.tile(patch.weights2d, (n3, 1)))
+    assert unit_cube(2, 3).plane_geometry().w is None
+
+
+def test_alpha_only_rule_answer_follows_the_weights():
+    """The fused-operator rule test is kept per rule and redone when the rule's weight
+    matrices are replaced (a rebuild of the rule sets new ones)."""
+    from igaops_amd.bands import test_weights_depend_on_alpha_only as alpha_only
+
+    q = WeightedQuadrature(3, make_open_knotvector(3, 6), quadtype="2")
+    assert alpha_only(q) and alpha_only(q)
+    w = list(q.weights)
+    q._weights = [w[0], 2.0 * w[1], w[2], w[3]]  # as a rebuild of the rule would
+    assert not alpha_only(q)
+    q._weights = w
+    assert alpha_only(q)
