@@ -570,6 +570,10 @@ class GmresRun:
             return self
         return None
 
+    def stepping(self) -> bool:
+        """Still taking Arnoldi steps (started, stopping step not yet known)."""
+        return self.started and self.result is None and self.kfinal is None
+
     @property
     def done(self) -> bool:
         return self.result is not None
@@ -662,6 +666,10 @@ class RefinementRun:
                 and self.inner.can_launch():
             return self.inner
         return None
+
+    def stepping(self) -> bool:
+        """An inner solve is taking Arnoldi steps."""
+        return self.phase == "inner" and self.inner is not None and self.inner.stepping()
 
     def advance(self, launch: bool = True) -> bool:
         if self.result is not None:
@@ -767,7 +775,7 @@ def launch_group(group, work: dict, batch_stream=None):
         r._launched_native(k)
 
 
-def run_batched(runs, max_group: int = 16, batch_stream=None):
+def run_batched(runs, max_group: int = 16, batch_stream=None, lockstep: bool = True):
     """Drive several solves of one shape to completion with their Arnoldi steps
     batched: ``runs`` are ``GmresRun`` or ``RefinementRun`` objects whose one-call
     steps (``native`` / ``native_lo``) share the FD plan and the mask box and whose
@@ -776,8 +784,13 @@ def run_batched(runs, max_group: int = 16, batch_stream=None):
     ``max_group`` per launch): the FD and operator launches go on ``batch_stream`` (the
     runs' stream when they all share one), each run's orthogonalisation on its own
     stream, so that the small kernels of different runs overlap when the runs have
-    streams of their own.  Everything else (starts, refinement residuals, solution
-    updates) stays per run.  Returns the results in order and the host clock
+    streams of their own.  With ``lockstep`` a group step goes out only when every
+    run that is still taking Arnoldi steps can take one, so that each FD and operator
+    launch serves all of them (the runs' records arrive at slightly different times,
+    and launching whatever is ready splits the steps into groups of one or two); no
+    run waits for anything but the device, since a run that cannot step is waiting for
+    its own queued steps' records.  Everything else (starts, refinement residuals,
+    solution updates) stays per run.  Returns the results in order and the host clock
     (``time.perf_counter``) at which each completed."""
     import time
 
@@ -811,6 +824,8 @@ def run_batched(runs, max_group: int = 16, batch_stream=None):
                 if runs[i].done:
                     done_at[i] = time.perf_counter()
             ready = [s for s in (runs[i].step_run() for i in pending if not runs[i].done) if s is not None]
+            if lockstep and len(ready) < sum(1 for i in pending if runs[i].stepping()):
+                ready = []
             for j in range(0, len(ready), max_group):
                 launch_group(ready[j:j + max_group], work, stream)
                 progressed = True

@@ -303,17 +303,19 @@ def test_one_call_arnoldi_step_inside_refinement():
     assert float(torch.linalg.vector_norm(r) / torch.linalg.vector_norm(b)) <= 1e-10
 
 
+@pytest.mark.parametrize("lockstep", [True, False])
 @pytest.mark.parametrize("mode", ["fp64", "mixed"])
-def test_batched_steps_of_several_right_hand_sides_match_single_runs(mode):
+def test_batched_steps_of_several_right_hand_sides_match_single_runs(mode, lockstep):
     """Five right-hand sides of one operator driven together (``run_batched``: the FD
     of the ready runs in one multi-right-hand-side launch, their operators in one
-    grouped launch, then each run's orthogonalisation): the same iteration counts and
-    solution bits as each solve driven alone, and a true residual of 1e-10."""
+    grouped launch, then each run's orthogonalisation; in lock step or as they become
+    ready): the same iteration counts and solution bits as each solve driven alone, and
+    a true residual of 1e-10."""
     import torch
 
     from igaops_amd.runs import GmresRun, NativeStep, RefinementRun, run_batched
 
-    print("case", mode, flush=True)
+    print("case", mode, lockstep, flush=True)
     patch, C, m, fd, mask, ops = _annulus_fd_problem(4, 8)
     n = patch.nbctrlpts_total
     dt = "f64" if mode == "fp64" else "f32"
@@ -334,7 +336,7 @@ def test_batched_steps_of_several_right_hand_sides_match_single_runs(mode):
                              native_lo=NativeStep(ops["f32"], plan, work, mask.box))
 
     alone = [make(b).run() for b in B]
-    together, done_at = run_batched([make(b) for b in B])
+    together, done_at = run_batched([make(b) for b in B], lockstep=lockstep)
     assert all(d is not None for d in done_at)
     for b, a, t in zip(B, alone, together):
         assert a.success and t.success
