@@ -177,6 +177,7 @@ int iga_arnoldi_set_increment(iga_arnoldi_t c, void* Z, int64_t ldz, double* gra
 namespace {
 
 constexpr int kStepGroupMax = 16;
+static_assert(kStepGroupMax <= kGmGroupMax, "a group step's orthogonalisations fit one grouped launch");
 
 template <typename T>
 struct StackArgs {
@@ -272,12 +273,31 @@ int iga_arnoldi_step_group(int g, const iga_arnoldi_t* ctxs, const int* ks, void
     }
     IGA_CHECK(iga_fused_apply_group(c0->op, g, packed, Y, 1, n, outs, c0->nurbs_w, op_work, c0->masked ? 3 : 0,
                                     c0->box_lo, c0->box_len, batch_stream));
+    // the members' orthogonalisations: one launch per kernel of the step for all of
+    // them on the batch stream, each member's stream ordered after it; members whose
+    // basis layouts differ take their own steps on their own streams instead
+    GmGroupMember mem[kStepGroupMax];
+    for (int i = 0; i < g; ++i) {
+      const iga_arnoldi_s* c = ctxs[i];
+      mem[i].k = ks[i];
+      mem[i].m = c->m;
+      mem[i].V = Vs[i];
+      mem[i].ldv = ldvs[i];
+      mem[i].w = c->w;
+      mem[i].ls = c->ls;
+      mem[i].work = c->work;
+      mem[i].rec = c->rec;
+      mem[i].tag = c->tag;
+    }
+    const bool grouped = gmres_step_group_impl(c0->dtype, n, g, mem, batch_stream);
     IGA_HIP(hipEventRecord(ev[g], bs));
     for (int i = 0; i < g; ++i) {
       const iga_arnoldi_s* c = ctxs[i];
       hipStream_t si = as_stream(streams[i]);
       if (si != bs) IGA_HIP(hipStreamWaitEvent(si, ev[g], 0));
-      gmres_step_impl(c->dtype, c->n, ks[i], c->m, Vs[i], ldvs[i], c->w, c->ls, c->work, c->rec, c->tag, streams[i]);
+      if (!grouped)
+        gmres_step_impl(c->dtype, c->n, ks[i], c->m, Vs[i], ldvs[i], c->w, c->ls, c->work, c->rec, c->tag,
+                        streams[i]);
     }
   });
 }

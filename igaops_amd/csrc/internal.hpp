@@ -27,6 +27,24 @@ void gmres_start_impl(int dtype, int64_t n, int m, const void* r, void* V0, doub
 void gmres_step_impl(int dtype, int64_t n, int k, int m, void* V, int64_t ldv, void* w, double* ls, void* work,
                      double* rec, int64_t tag, void* stream, const GmIncrement* inc = nullptr);
 
+// One GMRES orthogonalisation step (gmres_step_impl without the increment record) of
+// each of up to kGmGroupMax solves of one vector type and length, as one launch per
+// kernel of the step for all of them; the results are those of gmres_step_impl bit for
+// bit.  Returns false, launching nothing, when the solves' basis layouts differ (the
+// vector path of the many-dot pass must be the same for all).
+constexpr int kGmGroupMax = 16;
+struct GmGroupMember {
+  int k = 0, m = 0;
+  void* V = nullptr;
+  int64_t ldv = 0;
+  void* w = nullptr;
+  double* ls = nullptr;
+  void* work = nullptr;
+  double* rec = nullptr;
+  int64_t tag = 0;
+};
+bool gmres_step_group_impl(int dtype, int64_t n, int g, const GmGroupMember* mem, void* stream);
+
 // iga_dot_many (krylov.hip): out[i] = V[i] . x for i < k
 void dot_many_impl(int dtype, int64_t n, int k, const void* V, int64_t ldv, const void* x, double* out, void* work,
                    void* stream);

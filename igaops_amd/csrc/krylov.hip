@@ -121,15 +121,12 @@ __device__ inline double wave_sum_t(double (&acc)[N], int lane) {
 // (work-groups go round-robin to the eight XCDs) one after the other and read
 // the chunk of x0, x1 from that XCD's L2.
 template <typename T, bool VECTOR, int NX>
-__global__ void __launch_bounds__(kRedThreads) __attribute__((amdgpu_waves_per_eu(1, 4))) dot_many_stage1(int64_t n, int k, int nb, int groups,
-                                                               const T* __restrict__ V, int64_t ldv,
-                                                               const T* __restrict__ x0,
-                                                               const T* __restrict__ x1,
-                                                               double* __restrict__ partial) {
+__device__ __forceinline__ void dot_many_stage1_body(int L, int64_t n, int k, int nb, int groups,
+                                                     const T* __restrict__ V, int64_t ldv, const T* __restrict__ x0,
+                                                     const T* __restrict__ x1, double* __restrict__ partial) {
   constexpr int NA = kDotAcc;  // accumulators per thread
   constexpr int G = NA / NX;    // rows per group
   __shared__ double red[kRedThreads / 64][NA];
-  const int L = blockIdx.x;
   const int bx = (L / 8 / groups) * 8 + L % 8;
   if (bx >= nb) return;
   const int by = (L / 8) % groups;
@@ -197,17 +194,29 @@ __global__ void __launch_bounds__(kRedThreads) __attribute__((amdgpu_waves_per_e
   }
 }
 
+template <typename T, bool VECTOR, int NX>
+__global__ void __launch_bounds__(kRedThreads) __attribute__((amdgpu_waves_per_eu(1, 4)))
+dot_many_stage1(int64_t n, int k, int nb, int groups, const T* __restrict__ V, int64_t ldv, const T* __restrict__ x0,
+                const T* __restrict__ x1, double* __restrict__ partial) {
+  dot_many_stage1_body<T, VECTOR, NX>(blockIdx.x, n, k, nb, groups, V, ldv, x0, x1, partial);
+}
+
 // out[x * ostride + j] = sum over b of partial[(x * k + j) * nb + b]: one wave per output
-__global__ void __launch_bounds__(kRedThreads) dot_many_stage2(int nb, int k, int nx,
-                                                               const double* __restrict__ partial,
-                                                               double* __restrict__ out, int64_t ostride) {
-  const int o = blockIdx.x * (kRedThreads / 64) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+__device__ __forceinline__ void dot_many_stage2_body(int blk, int nb, int k, int nx, const double* __restrict__ partial,
+                                                     double* __restrict__ out, int64_t ostride) {
+  const int o = blk * (kRedThreads / 64) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (o >= nx * k) return;
   const double* po = partial + (int64_t)o * nb;
   double s = 0.0;
   for (int b = lane; b < nb; b += 64) s += po[b];
   s = wave_sum(s);
   if (lane == 0) out[(int64_t)(o / k) * ostride + o % k] = s;
+}
+
+__global__ void __launch_bounds__(kRedThreads) dot_many_stage2(int nb, int k, int nx,
+                                                               const double* __restrict__ partial,
+                                                               double* __restrict__ out, int64_t ostride) {
+  dot_many_stage2_body(blockIdx.x, nb, k, nx, partial, out, ostride);
 }
 
 // Both stages; nx = 1 (x1 unused) or 2.
@@ -387,9 +396,9 @@ constexpr int kGmHsFrom = 24;
 // y_i = sum_l H[i][l] s_l for i < j with column j - 1 taken final (H[i][j-1] + s_i),
 // s = U[0..j); the Hessenberg rows start at column i - 1.  Row j (rho s_{j-1}) is
 // left to the small dense kernel.  One wave per row.
-__global__ void __launch_bounds__(256) gm_hs_kernel(GmState st, int j, double* ls) {
+__device__ __forceinline__ void gm_hs_body(int blk, GmState st, int j, double* ls) {
   const int lane = threadIdx.x & 63;
-  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int i = blk * 4 + (threadIdx.x >> 6);
   if (i >= j) return;
   const double* H = ls + st.H();
   const double* s = ls + st.U();
@@ -403,6 +412,8 @@ __global__ void __launch_bounds__(256) gm_hs_kernel(GmState st, int j, double* l
   acc = wave_sum(acc);
   if (lane == 0) ls[st.Y() + i] = acc;
 }
+
+__global__ void __launch_bounds__(256) gm_hs_kernel(GmState st, int j, double* ls) { gm_hs_body(blockIdx.x, st, j, ls); }
 
 __global__ void gm_zero_kernel(int64_t n, double* __restrict__ p) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -444,9 +455,9 @@ __global__ void gm_start_kernel(GmState st, double* ls, double* rec, int64_t tag
 // (clamped indices), so that they arrive in one memory round trip.
 constexpr int kGmPer = (kGmMaxDim + kGmThreads - 1) / kGmThreads;  // indices l per thread
 
-__global__ void __launch_bounds__(kGmThreads) __attribute__((amdgpu_waves_per_eu(1, 1)))
-gm_dcgs2_kernel(GmState st, int j, double* ls, double* rec, int64_t tag, GmIncrement inc) {
-  extern __shared__ double sh[];  // s[j] | col[j+1] | cr[j] | sr[j]   (y[j+1] reuses col)
+// sh: s[j] | col[j+1] | cr[j] | sr[j]   (y[j+1] reuses col), dynamic LDS of the kernel
+__device__ __forceinline__ void gm_dcgs2_body(GmState st, int j, double* ls, double* rec, int64_t tag,
+                                              const GmIncrement& inc, double* sh) {
   __shared__ double red[2][kGmThreads / 64];
   __shared__ double pub[3];  // residual estimate, rho, final g[j - 1]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -638,12 +649,19 @@ gm_dcgs2_kernel(GmState st, int j, double* ls, double* rec, int64_t tag, GmIncre
   }
 }
 
+__global__ void __launch_bounds__(kGmThreads) __attribute__((amdgpu_waves_per_eu(1, 1)))
+gm_dcgs2_kernel(GmState st, int j, double* ls, double* rec, int64_t tag, GmIncrement inc) {
+  extern __shared__ double gm_sh[];
+  gm_dcgs2_body(st, j, ls, rec, tag, inc, gm_sh);
+}
+
 // q_j = inv u_j - Q cq -> V[j];  u_{j+1} = inv w - zinv u_j - Q cw -> V[j+1]
 template <typename T>
-__global__ void gm_update_kernel(int64_t n, int j, T* __restrict__ V, int64_t ldv, const T* __restrict__ w,
-                                 const double* __restrict__ cq, const double* __restrict__ cw,
-                                 const double* __restrict__ scal) {
-  extern __shared__ double hs[];  // cq[j], cw[j]
+__device__ __forceinline__ void gm_update_body(int64_t n, int j, T* __restrict__ V, int64_t ldv,
+                                               const T* __restrict__ w, const double* __restrict__ cq,
+                                               const double* __restrict__ cw, const double* __restrict__ scal,
+                                               double* hs) {
+  // hs: cq[j], cw[j] (dynamic LDS of the kernel)
   for (int l = threadIdx.x; l < j; l += blockDim.x) {
     hs[l] = cq[l];
     hs[j + l] = cw[l];
@@ -664,6 +682,14 @@ __global__ void gm_update_kernel(int64_t n, int j, T* __restrict__ V, int64_t ld
     Vj[i] = (T)(inv * u - aq);
     Vn[i] = (T)(inv * (double)w[i] - zinv * u - aw);
   }
+}
+
+template <typename T>
+__global__ void gm_update_kernel(int64_t n, int j, T* __restrict__ V, int64_t ldv, const T* __restrict__ w,
+                                 const double* __restrict__ cq, const double* __restrict__ cw,
+                                 const double* __restrict__ scal) {
+  extern __shared__ double gm_hs[];
+  gm_update_body<T>(n, j, V, ldv, w, cq, cw, scal, gm_hs);
 }
 
 }  // namespace iga
@@ -872,6 +898,158 @@ void gmres_step_impl(int dtype, int64_t n, int k, int m, void* V, int64_t ldv, v
                                                        ls + st.cw(), ls + st.scal());
     IGA_LAUNCH_CHECK();
   }
+}
+
+// Grouped launches of the step's kernels: member blockIdx.y (blockIdx.x for the small
+// dense kernel), each with its own arguments; blocks past a member's own grid exit.
+template <typename T>
+struct Dot1Member {
+  int k, nb, groups;
+  const T* V;
+  int64_t ldv;
+  const T* x0;
+  const T* x1;
+  double* partial;
+};
+template <typename T>
+struct Dot1Group {
+  Dot1Member<T> m[kGmGroupMax];
+};
+template <typename T, bool VECTOR>
+__global__ void __launch_bounds__(kRedThreads) __attribute__((amdgpu_waves_per_eu(1, 4)))
+dot_many_stage1_group(int64_t n, Dot1Group<T> a) {
+  const Dot1Member<T>& d = a.m[blockIdx.y];
+  dot_many_stage1_body<T, VECTOR, 2>(blockIdx.x, n, d.k, d.nb, d.groups, d.V, d.ldv, d.x0, d.x1, d.partial);
+}
+
+struct Dot2Member {
+  int nb, k;
+  const double* partial;
+  double* out;
+  int64_t ostride;
+};
+struct Dot2Group {
+  Dot2Member m[kGmGroupMax];
+};
+__global__ void __launch_bounds__(kRedThreads) dot_many_stage2_group(Dot2Group a) {
+  const Dot2Member& d = a.m[blockIdx.y];
+  dot_many_stage2_body(blockIdx.x, d.nb, d.k, 2, d.partial, d.out, d.ostride);
+}
+
+struct GmMember {
+  GmState st;
+  int j;
+  double* ls;
+  double* rec;
+  int64_t tag;
+};
+struct GmGroup {
+  GmMember m[kGmGroupMax];
+};
+// members below kGmHsFrom carry j = 0 here: their blocks exit at once
+__global__ void __launch_bounds__(256) gm_hs_group(GmGroup a) {
+  const GmMember& d = a.m[blockIdx.y];
+  gm_hs_body(blockIdx.x, d.st, d.j, d.ls);
+}
+
+__global__ void __launch_bounds__(kGmThreads) __attribute__((amdgpu_waves_per_eu(1, 1))) gm_dcgs2_group(GmGroup a) {
+  extern __shared__ double gm_sh[];
+  const GmMember& d = a.m[blockIdx.x];
+  gm_dcgs2_body(d.st, d.j, d.ls, d.rec, d.tag, GmIncrement{}, gm_sh);
+}
+
+template <typename T>
+struct UpdMember {
+  int j;
+  T* V;
+  int64_t ldv;
+  const T* w;
+  const double* ls;
+  GmState st;
+};
+template <typename T>
+struct UpdGroup {
+  UpdMember<T> m[kGmGroupMax];
+};
+template <typename T>
+__global__ void gm_update_group(int64_t n, UpdGroup<T> a) {
+  extern __shared__ double gm_hs[];
+  const UpdMember<T>& d = a.m[blockIdx.y];
+  gm_update_body<T>(n, d.j, d.V, d.ldv, d.w, d.ls + d.st.cq(), d.ls + d.st.cw(), d.ls + d.st.scal(), gm_hs);
+}
+
+template <typename T>
+static bool gmres_step_group_t(int64_t n, int g, const GmGroupMember* mem, hipStream_t s) {
+  constexpr int width = sizeof(T) == 4 ? 4 : 2;
+  constexpr int G = kDotAcc / 2;  // rows per group of the two-vector pass
+  auto aligned = [](const void* p) { return ((uintptr_t)p & 15) == 0; };
+  // the vector path of each member's own step, which the group must share
+  bool vec0 = false;
+  for (int i = 0; i < g; ++i) {
+    const GmGroupMember& d = mem[i];
+    const int j = d.k - 1;
+    const size_t es = sizeof(T);
+    const void* u = (const char*)d.V + (size_t)j * d.ldv * es;
+    const bool v = (j + 1 == 1 || d.ldv % width == 0) && aligned(d.V) && aligned(u) && aligned(d.w);
+    if (i == 0) vec0 = v;
+    if (v != vec0) return false;
+  }
+  Dot1Group<T> a1{};
+  Dot2Group a2{};
+  GmGroup gs{};
+  UpdGroup<T> au{};
+  unsigned grid1 = 1, grid2 = 1, gridh = 0;
+  int jmax = 0;
+  for (int i = 0; i < g; ++i) {
+    const GmGroupMember& d = mem[i];
+    const GmState st{d.m};
+    const int j = d.k - 1, k = j + 1;
+    const int groups = (k + G - 1) / G;
+    const int nb = dot_chunks(n, groups, sizeof(T));
+    a1.m[i] = Dot1Member<T>{k, nb, groups, (const T*)d.V, d.ldv, (const T*)d.V + (int64_t)j * d.ldv,
+                            (const T*)d.w, (double*)d.work};
+    grid1 = std::max(grid1, (unsigned)(((nb + 7) / 8) * 8 * groups));
+    a2.m[i] = Dot2Member{nb, k, (const double*)d.work, d.ls + st.U(), st.W() - st.U()};
+    grid2 = std::max(grid2, (unsigned)((2 * k + kRedThreads / 64 - 1) / (kRedThreads / 64)));
+    gs.m[i] = GmMember{st, j >= kGmHsFrom ? j : 0, d.ls, d.rec, d.tag};
+    if (j >= kGmHsFrom) gridh = std::max(gridh, (unsigned)((j + 3) / 4));
+    au.m[i] = UpdMember<T>{j, (T*)d.V, d.ldv, (const T*)d.w, d.ls, st};
+    jmax = std::max(jmax, j);
+  }
+  if (vec0)
+    dot_many_stage1_group<T, true><<<dim3(grid1, (unsigned)g), kRedThreads, 0, s>>>(n, a1);
+  else
+    dot_many_stage1_group<T, false><<<dim3(grid1, (unsigned)g), kRedThreads, 0, s>>>(n, a1);
+  IGA_LAUNCH_CHECK();
+  dot_many_stage2_group<<<dim3(grid2, (unsigned)g), kRedThreads, 0, s>>>(a2);
+  IGA_LAUNCH_CHECK();
+  if (gridh > 0) {
+    gm_hs_group<<<dim3(gridh, (unsigned)g), 256, 0, s>>>(gs);
+    IGA_LAUNCH_CHECK();
+  }
+  // the small dense kernel takes each member's own j (above, members below kGmHsFrom
+  // carried 0 for the product kernel)
+  for (int i = 0; i < g; ++i) gs.m[i].j = mem[i].k - 1;
+  gm_dcgs2_group<<<(unsigned)g, kGmThreads, sizeof(double) * (4 * (size_t)jmax + 1), s>>>(gs);
+  IGA_LAUNCH_CHECK();
+  const dim3 gridu(grid_for(n, 256, 8192), (unsigned)g);
+  gm_update_group<T><<<gridu, 256, sizeof(double) * 2 * (size_t)(jmax > 0 ? jmax : 1), s>>>(n, au);
+  IGA_LAUNCH_CHECK();
+  return true;
+}
+
+bool gmres_step_group_impl(int dtype, int64_t n, int g, const GmGroupMember* mem, void* stream) {
+  IGA_CHECK_ARG(g >= 1 && g <= kGmGroupMax && mem && n >= 1, "bad group");
+  for (int i = 0; i < g; ++i) {
+    const GmGroupMember& d = mem[i];
+    IGA_CHECK_ARG(d.V && d.w && d.ls && d.work && d.ldv >= n, "bad member");
+    IGA_CHECK_ARG(d.k >= 1 && d.k <= d.m + 1, "step outside the Krylov dimension");
+    IGA_CHECK_ARG(d.m <= kGmMaxDim, "Krylov dimension above 2000");
+  }
+  hipStream_t s = as_stream(stream);
+  if (dtype == IGA_F64) return gmres_step_group_t<double>(n, g, mem, s);
+  if (dtype == IGA_F32) return gmres_step_group_t<float>(n, g, mem, s);
+  throw Error(IGA_ERR_INVALID, "unknown dtype");
 }
 
 int64_t gmres_increment_doubles(int m) { return 1 + ((int64_t)m + 1) * gm_inc_stride(m); }
