@@ -743,9 +743,10 @@ def launch_group(group, work: dict, batch_stream=None):
     steps sharing the FD plan, the operator plan, the mask box and the vector type)
     queued by one library call (``iga_arnoldi_step_group``): their basis vectors as the
     columns of one multi-right-hand-side FD launch, one grouped operator launch into
-    each run's ``w``, then each run's orthogonalisation on its own stream.  The batch
-    goes on ``batch_stream`` (the first run's stream if None); ``work`` caches the
-    batch scratch per group size (allocated on the batch stream)."""
+    each run's ``w``, then the runs' orthogonalisations (one launch per kernel of the
+    step for all of them).  The batch goes on ``batch_stream`` (the first run's stream if
+    None), each run's stream ordered after it; ``work`` caches the batch scratch per
+    group size (allocated on the batch stream)."""
     t = dv.torch()
     g = len(group)
     if g < 1 or g > 16:

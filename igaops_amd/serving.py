@@ -421,8 +421,8 @@ class SolveServer:
         library call per group (``iga_arnoldi_step_group``): their preconditioner
         applications in one multi-right-hand-side FD launch and their operators in one
         grouped launch (one plan, each request's coefficients) on the batch stream,
-        then each request's orthogonalisation on its own stream; events order the batch
-        after each request's previous work and each request after the batch."""
+        then their orthogonalisations (one launch per kernel for the group); events order
+        the batch after each request's previous work and each request after the batch."""
         from .runs import launch_group
 
         groups: Dict[tuple, list] = {}

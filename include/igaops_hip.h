@@ -423,11 +423,13 @@ int iga_arnoldi_set_increment(iga_arnoldi_t ctx, void* Z, int64_t ldz, double* g
  * solve i on basis Vs[i] (row stride ldvs[i]).  The vectors V_i[k_i - 1] go, one
  * after the other, through one multi-right-hand-side FD apply (iga_fd_apply_rhs_major)
  * and the results through one
- * iga_fused_apply_group launch into each context's w (masked); then each solve's
- * orthogonalisation (iga_gmres_step, residual estimate published as by
- * iga_arnoldi_step) is queued on streams[i].  The batch runs on batch_stream, after
- * every member stream's earlier work and before the orthogonalisations (events; a
- * member stream equal to batch_stream needs none).  xy: scratch of
+ * iga_fused_apply_group launch into each context's w (masked); then the solves'
+ * orthogonalisations (iga_gmres_step's results bit for bit, residual estimates
+ * published as by iga_arnoldi_step), one launch per kernel of the step for all of them
+ * (per-member launches on streams[i] when the members' basis layouts differ).  All of it
+ * runs on batch_stream, after every member stream's earlier work, and each member
+ * stream is ordered after it (events; a member stream equal to batch_stream needs
+ * none).  xy: scratch of
  * iga_arnoldi_group_workspace_bytes(ctx, g) bytes; fd_work: iga_fd_workspace_bytes(fd,
  * g); op_work: iga_fused_workspace_bytes(op, g).  The reference has no counterpart: it
  * runs one solve at a time (solvers/linear_solver.py:438-525); this entry point
