@@ -207,7 +207,7 @@ def solve_section(patch, C, m, nrhs, tol=1e-10):
 
     from igaops_amd.fastdiag import SingleFD
     from igaops_amd.operators import StiffnessMassOperator
-    from igaops_amd.runs import GmresRun, NativeStep, RefinementRun
+    from igaops_amd.runs import REFINEMENT_INNER_TOL, GmresRun, NativeStep, RefinementRun
     from igaops_amd.solvers import Mask
 
     qs = patch.quadrule_list
@@ -246,7 +246,8 @@ def solve_section(patch, C, m, nrhs, tol=1e-10):
                 out = GmresRun(a64, P["f64"], b, tol, 400, mask=mask,
                                native=NativeStep(A64, plans["f64"], works["f64"], mask.box)).run()
             else:
-                out = RefinementRun(a64, a32, P["f32"], b, tol=tol, inner_tol=1e-6, inner_maxiters=300, mask=mask,
+                out = RefinementRun(a64, a32, P["f32"], b, tol=tol, inner_tol=REFINEMENT_INNER_TOL, inner_maxiters=300,
+                                    mask=mask,
                                     native_lo=NativeStep(A32, plans["f32"], works["f32"], mask.box)).run()
             torch.cuda.synchronize()
             if warm:
@@ -291,7 +292,8 @@ def solve_section(patch, C, m, nrhs, tol=1e-10):
             if mode == "fp64":
                 return GmresRun(a64s, Pi["f64"], b, tol, 400, mask=mask, stream=s,
                                 native=NativeStep(A64, plans["f64"], fd_works["f64"][j], mask.box))
-            return RefinementRun(a64s, a32, Pi["f32"], b, tol=tol, inner_tol=1e-6, inner_maxiters=300, mask=mask,
+            return RefinementRun(a64s, a32, Pi["f32"], b, tol=tol, inner_tol=REFINEMENT_INNER_TOL, inner_maxiters=300,
+                                 mask=mask,
                                  stream=s, native_lo=NativeStep(A32, plans["f32"], fd_works["f32"][j], mask.box))
 
     for mode in ("fp64", "mixed"):
@@ -416,7 +418,7 @@ def c3_section(degrees=(2, 4, 6), nel=128, tol=1e-10):
     from igaops_amd.fastdiag import SingleFD
     from igaops_amd.operators import StiffnessMassOperator
     from igaops_amd.quadrature import StandardGauss
-    from igaops_amd.runs import GmresRun, NativeStep, RefinementRun
+    from igaops_amd.runs import REFINEMENT_INNER_TOL, GmresRun, NativeStep, RefinementRun
     from igaops_amd.solvers import Mask
     from oracle import fastdiag as ofd
     from oracle import operators as oo
@@ -481,7 +483,8 @@ def c3_section(degrees=(2, 4, 6), nel=128, tol=1e-10):
                 if plan_name == "f64":
                     res = GmresRun(a64, P, b, tol, 600, mask=mask, native=NativeStep(A64, plan, work, mask.box)).run()
                 else:
-                    res = RefinementRun(a64, a32, P, b, tol=tol, inner_tol=1e-6, inner_maxiters=400, mask=mask,
+                    res = RefinementRun(a64, a32, P, b, tol=tol, inner_tol=REFINEMENT_INNER_TOL, inner_maxiters=400,
+                                        mask=mask,
                                         native_lo=NativeStep(A32, plan, work, mask.box)).run()
                 torch.cuda.synchronize()
             fdr[plan_name]["solve"] = {"s": time.perf_counter() - t0, "iterations": int(res.iterations),

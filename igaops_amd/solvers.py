@@ -409,7 +409,7 @@ def refinement_solve(
                       history={"inner_iterations": inner_its})
 
 
-def mixed_solve(A64: Callable, A_lo: Callable, P_lo: Callable, b, tol: float = 1e-10, inner_tol: float = 1e-6,
+def mixed_solve(A64: Callable, A_lo: Callable, P_lo: Callable, b, tol: float = 1e-10, inner_tol: float = 1e-4,
                 inner_maxiters: int = 200, max_refinements: int = 30, mask: Optional[Mask] = None,
                 lo_dtype: str = "f32") -> OutputArgs:
     """Iterative refinement with float64 residuals around inner GMRES solves whose
