@@ -599,18 +599,27 @@ __device__ __attribute__((always_inline)) inline void fused_apply_body(const Fus
 #endif
 constexpr int kWidePrefetchF64 = IGA_FUSED_WIDE_PD_F64;
 
+// Coefficient prefetch depth (q3 planes) of the FP32 sweep: two for the ten-field NURBS
+// form up to degree 5 (p=4 n=64: 78.3 -> 75.3 us, p=5: 88.0 -> 78.0 us against three),
+// three otherwise (the unit cube's seven fields at p=4 and the NURBS form at p=6 were
+// 1.5-4% slower with two).
+template <int P, int FORM>
+constexpr int prefetch_f32() {
+  return FORM == FORM_SYM10 && P <= 6 ? 2 : 3;
+}
+
 // Two launch variants of the same sweep: with an occupancy target of three waves
 // per SIMD for FP64 and four for FP32 (used when the band is narrow enough to fit
 // without spilling) and without (wider bands, one wave per SIMD fewer).
 template <typename T, int P, int FORM, int NR>
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 3, 8)))
 fused_apply_kernel(FusedArgs<T> a) {
-  fused_apply_body<T, P, FORM, NR, sizeof(T) == 8 ? 2 : 3>(a);
+  fused_apply_body<T, P, FORM, NR, sizeof(T) == 8 ? 2 : prefetch_f32<P, FORM>()>(a);
 }
 
 template <typename T, int P, int FORM, int NR>
 __global__ void __launch_bounds__(kThreads) fused_apply_kernel_wide(FusedArgs<T> a) {
-  fused_apply_body<T, P, FORM, NR, sizeof(T) == 8 ? kWidePrefetchF64 : 3>(a);
+  fused_apply_body<T, P, FORM, NR, sizeof(T) == 8 ? kWidePrefetchF64 : prefetch_f32<P, FORM>()>(a);
 }
 
 // forms with a two-right-hand-side variant (the solver and benchmark forms)
