@@ -790,10 +790,10 @@ def run_batched(runs, max_group: int = 16, batch_stream=None, lockstep: bool = T
     steps (``native`` / ``native_lo``) share the FD plan and the mask box and whose
     operators share one fused plan (the same operator for several right-hand sides, or
     one per solve).  Every pass, the runs that can take a step take it together (up to
-    ``max_group`` per launch): the FD and operator launches go on ``batch_stream`` (the
-    runs' stream when they all share one), each run's orthogonalisation on its own
-    stream, so that the small kernels of different runs overlap when the runs have
-    streams of their own.  With ``lockstep`` a group step goes out only when every
+    ``max_group`` per launch): the FD, operator and orthogonalisation launches of a
+    group go on ``batch_stream`` (the runs' stream when they all share one), and each
+    run's own later work (refinement residuals, solution updates) on its stream is
+    ordered after them.  With ``lockstep`` a group step goes out only when every
     run that is still taking Arnoldi steps can take one, so that each FD and operator
     launch serves all of them (the runs' records arrive at slightly different times,
     and launching whatever is ready splits the steps into groups of one or two); no
