@@ -273,9 +273,8 @@ def solve_section(patch, C, m, nrhs, tol=1e-10):
     from igaops_amd import device as dv
     from igaops_amd.runs import run_batched
 
-    # each solve's orthogonalisation on one of four streams (the box's hardware queues),
-    # so that the small single-work-group kernels of different solves overlap; the FD
-    # and operator launches of a group go on a batch stream
+    # the solves on four streams (their starts, refinement residuals and solution updates);
+    # a group step's FD, operator and orthogonalisation launches go on a batch stream
     streams = [torch.cuda.Stream() for _ in range(4)]
     bstream = torch.cuda.Stream()
     # scratch per stream: the FD workspace and the operators' partial blocks of work on
