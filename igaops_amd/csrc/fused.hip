@@ -1320,7 +1320,12 @@ int iga_fused_create(iga_fused_t* out, const int32_t* nc, const int32_t* nq, int
       if (ns <= 0) {
         const int64_t tiles = (int64_t)f->nt1 * f->nt2;
         int dev = 0, ncu = 0;
-        const int per_cu = dtype == IGA_F64 ? resident_blocks<double>(P, form) : resident_blocks<float>(P, form);
+        // FP64 work-groups count two per compute unit whatever the occupancy allows: a
+        // third resident one adds no throughput (over p 2..5, nel 32..96, both
+        // geometries, sweep + gather within 2.2% of the best slab count on average
+        // with two, 4.4% and up to 31% slower with the occupancy's three)
+        int per_cu = dtype == IGA_F64 ? resident_blocks<double>(P, form) : resident_blocks<float>(P, form);
+        if (dtype == IGA_F64 && per_cu > 2) per_cu = 2;
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
           ncu = 0;
