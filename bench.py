@@ -685,7 +685,7 @@ def pmc_child(args):
     right-hand side, then three 512 MiB device copies (the counters' scale)."""
     import torch
 
-    from igaops_amd.operators import StiffnessMassOperator
+    from igaops_amd.operators import BATCH_PLAN_MIN_RHS, StiffnessMassOperator
 
     cfg = dict(WORKLOADS[args.workload])
     patch, C, m = build_patch(cfg)
@@ -805,7 +805,7 @@ def main():
 
     import torch
 
-    from igaops_amd.operators import StiffnessMassOperator
+    from igaops_amd.operators import BATCH_PLAN_MIN_RHS, StiffnessMassOperator
 
     cfg = dict(WORKLOADS[args.workload])
     if args.nrhs:
@@ -856,16 +856,18 @@ def main():
         # barrier serves both; equal to rounding); the faster is reported
         Xf, Yf = X.reshape(-1), Y.reshape(-1)
         by_nr, outs = {}, {}
+        # (the operator's multi-right-hand-side launches run on its fewest-slab plan)
         for nr in (1, 2):
-            plan.set_rhs_block(nr)
+            op.set_rhs_block(nr)
             bms = time_events(lambda: op.apply(Xf, out=Yf, nrhs=r_batch), max(3, args.steps // 2), 1)
             by_nr[nr] = float(np.mean(bms))
             outs[nr] = Yf.clone()
         best = min(by_nr, key=by_nr.get)
-        plan.set_rhs_block(1)
+        op.set_rhs_block(1)
         tb = by_nr[best] * 1e-3
         batched = {
             "nrhs": r_batch,
+            "q3_slabs": op.batch_plan().nslab if r_batch >= BATCH_PLAN_MIN_RHS else plan.nslab,
             "ms": tb * 1e3,
             "gdofs": n * r_batch / tb / 1e9,
             "pct_hbm_roofline": op.algorithmic_bytes(r_batch) / tb / 1e9 / HBM_PEAK_GBS,

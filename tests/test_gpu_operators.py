@@ -297,6 +297,44 @@ def test_first_order_terms_in_one_sweep(kind, geom):
 
 
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_fewest_slab_plan_of_several_right_hand_sides(dtype):
+    """Several right-hand sides run on the operator's fewest-slab plan (the same packed
+    coefficients): the same values as the single-right-hand-side plan to rounding (the
+    partial blocks are summed over other slabs), each column against the oracle; the
+    batch view applies one right-hand side on that plan too."""
+    import torch
+
+    from igaops_amd.operators import BATCH_PLAN_MIN_RHS, StiffnessMassOperator
+
+    patch = quarter_annulus(4, 12)
+    qs = patch.quadrule_list
+    w = patch.nurbs_weights
+    C, m = patch.conductivity_coefficients(lognormal_conductivity(patch, 0.5, seed=6))
+    op = StiffnessMassOperator(qs, C=C, m=m, nurbs_weights=w, dtype=dtype)
+    bp = op.batch_plan()
+    print("case", dtype, "slabs", op.plan.nslab, bp.nslab, flush=True)
+    assert bp.nslab <= op.plan.nslab and bp.nslab == -(-int(patch.nbquadpts[2]) // 128)
+    n = patch.nbctrlpts_total
+    r = BATCH_PLAN_MIN_RHS + 1
+    tdt = torch.float64 if dtype == "f64" else torch.float32
+    X = torch.randn(n, r, dtype=torch.float64, device="cuda").to(tdt)
+    xf = X.reshape(-1).contiguous()
+    y_batch = op.apply(xf, nrhs=r).reshape(n, r)
+    y_plan = op.apply(xf, nrhs=r, work=op.workspace(r)).reshape(n, r)
+    d = float((y_batch - y_plan).abs().max() / y_plan.abs().max())
+    assert d <= (1e-14 if dtype == "f64" else 1e-6), d
+    Xh = X.double().cpu().numpy()
+    tol = F64_TOL if dtype == "f64" else F32_TOL
+    for k in (0, r - 1):
+        ref = oo.stiffness_plus_mass(qs, C, m, Xh[:, k], nurbs_weights=w)
+        assert rel(y_batch[:, k].double().cpu().numpy(), ref) <= tol
+    v = op.batch_view()
+    assert v.plan is bp and v.packed is op.packed
+    y1 = v.apply(X[:, 0].contiguous())
+    assert float((y1 - y_plan[:, 0]).abs().max() / y_plan[:, 0].abs().max()) <= (1e-14 if dtype == "f64" else 1e-6)
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
 @pytest.mark.parametrize("geom,mass", [("annulus", True), ("cube", True), ("cube", False)])
 def test_two_right_hand_sides_per_workgroup_same_bits(dtype, geom, mass):
     """Right-hand sides sharing one coefficient set swept two per work-group (the
@@ -318,9 +356,9 @@ def test_two_right_hand_sides_per_workgroup_same_bits(dtype, geom, mass):
     X = torch.randn(n, 6, dtype=torch.float64, device="cuda").to(tdt)
     outs = {}
     for nr in (1, 2):
-        op.plan.set_rhs_block(nr)
+        op.set_rhs_block(nr)
         outs[nr] = op.apply(X.reshape(-1).contiguous(), nrhs=6).reshape(n, 6)
-    op.plan.set_rhs_block(2)
+    op.set_rhs_block(1)
     d = float((outs[1] - outs[2]).abs().max() / outs[1].abs().max())
     print("two-per-work-group difference", d, flush=True)
     assert d <= (1e-14 if dtype == "f64" else 1e-6)
