@@ -55,13 +55,15 @@ static int reduce_blocks(int64_t n) {
 #define IGA_DOT_BLOCKS 4096
 #endif
 #ifndef IGA_DOT_EPT
-#define IGA_DOT_EPT 4
+#define IGA_DOT_EPT 8
 #endif
 constexpr int kDotAcc = 16;
 constexpr int kDotBlocksTarget = IGA_DOT_BLOCKS;
 constexpr int kDotMaxVectors = 16384;
-// 16-byte vectors per thread and chunk: 4 took the GMRES step at n = 314 432 from 77 to 68 us
-// (FP64, 65 rows) against 2; 1 was slower, 8 no faster
+// 16-byte vectors per thread and chunk.  GMRES step at n = 314 432: 4 took FP64 at basis
+// size 64 from 77 to 68 us against 2 (1 was slower); 8 against 4, at basis sizes 16 / 32 /
+// 64: FP64 34.1 -> 31.8, 48.0 -> 44.1, 67.9 -> 67.9 us, FP32 24.9 -> 26.0, 32.2 -> 30.0,
+// 46.1 -> 43.8 us (the 2000-request serving stream: FP64 49.1 -> 50.0 solves/s, mixed equal)
 constexpr int kDotElemsPerThread = IGA_DOT_EPT;
 
 template <typename T> struct Vec16;
