@@ -47,6 +47,7 @@ from typing import Dict, List, Optional, Tuple
 import numpy as np
 
 from . import device as dv
+from .runs import REFINEMENT_INNER_TOL
 
 _RULES: Dict[Tuple[int, int], object] = {}
 _PENCILS: Dict[Tuple[int, int], tuple] = {}
@@ -351,7 +352,8 @@ class SolveServer:
     """Continuous admission over a stream of requests (see the module docstring)."""
 
     def __init__(self, mode: str = "mixed", hbm_budget_gb: float = 200.0, max_active: int = 16, tol: float = 1e-10,
-                 maxiters: int = 800, inner_tol: float = 1e-4, inner_maxiters: int = 300, lookahead: int = 2,
+                 maxiters: int = 800, inner_tol: float = REFINEMENT_INNER_TOL, inner_maxiters: int = 300,
+                 lookahead: int = 2,
                  keep_solutions: bool = False, batch_steps: bool = True):
         if mode not in ("mixed", "fp64"):
             raise ValueError("mode must be 'mixed' or 'fp64'")

@@ -409,14 +409,16 @@ def refinement_solve(
                       history={"inner_iterations": inner_its})
 
 
-def mixed_solve(A64: Callable, A_lo: Callable, P_lo: Callable, b, tol: float = 1e-10, inner_tol: float = 1e-4,
-                inner_maxiters: int = 200, max_refinements: int = 30, mask: Optional[Mask] = None,
+def mixed_solve(A64: Callable, A_lo: Callable, P_lo: Callable, b, tol: float = 1e-10,
+                inner_tol: Optional[float] = None, inner_maxiters: int = 200, max_refinements: int = 30, mask: Optional[Mask] = None,
                 lo_dtype: str = "f32") -> OutputArgs:
     """Iterative refinement with float64 residuals around inner GMRES solves whose
-    vectors, operator ``A_lo`` and preconditioner ``P_lo`` are ``lo_dtype``."""
-    from .runs import RefinementRun
+    vectors, operator ``A_lo`` and preconditioner ``P_lo`` are ``lo_dtype``
+    (``inner_tol`` None: ``runs.REFINEMENT_INNER_TOL``)."""
+    from .runs import REFINEMENT_INNER_TOL, RefinementRun
 
-    return RefinementRun(A64, A_lo, P_lo, b, tol, inner_tol, inner_maxiters, max_refinements, mask=mask,
+    itol = REFINEMENT_INNER_TOL if inner_tol is None else inner_tol
+    return RefinementRun(A64, A_lo, P_lo, b, tol, itol, inner_maxiters, max_refinements, mask=mask,
                          lo_dtype=lo_dtype).run()
 
 
